@@ -1,0 +1,25 @@
+"""dygrass_amd -- MI355X-native dyGRASS random-walk sparsifier updaters.
+
+The compute path is libdygrass.so (hand-written HIP for gfx950 behind a C
+ABI, include/dygrass.h).  This package is its Python host interface; it has
+no CPU implementation of any walk.
+"""
+from ._lib import DygError, WalkParams, WalkStats, RunStats, LIB_PATH, BIN_DIR  # noqa: F401
+from .core import (  # noqa: F401
+    Context,
+    DeviceGraph,
+    DualGraph,
+    WalkResult,
+    csr_from_edges,
+    incremental_update,
+    params,
+    walk_decremental,
+    walk_heuristic,
+    walk_incremental,
+)
+
+__all__ = [
+    "Context", "DeviceGraph", "DualGraph", "WalkResult", "csr_from_edges", "incremental_update",
+    "params", "walk_decremental", "walk_heuristic", "walk_incremental", "DygError", "WalkParams",
+    "WalkStats", "RunStats", "LIB_PATH", "BIN_DIR",
+]

@@ -1,0 +1,298 @@
+"""Python host interface over the C ABI.
+
+Mirrors the reference's units one to one (citations on each method):
+``DeviceGraph`` is the device graph store (gpu_graph / GPU_Dual_Graph's dense
+half), ``walk_incremental`` / ``walk_decremental`` / ``walk_heuristic`` are the
+three kernels with their result copies, ``incremental_update`` is the
+incremental CLI's batch loop and ``DualGraph`` the decremental dual graph.
+All compute runs in libdygrass.so on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import check, ptr
+
+
+def _u32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def _u64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.uint64)
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+@dataclass
+class WalkResult:
+    found_at: np.ndarray  # int32, winner's hop index or -1
+    cond: np.ndarray      # float32, winner's 1/R or -1
+    winner: np.ndarray    # uint32, winning walker
+    hops: np.ndarray      # uint32, neighbour draws of each edge
+    path: np.ndarray | None = None  # uint32 [n, max_steps] (deletions)
+    stats: dict | None = None
+
+
+def params(walkers: int, max_steps: int = 100, threshold: float = 0.0, seed: int = 0,
+           prune: bool = True) -> L.WalkParams:
+    return L.WalkParams(walkers, max_steps, threshold, 0 if prune else L.DYG_WALK_NO_PRUNE, seed)
+
+
+class Context:
+    """One HIP device + stream (dyg_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self._lib = L.lib()
+        h = C.c_void_p()
+        check(self._lib.dyg_ctx_create(device, C.byref(h)))
+        self.handle = h
+
+    def sync(self) -> None:
+        check(self._lib.dyg_ctx_sync(self.handle), self.handle)
+
+    @property
+    def stream(self) -> int:
+        return self._lib.dyg_ctx_stream(self.handle)
+
+    def close(self) -> None:
+        if self.handle:
+            self._lib.dyg_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def csr_from_edges(V: int, u, v, w) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """CSR in the reference's slot order: line (a,b,w) appends b to a's row and
+    then a to b's row (random_walk_incremental/functions.cpp:168-232)."""
+    u = np.asarray(u, dtype=np.int64)
+    v = np.asarray(v, dtype=np.int64)
+    w = np.asarray(w, dtype=np.float32)
+    m = len(u)
+    src = np.empty(2 * m, dtype=np.int64)
+    dst = np.empty(2 * m, dtype=np.int64)
+    ww = np.empty(2 * m, dtype=np.float32)
+    src[0::2], src[1::2] = u, v
+    dst[0::2], dst[1::2] = v, u
+    ww[0::2], ww[1::2] = w, w
+    order = np.argsort(src, kind="stable")
+    row_ptr = np.zeros(V + 1, dtype=np.uint64)
+    np.add.at(row_ptr, src + 1, 1)
+    row_ptr = np.cumsum(row_ptr, dtype=np.uint64)
+    return row_ptr, dst[order].astype(np.uint32), ww[order]
+
+
+class DeviceGraph:
+    """Device graph store with per-row slack (dyg_graph)."""
+
+    def __init__(self, ctx: Context, V: int, row_ptr, col, w, slack: int = 2, keyed: bool = False):
+        self.ctx = ctx
+        self._lib = ctx._lib
+        self.V = int(V)
+        rp, cl, ww = _u64(row_ptr), _u32(col), _f32(w)
+        h = C.c_void_p()
+        check(self._lib.dyg_graph_upload(ctx.handle, self.V, ptr(rp, L.u64p), ptr(cl, L.u32p),
+                                         ptr(ww, L.f32p), slack, int(keyed), C.byref(h)),
+              ctx.handle)
+        self.handle = h
+
+    @classmethod
+    def from_edges(cls, ctx: Context, V: int, u, v, w, slack: int = 2, keyed: bool = False):
+        rp, cl, ww = csr_from_edges(V, u, v, w)
+        return cls(ctx, V, rp, cl, ww, slack=slack, keyed=keyed)
+
+    def info(self) -> tuple[int, int, int]:
+        V, n, cap = C.c_uint32(), C.c_uint64(), C.c_uint64()
+        check(self._lib.dyg_graph_info(self.handle, C.byref(V), C.byref(n), C.byref(cap)))
+        return V.value, n.value, cap.value
+
+    def download(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        _, n, _ = self.info()
+        rp = np.zeros(self.V + 1, dtype=np.uint64)
+        col = np.zeros(max(n, 1), dtype=np.uint32)
+        w = np.zeros(max(n, 1), dtype=np.float32)
+        check(self._lib.dyg_graph_download(self.handle, ptr(rp, L.u64p), ptr(col, L.u32p),
+                                           ptr(w, L.f32p)), self.ctx.handle)
+        return rp, col[:n], w[:n]
+
+    def insert(self, u, v, w) -> None:
+        """random_walk_incremental/gpu_graph.cuh:319-325,357-384 (append in order)."""
+        u, v, w = _u32(u), _u32(v), _f32(w)
+        check(self._lib.dyg_graph_insert(self.handle, ptr(u, L.u32p), ptr(v, L.u32p),
+                                         ptr(w, L.f32p), len(u)), self.ctx.handle)
+
+    def delete(self, u, v) -> np.ndarray:
+        """random_walk_decremental/gpu_graph.cuh:338-400 (swap-with-last)."""
+        u, v = _u32(u), _u32(v)
+        hit = np.zeros(max(len(u), 1), dtype=np.int8)
+        check(self._lib.dyg_graph_delete(self.handle, ptr(u, L.u32p), ptr(v, L.u32p), len(u),
+                                         ptr(hit, L.i8p)), self.ctx.handle)
+        return hit[: len(u)]
+
+    def close(self) -> None:
+        if self.handle:
+            self._lib.dyg_graph_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def walk_incremental(H: DeviceGraph, src, dst, w, p: L.WalkParams, edge_id=None) -> WalkResult:
+    """random_walk_incremental/main.cu:70-211 + get_result (gpu_graph.cuh:551-554)."""
+    src, dst, w = _u32(src), _u32(dst), _f32(w)
+    n = len(src)
+    eid = None if edge_id is None else _u64(edge_id)
+    f = np.zeros(max(n, 1), np.int32)
+    c = np.zeros(max(n, 1), np.float32)
+    win = np.zeros(max(n, 1), np.uint32)
+    hops = np.zeros(max(n, 1), np.uint32)
+    st = L.WalkStats()
+    check(H._lib.dyg_walk_incremental(H.handle, ptr(src, L.u32p), ptr(dst, L.u32p),
+                                      ptr(w, L.f32p), ptr(eid, L.u64p), n, C.byref(p),
+                                      ptr(f, L.i32p), ptr(c, L.f32p), ptr(win, L.u32p),
+                                      ptr(hops, L.u32p), C.byref(st)), H.ctx.handle)
+    return WalkResult(f[:n], c[:n], win[:n], hops[:n], None,
+                      dict(hops=st.hops, redirects=st.redirects, leaf_stops=st.leaf_stops,
+                           kernel_ms=st.kernel_ms))
+
+
+def walk_decremental(G: DeviceGraph, src, dst, p: L.WalkParams, edge_id=None) -> WalkResult:
+    """random_walk_decremental/main.cu:77-220 + get_result (gpu_graph.cuh:831-841)."""
+    src, dst = _u32(src), _u32(dst)
+    n = len(src)
+    eid = None if edge_id is None else _u64(edge_id)
+    f = np.zeros(max(n, 1), np.int32)
+    c = np.zeros(max(n, 1), np.float32)
+    win = np.zeros(max(n, 1), np.uint32)
+    hops = np.zeros(max(n, 1), np.uint32)
+    path = np.zeros(max(n, 1) * p.max_steps, np.uint32)
+    st = L.WalkStats()
+    check(G._lib.dyg_walk_decremental(G.handle, ptr(src, L.u32p), ptr(dst, L.u32p),
+                                      ptr(eid, L.u64p), n, C.byref(p), ptr(f, L.i32p),
+                                      ptr(c, L.f32p), ptr(win, L.u32p), ptr(hops, L.u32p),
+                                      ptr(path, L.u32p), C.byref(st)), G.ctx.handle)
+    return WalkResult(f[:n], c[:n], win[:n], hops[:n], path.reshape(-1, p.max_steps)[:n],
+                      dict(hops=st.hops, redirects=st.redirects, leaf_stops=st.leaf_stops,
+                           kernel_ms=st.kernel_ms))
+
+
+def walk_heuristic(G: DeviceGraph, endpoints, keys, walkers: int = 16, seed: int = 0) -> np.ndarray:
+    """random_walk_decremental/main.cu:243-290 (hop two from n1's row)."""
+    ep, k = _u32(endpoints), _u64(keys)
+    n = len(ep)
+    out = np.zeros(max(n, 1) * walkers * 3, np.uint32)
+    check(G._lib.dyg_walk_heuristic(G.handle, ptr(ep, L.u32p), ptr(k, L.u64p), n, walkers, seed,
+                                    ptr(out, L.u32p)), G.ctx.handle)
+    return out[: n * walkers * 3].reshape(n, walkers, 3)
+
+
+def incremental_update(H: DeviceGraph, src, dst, w, p: L.WalkParams, divisor: int = 10):
+    """The incremental CLI's batch loop (random_walk_incremental/main.cu:297-340)."""
+    src, dst, w = _u32(src), _u32(dst), _f32(w)
+    m = len(src)
+    f = np.zeros(max(m, 1), np.int32)
+    st = L.RunStats()
+    check(H._lib.dyg_incremental_update(H.handle, ptr(src, L.u32p), ptr(dst, L.u32p),
+                                        ptr(w, L.f32p), m, divisor, C.byref(p), ptr(f, L.i32p),
+                                        C.byref(st)), H.ctx.handle)
+    stats = {k: getattr(st, k) for k, _ in L.RunStats._fields_}
+    return f[:m], stats
+
+
+class DualGraph:
+    """GPU_Dual_Graph (random_walk_decremental/gpu_graph.cuh:79-687): G on the
+    device with its edge map, H as a host key set."""
+
+    def __init__(self, ctx: Context, V: int, g_row_ptr, g_col, g_w, h_u, h_v):
+        self.ctx = ctx
+        self._lib = ctx._lib
+        rp, cl, ww = _u64(g_row_ptr), _u32(g_col), _f32(g_w)
+        hu, hv = _u32(h_u), _u32(h_v)
+        h = C.c_void_p()
+        check(self._lib.dyg_dual_create(ctx.handle, V, ptr(rp, L.u64p), ptr(cl, L.u32p),
+                                        ptr(ww, L.f32p), ptr(hu, L.u32p), ptr(hv, L.u32p),
+                                        len(hu), C.byref(h)), ctx.handle)
+        self.handle = h
+
+    @property
+    def multiplier(self) -> int:
+        return self._lib.dyg_dual_key_multiplier(self.handle)
+
+    def remove(self, u, v, ids=None) -> tuple[np.ndarray, int]:
+        """removeAndUpdateDelEdges (gpu_graph.cuh:328-477)."""
+        u, v = _u32(u), _u32(v)
+        n = len(u)
+        idv = None if ids is None else _u64(ids)
+        status = np.zeros(max(n, 1), np.int8)
+        q = C.c_uint32()
+        check(self._lib.dyg_dual_remove(self.handle, ptr(u, L.u32p), ptr(v, L.u32p),
+                                        ptr(idv, L.u64p), n, ptr(status, L.i8p), C.byref(q)),
+              self.ctx.handle)
+        return status[:n], q.value
+
+    def walk(self, p: L.WalkParams) -> tuple[int, int]:
+        """random_walk_decremental + incrementalUpdateSparse (gpu_graph.cuh:509-559)."""
+        nop, add = C.c_uint32(), C.c_uint64()
+        check(self._lib.dyg_dual_walk(self.handle, C.byref(p), C.byref(nop), C.byref(add), None),
+              self.ctx.handle)
+        return nop.value, add.value
+
+    def heuristic(self, walkers: int = 16, seed: int = 0) -> None:
+        """random_walk_small + heuristicRecovery (gpu_graph.cuh:577-615)."""
+        check(self._lib.dyg_dual_heuristic(self.handle, walkers, seed), self.ctx.handle)
+
+    @property
+    def sparse_size(self) -> int:
+        return self._lib.dyg_dual_sparse_size(self.handle)
+
+    def added_edges(self) -> np.ndarray:
+        n = self._lib.dyg_dual_added_count(self.handle)
+        u = np.zeros(max(n, 1), np.uint32)
+        v = np.zeros(max(n, 1), np.uint32)
+        check(self._lib.dyg_dual_added_edges(self.handle, ptr(u, L.u32p), ptr(v, L.u32p)))
+        return np.stack([u[:n], v[:n]], axis=1)
+
+    def run(self, du, dv, p: L.WalkParams, divisor: int = 10, heuristic_walkers: int = 16):
+        """Whole decremental loop (random_walk_decremental/main.cu:363-444)."""
+        du, dv = _u32(du), _u32(dv)
+        M = len(du)
+        I = (M + divisor - 1) // divisor if M else 0
+        log = []
+        b0 = 0
+        while True:
+            n = min(I, M - b0)
+            status, q = self.remove(du[b0:b0 + n], dv[b0:b0 + n], np.arange(b0, b0 + n))
+            nop, add = self.walk(p)
+            log.append(dict(batch_start=b0, queued=q, no_path=nop, added=add))
+            if nop:
+                self.heuristic(heuristic_walkers, p.seed)
+            b0 += n
+            if b0 >= M:
+                break
+        return log
+
+    def close(self) -> None:
+        if self.handle:
+            self._lib.dyg_dual_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

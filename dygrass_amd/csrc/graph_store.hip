@@ -1,0 +1,432 @@
+// graph_store.hip -- device graph store with per-row slack and in-place edits.
+//
+// HBM layout (one graph):
+//   rows[V]        uint2 (first slot, degree)           8 B per vertex
+//   adj[capacity]  uint2 (neighbour, weight bits)        8 B per slot
+// Each row owns `cap` consecutive slots; the first `deg` are live, in the
+// reference's slot order.  Inserts append at slot deg; a full row moves to
+// the arena tail with its order kept (the reference's relocation,
+// random_walk_incremental/gpu_graph.cuh:357-384).  Deletes move the row's
+// last slot into the hole (random_walk_decremental/gpu_graph.cuh:368-399).
+//
+// The host keeps a mirror of rows and slots.  An edit is planned on the host
+// in reference order and shipped as a delta (copies, slot writes, row
+// writes) that one kernel applies; the reference instead re-uploads the whole
+// arena after every batch (gpu_graph.cuh:333-335, :462-465).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "internal.h"
+
+namespace dyg {
+
+hipError_t Pinned::reserve(size_t n) {
+    if (n <= bytes) return hipSuccess;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+    size_t want = std::max(n, (size_t)1 << 20);
+    hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+    if (e == hipSuccess) bytes = want;
+    return e;
+}
+Pinned::~Pinned() {
+    if (p) (void)hipHostFree(p);
+}
+
+hipError_t DevBuf::reserve(size_t n) {
+    if (n <= bytes) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    size_t want = std::max(n, (size_t)1 << 20);
+    hipError_t e = hipMalloc(&p, want);
+    if (e == hipSuccess) bytes = want;
+    return e;
+}
+DevBuf::~DevBuf() {
+    if (p) (void)hipFree(p);
+}
+
+long key_multiplier(uint64_t V) {
+    // 10^digits(V) (random_walk_decremental/functions.cpp:129-130)
+    long m = 1;
+    for (uint64_t n = V; n != 0; n /= 10) m *= 10;
+    return m;
+}
+
+namespace {
+
+inline uint32_t fbits(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return u;
+}
+
+// Grow the slot arena to hold at least `need` slots (device copy of the used
+// prefix; host mirror resized).
+dyg_status grow_arena(dyg_graph g, uint64_t need) {
+    if (need <= g->cap_slots) return DYG_OK;
+    uint64_t ncap = std::max<uint64_t>(need, g->cap_slots + g->cap_slots / 2 + 1024);
+    if (ncap > 0xFFFFFFFFull) ncap = 0xFFFFFFFFull;
+    if (need > ncap) {
+        set_error(g->ctx, "slot arena exceeds 2^32 slots");
+        return DYG_ERR_NOMEM;
+    }
+    uint2* nadj = nullptr;
+    if (hipMalloc(&nadj, ncap * sizeof(uint2)) != hipSuccess) {
+        set_error(g->ctx, "device allocation of the slot arena failed");
+        return DYG_ERR_NOMEM;
+    }
+    DYG_HIP(g->ctx, hipMemcpyAsync(nadj, g->d_adj, g->used_slots * sizeof(uint2),
+                                   hipMemcpyDeviceToDevice, g->ctx->stream));
+    DYG_HIP(g->ctx, hipStreamSynchronize(g->ctx->stream));
+    DYG_HIP(g->ctx, hipFree(g->d_adj));
+    g->d_adj = nadj;
+    g->cap_slots = ncap;
+    g->adj.resize(ncap, make_uint2(0, 0));
+    return DYG_OK;
+}
+
+dyg_status ship_delta(dyg_graph g, uint32_t n_copy, uint32_t n_write, uint32_t n_row) {
+    const size_t bytes = g->delta.size() * sizeof(uint32_t);
+    if (bytes == 0) return DYG_OK;
+    dyg_ctx ctx = g->ctx;
+    // the staging buffers are shared per context: wait for earlier users
+    DYG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    DYG_HIP(ctx, g->d_delta.reserve(bytes));
+    DYG_HIP(ctx, ctx->stage.reserve(bytes));
+    std::memcpy(ctx->stage.p, g->delta.data(), bytes);
+    DYG_HIP(ctx, hipMemcpyAsync(g->d_delta.p, ctx->stage.p, bytes, hipMemcpyHostToDevice,
+                                ctx->stream));
+    DYG_HIP(ctx, launch_apply_delta(g->d_rows, g->d_adj, (const uint32_t*)g->d_delta.p, n_copy,
+                                    n_write, n_row, ctx->stream));
+    return DYG_OK;
+}
+
+}  // namespace
+
+dyg_status graph_apply_inserts(dyg_graph g, const uint32_t* u, const uint32_t* v, const float* w,
+                               uint32_t n) {
+    if (n == 0) return DYG_OK;
+    for (uint32_t i = 0; i < n; ++i)
+        if (u[i] >= g->V || v[i] >= g->V) {
+            set_error(g->ctx, "insert: vertex id out of range");
+            return DYG_ERR_ARG;
+        }
+    // appends in reference order: edge i gives (u->v) then (v->u)
+    struct App {
+        uint32_t row, seq, nbr;
+        float w;
+    };
+    std::vector<App> apps(2 * (size_t)n);
+    for (uint32_t i = 0; i < n; ++i) {
+        apps[2 * i] = {u[i], 2 * i, v[i], w[i]};
+        apps[2 * i + 1] = {v[i], 2 * i + 1, u[i], w[i]};
+    }
+    std::stable_sort(apps.begin(), apps.end(),
+                     [](const App& a, const App& b) { return a.row < b.row; });
+    // arena space needed by relocations
+    uint64_t extra = 0;
+    for (size_t p = 0; p < apps.size();) {
+        size_t q = p;
+        while (q < apps.size() && apps[q].row == apps[p].row) ++q;
+        const uint32_t r = apps[p].row;
+        const uint64_t need = (uint64_t)g->deg[r] + (q - p);
+        if (need > g->cap[r])
+            extra += std::max<uint64_t>({2ull * g->cap[r], need + g->slack, 4ull});
+        p = q;
+    }
+    dyg_status st = grow_arena(g, g->used_slots + extra);
+    if (st != DYG_OK) return st;
+
+    std::vector<uint32_t> copies, writes, rowops;
+    std::vector<uint32_t> slot_of(2 * (size_t)n);
+    for (size_t p = 0; p < apps.size();) {
+        size_t q = p;
+        while (q < apps.size() && apps[q].row == apps[p].row) ++q;
+        const uint32_t r = apps[p].row;
+        const uint32_t d0 = g->deg[r];
+        const uint64_t need = (uint64_t)d0 + (q - p);
+        if (need > g->cap[r]) {
+            const uint64_t ncap = std::max<uint64_t>({2ull * g->cap[r], need + g->slack, 4ull});
+            const uint32_t ns = (uint32_t)g->used_slots;
+            g->used_slots += ncap;
+            copies.push_back(g->start[r]);
+            copies.push_back(ns);
+            copies.push_back(d0);
+            std::copy(g->adj.begin() + g->start[r], g->adj.begin() + g->start[r] + d0,
+                      g->adj.begin() + ns);
+            g->start[r] = ns;
+            g->cap[r] = (uint32_t)ncap;
+        }
+        for (size_t k = p; k < q; ++k) {
+            const uint32_t slot = g->start[r] + d0 + (uint32_t)(k - p);
+            g->adj[slot] = make_uint2(apps[k].nbr, fbits(apps[k].w));
+            writes.push_back(slot);
+            writes.push_back(apps[k].nbr);
+            writes.push_back(fbits(apps[k].w));
+            slot_of[apps[k].seq] = (uint32_t)(d0 + (k - p));
+        }
+        g->deg[r] = (uint32_t)need;
+        rowops.push_back(r);
+        rowops.push_back(g->start[r]);
+        rowops.push_back(g->deg[r]);
+        p = q;
+    }
+    g->nslots += 2ull * n;
+    if (g->keyed) {
+        // first occurrence wins, as std::unordered_map::insert in the reference
+        for (uint32_t i = 0; i < n; ++i) {
+            const long a = u[i], b = v[i];
+            const long key = key_of(a, b, g->mult);
+            const uint32_t sa = slot_of[2 * i], sb = slot_of[2 * i + 1];
+            if (a <= b) g->key_map.insert({key, {sa, sb}});
+            else g->key_map.insert({key, {sb, sa}});
+        }
+    }
+    g->delta.clear();
+    g->delta.insert(g->delta.end(), copies.begin(), copies.end());
+    g->delta.insert(g->delta.end(), writes.begin(), writes.end());
+    g->delta.insert(g->delta.end(), rowops.begin(), rowops.end());
+    return ship_delta(g, (uint32_t)(copies.size() / 3), (uint32_t)(writes.size() / 3),
+                      (uint32_t)(rowops.size() / 3));
+}
+
+dyg_status graph_apply_deletes(dyg_graph g, const uint32_t* u, const uint32_t* v, uint32_t n,
+                               int8_t* hit) {
+    if (!g->keyed) {
+        set_error(g->ctx, "delete: graph was uploaded without its edge map");
+        return DYG_ERR_ARG;
+    }
+    for (uint32_t i = 0; i < n; ++i)
+        if (u[i] >= g->V || v[i] >= g->V) {
+            set_error(g->ctx, "delete: vertex id out of range");
+            return DYG_ERR_ARG;
+        }
+    // validation pass: no vertex may reach degree 0 (the reference asserts,
+    // random_walk_decremental/gpu_graph.cuh:362-363); nothing changes on failure
+    {
+        std::unordered_map<long, char> gone;
+        std::unordered_map<uint32_t, uint32_t> dec;
+        for (uint32_t i = 0; i < n; ++i) {
+            const long key = key_of(u[i], v[i], g->mult);
+            if (!g->key_map.count(key) || gone.count(key)) continue;
+            gone[key] = 1;
+            const uint32_t a = std::min(u[i], v[i]), b = std::max(u[i], v[i]);
+            if (++dec[a] >= g->deg[a] || ++dec[b] >= g->deg[b]) {
+                set_error(g->ctx, "delete: a vertex would be left with degree 0");
+                return DYG_ERR_STATE;
+            }
+        }
+    }
+    std::vector<uint32_t> dirty_slots, dirty_rows;
+    for (uint32_t i = 0; i < n; ++i) {
+        long a = u[i], b = v[i];
+        if (a > b) std::swap(a, b);
+        const long key = a * g->mult + b;
+        auto it = g->key_map.find(key);
+        if (it == g->key_map.end()) {
+            if (hit) hit[i] = 0;
+            continue;
+        }
+        if (hit) hit[i] = 1;
+        const uint32_t ib_a = it->second.first, ia_b = it->second.second;
+        g->key_map.erase(it);
+        const uint32_t da = --g->deg[a];
+        const uint32_t db = --g->deg[b];
+        if (da != ib_a) {
+            const uint2 last = g->adj[g->start[a] + da];
+            g->adj[g->start[a] + ib_a] = last;
+            const long al = last.x;
+            if (a < al) g->key_map.at(a * g->mult + al).first = ib_a;
+            else g->key_map.at(al * g->mult + a).second = ib_a;
+            dirty_slots.push_back(g->start[a] + ib_a);
+        }
+        if (db != ia_b) {
+            const uint2 last = g->adj[g->start[b] + db];
+            g->adj[g->start[b] + ia_b] = last;
+            const long bl = last.x;
+            if (b < bl) g->key_map.at(b * g->mult + bl).first = ia_b;
+            else g->key_map.at(bl * g->mult + b).second = ia_b;
+            dirty_slots.push_back(g->start[b] + ia_b);
+        }
+        dirty_rows.push_back((uint32_t)a);
+        dirty_rows.push_back((uint32_t)b);
+        g->nslots -= 2;
+    }
+    std::sort(dirty_slots.begin(), dirty_slots.end());
+    dirty_slots.erase(std::unique(dirty_slots.begin(), dirty_slots.end()), dirty_slots.end());
+    std::sort(dirty_rows.begin(), dirty_rows.end());
+    dirty_rows.erase(std::unique(dirty_rows.begin(), dirty_rows.end()), dirty_rows.end());
+    g->delta.clear();
+    for (uint32_t sl : dirty_slots) {
+        g->delta.push_back(sl);
+        g->delta.push_back(g->adj[sl].x);
+        g->delta.push_back(g->adj[sl].y);
+    }
+    for (uint32_t r : dirty_rows) {
+        g->delta.push_back(r);
+        g->delta.push_back(g->start[r]);
+        g->delta.push_back(g->deg[r]);
+    }
+    return ship_delta(g, 0, (uint32_t)dirty_slots.size(), (uint32_t)dirty_rows.size());
+}
+
+}  // namespace dyg
+
+using namespace dyg;
+
+extern "C" dyg_status dyg_graph_upload(dyg_ctx ctx, uint32_t V, const uint64_t* row_ptr,
+                                       const uint32_t* col, const float* w, uint32_t slack,
+                                       int keyed, dyg_graph* out) {
+    if (!ctx || !out || (!row_ptr && V) || V == 0) {
+        set_error(ctx, "dyg_graph_upload: invalid argument");
+        return DYG_ERR_ARG;
+    }
+    *out = nullptr;
+    if (row_ptr[0] != 0) {
+        set_error(ctx, "dyg_graph_upload: row_ptr[0] must be 0");
+        return DYG_ERR_ARG;
+    }
+    for (uint32_t v = 0; v < V; ++v)
+        if (row_ptr[v + 1] < row_ptr[v]) {
+            set_error(ctx, "dyg_graph_upload: row_ptr not monotone");
+            return DYG_ERR_ARG;
+        }
+    const uint64_t E = row_ptr[V];
+    for (uint64_t p = 0; p < E; ++p)
+        if (col[p] >= V) {
+            set_error(ctx, "dyg_graph_upload: neighbour id out of range");
+            return DYG_ERR_ARG;
+        }
+    auto* g = new dyg_graph_s;
+    g->ctx = ctx;
+    g->V = V;
+    g->slack = slack;
+    g->start.resize(V);
+    g->deg.resize(V);
+    g->cap.resize(V);
+    uint64_t total = 0;
+    for (uint32_t v = 0; v < V; ++v) {
+        g->deg[v] = (uint32_t)(row_ptr[v + 1] - row_ptr[v]);
+        g->cap[v] = g->deg[v] + slack;
+        g->start[v] = (uint32_t)total;
+        total += g->cap[v];
+    }
+    const uint64_t cap_slots = total + std::max<uint64_t>(4096, total / 16);
+    if (cap_slots > 0xFFFFFFFFull) {
+        delete g;
+        set_error(ctx, "dyg_graph_upload: more than 2^32 slots");
+        return DYG_ERR_NOMEM;
+    }
+    g->used_slots = total;
+    g->cap_slots = cap_slots;
+    g->nslots = E;
+    g->adj.assign(cap_slots, make_uint2(0, 0));
+    for (uint32_t v = 0; v < V; ++v)
+        for (uint32_t k = 0; k < g->deg[v]; ++k)
+            g->adj[g->start[v] + k] = make_uint2(col[row_ptr[v] + k], fbits(w[row_ptr[v] + k]));
+    std::vector<uint2> rows(V);
+    for (uint32_t v = 0; v < V; ++v) rows[v] = make_uint2(g->start[v], g->deg[v]);
+    if (hipMalloc(&g->d_rows, (size_t)V * sizeof(uint2)) != hipSuccess ||
+        hipMalloc(&g->d_adj, cap_slots * sizeof(uint2)) != hipSuccess) {
+        if (g->d_rows) (void)hipFree(g->d_rows);
+        if (g->d_adj) (void)hipFree(g->d_adj);
+        delete g;
+        set_error(ctx, "dyg_graph_upload: device allocation failed");
+        return DYG_ERR_NOMEM;
+    }
+    hipError_t e1 = hipMemcpy(g->d_rows, rows.data(), (size_t)V * sizeof(uint2),
+                              hipMemcpyHostToDevice);
+    hipError_t e2 = hipMemcpy(g->d_adj, g->adj.data(), cap_slots * sizeof(uint2),
+                              hipMemcpyHostToDevice);
+    if (e1 != hipSuccess || e2 != hipSuccess) {
+        set_error(ctx, std::string("dyg_graph_upload: ") +
+                           hipGetErrorString(e1 != hipSuccess ? e1 : e2));
+        dyg_graph_free(g);
+        return DYG_ERR_HIP;
+    }
+    if (keyed) {
+        g->keyed = true;
+        g->mult = key_multiplier(V);
+        g->key_map.reserve((size_t)(E / 2 * 1.5) + 16);
+        // (slot of b in a's row, slot of a in b's row) for key a*mult+b, a<b;
+        // the first occurrence in each row wins, as the first file line does
+        for (uint32_t a = 0; a < V; ++a)
+            for (uint32_t k = 0; k < g->deg[a]; ++k) {
+                const uint32_t b = g->adj[g->start[a] + k].x;
+                const long key = key_of(a, b, g->mult);
+                auto it = g->key_map.find(key);
+                if (a <= b) {
+                    if (it == g->key_map.end()) g->key_map.insert({key, {k, 0xFFFFFFFFu}});
+                    else if (it->second.first == 0xFFFFFFFFu) it->second.first = k;
+                }
+                if (a >= b) {
+                    if (it == g->key_map.end()) g->key_map.insert({key, {0xFFFFFFFFu, k}});
+                    else if (it->second.second == 0xFFFFFFFFu) it->second.second = k;
+                }
+            }
+    }
+    *out = g;
+    return DYG_OK;
+}
+
+extern "C" dyg_status dyg_graph_info(dyg_graph g, uint32_t* V, uint64_t* nslots,
+                                     uint64_t* capacity) {
+    if (!g) return DYG_ERR_ARG;
+    if (V) *V = g->V;
+    if (nslots) *nslots = g->nslots;
+    if (capacity) *capacity = g->cap_slots;
+    return DYG_OK;
+}
+
+extern "C" dyg_status dyg_graph_download(dyg_graph g, uint64_t* row_ptr, uint32_t* col,
+                                         float* w) {
+    if (!g || !row_ptr) return DYG_ERR_ARG;
+    dyg_ctx ctx = g->ctx;
+    std::vector<uint2> rows(g->V), adj(g->used_slots);
+    DYG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    DYG_HIP(ctx, hipMemcpy(rows.data(), g->d_rows, (size_t)g->V * sizeof(uint2),
+                           hipMemcpyDeviceToHost));
+    DYG_HIP(ctx, hipMemcpy(adj.data(), g->d_adj, g->used_slots * sizeof(uint2),
+                           hipMemcpyDeviceToHost));
+    uint64_t p = 0;
+    for (uint32_t v = 0; v < g->V; ++v) {
+        row_ptr[v] = p;
+        for (uint32_t k = 0; k < rows[v].y; ++k) {
+            const uint2 sl = adj[rows[v].x + k];
+            if (col) col[p] = sl.x;
+            if (w) std::memcpy(&w[p], &sl.y, 4);
+            ++p;
+        }
+    }
+    row_ptr[g->V] = p;
+    return DYG_OK;
+}
+
+extern "C" dyg_status dyg_graph_insert(dyg_graph g, const uint32_t* u, const uint32_t* v,
+                                       const float* w, uint32_t n) {
+    if (!g || (n && (!u || !v || !w))) return DYG_ERR_ARG;
+    dyg_status st = graph_apply_inserts(g, u, v, w, n);
+    if (st != DYG_OK) return st;
+    DYG_HIP(g->ctx, hipStreamSynchronize(g->ctx->stream));
+    return DYG_OK;
+}
+
+extern "C" dyg_status dyg_graph_delete(dyg_graph g, const uint32_t* u, const uint32_t* v,
+                                       uint32_t n, int8_t* hit) {
+    if (!g || (n && (!u || !v))) return DYG_ERR_ARG;
+    dyg_status st = graph_apply_deletes(g, u, v, n, hit);
+    if (st != DYG_OK) return st;
+    DYG_HIP(g->ctx, hipStreamSynchronize(g->ctx->stream));
+    return DYG_OK;
+}
+
+extern "C" void dyg_graph_free(dyg_graph g) {
+    if (!g) return;
+    if (g->d_rows) (void)hipFree(g->d_rows);
+    if (g->d_adj) (void)hipFree(g->d_adj);
+    delete g;
+}

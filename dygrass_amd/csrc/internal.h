@@ -1,0 +1,127 @@
+// internal.h -- shared internals of libdygrass (context, graph store, helpers).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <unordered_map>
+#include <utility>
+#include <vector>
+
+#include "../../include/dygrass.h"
+
+namespace dyg {
+
+// Random stream tags: top byte of Philox counter word 3.
+constexpr uint32_t kTagInsert = 1u;
+constexpr uint32_t kTagDelete = 2u;
+constexpr uint32_t kTagHeuristic = 3u;
+
+void set_error(dyg_ctx ctx, const std::string& msg);
+
+// Pinned host staging buffer that grows on demand.
+struct Pinned {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t reserve(size_t n);
+    ~Pinned();
+};
+
+// Device buffer that grows on demand (contents not preserved on growth).
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t reserve(size_t n);
+    ~DevBuf();
+};
+
+}  // namespace dyg
+
+struct dyg_ctx_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string last_error;
+    dyg::Pinned stage;         // H2D / D2H staging
+    dyg::DevBuf dscratch;      // per-call device staging of inputs/outputs
+    dyg::DevBuf dstats;        // device counters (hops, redirects, leaf stops)
+};
+
+// One slot of a row: neighbour id and weight bits, 8 bytes, read in one load.
+// One row descriptor: first slot and degree, 8 bytes, read in one load.
+struct dyg_graph_s {
+    dyg_ctx ctx = nullptr;
+    uint32_t V = 0;
+    uint2* d_rows = nullptr;  // [V] (start, deg)
+    uint2* d_adj = nullptr;   // [cap_slots] (col, float bits of w)
+    uint64_t cap_slots = 0;   // slots allocated on the device
+    uint64_t used_slots = 0;  // bump pointer of the slot arena
+    uint32_t slack = 0;       // spare slots given to a relocated row
+    // host mirror of the device state (same slot positions)
+    std::vector<uint32_t> start, deg, cap;
+    std::vector<uint2> adj;
+    uint64_t nslots = 0;  // sum of degrees
+    // reference edge map (random_walk_decremental/functions.cpp:243-258)
+    bool keyed = false;
+    long mult = 1;
+    std::unordered_map<long, std::pair<uint32_t, uint32_t>> key_map;
+    // delta staging
+    std::vector<uint32_t> delta;
+    dyg::DevBuf d_delta;
+};
+
+namespace dyg {
+
+inline long key_of(long a, long b, long mult) { return a < b ? a * mult + b : b * mult + a; }
+long key_multiplier(uint64_t V);
+
+// Graph edit plans (graph_store.hip).  Both build a delta in g->delta and
+// apply it on the device; the host mirror is updated in place.
+dyg_status graph_apply_inserts(dyg_graph g, const uint32_t* u, const uint32_t* v, const float* w,
+                               uint32_t n);
+dyg_status graph_apply_deletes(dyg_graph g, const uint32_t* u, const uint32_t* v, uint32_t n,
+                               int8_t* hit);
+
+// Walk launchers (walk_kernels.hip).  All pointers are device pointers.
+struct WalkLaunch {
+    const uint2* rows;
+    const uint2* adj;
+    const uint32_t* src;
+    const uint32_t* dst;
+    const float* w;          // insertion weights (nullptr for deletions)
+    const uint64_t* eid;     // nullptr -> eid_base + index
+    uint64_t eid_base;
+    uint32_t n;
+    uint32_t walkers;
+    uint32_t max_steps;
+    float thr;
+    uint32_t flags;
+    uint64_t seed;
+    int32_t* found;
+    float* cond;
+    uint32_t* winner;        // optional
+    uint32_t* hops;          // optional, per edge
+    uint32_t* path;          // deletions: [n*max_steps]
+    unsigned long long* stats;  // optional [3]: hops, redirects, leaf stops
+};
+hipError_t launch_walk_incremental(const WalkLaunch& a, hipStream_t s);
+hipError_t launch_walk_decremental(const WalkLaunch& a, hipStream_t s);
+hipError_t launch_heuristic(const uint2* rows, const uint2* adj, const uint32_t* endpoint,
+                            const uint64_t* key, uint32_t n, uint32_t walkers, uint64_t seed,
+                            uint32_t* path3, hipStream_t s);
+hipError_t launch_apply_delta(uint2* rows, uint2* adj, const uint32_t* delta, uint32_t n_copy,
+                              uint32_t n_write, uint32_t n_row, hipStream_t s);
+
+bool valid_walkers(uint32_t w);
+
+}  // namespace dyg
+
+#define DYG_HIP(ctx, expr)                                                              \
+    do {                                                                                \
+        hipError_t _e = (expr);                                                         \
+        if (_e != hipSuccess) {                                                         \
+            dyg::set_error((ctx), std::string(#expr) + ": " + hipGetErrorString(_e));   \
+            return DYG_ERR_HIP;                                                         \
+        }                                                                               \
+    } while (0)

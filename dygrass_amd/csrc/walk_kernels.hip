@@ -1,0 +1,361 @@
+// walk_kernels.hip -- non-backtracking random-walk kernels for CDNA4 (gfx950).
+//
+// Layout read by every kernel (device graph store, 8-byte records):
+//   rows[v] = (first slot, degree)          one 8 B load per visited vertex
+//   adj[p]  = (neighbour id, weight bits)   one 8 B load per draw
+// A walk hop is two dependent loads (row, then slot).  The slot that the
+// non-backtracking rule may redirect to (the row's last slot) is loaded in the
+// same step as the drawn slot, so a redirect costs no extra latency.
+//
+// One workgroup per updated edge, one wave64 lane per walker (W/64 waves).
+// The winner of an edge is the walker with the largest conductance 1/R; ties
+// go to the walker whose index, bit-reversed over log2(W) bits, is smallest.
+// That is the closed form of the reference's tree reduction
+// (random_walk_incremental/main.cu:179-201).  Here it is one 64-bit max per
+// lane: key = (bits(1/R)+1) << 32 | (0xffff - bitrev(j)) << 16 | found_at,
+// reduced by wave64 butterflies and then across waves through LDS.
+//
+// Draws come from Philox4x32-10 keyed by (seed, tag, edge id, walker): each
+// 10-round block yields the next four draws of a walker, so results do not
+// depend on how walkers are mapped to lanes, workgroups or launches.
+#include "internal.h"
+
+namespace dyg {
+namespace {
+
+constexpr uint32_t kM0 = 0xD2511F53u, kM1 = 0xCD9E8D57u;
+constexpr uint32_t kW0 = 0x9E3779B9u, kW1 = 0xBB67AE85u;
+
+struct Draws {
+    uint32_t k0, k1, c1, c2, c3, blk;
+    uint32_t b0, b1, b2, b3, avail;
+
+    __device__ __forceinline__ Draws(uint64_t seed, uint32_t tag, uint64_t id, uint32_t walker)
+        : k0((uint32_t)seed),
+          k1((uint32_t)(seed >> 32)),
+          c1(walker),
+          c2((uint32_t)id),
+          c3((tag << 24) | ((uint32_t)(id >> 32) & 0xFFFFFFu)),
+          blk(0),
+          b0(0), b1(0), b2(0), b3(0),
+          avail(0) {}
+
+    __device__ __forceinline__ void refill() {
+        uint32_t x0 = blk, x1 = c1, x2 = c2, x3 = c3, q0 = k0, q1 = k1;
+#pragma unroll
+        for (int r = 0; r < 10; ++r) {
+            const uint32_t hi0 = __umulhi(kM0, x0), lo0 = kM0 * x0;
+            const uint32_t hi1 = __umulhi(kM1, x2), lo1 = kM1 * x2;
+            x0 = hi1 ^ x1 ^ q0;
+            x1 = lo1;
+            x2 = hi0 ^ x3 ^ q1;
+            x3 = lo0;
+            q0 += kW0;
+            q1 += kW1;
+        }
+        b0 = x0; b1 = x1; b2 = x2; b3 = x3;
+        avail = 4;
+        ++blk;
+    }
+
+    __device__ __forceinline__ uint32_t next() {
+        if (avail == 0) refill();
+        const uint32_t r = b0;
+        b0 = b1; b1 = b2; b2 = b3;
+        --avail;
+        return r;
+    }
+};
+
+struct WalkOut {
+    int found;
+    float R;
+    uint32_t hops, redir, leaf;
+};
+
+__device__ __forceinline__ uint2 ld(const uint2* __restrict__ p, uint32_t i) { return p[i]; }
+
+// One walker.  INSERT: distortion test R*w_e >= thr before the target test
+// (random_walk_incremental/main.cu:97-163).  Otherwise the deletion walk
+// (random_walk_decremental/main.cu:104-170).  RECORD writes path[0..].
+// PRUNE: stop once fl(1/R) is strictly below the best conductance already
+// found by a walker of the same edge; R never decreases, so such a walker can
+// no longer win and the edge's winner is unchanged.
+template <bool INSERT, bool RECORD, bool PRUNE>
+__device__ __forceinline__ void walk_lane(const uint2* __restrict__ rows,
+                                          const uint2* __restrict__ adj, uint32_t s, uint32_t t,
+                                          float we, float thr, int max_steps, Draws& rng,
+                                          uint32_t* best, uint32_t* __restrict__ path,
+                                          WalkOut& o) {
+    o.found = -1;
+    o.R = 0.0f;
+    o.hops = 0;
+    o.redir = 0;
+    o.leaf = 0;
+    if (RECORD) path[0] = s;
+    uint2 row = ld(rows, s);
+    if (row.y == 0) return;  // isolated source: nothing to find
+    uint32_t i = rng.next() % row.y;
+    o.hops = 1;
+    uint2 sl = ld(adj, row.x + i);
+    float R = 1.0f / __uint_as_float(sl.y);
+    if (RECORD) path[1] = sl.x;
+    if (sl.x == t) {
+        o.found = 1;
+        o.R = R;
+        if (PRUNE) atomicMax(best, __float_as_uint(1.0f / R) + 1u);
+        return;
+    }
+    uint32_t prev = s, cur = sl.x;
+    int step = 2;
+    while (step < max_steps) {
+        row = ld(rows, cur);
+        if (row.y <= 1) {  // leaf (degree 0 cannot occur: deletions keep degree >= 1)
+            o.leaf = 1;
+            break;
+        }
+        i = rng.next() % (row.y - 1);
+        ++o.hops;
+        const uint2 a = ld(adj, row.x + i);
+        const uint2 b = ld(adj, row.x + row.y - 1);
+        const bool back = (a.x == prev);
+        o.redir += back ? 1u : 0u;
+        sl = back ? b : a;
+        R += 1.0f / __uint_as_float(sl.y);
+        if (RECORD) path[step] = sl.x;
+        if (INSERT && R * we >= thr) break;
+        if (sl.x == t) {
+            o.found = step;
+            if (PRUNE) atomicMax(best, __float_as_uint(1.0f / R) + 1u);
+            break;
+        }
+        if (PRUNE) {
+            const uint32_t bb = __hip_atomic_load(best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (bb != 0u && __float_as_uint(1.0f / R) + 1u < bb) break;
+        }
+        prev = cur;
+        cur = sl.x;
+        ++step;
+    }
+    o.R = R;
+}
+
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <int LOGW>
+__device__ __forceinline__ uint32_t tie_field(uint32_t j) {
+    if constexpr (LOGW == 0) {
+        return 0xFFFFu;
+    } else {
+        return 0xFFFFu - (__brev(j) >> (32 - LOGW));
+    }
+}
+
+template <int LOGW>
+__device__ __forceinline__ uint32_t tie_to_walker(uint32_t tie) {
+    if constexpr (LOGW == 0) {
+        return 0;
+    } else {
+        return __brev((0xFFFFu - tie) << (32 - LOGW));
+    }
+}
+
+// Block of max(W,64) threads; threads j >= W hold no walker.
+template <int LOGW, bool INSERT, bool PRUNE>
+__global__ __launch_bounds__((1 << LOGW) < 64 ? 64 : (1 << LOGW)) void k_walk(WalkLaunch a) {
+    constexpr int W = 1 << LOGW;
+    constexpr int BS = W < 64 ? 64 : W;
+    constexpr int NWAVE = BS / 64;
+    __shared__ unsigned long long s_key[NWAVE];
+    __shared__ uint32_t s_cnt[NWAVE][3];
+    __shared__ uint32_t s_best;
+
+    const uint32_t e = blockIdx.x;
+    const uint32_t j = threadIdx.x;
+    const uint32_t lane = j & 63u, wave = j >> 6;
+    if (j == 0) s_best = 0u;
+    const uint32_t s = a.src[e], t = a.dst[e];
+    const float we = INSERT ? a.w[e] : 0.0f;
+    const uint64_t eid = a.eid ? a.eid[e] : a.eid_base + e;
+    __syncthreads();
+
+    WalkOut o;
+    o.found = -1; o.R = 0.0f; o.hops = 0; o.redir = 0; o.leaf = 0;
+    if (j < (uint32_t)W) {
+        Draws rng(a.seed, INSERT ? kTagInsert : kTagDelete, eid, j);
+        walk_lane<INSERT, false, PRUNE>(a.rows, a.adj, s, t, we, a.thr, (int)a.max_steps, rng,
+                                        &s_best, nullptr, o);
+    }
+    const bool found = (o.found != -1) && (j < (uint32_t)W);
+    const uint32_t hi = found ? __float_as_uint(1.0f / o.R) + 1u : 0u;
+    const uint32_t tie = (j < (uint32_t)W) ? tie_field<LOGW>(j) : 0u;
+    unsigned long long key =
+        ((unsigned long long)hi << 32) | ((unsigned long long)tie << 16) |
+        (unsigned long long)((uint32_t)o.found & 0xFFFFu);
+    key = wave_max_u64(key);
+    const uint32_t h = wave_sum_u32(o.hops);
+    const uint32_t r = wave_sum_u32(o.redir);
+    const uint32_t l = wave_sum_u32(o.leaf);
+    if (lane == 0) {
+        s_key[wave] = key;
+        s_cnt[wave][0] = h;
+        s_cnt[wave][1] = r;
+        s_cnt[wave][2] = l;
+    }
+    __syncthreads();
+    if (j == 0) {
+        unsigned long long best = s_key[0];
+        uint32_t th = s_cnt[0][0], tr = s_cnt[0][1], tl = s_cnt[0][2];
+#pragma unroll
+        for (int k = 1; k < NWAVE; ++k) {
+            best = s_key[k] > best ? s_key[k] : best;
+            th += s_cnt[k][0];
+            tr += s_cnt[k][1];
+            tl += s_cnt[k][2];
+        }
+        const int f = (int)(int16_t)(uint16_t)(best & 0xFFFFu);
+        const uint32_t win = tie_to_walker<LOGW>((uint32_t)(best >> 16) & 0xFFFFu);
+        a.found[e] = f;
+        a.cond[e] = (f != -1) ? __uint_as_float((uint32_t)(best >> 32) - 1u) : -1.0f;
+        if (a.winner) a.winner[e] = win;
+        if (a.hops) a.hops[e] = th;
+        if (a.stats) {
+            atomicAdd(&a.stats[0], (unsigned long long)th);
+            atomicAdd(&a.stats[1], (unsigned long long)tr);
+            atomicAdd(&a.stats[2], (unsigned long long)tl);
+        }
+        if (!INSERT && f > 0) {
+            // The winner's path, replayed: same draws, same graph snapshot,
+            // so the same trajectory (the winner is never pruned).
+            Draws rng(a.seed, kTagDelete, eid, win);
+            WalkOut ro;
+            walk_lane<false, true, false>(a.rows, a.adj, s, t, 0.0f, 0.0f, (int)a.max_steps, rng,
+                                          nullptr, a.path + (uint64_t)e * a.max_steps, ro);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_heuristic(const uint2* __restrict__ rows,
+                                                   const uint2* __restrict__ adj,
+                                                   const uint32_t* __restrict__ endpoint,
+                                                   const uint64_t* __restrict__ key, uint32_t n,
+                                                   uint32_t walkers, uint64_t seed,
+                                                   uint32_t* __restrict__ path3) {
+    const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (uint64_t)n * walkers) return;
+    const uint32_t i = (uint32_t)(gid / walkers), j = (uint32_t)(gid % walkers);
+    const uint32_t u = endpoint[i];
+    Draws rng(seed, kTagHeuristic, key[i], j);
+    uint32_t n1 = u, n2 = u;
+    const uint2 ru = rows[u];
+    if (ru.y != 0) {
+        n1 = adj[ru.x + rng.next() % ru.y].x;
+        const uint2 r1 = rows[n1];
+        n2 = adj[r1.x + rng.next() % r1.y].x;  // r1.y >= 1: n1 is a neighbour of u
+        if (n2 == u) n2 = adj[r1.x + r1.y - 1].x;
+    }
+    path3[gid * 3 + 0] = u;
+    path3[gid * 3 + 1] = n1;
+    path3[gid * 3 + 2] = n2;
+}
+
+// Graph edits: relocation copies, slot writes, row descriptor writes.  The
+// three sets touch disjoint memory (copies read abandoned regions, writes
+// land beyond the copied prefix or in rows not relocated), so one launch
+// applies them all.  Delta layout (uint32 words): copies (src, dst, len),
+// writes (slot, col, wbits), rows (v, start, deg).
+__global__ __launch_bounds__(256) void k_apply_delta(uint2* __restrict__ rows,
+                                                     uint2* __restrict__ adj,
+                                                     const uint32_t* __restrict__ d,
+                                                     uint32_t n_copy, uint32_t n_write,
+                                                     uint32_t n_row) {
+    const uint32_t copy_blocks = (n_copy + 3u) / 4u;
+    if (blockIdx.x < copy_blocks) {
+        const uint32_t op = blockIdx.x * 4u + (threadIdx.x >> 6);
+        const uint32_t lane = threadIdx.x & 63u;
+        if (op < n_copy) {
+            const uint32_t src = d[op * 3 + 0], dst = d[op * 3 + 1], len = d[op * 3 + 2];
+            for (uint32_t k = lane; k < len; k += 64u) adj[dst + k] = adj[src + k];
+        }
+        return;
+    }
+    const uint32_t g = (blockIdx.x - copy_blocks) * 256u + threadIdx.x;
+    const uint32_t* wr = d + (uint64_t)n_copy * 3;
+    const uint32_t* rw = wr + (uint64_t)n_write * 3;
+    if (g < n_write) {
+        adj[wr[g * 3 + 0]] = make_uint2(wr[g * 3 + 1], wr[g * 3 + 2]);
+    } else if (g - n_write < n_row) {
+        const uint32_t q = g - n_write;
+        rows[rw[q * 3 + 0]] = make_uint2(rw[q * 3 + 1], rw[q * 3 + 2]);
+    }
+}
+
+template <bool INSERT, bool PRUNE>
+hipError_t launch_walk_t(const WalkLaunch& a, hipStream_t s) {
+    if (a.n == 0) return hipSuccess;
+    const dim3 grid(a.n);
+    switch (a.walkers) {
+#define DYG_CASE(LW)                                                                       \
+    case (1u << LW):                                                                       \
+        hipLaunchKernelGGL((k_walk<LW, INSERT, PRUNE>), grid,                              \
+                           dim3((1 << LW) < 64 ? 64 : (1 << LW)), 0, s, a);                \
+        break;
+        DYG_CASE(0) DYG_CASE(1) DYG_CASE(2) DYG_CASE(3) DYG_CASE(4) DYG_CASE(5)
+        DYG_CASE(6) DYG_CASE(7) DYG_CASE(8) DYG_CASE(9) DYG_CASE(10)
+#undef DYG_CASE
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace
+
+bool valid_walkers(uint32_t w) { return w >= 1 && w <= 1024 && (w & (w - 1)) == 0; }
+
+hipError_t launch_walk_incremental(const WalkLaunch& a, hipStream_t s) {
+    return (a.flags & DYG_WALK_NO_PRUNE) ? launch_walk_t<true, false>(a, s)
+                                         : launch_walk_t<true, true>(a, s);
+}
+
+hipError_t launch_walk_decremental(const WalkLaunch& a, hipStream_t s) {
+    return (a.flags & DYG_WALK_NO_PRUNE) ? launch_walk_t<false, false>(a, s)
+                                         : launch_walk_t<false, true>(a, s);
+}
+
+hipError_t launch_heuristic(const uint2* rows, const uint2* adj, const uint32_t* endpoint,
+                            const uint64_t* key, uint32_t n, uint32_t walkers, uint64_t seed,
+                            uint32_t* path3, hipStream_t s) {
+    const uint64_t total = (uint64_t)n * walkers;
+    if (total == 0) return hipSuccess;
+    const uint32_t blocks = (uint32_t)((total + 255) / 256);
+    hipLaunchKernelGGL(k_heuristic, dim3(blocks), dim3(256), 0, s, rows, adj, endpoint, key, n,
+                       walkers, seed, path3);
+    return hipGetLastError();
+}
+
+hipError_t launch_apply_delta(uint2* rows, uint2* adj, const uint32_t* delta, uint32_t n_copy,
+                              uint32_t n_write, uint32_t n_row, hipStream_t s) {
+    const uint32_t copy_blocks = (n_copy + 3u) / 4u;
+    const uint32_t rest_blocks = (uint32_t)(((uint64_t)n_write + n_row + 255) / 256);
+    const uint32_t blocks = copy_blocks + rest_blocks;
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_apply_delta, dim3(blocks), dim3(256), 0, s, rows, adj, delta, n_copy,
+                       n_write, n_row);
+    return hipGetLastError();
+}
+
+}  // namespace dyg

@@ -1,0 +1,192 @@
+"""Seeded synthetic workloads (libdygrass_synth.so) and the named configurations.
+
+A workload is a dense graph G, a sparsifier H (maximum-weight spanning tree
+of G plus a sample of off-tree edges) and an update stream.  Everything is
+deterministic in (configuration, seed).  `scale` shrinks a configuration for
+tests; 1.0 is the full size.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import ptr
+
+
+@dataclass
+class Edges:
+    V: int
+    u: np.ndarray
+    v: np.ndarray
+    w: np.ndarray
+    xy: np.ndarray | None = None
+
+    def __len__(self) -> int:
+        return len(self.u)
+
+
+def _take(h) -> Edges:
+    s = L.synth_lib()
+    try:
+        n = s.dyg_synth_ne(h)
+        V = s.dyg_synth_nv(h)
+        u = np.ctypeslib.as_array(s.dyg_synth_u(h), (n,)).copy() if n else np.zeros(0, np.uint32)
+        v = np.ctypeslib.as_array(s.dyg_synth_v(h), (n,)).copy() if n else np.zeros(0, np.uint32)
+        w = np.ctypeslib.as_array(s.dyg_synth_w(h), (n,)).copy() if n else np.zeros(0, np.float32)
+        xyp = s.dyg_synth_xy(h)
+        xy = np.ctypeslib.as_array(xyp, (2 * V,)).copy() if xyp else None
+    finally:
+        s.dyg_synth_free(h)
+    return Edges(V, u, v, w, xy)
+
+
+def grid2d(nx: int, ny: int, wlo: float, whi: float, seed: int) -> Edges:
+    return _take(L.synth_lib().dyg_synth_grid2d(nx, ny, wlo, whi, seed))
+
+
+def rgg_knn(n: int, k: int, seed: int) -> Edges:
+    return _take(L.synth_lib().dyg_synth_rgg_knn(n, k, seed))
+
+
+def trimesh(nx: int, ny: int, wlo: float, whi: float, seed: int) -> Edges:
+    return _take(L.synth_lib().dyg_synth_trimesh(nx, ny, wlo, whi, seed))
+
+
+def grid3d_shortcuts(n: int, shortcuts: int, wlo: float, whi: float, seed: int) -> Edges:
+    return _take(L.synth_lib().dyg_synth_grid3d_sc(n, shortcuts, wlo, whi, seed))
+
+
+def sparsifier(G: Edges, extra: int, seed: int) -> Edges:
+    h = L.synth_lib().dyg_synth_sparsifier(G.V, ptr(G.u, L.u32p), ptr(G.v, L.u32p),
+                                            ptr(G.w, L.f32p), len(G), extra, seed)
+    H = _take(h)
+    H.xy = G.xy
+    return H
+
+
+def insertions(G: Edges, count: int, dlo: int, dhi: int, wlo: float, whi: float, seed: int,
+               geometric: bool = False) -> Edges:
+    xy = G.xy if geometric else None
+    h = L.synth_lib().dyg_synth_insertions(G.V, ptr(G.u, L.u32p), ptr(G.v, L.u32p), len(G),
+                                            ptr(xy, L.f32p), count, dlo, dhi, wlo, whi, seed)
+    return _take(h)
+
+
+def deletions(G: Edges, count: int, seed: int) -> Edges:
+    h = L.synth_lib().dyg_synth_deletions(G.V, ptr(G.u, L.u32p), ptr(G.v, L.u32p),
+                                           ptr(G.w, L.f32p), len(G), count, seed)
+    return _take(h)
+
+
+def write_mtx(path: str, u, v, w=None, base: int = 1) -> None:
+    u = np.ascontiguousarray(u, np.uint32)
+    v = np.ascontiguousarray(v, np.uint32)
+    wp = None if w is None else np.ascontiguousarray(w, np.float32)
+    rc = L.synth_lib().dyg_synth_write_mtx(path.encode(), ptr(u, L.u32p), ptr(v, L.u32p),
+                                           ptr(wp, L.f32p), len(u), base)
+    if rc != 0:
+        raise OSError(f"cannot write {path}")
+
+
+def _seed(config: int, purpose: int, seed: int) -> int:
+    """SplitMix64 of (seed, config, purpose)."""
+    z = (seed * 0x9E3779B97F4A7C15 + config * 0xBF58476D1CE4E5B9 + purpose * 0x94D049BB133111EB
+         + 0x2545F4914F6CDD1D) & 0xFFFFFFFFFFFFFFFF
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+    return z ^ (z >> 31)
+
+
+@dataclass
+class Workload:
+    name: str
+    G: Edges
+    H: Edges
+    ins: Edges | None = None
+    dels: Edges | None = None
+    threshold: float = 0.0
+    walkers: int = 512
+    batches: int = 10
+    meta: dict = field(default_factory=dict)
+
+
+def config_grid_insert(scale: float = 1.0, seed: int = 0) -> Workload:
+    """128x128 grid, weights log-uniform [0.1,10], H = tree + 5% V, 2000
+    insertions at hop distance 2..6, threshold 10, 512 walkers."""
+    side = max(8, int(round(128 * math.sqrt(scale))))
+    G = grid2d(side, side, 0.1, 10.0, _seed(1, 0, seed))
+    H = sparsifier(G, int(0.05 * G.V), _seed(1, 1, seed))
+    ins = insertions(G, max(1, int(2000 * scale)), 2, 6, 0.1, 10.0, _seed(1, 2, seed))
+    return Workload(f"grid{side}x{side}-ins", G, H, ins=ins, threshold=10.0, walkers=512)
+
+
+def config_rgg_insert(scale: float = 1.0, seed: int = 0) -> Workload:
+    """1M-point 8-NN random geometric graph (w = 1/dist), H = tree + 2% V,
+    10 batches of 20k insertions within 5 hops, threshold 100, 256 walkers."""
+    n = max(1000, int(1_000_000 * scale))
+    G = rgg_knn(n, 8, _seed(2, 0, seed))
+    H = sparsifier(G, int(0.02 * n), _seed(2, 1, seed))
+    ins = insertions(G, max(10, int(200_000 * scale)), 2, 5, 0.0, 0.0, _seed(2, 2, seed),
+                     geometric=True)
+    return Workload(f"rgg{n}-knn8-ins", G, H, ins=ins, threshold=100.0, walkers=256)
+
+
+def config_mesh_delete(scale: float = 1.0, seed: int = 0) -> Workload:
+    """2M-node triangulated mesh, weights log-uniform [0.5,2], H = tree + 3% V,
+    10 batches of 10k uniform deletions, 512 walkers, 100 steps."""
+    side = max(16, int(round(1414 * math.sqrt(scale))))
+    G = trimesh(side, side, 0.5, 2.0, _seed(3, 0, seed))
+    H = sparsifier(G, int(0.03 * G.V), _seed(3, 1, seed))
+    dels = deletions(G, max(10, int(100_000 * scale)), _seed(3, 2, seed))
+    return Workload(f"mesh{side}x{side}-del", G, H, dels=dels, walkers=512)
+
+
+def config_rgg12(scale: float = 1.0, seed: int = 0) -> Workload:
+    """4M-point 12-NN random geometric graph, H = tree + 2% V."""
+    n = max(1000, int(4_000_000 * scale))
+    G = rgg_knn(n, 12, _seed(4, 0, seed))
+    H = sparsifier(G, int(0.02 * n), _seed(4, 1, seed))
+    return Workload(f"rgg{n}-knn12", G, H, walkers=256, threshold=100.0)
+
+
+def config_grid3d(scale: float = 1.0, seed: int = 0) -> Workload:
+    """200^3 6-neighbour grid plus 16M shortcuts, weights log-uniform [0.1,10],
+    H = tree + about 2M edges."""
+    n = max(8, int(round(200 * scale ** (1 / 3))))
+    V = n ** 3
+    G = grid3d_shortcuts(n, int(16_000_000 * V / 8_000_000), 0.1, 10.0, _seed(5, 0, seed))
+    H = sparsifier(G, int(0.25 * V), _seed(5, 1, seed))
+    return Workload(f"grid3d{n}-sc", G, H, walkers=256, threshold=100.0)
+
+
+CONFIGS = {
+    "grid-insert": config_grid_insert,
+    "rgg-insert": config_rgg_insert,
+    "mesh-delete": config_mesh_delete,
+    "rgg12": config_rgg12,
+    "grid3d": config_grid3d,
+}
+
+
+def write_dataset(root: str, wl: Workload) -> str:
+    """Write the reference's file set for `wl` under root/dataset/<name>/ and
+    return that directory: adj_sparse.mtx (0-based H), ext.mtx (1-based
+    insertions), dense.mtx (1-based G); for deletions updated_adj_sparse.mtx,
+    updated_dense.mtx and del.mtx (1-based)."""
+    import os
+
+    d = os.path.join(root, "dataset", wl.name)
+    os.makedirs(d, exist_ok=True)
+    write_mtx(os.path.join(d, "dense.mtx"), wl.G.u, wl.G.v, wl.G.w, base=1)
+    if wl.ins is not None:
+        write_mtx(os.path.join(d, "adj_sparse.mtx"), wl.H.u, wl.H.v, wl.H.w, base=0)
+        write_mtx(os.path.join(d, "ext.mtx"), wl.ins.u, wl.ins.v, wl.ins.w, base=1)
+    if wl.dels is not None:
+        write_mtx(os.path.join(d, "updated_adj_sparse.mtx"), wl.H.u, wl.H.v, wl.H.w, base=1)
+        write_mtx(os.path.join(d, "updated_dense.mtx"), wl.G.u, wl.G.v, wl.G.w, base=1)
+        write_mtx(os.path.join(d, "del.mtx"), wl.dels.u, wl.dels.v, None, base=1)
+    return d

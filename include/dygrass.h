@@ -1,0 +1,240 @@
+/*
+ * dygrass.h -- C ABI of the MI355X-native dyGRASS random-walk updaters.
+ *
+ * The reference has no library interface: its boundary is the two CLIs
+ * (random_walk_incremental/main.cu:240-362, random_walk_decremental/main.cu:318-468)
+ * and, inside them, kernel launches on a bitwise copy of a host object.
+ * Each entry point below replaces one reference unit; the unit is cited on
+ * the declaration.  Conventions:
+ *   - every call returns dyg_status (0 = success, < 0 = error) and never
+ *     exits the process; dyg_last_error() describes the last failure;
+ *   - vertex ids are 0-based uint32, weights are IEEE float32;
+ *   - host-pointer entry points copy inputs in and results out and return
+ *     after the device work has finished; *_dev entry points take device
+ *     pointers, enqueue on the context's stream and return at once;
+ *   - the caller owns every array it passes; handles are opaque and owned
+ *     by the library until the matching *_free / *_destroy;
+ *   - one context = one HIP stream = one host thread.  Distinct contexts
+ *     may be used from distinct threads concurrently.
+ */
+#ifndef DYGRASS_H
+#define DYGRASS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t dyg_status;
+#define DYG_OK 0
+#define DYG_ERR_ARG (-1)       /* invalid argument (sizes, ids, walker count) */
+#define DYG_ERR_HIP (-2)       /* HIP runtime failure */
+#define DYG_ERR_NOMEM (-3)     /* device or host allocation failed */
+#define DYG_ERR_IO (-4)        /* file could not be opened / written */
+#define DYG_ERR_FORMAT (-5)    /* malformed .mtx input */
+#define DYG_ERR_STATE (-6)     /* graph invariant violated (e.g. degree would reach 0) */
+#define DYG_ERR_ADMISSION (-7) /* request rejected by the memory budget */
+
+typedef struct dyg_ctx_s* dyg_ctx;
+typedef struct dyg_graph_s* dyg_graph;
+typedef struct dyg_dual_s* dyg_dual;
+
+/* ------------------------------------------------------------------ */
+/* Context                                                            */
+/* ------------------------------------------------------------------ */
+/* Replaces the implicit CUDA context / default stream the reference uses
+ * (cudaMalloc at random_walk_incremental/main.cu:276). */
+dyg_status dyg_ctx_create(int device, dyg_ctx* out);
+void dyg_ctx_destroy(dyg_ctx ctx);
+/* Message of the last failed call on ctx (ctx may be NULL for calls that
+ * failed before a context existed).  Never NULL. */
+const char* dyg_last_error(dyg_ctx ctx);
+const char* dyg_version(void);
+/* Stream of the context (a hipStream_t), for callers that enqueue their own
+ * work around the *_dev entry points. */
+void* dyg_ctx_stream(dyg_ctx ctx);
+/* Forces all work enqueued on ctx to finish. */
+dyg_status dyg_ctx_sync(dyg_ctx ctx);
+
+/* ------------------------------------------------------------------ */
+/* Device graph store                                                 */
+/* ------------------------------------------------------------------ */
+/* Upload an undirected graph given as per-vertex slot lists (CSR).  Slot
+ * order is preserved and is what the walks index.  Each row gets
+ * `slack` spare slots so inserts append in place.  keyed != 0 also builds
+ * the reference's edge map (key a*10^digits(V)+b -> slot pair) needed by
+ * dyg_graph_delete.
+ * Replaces gpu_graph::gpu_graph + createNeighborArray + copyToDevice
+ * (random_walk_incremental/gpu_graph.cuh:123-162,205-275) and
+ * GPU_Dual_Graph's dense layout (random_walk_decremental/gpu_graph.cuh:179-268). */
+dyg_status dyg_graph_upload(dyg_ctx ctx, uint32_t V, const uint64_t* row_ptr,
+                            const uint32_t* col, const float* w, uint32_t slack, int keyed,
+                            dyg_graph* out);
+/* V, directed slots in use (sum of degrees), slots allocated on the device. */
+dyg_status dyg_graph_info(dyg_graph g, uint32_t* V, uint64_t* nslots, uint64_t* capacity);
+/* Copy the current state back as compact CSR in slot order (row_ptr[V+1],
+ * col/w[nslots]).  Reads the device copy, not the host mirror. */
+dyg_status dyg_graph_download(dyg_graph g, uint64_t* row_ptr, uint32_t* col, float* w);
+/* Append edges in order: (u,v,w) goes to slot deg(u) of u, then (v,u,w) to
+ * slot deg(v) of v; a full row is relocated with its order kept.
+ * Replaces updateGraphFromResult/manageExtraNeighborsData
+ * (random_walk_incremental/gpu_graph.cuh:292-384) without the full re-upload. */
+dyg_status dyg_graph_insert(dyg_graph g, const uint32_t* u, const uint32_t* v, const float* w,
+                            uint32_t n);
+/* Delete edges in order by swapping the row's last slot into the hole (keyed
+ * graphs only).  hit[i] (optional) = 1 when edge i existed and was removed.
+ * Fails with DYG_ERR_STATE, leaving the graph unchanged, when a deletion
+ * would leave a vertex with degree 0.
+ * Replaces the dense half of removeAndUpdateDelEdges
+ * (random_walk_decremental/gpu_graph.cuh:338-400). */
+dyg_status dyg_graph_delete(dyg_graph g, const uint32_t* u, const uint32_t* v, uint32_t n,
+                            int8_t* hit);
+void dyg_graph_free(dyg_graph g);
+
+/* ------------------------------------------------------------------ */
+/* Walks                                                              */
+/* ------------------------------------------------------------------ */
+#define DYG_WALK_NO_PRUNE 1u /* run every walker to its own end (no early stop) */
+
+typedef struct {
+    uint32_t walkers;   /* walkers per edge, power of two in [64,1024] (argv[3]; 512 in decremental) */
+    uint32_t max_steps; /* walk bound, reference: 100 (random_walk_incremental/main.cu:44) */
+    float threshold;    /* insertion distortion threshold (argv[2]) */
+    uint32_t flags;     /* DYG_WALK_* */
+    uint64_t seed;      /* Philox key of the draw streams */
+} dyg_walk_params;
+
+typedef struct {
+    uint64_t hops;       /* neighbour draws executed over all walkers */
+    uint64_t redirects;  /* non-backtracking redirects to the row's last slot */
+    uint64_t leaf_stops; /* walks ended by a degree-1 row */
+    double kernel_ms;    /* device time of the walk kernel(s), HIP events */
+} dyg_walk_stats;
+
+/* Insertion walks, one edge per workgroup, walkers as wave64 lanes.
+ * Per edge: found_at = winner's hop index or -1, cond = winner's 1/R or -1,
+ * winner (optional) = winning walker, hops (optional) = draws of the edge.
+ * edge_id (optional, else 0..n-1) keys the draw streams.
+ * Replaces random_walk (random_walk_incremental/main.cu:70-211) + get_result
+ * (random_walk_incremental/gpu_graph.cuh:551-554). */
+dyg_status dyg_walk_incremental(dyg_graph H, const uint32_t* src, const uint32_t* dst,
+                                const float* w, const uint64_t* edge_id, uint32_t n,
+                                const dyg_walk_params* p, int32_t* found_at, float* cond,
+                                uint32_t* winner, uint32_t* hops, dyg_walk_stats* st);
+/* Deletion walks; path[i*max_steps + 0..found_at[i]] receives the winner's
+ * vertex path when found.  Replaces random_walk_decremental
+ * (random_walk_decremental/main.cu:77-220) + get_result (gpu_graph.cuh:831-841). */
+dyg_status dyg_walk_decremental(dyg_graph G, const uint32_t* src, const uint32_t* dst,
+                                const uint64_t* edge_id, uint32_t n, const dyg_walk_params* p,
+                                int32_t* found_at, float* cond, uint32_t* winner, uint32_t* hops,
+                                uint32_t* path, dyg_walk_stats* st);
+/* Two-hop recovery walks: `walkers` per endpoint, path3[(i*walkers+j)*3+0..2]
+ * = (u, n1, n2); key[i] keys endpoint i's draws.  Replaces random_walk_small
+ * (random_walk_decremental/main.cu:243-290) with its hop-two row fixed. */
+dyg_status dyg_walk_heuristic(dyg_graph G, const uint32_t* endpoint, const uint64_t* key,
+                              uint32_t n, uint32_t walkers, uint64_t seed, uint32_t* path3);
+
+/* Device-pointer forms of the two walks: all arrays are device pointers,
+ * work is enqueued on the context stream; `hops_total` (device, optional)
+ * accumulates draws.  Same results as the host forms. */
+dyg_status dyg_walk_incremental_dev(dyg_graph H, const uint32_t* src, const uint32_t* dst,
+                                    const float* w, const uint64_t* edge_id, uint32_t n,
+                                    const dyg_walk_params* p, int32_t* found_at, float* cond,
+                                    uint32_t* winner, uint32_t* hops,
+                                    unsigned long long* hops_total);
+dyg_status dyg_walk_decremental_dev(dyg_graph G, const uint32_t* src, const uint32_t* dst,
+                                    const uint64_t* edge_id, uint32_t n,
+                                    const dyg_walk_params* p, int32_t* found_at, float* cond,
+                                    uint32_t* winner, uint32_t* hops, uint32_t* path,
+                                    unsigned long long* hops_total);
+
+/* ------------------------------------------------------------------ */
+/* Whole incremental update (the incremental CLI's batch loop)        */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    uint64_t updates;      /* edges processed */
+    uint64_t added;        /* edges added to the sparsifier (no path found) */
+    uint64_t batches;      /* snapshot batches */
+    uint64_t hops;         /* neighbour draws over all walkers */
+    uint64_t redirects;
+    uint64_t leaf_stops;
+    double walk_ms;        /* device time in walk kernels (HIP events) */
+    double update_ms;      /* host planning + device apply of graph edits */
+    double total_ms;       /* wall time of the whole loop */
+} dyg_run_stats;
+
+/* Split m insertions into ceil(m/divisor)-sized snapshot batches; for each
+ * batch run the walks on H as it stood at batch start, then append the
+ * no-path edges in batch order.  found_at[i] (optional) = outcome of edge i;
+ * edge i's draws are keyed by i.  Replaces the loop of
+ * random_walk_incremental/main.cu:297-340 with gpu_graph.cuh:292-413. */
+dyg_status dyg_incremental_update(dyg_graph H, const uint32_t* src, const uint32_t* dst,
+                                  const float* w, uint32_t m, uint32_t divisor,
+                                  const dyg_walk_params* p, int32_t* found_at,
+                                  dyg_run_stats* st);
+
+/* ------------------------------------------------------------------ */
+/* Dual graph for deletions (GPU_Dual_Graph)                          */
+/* ------------------------------------------------------------------ */
+/* G (dense, on the device, keyed) plus H as a host key set, as in
+ * GPU_Dual_Graph (random_walk_decremental/gpu_graph.cuh:79-268).  G is given
+ * as CSR in slot order; H as an edge list (duplicates collapse). */
+dyg_status dyg_dual_create(dyg_ctx ctx, uint32_t V, const uint64_t* g_row_ptr,
+                           const uint32_t* g_col, const float* g_w, const uint32_t* h_u,
+                           const uint32_t* h_v, uint64_t h_m, dyg_dual* out);
+/* Key multiplier 10^digits(V) of the edge keys a*mult+b (a<b). */
+int64_t dyg_dual_key_multiplier(dyg_dual d);
+/* Process deletions in order (removeAndUpdateDelEdges,
+ * random_walk_decremental/gpu_graph.cuh:328-477): status[i] = 0 not in G,
+ * 1 removed from G only, 2 removed from G and H (queued for a walk).  ids[i]
+ * keys the walk of deletion i.  *n_queued = deletions queued for the walk. */
+dyg_status dyg_dual_remove(dyg_dual d, const uint32_t* u, const uint32_t* v, const uint64_t* ids,
+                           uint32_t n, int8_t* status, uint32_t* n_queued);
+/* Walk the queued deletions on G, then add the winners' path edges to H and
+ * to the added set in queue order (random_walk_decremental/main.cu:392-397,
+ * gpu_graph.cuh:509-559).  *no_path = queued edges without a path (they are
+ * queued for dyg_dual_heuristic), *added = edges newly added to H. */
+dyg_status dyg_dual_walk(dyg_dual d, const dyg_walk_params* p, uint32_t* no_path,
+                         uint64_t* added, dyg_walk_stats* st);
+/* Two-hop recovery for the no-path endpoints (main.cu:400-408,
+ * gpu_graph.cuh:577-615): inserts (u,n1),(n1,n2) into H only. */
+dyg_status dyg_dual_heuristic(dyg_dual d, uint32_t walkers, uint64_t seed);
+uint64_t dyg_dual_sparse_size(dyg_dual d);
+uint64_t dyg_dual_added_count(dyg_dual d);
+/* The added set as (u,v) pairs, u < v, sorted by key. */
+dyg_status dyg_dual_added_edges(dyg_dual d, uint32_t* u, uint32_t* v);
+dyg_graph dyg_dual_dense(dyg_dual d);
+void dyg_dual_free(dyg_dual d);
+
+/* ------------------------------------------------------------------ */
+/* .mtx I/O with the reference's parsing rules                        */
+/* ------------------------------------------------------------------ */
+typedef struct dyg_mtx_s* dyg_mtx;
+/* base 0: random_walk_incremental/functions.cpp:58-239 (read_graph);
+ * base 1: random_walk_decremental/functions.cpp:69-281 (V = v_max+1).
+ * Slots are laid out in file order; each line (a,b,w) appends b to a's
+ * row then a to b's row. */
+dyg_status dyg_mtx_read_graph(const char* path, int base, int weighted, dyg_mtx* out);
+/* Edge list in file order: 1-based ids made 0-based
+ * (random_walk_incremental/functions.cpp:261-370). */
+dyg_status dyg_mtx_read_edges(const char* path, int weighted, dyg_mtx* out);
+uint32_t dyg_mtx_nv(dyg_mtx m);
+uint64_t dyg_mtx_nlines(dyg_mtx m);
+const uint64_t* dyg_mtx_row_ptr(dyg_mtx m); /* [V+1], graphs only */
+const uint32_t* dyg_mtx_col(dyg_mtx m);     /* [2*lines], graphs only */
+const float* dyg_mtx_w(dyg_mtx m);          /* [2*lines], graphs only */
+const uint32_t* dyg_mtx_line_u(dyg_mtx m);  /* [lines] first id of each line (0-based) */
+const uint32_t* dyg_mtx_line_v(dyg_mtx m);  /* [lines] */
+const float* dyg_mtx_line_w(dyg_mtx m);     /* [lines] (1.0 when unweighted) */
+void dyg_mtx_free(dyg_mtx m);
+/* Write lines "u+1 v+1 w" with w printed as C++ `ostream << float` does
+ * (6 significant digits) -- random_walk_incremental/gpu_graph.cuh:427-454. */
+dyg_status dyg_mtx_write_weighted(const char* path, const uint32_t* u, const uint32_t* v,
+                                  const float* w, uint64_t n, int append);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DYGRASS_H */

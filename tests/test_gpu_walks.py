@@ -1,0 +1,202 @@
+"""GPU parity: the HIP walks against the oracle on the same inputs.
+
+Bit-exact is required for everything integer (found_at, winner, hop counts,
+paths, graph slot order) and for the float32 conductance, which both sides
+compute with IEEE division (compared as bit patterns).
+"""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+from conftest import csr, f32bits
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle():
+    from oracle import pyoracle
+
+    return pyoracle
+
+
+def _first_batch(wl, n=None):
+    I = (len(wl.ins) + 9) // 10
+    n = I if n is None else n
+    return wl.ins.u[:n], wl.ins.v[:n], wl.ins.w[:n]
+
+
+@pytest.mark.parametrize("prune", [True, False])
+def test_incremental_walks_match_oracle_grid(ctx, grid_wl, prune):
+    from dygrass_amd import DeviceGraph, params, walk_incremental
+
+    po = _oracle()
+    rp, col, w = csr(grid_wl.H)
+    H = DeviceGraph(ctx, grid_wl.H.V, rp, col, w)
+    s, t, we = _first_batch(grid_wl)
+    p = params(512, 100, 10.0, seed=0, prune=prune)
+    got = walk_incremental(H, s, t, we, p)
+    og = po.Graph(grid_wl.H.V, rp, col, w)
+    f, c, win, hops = og.walk_inc_batch(s, t, we, 10.0, 512, seed=0, threads=8)
+    assert np.array_equal(got.found_at, f)
+    assert np.array_equal(f32bits(got.cond), f32bits(c))
+    assert np.array_equal(got.winner, win)
+    if not prune:
+        assert np.array_equal(got.hops.astype(np.uint64), hops)
+        assert got.stats["hops"] == int(hops.sum())
+    else:
+        assert got.stats["hops"] <= int(hops.sum())
+
+
+@pytest.mark.parametrize("W", [1, 2, 8, 32, 64, 128, 256, 1024])
+def test_incremental_walker_counts(ctx, grid_wl, W):
+    from dygrass_amd import DeviceGraph, params, walk_incremental
+
+    po = _oracle()
+    rp, col, w = csr(grid_wl.H)
+    H = DeviceGraph(ctx, grid_wl.H.V, rp, col, w)
+    s, t, we = _first_batch(grid_wl, 64)
+    ids = np.arange(1000, 1064, dtype=np.uint64)
+    got = walk_incremental(H, s, t, we, params(W, 100, 10.0, seed=7, prune=False), edge_id=ids)
+    og = po.Graph(grid_wl.H.V, rp, col, w)
+    f, c, win, hops = og.walk_inc_batch(s, t, we, 10.0, W, seed=7, edge_id=ids, threads=8)
+    assert np.array_equal(got.found_at, f)
+    assert np.array_equal(f32bits(got.cond), f32bits(c))
+    assert np.array_equal(got.winner, win)
+    assert np.array_equal(got.hops.astype(np.uint64), hops)
+
+
+def test_incremental_update_matches_oracle_run(ctx, grid_wl):
+    """Whole snapshot-batched run: every decision and the final slot order of H."""
+    from dygrass_amd import DeviceGraph, incremental_update, params
+
+    po = _oracle()
+    rp, col, w = csr(grid_wl.H)
+    H = DeviceGraph(ctx, grid_wl.H.V, rp, col, w, slack=1)
+    found, stats = incremental_update(H, grid_wl.ins.u, grid_wl.ins.v, grid_wl.ins.w,
+                                      params(512, 100, 10.0, seed=0))
+    og = po.Graph(grid_wl.H.V, rp, col, w)
+    starts = po.batches(len(grid_wl.ins), 10)
+    expect = np.zeros(len(grid_wl.ins), np.int32)
+    for b0, b1 in zip(starts[:-1], starts[1:]):
+        s, t, we = grid_wl.ins.u[b0:b1], grid_wl.ins.v[b0:b1], grid_wl.ins.w[b0:b1]
+        f, _, _, _ = og.walk_inc_batch(s, t, we, 10.0, 512, seed=0,
+                                       edge_id=np.arange(b0, b1), threads=8)
+        expect[b0:b1] = f
+        for i in np.nonzero(f == -1)[0]:
+            og.insert(int(s[i]), int(t[i]), float(we[i]))
+    assert np.array_equal(found, expect)
+    assert stats["added"] == int((expect == -1).sum())
+    assert stats["batches"] == 10
+    grp, gcol, gw = H.download()
+    orp, ocol, ow = og.export()
+    assert np.array_equal(grp, orp)
+    assert np.array_equal(gcol, ocol)
+    assert np.array_equal(f32bits(gw), f32bits(ow))
+
+
+def test_insert_relocation_keeps_slot_order(ctx):
+    """Many appends to one row force relocations; order must be file order."""
+    from dygrass_amd import DeviceGraph
+
+    po = _oracle()
+    V = 40
+    u = np.arange(V - 1, dtype=np.uint32)
+    v = u + 1
+    w = np.linspace(0.5, 2.0, V - 1).astype(np.float32)
+    rp, col, ww = __import__("dygrass_amd").csr_from_edges(V, u, v, w)
+    G = DeviceGraph(ctx, V, rp, col, ww, slack=0)
+    og = po.Graph(V, rp, col, ww)
+    rng = np.random.default_rng(3)
+    for _ in range(6):
+        a = rng.integers(0, V, 25).astype(np.uint32)
+        b = ((a + rng.integers(2, V - 1, 25)) % V).astype(np.uint32)
+        x = rng.uniform(0.1, 10, 25).astype(np.float32)
+        a[:10] = 5  # hammer one row
+        G.insert(a, b, x)
+        for i in range(25):
+            og.insert(int(a[i]), int(b[i]), float(x[i]))
+    g = G.download()
+    o = og.export()
+    assert np.array_equal(g[0], o[0]) and np.array_equal(g[1], o[1])
+    assert np.array_equal(f32bits(g[2]), f32bits(o[2]))
+
+
+def test_delete_swap_with_last_matches_oracle(ctx, mesh_wl):
+    from dygrass_amd import DeviceGraph, synth
+
+    po = _oracle()
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "g.mtx")
+        synth.write_mtx(path, mesh_wl.G.u, mesh_wl.G.v, mesh_wl.G.w, base=1)
+        od = po.Dense(path)
+        rp, col, w = csr(mesh_wl.G)
+        G = DeviceGraph(ctx, mesh_wl.G.V, rp, col, w, slack=0, keyed=True)
+        du, dv = mesh_wl.dels.u, mesh_wl.dels.v
+        # include a repeated deletion and a non-edge: both must miss
+        du = np.concatenate([du, du[:3], [0]]).astype(np.uint32)
+        dv = np.concatenate([dv, dv[:3], [mesh_wl.G.V - 1]]).astype(np.uint32)
+        hit = G.delete(du, dv)
+        ohit = np.array([od.remove(int(a), int(b)) for a, b in zip(du, dv)])
+        assert np.array_equal(hit == 1, ohit == 1)
+        g = G.download()
+        o = od.graph.export()
+        assert np.array_equal(g[0], o[0]) and np.array_equal(g[1], o[1])
+        assert np.array_equal(f32bits(g[2]), f32bits(o[2]))
+
+
+@pytest.mark.parametrize("prune", [True, False])
+def test_decremental_walks_match_oracle(ctx, mesh_wl, prune):
+    from dygrass_amd import DeviceGraph, params, walk_decremental
+
+    po = _oracle()
+    rp, col, w = csr(mesh_wl.G)
+    G = DeviceGraph(ctx, mesh_wl.G.V, rp, col, w, slack=0, keyed=True)
+    n = min(300, len(mesh_wl.dels))
+    du, dv = mesh_wl.dels.u[:n], mesh_wl.dels.v[:n]
+    G.delete(du, dv)
+    a, b = np.minimum(du, dv), np.maximum(du, dv)
+    ids = np.arange(n, dtype=np.uint64) * 3 + 11
+    got = walk_decremental(G, a, b, params(512, 100, seed=5, prune=prune), edge_id=ids)
+    grp, gcol, gw = G.download()
+    og = po.Graph(mesh_wl.G.V, grp, gcol, gw)
+    f, c, win, hops, path = og.walk_dec_batch(a, b, 512, seed=5, edge_id=ids, threads=8)
+    assert np.array_equal(got.found_at, f)
+    assert np.array_equal(f32bits(got.cond), f32bits(c))
+    assert np.array_equal(got.winner, win)
+    for i in range(n):
+        if f[i] > 0:
+            assert np.array_equal(got.path[i, : f[i] + 1], path[i, : f[i] + 1])
+            assert got.path[i, 0] == a[i] and got.path[i, f[i]] == b[i]
+    if not prune:
+        assert np.array_equal(got.hops.astype(np.uint64), hops)
+
+
+def test_heuristic_matches_oracle(ctx, mesh_wl):
+    from dygrass_amd import DeviceGraph, walk_heuristic
+
+    po = _oracle()
+    rp, col, w = csr(mesh_wl.G)
+    G = DeviceGraph(ctx, mesh_wl.G.V, rp, col, w, slack=0)
+    ep = np.arange(0, mesh_wl.G.V, 97, dtype=np.uint32)
+    keys = (np.arange(len(ep), dtype=np.uint64) << np.uint64(1)) | np.uint64(1)
+    got = walk_heuristic(G, ep, keys, 16, seed=9)
+    og = po.Graph(mesh_wl.G.V, rp, col, w)
+    for i, (u, k) in enumerate(zip(ep, keys)):
+        assert np.array_equal(got[i], og.heuristic(int(u), 16, 9, int(k)))
+
+
+def test_walks_are_deterministic_and_prune_invariant(ctx, grid_wl):
+    from dygrass_amd import DeviceGraph, params, walk_incremental
+
+    rp, col, w = csr(grid_wl.H)
+    H = DeviceGraph(ctx, grid_wl.H.V, rp, col, w)
+    s, t, we = grid_wl.ins.u, grid_wl.ins.v, grid_wl.ins.w
+    a = walk_incremental(H, s, t, we, params(256, 100, 10.0, seed=3, prune=True))
+    b = walk_incremental(H, s, t, we, params(256, 100, 10.0, seed=3, prune=True))
+    c = walk_incremental(H, s, t, we, params(256, 100, 10.0, seed=3, prune=False))
+    for x in (b, c):
+        assert np.array_equal(a.found_at, x.found_at)
+        assert np.array_equal(f32bits(a.cond), f32bits(x.cond))
+        assert np.array_equal(a.winner, x.winner)
