@@ -750,6 +750,46 @@ int64_t orc_read_targets(const char* path, int weighted, uint32_t divisor, uint3
     return (int64_t)cnt;
 }
 
+/* The incremental batch loop on arrays (random_walk_incremental/main.cu:297-340,
+ * gpu_graph.cuh:292-336): snapshot batches of ceil(m/divisor), walks of a
+ * batch on H as it stood at batch start, then the no-path edges appended in
+ * batch order.  Edge i's draws are keyed by i. */
+int orc_incremental_arrays(orc_graph* H, uint32_t m, const uint32_t* src, const uint32_t* dst,
+                           const float* w, uint32_t divisor, float thr, uint32_t W,
+                           uint32_t max_steps, uint64_t seed, int threads, int32_t* found_at,
+                           orc_run_stats* st) {
+    std::vector<uint32_t> starts = batch_starts(m, divisor);
+    const uint32_t nb = (uint32_t)starts.size() - 1;
+    std::vector<uint64_t> id(m);
+    for (uint32_t i = 0; i < m; ++i) id[i] = i;
+    std::vector<float> c(m);
+    std::vector<uint64_t> h(m);
+    uint64_t added = 0;
+    double t_walk = 0, t0 = now_s();
+    for (uint32_t k = 0; k < nb; ++k) {
+        const uint32_t b0 = starts[k], n = starts[k + 1] - starts[k];
+        double tw = now_s();
+        orc_walk_inc_batch(H, n, src + b0, dst + b0, w + b0, id.data() + b0, thr, W, max_steps,
+                           seed, threads, found_at + b0, c.data() + b0, nullptr, h.data() + b0);
+        t_walk += now_s() - tw;
+        for (uint32_t i = b0; i < b0 + n; ++i)
+            if (found_at[i] == -1) {
+                orc_graph_insert(H, src[i], dst[i], w[i]);
+                ++added;
+            }
+    }
+    if (st) {
+        std::memset(st, 0, sizeof(*st));
+        st->n_updates = m;
+        st->n_added = added;
+        st->n_batches = nb;
+        for (uint32_t i = 0; i < m; ++i) st->hops += h[i];
+        st->walk_seconds = t_walk;
+        st->total_seconds = now_s() - t0;
+    }
+    return 0;
+}
+
 /* Whole incremental run: random_walk_incremental/main.cu:240-362 with the
  * host update of gpu_graph.cuh:292-336 and the writer :427-454. */
 int orc_incremental_run(const char* dir, float thr, uint32_t W, uint64_t seed, int threads,

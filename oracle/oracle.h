@@ -116,6 +116,13 @@ typedef struct {
     double total_seconds;   /* wall time of the batch loop */
 } orc_run_stats;
 
+/* The incremental batch loop on arrays (random_walk_incremental/main.cu:297-340):
+ * H is modified in place; found_at[m] receives every decision. */
+int orc_incremental_arrays(orc_graph* H, uint32_t m, const uint32_t* src, const uint32_t* dst,
+                           const float* w, uint32_t divisor, float thr, uint32_t W,
+                           uint32_t max_steps, uint64_t seed, int threads, int32_t* found_at,
+                           orc_run_stats* st);
+
 /* random_walk_incremental/main.cu:240-362 */
 int orc_incremental_run(const char* dir, float thr, uint32_t W, uint64_t seed, int threads,
                         const char* log_path, orc_run_stats* st);

@@ -56,6 +56,9 @@ _SIGS = {
                                   C.c_uint64, C.c_int, i32p, f32p, u32p, u64p, u32p]),
     "orc_batches": (C.c_uint32, [C.c_uint32, C.c_uint32, u32p]),
     "orc_format_float": (C.c_int, [C.c_float, C.c_char_p, C.c_int]),
+    "orc_incremental_arrays": (C.c_int, [vp, C.c_uint32, u32p, u32p, f32p, C.c_uint32, C.c_float,
+                                         C.c_uint32, C.c_uint32, C.c_uint64, C.c_int, i32p,
+                                         C.POINTER(RunStats)]),
     "orc_dense_open": (vp, [C.c_char_p]),
     "orc_dense_remove": (C.c_int, [vp, C.c_uint32, C.c_uint32]),
     "orc_dense_graph": (vp, [vp]),
@@ -190,6 +193,19 @@ class Graph:
                                  _p(eid, u64p), thr, W, max_steps, seed, threads, _p(f, i32p),
                                  _p(c, f32p), _p(win, u32p), _p(h, u64p))
         return f[:n], c[:n], win[:n], h[:n]
+
+    def incremental(self, src, dst, w, thr, W, seed=0, threads=1, divisor=10, max_steps=100):
+        """Whole snapshot-batched insertion run; modifies this graph."""
+        src = np.ascontiguousarray(src, np.uint32)
+        dst = np.ascontiguousarray(dst, np.uint32)
+        w = np.ascontiguousarray(w, np.float32)
+        m = len(src)
+        f = np.zeros(max(m, 1), np.int32)
+        st = RunStats()
+        lib().orc_incremental_arrays(self.h, m, _p(src, u32p), _p(dst, u32p), _p(w, f32p),
+                                     divisor, thr, W, max_steps, seed, threads, _p(f, i32p),
+                                     C.byref(st))
+        return f[:m], st
 
     def walk_dec_batch(self, src, dst, W, seed=0, edge_id=None, threads=1, max_steps=100):
         src = np.ascontiguousarray(src, np.uint32)
