@@ -12,6 +12,7 @@ from .core import (  # noqa: F401
     WalkResult,
     csr_from_edges,
     incremental_update,
+    incremental_update_dev,
     params,
     walk_decremental,
     walk_heuristic,
@@ -19,7 +20,7 @@ from .core import (  # noqa: F401
 )
 
 __all__ = [
-    "Context", "DeviceGraph", "DualGraph", "WalkResult", "csr_from_edges", "incremental_update",
+    "Context", "DeviceGraph", "DualGraph", "WalkResult", "csr_from_edges", "incremental_update", "incremental_update_dev",
     "params", "walk_decremental", "walk_heuristic", "walk_incremental", "DygError", "WalkParams",
     "WalkStats", "RunStats", "LIB_PATH", "BIN_DIR",
 ]

@@ -23,6 +23,8 @@ f32p = C.POINTER(C.c_float)
 vp = C.c_void_p
 
 DYG_WALK_NO_PRUNE = 1
+DYG_WALK_PER_EDGE = 2
+DYG_RUN_HOST_EDITS = 4
 
 STATUS = {
     0: "DYG_OK", -1: "DYG_ERR_ARG", -2: "DYG_ERR_HIP", -3: "DYG_ERR_NOMEM", -4: "DYG_ERR_IO",
@@ -81,6 +83,10 @@ _SIGS = {
                                              C.POINTER(WalkParams), vp, vp, vp, vp, vp, vp]),
     "dyg_incremental_update": (C.c_int32, [vp, u32p, u32p, f32p, C.c_uint32, C.c_uint32,
                                            C.POINTER(WalkParams), i32p, C.POINTER(RunStats)]),
+    "dyg_incremental_update_dev": (C.c_int32, [vp, vp, vp, vp, C.c_uint32, C.c_uint32,
+                                               C.POINTER(WalkParams), vp, vp,
+                                               C.POINTER(RunStats)]),
+    "dyg_graph_reserve": (C.c_int32, [vp, C.c_uint64, C.c_uint32]),
     "dyg_dual_create": (C.c_int32, [vp, C.c_uint32, u64p, u32p, f32p, u32p, u32p, C.c_uint64,
                                     C.POINTER(vp)]),
     "dyg_dual_key_multiplier": (C.c_int64, [vp]),

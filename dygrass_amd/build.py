@@ -45,7 +45,10 @@ HIP_FLAGS = [
 ]
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall"]
 
-LIB_SOURCES = ["walk_kernels.hip", "graph_store.hip", "api.hip", "mtx_io.cpp"]
+LIB_SOURCES = [
+    "walk_kernels.hip", "graph_store.hip", "device_edits.hip", "incremental.hip", "api.hip",
+    "mtx_io.cpp",
+]
 CLIS = {"incremental_update": "cli_incremental.cpp", "decremental_update": "cli_decremental.cpp"}
 
 

@@ -41,8 +41,13 @@ class WalkResult:
 
 
 def params(walkers: int, max_steps: int = 100, threshold: float = 0.0, seed: int = 0,
-           prune: bool = True) -> L.WalkParams:
-    return L.WalkParams(walkers, max_steps, threshold, 0 if prune else L.DYG_WALK_NO_PRUNE, seed)
+           prune: bool = True, per_edge: bool = False, host_edits: bool = False) -> L.WalkParams:
+    """Walk parameters.  per_edge: one workgroup per edge (the reference's
+    launch shape) instead of the work-queue kernel; host_edits: plan each
+    batch's appends on the host (dyg_incremental_update only)."""
+    flags = ((0 if prune else L.DYG_WALK_NO_PRUNE) | (L.DYG_WALK_PER_EDGE if per_edge else 0)
+             | (L.DYG_RUN_HOST_EDITS if host_edits else 0))
+    return L.WalkParams(walkers, max_steps, threshold, flags, seed)
 
 
 class Context:
@@ -125,6 +130,10 @@ class DeviceGraph:
         check(self._lib.dyg_graph_download(self.handle, ptr(rp, L.u64p), ptr(col, L.u32p),
                                            ptr(w, L.f32p)), self.ctx.handle)
         return rp, col[:n], w[:n]
+
+    def reserve(self, max_inserts: int, max_batch: int) -> None:
+        """Pre-allocate arena and device-edit buffers (dyg_graph_reserve)."""
+        check(self._lib.dyg_graph_reserve(self.handle, max_inserts, max_batch), self.ctx.handle)
 
     def insert(self, u, v, w) -> None:
         """random_walk_incremental/gpu_graph.cuh:319-325,357-384 (append in order)."""
@@ -212,6 +221,18 @@ def incremental_update(H: DeviceGraph, src, dst, w, p: L.WalkParams, divisor: in
                                         C.byref(st)), H.ctx.handle)
     stats = {k: getattr(st, k) for k, _ in L.RunStats._fields_}
     return f[:m], stats
+
+
+def incremental_update_dev(H: DeviceGraph, src_ptr: int, dst_ptr: int, w_ptr: int, m: int,
+                           found_ptr: int, cond_ptr: int, p: L.WalkParams, divisor: int = 10):
+    """The batch loop on inputs already in device memory (dyg_incremental_update_dev).
+    Pointers are device addresses (e.g. torch tensor data_ptr()); the caller
+    makes sure the data is ready before the call."""
+    st = L.RunStats()
+    check(H._lib.dyg_incremental_update_dev(H.handle, src_ptr, dst_ptr, w_ptr, m, divisor,
+                                            C.byref(p), found_ptr, cond_ptr, C.byref(st)),
+          H.ctx.handle)
+    return {k: getattr(st, k) for k, _ in L.RunStats._fields_}
 
 
 class DualGraph:

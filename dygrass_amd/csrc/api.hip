@@ -1,5 +1,5 @@
-// api.hip -- context, walk entry points, the incremental batch loop and the
-// decremental dual graph (C ABI declared in include/dygrass.h).
+// api.hip -- context, walk entry points and the decremental dual graph
+// (C ABI declared in include/dygrass.h).
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -19,14 +19,6 @@ void set_error(dyg_ctx ctx, const std::string& msg) {
     g_tls_error = msg;
 }
 
-namespace {
-
-double now_ms() {
-    return std::chrono::duration<double, std::milli>(
-               std::chrono::steady_clock::now().time_since_epoch())
-        .count();
-}
-
 dyg_status check_params(dyg_ctx ctx, const dyg_walk_params* p) {
     if (!p || !valid_walkers(p->walkers)) {
         set_error(ctx, "walk params: walkers must be a power of two in [1,1024]");
@@ -38,6 +30,8 @@ dyg_status check_params(dyg_ctx ctx, const dyg_walk_params* p) {
     }
     return DYG_OK;
 }
+
+namespace {
 
 // Device staging area for one host-pointer walk call, carved from ctx->dscratch.
 struct Carve {
@@ -303,130 +297,6 @@ extern "C" dyg_status dyg_walk_heuristic(dyg_graph G, const uint32_t* endpoint,
     DYG_HIP(ctx, launch_heuristic(G->d_rows, G->d_adj, d_ep, d_key, n, walkers, seed, d_out, s));
     DYG_HIP(ctx, hipMemcpyAsync(path3, d_out, out * 4, hipMemcpyDeviceToHost, s));
     DYG_HIP(ctx, hipStreamSynchronize(s));
-    return DYG_OK;
-}
-
-/* ------------------------------------------------------------------ */
-/* Incremental batch loop (random_walk_incremental/main.cu:297-340)   */
-/* ------------------------------------------------------------------ */
-extern "C" dyg_status dyg_incremental_update(dyg_graph H, const uint32_t* src, const uint32_t* dst,
-                                             const float* w, uint32_t m, uint32_t divisor,
-                                             const dyg_walk_params* p, int32_t* found_at,
-                                             dyg_run_stats* st) {
-    if (!H || divisor == 0 || (m && (!src || !dst || !w))) return DYG_ERR_ARG;
-    dyg_ctx ctx = H->ctx;
-    dyg_status cs = check_params(ctx, p);
-    if (cs != DYG_OK) return cs;
-    for (uint32_t i = 0; i < m; ++i)
-        if (src[i] >= H->V || dst[i] >= H->V) {
-            set_error(ctx, "incremental: vertex id out of range");
-            return DYG_ERR_ARG;
-        }
-    dyg_run_stats rs{};
-    const double t0 = now_ms();
-    const uint32_t I = (m + divisor - 1) / divisor;
-    // inputs resident on the device for the whole run
-    uint32_t *d_src = nullptr, *d_dst = nullptr;
-    float *d_w = nullptr, *d_cond = nullptr;
-    int32_t* d_found = nullptr;
-    unsigned long long* d_stats = nullptr;
-    hipStream_t s = ctx->stream;
-    auto cleanup = [&]() {
-        if (d_src) (void)hipFree(d_src);
-        if (d_dst) (void)hipFree(d_dst);
-        if (d_w) (void)hipFree(d_w);
-        if (d_cond) (void)hipFree(d_cond);
-        if (d_found) (void)hipFree(d_found);
-        if (d_stats) (void)hipFree(d_stats);
-    };
-    const size_t mb = std::max<size_t>(m, 1);
-    if (hipMalloc(&d_src, mb * 4) != hipSuccess || hipMalloc(&d_dst, mb * 4) != hipSuccess ||
-        hipMalloc(&d_w, mb * 4) != hipSuccess || hipMalloc(&d_cond, mb * 4) != hipSuccess ||
-        hipMalloc(&d_found, mb * 4) != hipSuccess || hipMalloc(&d_stats, 64) != hipSuccess) {
-        cleanup();
-        set_error(ctx, "incremental: device allocation failed");
-        return DYG_ERR_NOMEM;
-    }
-    std::vector<int32_t> found(mb);
-    hipError_t e = hipMemcpyAsync(d_src, src, m * 4ull, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(d_dst, dst, m * 4ull, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(d_w, w, m * 4ull, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = hipMemsetAsync(d_stats, 0, 64, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (e != hipSuccess) {
-        cleanup();
-        set_error(ctx, std::string("incremental: ") + hipGetErrorString(e));
-        return DYG_ERR_HIP;
-    }
-    std::vector<uint32_t> au, av;
-    std::vector<float> aw;
-    dyg_status ret = DYG_OK;
-    for (uint32_t b0 = 0; b0 < m && ret == DYG_OK; b0 += I) {
-        const uint32_t n = std::min(I, m - b0);
-        WalkLaunch a{};
-        a.rows = H->d_rows;
-        a.adj = H->d_adj;
-        a.src = d_src + b0;
-        a.dst = d_dst + b0;
-        a.w = d_w + b0;
-        a.eid = nullptr;
-        a.eid_base = b0;
-        a.n = n;
-        a.walkers = p->walkers;
-        a.max_steps = p->max_steps;
-        a.thr = p->threshold;
-        a.flags = p->flags;
-        a.seed = p->seed;
-        a.found = d_found + b0;
-        a.cond = d_cond + b0;
-        a.stats = d_stats;
-        e = hipEventRecord(ctx->ev0, s);
-        if (e == hipSuccess) e = launch_walk_incremental(a, s);
-        if (e == hipSuccess) e = hipEventRecord(ctx->ev1, s);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(found.data() + b0, d_found + b0, n * 4ull, hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-        float ms = 0.f;
-        if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
-        if (e != hipSuccess) {
-            set_error(ctx, std::string("incremental walk: ") + hipGetErrorString(e));
-            ret = DYG_ERR_HIP;
-            break;
-        }
-        rs.walk_ms += ms;
-        const double tu = now_ms();
-        au.clear();
-        av.clear();
-        aw.clear();
-        for (uint32_t i = 0; i < n; ++i)
-            if (found[b0 + i] == -1) {
-                au.push_back(src[b0 + i]);
-                av.push_back(dst[b0 + i]);
-                aw.push_back(w[b0 + i]);
-            }
-        rs.added += au.size();
-        ret = graph_apply_inserts(H, au.data(), av.data(), aw.data(), (uint32_t)au.size());
-        rs.update_ms += now_ms() - tu;
-        rs.batches++;
-    }
-    unsigned long long hst[3] = {0, 0, 0};
-    if (ret == DYG_OK) {
-        e = hipMemcpyAsync(hst, d_stats, sizeof(hst), hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-        if (e != hipSuccess) {
-            set_error(ctx, std::string("incremental: ") + hipGetErrorString(e));
-            ret = DYG_ERR_HIP;
-        }
-    }
-    cleanup();
-    if (ret != DYG_OK) return ret;
-    if (found_at) std::copy(found.begin(), found.begin() + m, found_at);
-    rs.updates = m;
-    rs.hops = hst[0];
-    rs.redirects = hst[1];
-    rs.leaf_stops = hst[2];
-    rs.total_ms = now_ms() - t0;
-    if (st) *st = rs;
     return DYG_OK;
 }
 

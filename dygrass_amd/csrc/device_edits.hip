@@ -1,0 +1,273 @@
+// device_edits.hip -- snapshot-batch inserts applied on the device.
+//
+// After a batch's walks, the edges whose walks found no path are appended to
+// H in batch order, (u,v) to u's row then (v,u) to v's row, exactly as the
+// host plan does (random_walk_incremental/gpu_graph.cuh:297-329), but without
+// a host round trip:
+//   k_compact   one workgroup: ordered compaction of the no-path edges and
+//               the list of appends keyed by row, padded to 2n
+//   radix sort  stable sort of appends by row (hipCUB), so each row's
+//               appends stay in batch order
+//   k_heads     first append of each row records its position; counts per row
+//   k_alloc     per row: relocate to the arena tail when the row is full
+//               (device bump allocator, order kept) and publish (start, deg)
+//   k_write     each append writes its slot at start + old degree + rank
+// Slot order is what the walks index, and it equals the host plan's; the
+// physical position of a relocated row may differ, which no walk can see.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cstring>
+
+#include "internal.h"
+
+namespace dyg {
+namespace {
+
+constexpr int kCompactThreads = 1024;
+
+__global__ __launch_bounds__(kCompactThreads) void k_compact(
+    const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
+    const float* __restrict__ w, const int32_t* __restrict__ found, uint32_t n,
+    uint32_t* __restrict__ au, uint32_t* __restrict__ av, float* __restrict__ aw,
+    uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, uint32_t* __restrict__ count,
+    unsigned long long* __restrict__ added_total) {
+    __shared__ uint32_t s_scan[kCompactThreads];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t chunk = (n + kCompactThreads - 1) / kCompactThreads;
+    const uint32_t b = min(n, tid * chunk), e = min(n, b + chunk);
+    uint32_t c = 0;
+    for (uint32_t i = b; i < e; ++i) c += (found[i] == -1) ? 1u : 0u;
+    s_scan[tid] = c;
+    __syncthreads();
+    // inclusive Hillis-Steele scan over the per-thread counts
+    for (uint32_t off = 1; off < kCompactThreads; off <<= 1) {
+        const uint32_t v = tid >= off ? s_scan[tid - off] : 0u;
+        __syncthreads();
+        s_scan[tid] += v;
+        __syncthreads();
+    }
+    uint32_t pos = s_scan[tid] - c;
+    const uint32_t total = s_scan[kCompactThreads - 1];
+    for (uint32_t i = b; i < e; ++i)
+        if (found[i] == -1) {
+            au[pos] = src[i];
+            av[pos] = dst[i];
+            aw[pos] = w[i];
+            keys[2 * pos] = src[i];
+            vals[2 * pos] = 2 * pos;
+            keys[2 * pos + 1] = dst[i];
+            vals[2 * pos + 1] = 2 * pos + 1;
+            ++pos;
+        }
+    // padding sorts after every real row
+    for (uint32_t p = 2 * total + tid; p < 2 * n; p += kCompactThreads) {
+        keys[p] = 0xFFFFFFFFu;
+        vals[p] = 0xFFFFFFFFu;
+    }
+    if (tid == 0) {
+        *count = total;
+        *added_total += total;
+    }
+}
+
+__global__ void k_heads(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ count,
+                        uint32_t* __restrict__ segstart, uint32_t* __restrict__ segcnt) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= 2 * *count) return;
+    const uint32_t r = keys[p];
+    if (p == 0 || keys[p - 1] != r) segstart[r] = p;
+    atomicAdd(&segcnt[r], 1u);
+}
+
+__global__ void k_alloc(uint2* __restrict__ rows, uint2* __restrict__ adj,
+                        uint32_t* __restrict__ cap, const uint32_t* __restrict__ keys,
+                        const uint32_t* __restrict__ count, const uint32_t* __restrict__ segstart,
+                        const uint32_t* __restrict__ segcnt, uint32_t* __restrict__ rowbase,
+                        unsigned long long* __restrict__ used, unsigned long long cap_slots,
+                        uint32_t slack, uint32_t* __restrict__ err) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= 2 * *count) return;
+    const uint32_t r = keys[p];
+    if (segstart[r] != p) return;
+    const uint32_t k = segcnt[r];
+    uint2 row = rows[r];
+    const uint32_t d0 = row.y;
+    const uint32_t c = cap[r];
+    if ((unsigned long long)d0 + k > c) {
+        unsigned long long ncap = 2ull * c;
+        ncap = max(ncap, (unsigned long long)d0 + k + slack);
+        ncap = max(ncap, 4ull);
+        const unsigned long long ns = atomicAdd(used, ncap);
+        if (ns + ncap > cap_slots) {
+            atomicOr(err, 1u);
+            rowbase[r] = 0xFFFFFFFFu;
+            return;
+        }
+        for (uint32_t q = 0; q < d0; ++q) adj[ns + q] = adj[row.x + q];
+        row.x = (uint32_t)ns;
+        cap[r] = (uint32_t)ncap;
+    }
+    rowbase[r] = row.x + d0;
+    rows[r] = make_uint2(row.x, d0 + k);
+}
+
+__global__ void k_write(uint2* __restrict__ adj, const uint32_t* __restrict__ keys,
+                        const uint32_t* __restrict__ vals, const uint32_t* __restrict__ count,
+                        uint32_t* __restrict__ segstart, uint32_t* __restrict__ segcnt,
+                        const uint32_t* __restrict__ rowbase, const uint32_t* __restrict__ au,
+                        const uint32_t* __restrict__ av, const float* __restrict__ aw) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= 2 * *count) return;
+    const uint32_t r = keys[p];
+    const uint32_t base = rowbase[r];
+    const uint32_t st = segstart[r];
+    if (base != 0xFFFFFFFFu) {
+        const uint32_t o = vals[p];
+        const uint32_t k = o >> 1;
+        const uint32_t nbr = (o & 1u) ? au[k] : av[k];
+        adj[base + (p - st)] = make_uint2(nbr, __float_as_uint(aw[k]));
+    }
+    if (p == st) segcnt[r] = 0;  // ready for the next batch
+}
+
+}  // namespace
+
+// Device state of a graph in device-edit mode (see internal.h).
+dyg_status device_edits_prepare(dyg_graph g, uint32_t max_batch, uint64_t max_inserts) {
+    dyg_ctx ctx = g->ctx;
+    DevEdits& d = g->dev;
+    // worst-case arena need: every row's lifetime allocations form a doubling
+    // series, bounded by 4x its final size, plus per-relocation slack
+    // (per relocated row: allocations <= 2*last <= 4*final + 2*(slack+4));
+    // the slots already in use stay allocated
+    const uint64_t need = g->used_slots + 4 * (g->nslots + 2 * max_inserts) +
+                          4 * max_inserts * (uint64_t)(g->slack + 4);
+    dyg_status st = graph_reserve_slots(g, std::min<uint64_t>(need, 0xFFFFFFFFull));
+    if (st != DYG_OK) return st;
+    const size_t n2 = 2 * (size_t)std::max<uint32_t>(max_batch, 1);
+    if (!d.ready || d.max_batch < max_batch) {
+        if (g->mirror_stale) {  // capacities live only in the buffers about to go
+            st = graph_refresh_mirror(g);
+            if (st != DYG_OK) return st;
+        }
+        d.release();
+        bool ok = hipMalloc(&d.cap, (size_t)g->V * 4) == hipSuccess &&
+                  hipMalloc(&d.segstart, (size_t)g->V * 4) == hipSuccess &&
+                  hipMalloc(&d.segcnt, (size_t)g->V * 4) == hipSuccess &&
+                  hipMalloc(&d.rowbase, (size_t)g->V * 4) == hipSuccess &&
+                  hipMalloc(&d.au, n2 / 2 * 4) == hipSuccess &&
+                  hipMalloc(&d.av, n2 / 2 * 4) == hipSuccess &&
+                  hipMalloc(&d.aw, n2 / 2 * 4) == hipSuccess &&
+                  hipMalloc(&d.keys_in, n2 * 4) == hipSuccess &&
+                  hipMalloc(&d.keys_out, n2 * 4) == hipSuccess &&
+                  hipMalloc(&d.vals_in, n2 * 4) == hipSuccess &&
+                  hipMalloc(&d.vals_out, n2 * 4) == hipSuccess &&
+                  hipMalloc(&d.meta, 64) == hipSuccess;
+        if (!ok) {
+            d.release();
+            set_error(ctx, "device edits: allocation failed");
+            return DYG_ERR_NOMEM;
+        }
+        size_t tb = 0;
+        DYG_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, tb, d.keys_in, d.keys_out,
+                                                        d.vals_in, d.vals_out, (int)n2, 0, 32,
+                                                        ctx->stream));
+        if (hipMalloc(&d.temp, tb) != hipSuccess) {
+            d.release();
+            set_error(ctx, "device edits: allocation failed");
+            return DYG_ERR_NOMEM;
+        }
+        d.temp_bytes = tb;
+        d.max_batch = max_batch;
+        d.ready = true;
+    }
+    return DYG_OK;
+}
+
+dyg_status device_edits_begin(dyg_graph g, uint32_t max_batch, uint64_t max_inserts) {
+    dyg_status st = device_edits_prepare(g, max_batch, max_inserts);
+    if (st != DYG_OK) return st;
+    dyg_ctx ctx = g->ctx;
+    DevEdits& d = g->dev;
+    hipStream_t s = ctx->stream;
+    // capacities: the host's are current unless an earlier device run changed them
+    if (!g->mirror_stale || !d.cap_valid)
+        DYG_HIP(ctx, hipMemcpyAsync(d.cap, g->cap.data(), (size_t)g->V * 4, hipMemcpyHostToDevice,
+                                    s));
+    d.cap_valid = true;
+    DYG_HIP(ctx, hipMemsetAsync(d.segcnt, 0, (size_t)g->V * 4, s));
+    // meta: [0] used slots (u64), [1] added edges (u64), [2] error flag | count (u32 x2)
+    unsigned long long meta[4] = {g->used_slots, 0, 0, 0};
+    DYG_HIP(ctx, ctx->stage.reserve(64));
+    DYG_HIP(ctx, hipStreamSynchronize(s));
+    std::memcpy(ctx->stage.p, meta, sizeof(meta));
+    DYG_HIP(ctx, hipMemcpyAsync(d.meta, ctx->stage.p, sizeof(meta), hipMemcpyHostToDevice, s));
+    g->mirror_stale = true;
+    return DYG_OK;
+}
+
+dyg_status device_edits_batch(dyg_graph g, const uint32_t* src, const uint32_t* dst,
+                              const float* w, const int32_t* found, uint32_t n) {
+    if (n == 0) return DYG_OK;
+    dyg_ctx ctx = g->ctx;
+    DevEdits& d = g->dev;
+    hipStream_t s = ctx->stream;
+    unsigned long long* used = d.meta;
+    unsigned long long* added = d.meta + 1;
+    uint32_t* err = (uint32_t*)(d.meta + 2);
+    uint32_t* count = err + 1;
+    hipLaunchKernelGGL(k_compact, dim3(1), dim3(kCompactThreads), 0, s, src, dst, w, found, n,
+                       d.au, d.av, d.aw, d.keys_in, d.vals_in, count, added);
+    DYG_HIP(ctx, hipGetLastError());
+    const int end_bit = std::min(32, 33 - __builtin_clz(std::max<uint32_t>(g->V, 1)));
+    size_t tb = d.temp_bytes;
+    DYG_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(d.temp, tb, d.keys_in, d.keys_out, d.vals_in,
+                                                    d.vals_out, (int)(2 * n), 0, end_bit, s));
+    const uint32_t blocks = (2 * n + 255) / 256;
+    hipLaunchKernelGGL(k_heads, dim3(blocks), dim3(256), 0, s, d.keys_out, count, d.segstart,
+                       d.segcnt);
+    hipLaunchKernelGGL(k_alloc, dim3(blocks), dim3(256), 0, s, g->d_rows, g->d_adj, d.cap,
+                       d.keys_out, count, d.segstart, d.segcnt, d.rowbase, used,
+                       (unsigned long long)g->cap_slots, g->slack, err);
+    hipLaunchKernelGGL(k_write, dim3(blocks), dim3(256), 0, s, g->d_adj, d.keys_out, d.vals_out,
+                       count, d.segstart, d.segcnt, d.rowbase, d.au, d.av, d.aw);
+    DYG_HIP(ctx, hipGetLastError());
+    return DYG_OK;
+}
+
+// Waits for the stream, then brings the host's view of the arena up to date.
+dyg_status device_edits_end(dyg_graph g, uint64_t* added_out) {
+    dyg_ctx ctx = g->ctx;
+    unsigned long long meta[4];
+    DYG_HIP(ctx, hipMemcpyAsync(meta, g->dev.meta, sizeof(meta), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    DYG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const uint32_t err = (uint32_t)meta[2];
+    g->used_slots = meta[0];
+    g->nslots += 2 * meta[1];
+    if (added_out) *added_out = meta[1];
+    if (err) {
+        set_error(ctx, "device edits: slot arena exhausted");
+        return DYG_ERR_NOMEM;
+    }
+    return DYG_OK;
+}
+
+void DevEdits::release() {
+    void* ps[] = {cap, segstart, segcnt, rowbase, au, av, aw, keys_in, keys_out,
+                  vals_in, vals_out, meta, temp};
+    for (void* p : ps)
+        if (p) (void)hipFree(p);
+    cap = segstart = segcnt = rowbase = au = av = keys_in = keys_out = vals_in = vals_out =
+        nullptr;
+    aw = nullptr;
+    meta = nullptr;
+    temp = nullptr;
+    temp_bytes = 0;
+    max_batch = 0;
+    ready = false;
+    cap_valid = false;
+}
+
+}  // namespace dyg

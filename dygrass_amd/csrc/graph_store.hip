@@ -85,7 +85,7 @@ dyg_status grow_arena(dyg_graph g, uint64_t need) {
     DYG_HIP(g->ctx, hipFree(g->d_adj));
     g->d_adj = nadj;
     g->cap_slots = ncap;
-    g->adj.resize(ncap, make_uint2(0, 0));
+    // the host mirror grows when the host next plans an edit
     return DYG_OK;
 }
 
@@ -107,9 +107,39 @@ dyg_status ship_delta(dyg_graph g, uint32_t n_copy, uint32_t n_write, uint32_t n
 
 }  // namespace
 
+dyg_status graph_reserve_slots(dyg_graph g, uint64_t need) { return grow_arena(g, need); }
+
+dyg_status graph_refresh_mirror(dyg_graph g) {
+    if (!g->mirror_stale) return DYG_OK;
+    dyg_ctx ctx = g->ctx;
+    std::vector<uint2> rows(g->V);
+    DYG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    DYG_HIP(ctx, hipMemcpy(rows.data(), g->d_rows, (size_t)g->V * sizeof(uint2),
+                           hipMemcpyDeviceToHost));
+    if (g->dev.cap)
+        DYG_HIP(ctx, hipMemcpy(g->cap.data(), g->dev.cap, (size_t)g->V * 4, hipMemcpyDeviceToHost));
+    if (g->adj.size() < g->used_slots) g->adj.resize(g->used_slots, make_uint2(0, 0));
+    DYG_HIP(ctx, hipMemcpy(g->adj.data(), g->d_adj, g->used_slots * sizeof(uint2),
+                           hipMemcpyDeviceToHost));
+    for (uint32_t v = 0; v < g->V; ++v) {
+        g->start[v] = rows[v].x;
+        g->deg[v] = rows[v].y;
+    }
+    g->mirror_stale = false;
+    return DYG_OK;
+}
+
 dyg_status graph_apply_inserts(dyg_graph g, const uint32_t* u, const uint32_t* v, const float* w,
                                uint32_t n) {
     if (n == 0) return DYG_OK;
+    if (g->keyed && g->mirror_stale) {
+        set_error(g->ctx, "insert: keyed graph edited on the device");
+        return DYG_ERR_STATE;
+    }
+    {
+        dyg_status rs = graph_refresh_mirror(g);
+        if (rs != DYG_OK) return rs;
+    }
     for (uint32_t i = 0; i < n; ++i)
         if (u[i] >= g->V || v[i] >= g->V) {
             set_error(g->ctx, "insert: vertex id out of range");
@@ -140,6 +170,7 @@ dyg_status graph_apply_inserts(dyg_graph g, const uint32_t* u, const uint32_t* v
     }
     dyg_status st = grow_arena(g, g->used_slots + extra);
     if (st != DYG_OK) return st;
+    if (g->adj.size() < g->used_slots + extra) g->adj.resize(g->cap_slots, make_uint2(0, 0));
 
     std::vector<uint32_t> copies, writes, rowops;
     std::vector<uint32_t> slot_of(2 * (size_t)n);
@@ -199,6 +230,10 @@ dyg_status graph_apply_deletes(dyg_graph g, const uint32_t* u, const uint32_t* v
     if (!g->keyed) {
         set_error(g->ctx, "delete: graph was uploaded without its edge map");
         return DYG_ERR_ARG;
+    }
+    {
+        dyg_status rs = graph_refresh_mirror(g);
+        if (rs != DYG_OK) return rs;
     }
     for (uint32_t i = 0; i < n; ++i)
         if (u[i] >= g->V || v[i] >= g->V) {

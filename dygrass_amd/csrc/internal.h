@@ -46,7 +46,36 @@ struct dyg_ctx_s {
     dyg::Pinned stage;         // H2D / D2H staging
     dyg::DevBuf dscratch;      // per-call device staging of inputs/outputs
     dyg::DevBuf dstats;        // device counters (hops, redirects, leaf stops)
+    std::vector<hipEvent_t> evpool;  // timing events for multi-launch runs
+    void reserve_events(size_t n) {
+        while (evpool.size() < n) {
+            hipEvent_t e = nullptr;
+            if (hipEventCreate(&e) != hipSuccess) break;
+            evpool.push_back(e);
+        }
+    }
+    ~dyg_ctx_s() {
+        for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
+    }
 };
+
+namespace dyg {
+// Device-side state of the insert pipeline (device_edits.hip).
+struct DevEdits {
+    bool ready = false;
+    bool cap_valid = false;  // `cap` holds the current capacities
+    uint32_t max_batch = 0;
+    uint32_t *cap = nullptr, *segstart = nullptr, *segcnt = nullptr, *rowbase = nullptr;
+    uint32_t *au = nullptr, *av = nullptr;
+    float* aw = nullptr;
+    uint32_t *keys_in = nullptr, *keys_out = nullptr, *vals_in = nullptr, *vals_out = nullptr;
+    unsigned long long* meta = nullptr;  // used slots, added edges, error|count
+    void* temp = nullptr;
+    size_t temp_bytes = 0;
+    void release();
+    ~DevEdits() { release(); }
+};
+}  // namespace dyg
 
 // One slot of a row: neighbour id and weight bits, 8 bytes, read in one load.
 // One row descriptor: first slot and degree, 8 bytes, read in one load.
@@ -69,6 +98,10 @@ struct dyg_graph_s {
     // delta staging
     std::vector<uint32_t> delta;
     dyg::DevBuf d_delta;
+    // device-edit mode: rows/cap/slots were changed on the device and the
+    // host mirror must be re-read before the host plans an edit
+    dyg::DevEdits dev;
+    bool mirror_stale = false;
 };
 
 namespace dyg {
@@ -82,6 +115,23 @@ dyg_status graph_apply_inserts(dyg_graph g, const uint32_t* u, const uint32_t* v
                                uint32_t n);
 dyg_status graph_apply_deletes(dyg_graph g, const uint32_t* u, const uint32_t* v, uint32_t n,
                                int8_t* hit);
+// Grow the slot arena to at least `need` slots.
+dyg_status graph_reserve_slots(dyg_graph g, uint64_t need);
+// Re-read rows, capacities and slots from the device after device edits.
+dyg_status graph_refresh_mirror(dyg_graph g);
+
+// Device-side insert pipeline (device_edits.hip).  begin() reserves arena
+// space for up to `max_inserts` appends in batches of at most `max_batch`
+// edges; batch() enqueues one batch's edits (device pointers; found[i] == -1
+// marks an edge to add); end() waits and reports the edges added.
+dyg_status device_edits_begin(dyg_graph g, uint32_t max_batch, uint64_t max_inserts);
+// The allocation part of begin(): arena growth and pipeline buffers (no-op
+// when already large enough).  Lets a caller move all allocation out of a
+// timed or latency-critical run.
+dyg_status device_edits_prepare(dyg_graph g, uint32_t max_batch, uint64_t max_inserts);
+dyg_status device_edits_batch(dyg_graph g, const uint32_t* src, const uint32_t* dst,
+                              const float* w, const int32_t* found, uint32_t n);
+dyg_status device_edits_end(dyg_graph g, uint64_t* added);
 
 // Walk launchers (walk_kernels.hip).  All pointers are device pointers.
 struct WalkLaunch {
@@ -97,6 +147,7 @@ struct WalkLaunch {
     uint32_t max_steps;
     float thr;
     uint32_t flags;
+    uint32_t edges_per_block;  // work-queue kernel: edges per workgroup (0 = automatic)
     uint64_t seed;
     int32_t* found;
     float* cond;
@@ -114,6 +165,8 @@ hipError_t launch_apply_delta(uint2* rows, uint2* adj, const uint32_t* delta, ui
                               uint32_t n_write, uint32_t n_row, hipStream_t s);
 
 bool valid_walkers(uint32_t w);
+// Validates walkers (power of two, 1..1024) and max_steps (1..32767).
+dyg_status check_params(dyg_ctx ctx, const dyg_walk_params* p);
 
 }  // namespace dyg
 

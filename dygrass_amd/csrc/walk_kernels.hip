@@ -248,6 +248,195 @@ __global__ __launch_bounds__((1 << LOGW) < 64 ? 64 : (1 << LOGW)) void k_walk(Wa
     }
 }
 
+// ---------------------------------------------------------------------------
+// Work-queue form.  A workgroup of 256 lanes owns `E` consecutive edges and
+// their E*W (edge, walker) tasks, taken from an LDS counter; a lane whose
+// walker ends takes the next task at once, so a wave is never held by its
+// longest walker.  Tasks are edge-major, so the lanes that start an edge share
+// its first row.  Each lane advances one hop per loop turn; refills are
+// wave-aggregated (one LDS atomic per wave per turn).  Per-edge winner keys
+// and hop counts live in LDS (64-bit LDS max); pruning reads the edge's key.
+// Results equal the one-workgroup-per-edge kernel's bit for bit: a walker's
+// trajectory depends only on (edge, walker) and the graph.
+// ---------------------------------------------------------------------------
+constexpr int kPoolThreads = 256;
+constexpr int kPoolMaxEdges = 64;
+
+struct Lane {
+    uint32_t edge;  // local edge index
+    uint32_t walker;
+    uint32_t cur, prev, t;
+    float R, we;
+    int step;
+};
+
+template <int LOGW, bool INSERT, bool PRUNE>
+__global__ __launch_bounds__(kPoolThreads) void k_walk_pool(WalkLaunch a, uint32_t E) {
+    constexpr uint32_t W = 1u << LOGW;
+    __shared__ unsigned long long s_key[kPoolMaxEdges];
+    __shared__ uint32_t s_hops[kPoolMaxEdges];
+    __shared__ uint32_t s_src[kPoolMaxEdges], s_dst[kPoolMaxEdges];
+    __shared__ float s_w[kPoolMaxEdges];
+    __shared__ unsigned long long s_eid[kPoolMaxEdges];
+    __shared__ uint32_t s_next;
+    __shared__ uint32_t s_cnt[3];
+
+    const uint32_t e0 = blockIdx.x * E;
+    const uint32_t ne = min(E, a.n - e0);
+    const uint32_t tid = threadIdx.x;
+    if (tid < ne) {
+        const uint32_t e = e0 + tid;
+        s_src[tid] = a.src[e];
+        s_dst[tid] = a.dst[e];
+        s_w[tid] = INSERT ? a.w[e] : 0.0f;
+        s_eid[tid] = a.eid ? a.eid[e] : a.eid_base + e;
+        // walker 0 not found: the key every found walker beats
+        s_key[tid] = (0xFFFFull << 16) | 0xFFFFull;
+        s_hops[tid] = 0;
+    }
+    if (tid == 0) {
+        s_next = 0;
+        s_cnt[0] = s_cnt[1] = s_cnt[2] = 0;
+    }
+    __syncthreads();
+
+    const uint32_t total = ne * W;
+    const uint32_t lane = __lane_id();
+    bool active = false;
+    bool drained = false;
+    Lane L{};
+    Draws rng(0, 0, 0, 0);
+    uint32_t hops = 0, redir = 0, leaf = 0;  // this lane's running counts
+    uint32_t my_hops = 0;                    // current walker's draws
+
+    for (;;) {
+        // ---- refill idle lanes (one LDS atomic per wave) ----
+        const unsigned long long need = __ballot(!active && !drained);
+        if (need) {
+            const uint32_t cnt = __popcll(need);
+            const int leader = __ffsll((long long)need) - 1;
+            uint32_t base = 0;
+            if ((int)lane == leader) base = atomicAdd(&s_next, cnt);
+            base = __shfl(base, leader, 64);
+            if (!active && !drained) {
+                const uint32_t rank = __popcll(need & ((1ull << lane) - 1ull));
+                const uint32_t task = base + rank;
+                if (task < total) {
+                    L.edge = task >> LOGW;
+                    L.walker = task & (W - 1u);
+                    L.cur = s_src[L.edge];
+                    L.t = s_dst[L.edge];
+                    L.we = s_w[L.edge];
+                    L.prev = 0;
+                    L.R = 0.0f;
+                    L.step = 1;
+                    rng = Draws(a.seed, INSERT ? kTagInsert : kTagDelete, s_eid[L.edge], L.walker);
+                    my_hops = 0;
+                    active = true;
+                } else {
+                    drained = true;
+                }
+            }
+        }
+        if (!__any(active)) break;
+        if (!active) continue;
+
+        // ---- one hop ----
+        bool done = false;
+        int found = -1;
+        const uint2 row = ld(a.rows, L.cur);
+        if (L.step == 1) {
+            if (row.y == 0) {
+                done = true;  // isolated source
+            } else {
+                const uint32_t i = rng.next() % row.y;
+                ++my_hops;
+                const uint2 sl = ld(a.adj, row.x + i);
+                L.R = 1.0f / __uint_as_float(sl.y);
+                if (sl.x == L.t) {
+                    found = 1;
+                    done = true;
+                } else {
+                    L.prev = L.cur;
+                    L.cur = sl.x;
+                    L.step = 2;
+                    if (L.step >= (int)a.max_steps) done = true;
+                }
+            }
+        } else if (row.y <= 1) {
+            ++leaf;
+            done = true;
+        } else {
+            const uint32_t i = rng.next() % (row.y - 1);
+            ++my_hops;
+            const uint2 x = ld(a.adj, row.x + i);
+            const uint2 y = ld(a.adj, row.x + row.y - 1);
+            const bool back = (x.x == L.prev);
+            redir += back ? 1u : 0u;
+            const uint2 sl = back ? y : x;
+            L.R += 1.0f / __uint_as_float(sl.y);
+            if (INSERT && L.R * L.we >= a.thr) {
+                done = true;
+            } else if (sl.x == L.t) {
+                found = L.step;
+                done = true;
+            } else {
+                if (PRUNE) {
+                    const uint32_t bb = (uint32_t)(s_key[L.edge] >> 32);
+                    if (bb != 0u && __float_as_uint(1.0f / L.R) + 1u < bb) done = true;
+                }
+                L.prev = L.cur;
+                L.cur = sl.x;
+                ++L.step;
+                if (L.step >= (int)a.max_steps) done = true;
+            }
+        }
+        if (done) {
+            if (found != -1) {
+                const unsigned long long key =
+                    ((unsigned long long)(__float_as_uint(1.0f / L.R) + 1u) << 32) |
+                    ((unsigned long long)tie_field<LOGW>(L.walker) << 16) |
+                    (unsigned long long)((uint32_t)found & 0xFFFFu);
+                atomicMax(&s_key[L.edge], key);
+            }
+            atomicAdd(&s_hops[L.edge], my_hops);
+            hops += my_hops;
+            active = false;
+        }
+    }
+
+    // block totals
+    const uint32_t h = wave_sum_u32(hops), r = wave_sum_u32(redir), l = wave_sum_u32(leaf);
+    if (lane == 0) {
+        atomicAdd(&s_cnt[0], h);
+        atomicAdd(&s_cnt[1], r);
+        atomicAdd(&s_cnt[2], l);
+    }
+    __syncthreads();
+    if (tid < ne) {
+        const uint32_t e = e0 + tid;
+        const unsigned long long best = s_key[tid];
+        const int f = (int)(int16_t)(uint16_t)(best & 0xFFFFu);
+        const uint32_t win = tie_to_walker<LOGW>((uint32_t)(best >> 16) & 0xFFFFu);
+        a.found[e] = f;
+        a.cond[e] = (f != -1) ? __uint_as_float((uint32_t)(best >> 32) - 1u) : -1.0f;
+        if (a.winner) a.winner[e] = win;
+        if (a.hops) a.hops[e] = s_hops[tid];
+        if (!INSERT && f > 0) {
+            Draws rr(a.seed, kTagDelete, s_eid[tid], win);
+            WalkOut ro;
+            walk_lane<false, true, false>(a.rows, a.adj, s_src[tid], s_dst[tid], 0.0f, 0.0f,
+                                          (int)a.max_steps, rr, nullptr,
+                                          a.path + (uint64_t)e * a.max_steps, ro);
+        }
+    }
+    if (tid == 0 && a.stats) {
+        atomicAdd(&a.stats[0], (unsigned long long)s_cnt[0]);
+        atomicAdd(&a.stats[1], (unsigned long long)s_cnt[1]);
+        atomicAdd(&a.stats[2], (unsigned long long)s_cnt[2]);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_heuristic(const uint2* __restrict__ rows,
                                                    const uint2* __restrict__ adj,
                                                    const uint32_t* __restrict__ endpoint,
@@ -322,18 +511,52 @@ hipError_t launch_walk_t(const WalkLaunch& a, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Edges per workgroup of the work-queue kernel: about 16 walkers per lane.
+inline uint32_t pool_edges(uint32_t W, uint32_t requested) {
+    uint32_t E = requested ? requested : (uint32_t)(kPoolThreads * 16) / W;
+    if (E < 1) E = 1;
+    if (E > (uint32_t)kPoolMaxEdges) E = kPoolMaxEdges;
+    return E;
+}
+
+template <bool INSERT, bool PRUNE>
+hipError_t launch_pool_t(const WalkLaunch& a, hipStream_t s) {
+    if (a.n == 0) return hipSuccess;
+    const uint32_t E = pool_edges(a.walkers, a.edges_per_block);
+    const dim3 grid((a.n + E - 1) / E);
+    switch (a.walkers) {
+#define DYG_CASE(LW)                                                                        \
+    case (1u << LW):                                                                        \
+        hipLaunchKernelGGL((k_walk_pool<LW, INSERT, PRUNE>), grid, dim3(kPoolThreads), 0, s, \
+                           a, E);                                                           \
+        break;
+        DYG_CASE(0) DYG_CASE(1) DYG_CASE(2) DYG_CASE(3) DYG_CASE(4) DYG_CASE(5)
+        DYG_CASE(6) DYG_CASE(7) DYG_CASE(8) DYG_CASE(9) DYG_CASE(10)
+#undef DYG_CASE
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <bool INSERT>
+hipError_t launch_any(const WalkLaunch& a, hipStream_t s) {
+    const bool prune = !(a.flags & DYG_WALK_NO_PRUNE);
+    if (a.flags & DYG_WALK_PER_EDGE)
+        return prune ? launch_walk_t<INSERT, true>(a, s) : launch_walk_t<INSERT, false>(a, s);
+    return prune ? launch_pool_t<INSERT, true>(a, s) : launch_pool_t<INSERT, false>(a, s);
+}
+
 }  // namespace
 
 bool valid_walkers(uint32_t w) { return w >= 1 && w <= 1024 && (w & (w - 1)) == 0; }
 
 hipError_t launch_walk_incremental(const WalkLaunch& a, hipStream_t s) {
-    return (a.flags & DYG_WALK_NO_PRUNE) ? launch_walk_t<true, false>(a, s)
-                                         : launch_walk_t<true, true>(a, s);
+    return launch_any<true>(a, s);
 }
 
 hipError_t launch_walk_decremental(const WalkLaunch& a, hipStream_t s) {
-    return (a.flags & DYG_WALK_NO_PRUNE) ? launch_walk_t<false, false>(a, s)
-                                         : launch_walk_t<false, true>(a, s);
+    return launch_any<false>(a, s);
 }
 
 hipError_t launch_heuristic(const uint2* rows, const uint2* adj, const uint32_t* endpoint,

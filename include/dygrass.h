@@ -97,6 +97,17 @@ void dyg_graph_free(dyg_graph g);
 /* Walks                                                              */
 /* ------------------------------------------------------------------ */
 #define DYG_WALK_NO_PRUNE 1u /* run every walker to its own end (no early stop) */
+/* Launch shape.  Default: work-queue kernel (each workgroup owns a run of
+ * edges and hands their (edge, walker) tasks to lanes as lanes free up).
+ * DYG_WALK_PER_EDGE: one workgroup per edge, one lane per walker (the
+ * reference's <<<edges, walkers>>> shape).  Results are identical. */
+#define DYG_WALK_PER_EDGE 2u
+/* dyg_incremental_update: plan each batch's appends on the host and ship
+ * them as a delta (the reference's host-ordered update between batches).
+ * Default: the appends are compacted, ordered and applied on the device, so
+ * batches follow each other on the stream with no host round trip.  Same
+ * resulting graph either way (slot order per row). */
+#define DYG_RUN_HOST_EDITS 4u
 
 typedef struct {
     uint32_t walkers;   /* walkers per edge, power of two in [64,1024] (argv[3]; 512 in decremental) */
@@ -174,6 +185,18 @@ dyg_status dyg_incremental_update(dyg_graph H, const uint32_t* src, const uint32
                                   const float* w, uint32_t m, uint32_t divisor,
                                   const dyg_walk_params* p, int32_t* found_at,
                                   dyg_run_stats* st);
+/* Same loop with inputs already in device memory: src/dst/w and the outputs
+ * found_at/cond[m] are device pointers.  Returns when the run has finished. */
+dyg_status dyg_incremental_update_dev(dyg_graph H, const uint32_t* src, const uint32_t* dst,
+                                      const float* w, uint32_t m, uint32_t divisor,
+                                      const dyg_walk_params* p, int32_t* found_at, float* cond,
+                                      dyg_run_stats* st);
+/* Pre-allocate for device-side edits: slot-arena room for `max_inserts`
+ * appended edges and the ordering buffers for batches of up to `max_batch`
+ * edges, so that a following dyg_incremental_update* allocates nothing.
+ * No counterpart in the reference (its arena is fixed at 2x, unchecked:
+ * random_walk_incremental/gpu_graph.cuh:210,357-384). */
+dyg_status dyg_graph_reserve(dyg_graph g, uint64_t max_inserts, uint32_t max_batch);
 
 /* ------------------------------------------------------------------ */
 /* Dual graph for deletions (GPU_Dual_Graph)                          */

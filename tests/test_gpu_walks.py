@@ -27,15 +27,16 @@ def _first_batch(wl, n=None):
     return wl.ins.u[:n], wl.ins.v[:n], wl.ins.w[:n]
 
 
+@pytest.mark.parametrize("per_edge", [False, True])
 @pytest.mark.parametrize("prune", [True, False])
-def test_incremental_walks_match_oracle_grid(ctx, grid_wl, prune):
+def test_incremental_walks_match_oracle_grid(ctx, grid_wl, prune, per_edge):
     from dygrass_amd import DeviceGraph, params, walk_incremental
 
     po = _oracle()
     rp, col, w = csr(grid_wl.H)
     H = DeviceGraph(ctx, grid_wl.H.V, rp, col, w)
     s, t, we = _first_batch(grid_wl)
-    p = params(512, 100, 10.0, seed=0, prune=prune)
+    p = params(512, 100, 10.0, seed=0, prune=prune, per_edge=per_edge)
     got = walk_incremental(H, s, t, we, p)
     og = po.Graph(grid_wl.H.V, rp, col, w)
     f, c, win, hops = og.walk_inc_batch(s, t, we, 10.0, 512, seed=0, threads=8)
@@ -49,8 +50,9 @@ def test_incremental_walks_match_oracle_grid(ctx, grid_wl, prune):
         assert got.stats["hops"] <= int(hops.sum())
 
 
+@pytest.mark.parametrize("per_edge", [False, True])
 @pytest.mark.parametrize("W", [1, 2, 8, 32, 64, 128, 256, 1024])
-def test_incremental_walker_counts(ctx, grid_wl, W):
+def test_incremental_walker_counts(ctx, grid_wl, W, per_edge):
     from dygrass_amd import DeviceGraph, params, walk_incremental
 
     po = _oracle()
@@ -58,7 +60,8 @@ def test_incremental_walker_counts(ctx, grid_wl, W):
     H = DeviceGraph(ctx, grid_wl.H.V, rp, col, w)
     s, t, we = _first_batch(grid_wl, 64)
     ids = np.arange(1000, 1064, dtype=np.uint64)
-    got = walk_incremental(H, s, t, we, params(W, 100, 10.0, seed=7, prune=False), edge_id=ids)
+    got = walk_incremental(H, s, t, we, params(W, 100, 10.0, seed=7, prune=False,
+                                               per_edge=per_edge), edge_id=ids)
     og = po.Graph(grid_wl.H.V, rp, col, w)
     f, c, win, hops = og.walk_inc_batch(s, t, we, 10.0, W, seed=7, edge_id=ids, threads=8)
     assert np.array_equal(got.found_at, f)
@@ -67,7 +70,8 @@ def test_incremental_walker_counts(ctx, grid_wl, W):
     assert np.array_equal(got.hops.astype(np.uint64), hops)
 
 
-def test_incremental_update_matches_oracle_run(ctx, grid_wl):
+@pytest.mark.parametrize("per_edge,host_edits", [(False, False), (True, True), (False, True)])
+def test_incremental_update_matches_oracle_run(ctx, grid_wl, per_edge, host_edits):
     """Whole snapshot-batched run: every decision and the final slot order of H."""
     from dygrass_amd import DeviceGraph, incremental_update, params
 
@@ -75,7 +79,8 @@ def test_incremental_update_matches_oracle_run(ctx, grid_wl):
     rp, col, w = csr(grid_wl.H)
     H = DeviceGraph(ctx, grid_wl.H.V, rp, col, w, slack=1)
     found, stats = incremental_update(H, grid_wl.ins.u, grid_wl.ins.v, grid_wl.ins.w,
-                                      params(512, 100, 10.0, seed=0))
+                                      params(512, 100, 10.0, seed=0, per_edge=per_edge,
+                                             host_edits=host_edits))
     og = po.Graph(grid_wl.H.V, rp, col, w)
     starts = po.batches(len(grid_wl.ins), 10)
     expect = np.zeros(len(grid_wl.ins), np.int32)
@@ -94,6 +99,30 @@ def test_incremental_update_matches_oracle_run(ctx, grid_wl):
     assert np.array_equal(grp, orp)
     assert np.array_equal(gcol, ocol)
     assert np.array_equal(f32bits(gw), f32bits(ow))
+
+
+def test_device_edits_chain_with_host_edits(ctx, grid_wl):
+    """Two device-edited runs back to back, then host-planned inserts on the
+    same graph (the host re-reads the device state first)."""
+    from dygrass_amd import DeviceGraph, incremental_update, params
+
+    po = _oracle()
+    rp, col, w = csr(grid_wl.H)
+    H = DeviceGraph(ctx, grid_wl.H.V, rp, col, w, slack=0)
+    og = po.Graph(grid_wl.H.V, rp, col, w)
+    u, v, x = grid_wl.ins.u, grid_wl.ins.v, grid_wl.ins.w
+    halves = [(0, 1000), (1000, 2000)]
+    for b0, b1 in halves:
+        found, _ = incremental_update(H, u[b0:b1], v[b0:b1], x[b0:b1],
+                                      params(128, 100, 10.0, seed=b0))
+        ofound, _ = og.incremental(u[b0:b1], v[b0:b1], x[b0:b1], 10.0, 128, seed=b0, threads=8)
+        assert np.array_equal(found, ofound)
+    H.insert(u[:50], v[:50], x[:50])
+    for i in range(50):
+        og.insert(int(u[i]), int(v[i]), float(x[i]))
+    g, o = H.download(), og.export()
+    assert np.array_equal(g[0], o[0]) and np.array_equal(g[1], o[1])
+    assert np.array_equal(f32bits(g[2]), f32bits(o[2]))
 
 
 def test_insert_relocation_keeps_slot_order(ctx):
@@ -146,8 +175,9 @@ def test_delete_swap_with_last_matches_oracle(ctx, mesh_wl):
         assert np.array_equal(f32bits(g[2]), f32bits(o[2]))
 
 
+@pytest.mark.parametrize("per_edge", [False, True])
 @pytest.mark.parametrize("prune", [True, False])
-def test_decremental_walks_match_oracle(ctx, mesh_wl, prune):
+def test_decremental_walks_match_oracle(ctx, mesh_wl, prune, per_edge):
     from dygrass_amd import DeviceGraph, params, walk_decremental
 
     po = _oracle()
@@ -158,7 +188,8 @@ def test_decremental_walks_match_oracle(ctx, mesh_wl, prune):
     G.delete(du, dv)
     a, b = np.minimum(du, dv), np.maximum(du, dv)
     ids = np.arange(n, dtype=np.uint64) * 3 + 11
-    got = walk_decremental(G, a, b, params(512, 100, seed=5, prune=prune), edge_id=ids)
+    got = walk_decremental(G, a, b, params(512, 100, seed=5, prune=prune, per_edge=per_edge),
+                           edge_id=ids)
     grp, gcol, gw = G.download()
     og = po.Graph(mesh_wl.G.V, grp, gcol, gw)
     f, c, win, hops, path = og.walk_dec_batch(a, b, 512, seed=5, edge_id=ids, threads=8)
@@ -196,7 +227,45 @@ def test_walks_are_deterministic_and_prune_invariant(ctx, grid_wl):
     a = walk_incremental(H, s, t, we, params(256, 100, 10.0, seed=3, prune=True))
     b = walk_incremental(H, s, t, we, params(256, 100, 10.0, seed=3, prune=True))
     c = walk_incremental(H, s, t, we, params(256, 100, 10.0, seed=3, prune=False))
-    for x in (b, c):
+    d = walk_incremental(H, s, t, we, params(256, 100, 10.0, seed=3, per_edge=True))
+    for x in (b, c, d):
         assert np.array_equal(a.found_at, x.found_at)
         assert np.array_equal(f32bits(a.cond), f32bits(x.cond))
         assert np.array_equal(a.winner, x.winner)
+
+
+def test_incremental_update_dev_matches_host_entry(ctx, grid_wl):
+    """Device-pointer entry point (inputs resident in HBM) equals the host one."""
+    import torch
+
+    from dygrass_amd import DeviceGraph, incremental_update, incremental_update_dev, params
+
+    rp, col, w = csr(grid_wl.H)
+    p = params(512, 100, 10.0, seed=4)
+    H1 = DeviceGraph(ctx, grid_wl.H.V, rp, col, w)
+    f1, s1 = incremental_update(H1, grid_wl.ins.u, grid_wl.ins.v, grid_wl.ins.w, p)
+    H2 = DeviceGraph(ctx, grid_wl.H.V, rp, col, w)
+    m = len(grid_wl.ins)
+    H2.reserve(m, (m + 9) // 10)
+    dev = torch.device("cuda:0")
+    ts = torch.from_numpy(grid_wl.ins.u.astype(np.int32)).to(dev)
+    td = torch.from_numpy(grid_wl.ins.v.astype(np.int32)).to(dev)
+    tw = torch.from_numpy(grid_wl.ins.w).to(dev)
+    tf = torch.empty(m, dtype=torch.int32, device=dev)
+    tc = torch.empty(m, dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+    s2 = incremental_update_dev(H2, ts.data_ptr(), td.data_ptr(), tw.data_ptr(), m,
+                                tf.data_ptr(), tc.data_ptr(), p)
+    assert np.array_equal(tf.cpu().numpy(), f1)
+    assert s2["added"] == s1["added"] and s2["hops"] == s1["hops"]
+    g1, g2 = H1.download(), H2.download()
+    assert all(np.array_equal(a, b) for a, b in zip(g1, g2))
+    # an out-of-range id is rejected before any walk runs
+    bad = ts.clone()
+    bad[5] = grid_wl.H.V + 3
+    torch.cuda.synchronize()
+    from dygrass_amd import DygError
+
+    with pytest.raises(DygError):
+        incremental_update_dev(H2, bad.data_ptr(), td.data_ptr(), tw.data_ptr(), m,
+                               tf.data_ptr(), tc.data_ptr(), p)
