@@ -35,12 +35,14 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(stats: dict, updates: int) -> float:
-    """Bytes the walks must read: per neighbour draw a row record (8 B:
-    first slot + degree) and a slot (8 B: neighbour + weight); +8 B for each
-    non-backtracking redirect to the row's last slot; +8 B for each row read
-    that ends a walk at a leaf.  Per edge 12 B of input, 8 B of output."""
-    return (16.0 * stats["hops"] + 8.0 * stats["redirects"] + 8.0 * stats["leaf_stops"]
+def algorithmic_bytes(stats: dict, updates: int, walkers: int) -> float:
+    """Bytes the walks must read.  Per walker the source's row record (8 B:
+    first slot + degree); per neighbour draw one 16 B slot (neighbour 4 +
+    weight 4 + the neighbour's first slot 4 + degree 4, i.e. the row-start,
+    degree, column and weight words of a hop); +16 B for each
+    non-backtracking redirect to the row's last slot.  A leaf is recognised
+    from the degree already read.  Per edge 12 B of input and 8 B of output."""
+    return (8.0 * updates * walkers + 16.0 * stats["hops"] + 16.0 * stats["redirects"]
             + 20.0 * updates)
 
 
@@ -53,7 +55,12 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--mode", choices=["device", "plain"], default="device")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--block-edges", type=int, default=0,
+                    help="edges per workgroup of the work-queue walk kernel (0 = automatic)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="also run one device-to-device copy of this size (a known-size "
+                         "dispatch for scaling hardware byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -62,8 +69,8 @@ def main() -> int:
               "this build does not implement", file=sys.stderr)
         return 2
 
-    from dygrass_amd import (Context, DeviceGraph, csr_from_edges, incremental_update,
-                             incremental_update_dev, params)
+    from dygrass_amd import (Context, DeviceArray, DeviceGraph, csr_from_edges,
+                             incremental_update, incremental_update_dev, params)
     from dygrass_amd import synth
 
     t0 = time.time()
@@ -77,19 +84,15 @@ def main() -> int:
     def run_mode(mode: str, steps: int, warmup: int) -> dict:
         plain = mode == "plain"
         p = params(wl.walkers, 100, wl.threshold, seed=args.seed, per_edge=plain,
-                   host_edits=plain)
+                   host_edits=plain, block_edges=args.block_edges)
         I = (m + 9) // 10
         if not plain:
-            # inputs resident in HBM before the timed window (torch is only the allocator)
-            import torch
-
-            dev = torch.device("cuda:0")
-            t_src = torch.from_numpy(wl.ins.u.astype(np.int32)).to(dev)
-            t_dst = torch.from_numpy(wl.ins.v.astype(np.int32)).to(dev)
-            t_w = torch.from_numpy(wl.ins.w).to(dev)
-            t_found = torch.empty(m, dtype=torch.int32, device=dev)
-            t_cond = torch.empty(m, dtype=torch.float32, device=dev)
-            torch.cuda.synchronize()
+            # inputs resident in HBM before the timed window
+            t_src = DeviceArray.from_numpy(ctx, wl.ins.u)
+            t_dst = DeviceArray.from_numpy(ctx, wl.ins.v)
+            t_w = DeviceArray.from_numpy(ctx, wl.ins.w)
+            t_found = DeviceArray(ctx, m, np.int32)
+            t_cond = DeviceArray(ctx, m, np.float32)
 
         def one_step():
             H = DeviceGraph(ctx, wl.H.V, rp, col, w, slack=2)  # restore (untimed)
@@ -100,12 +103,12 @@ def main() -> int:
             if plain:
                 found, st = incremental_update(H, wl.ins.u, wl.ins.v, wl.ins.w, p)
             else:
-                st = incremental_update_dev(H, t_src.data_ptr(), t_dst.data_ptr(), t_w.data_ptr(),
-                                            m, t_found.data_ptr(), t_cond.data_ptr(), p)
+                st = incremental_update_dev(H, t_src.ptr, t_dst.ptr, t_w.ptr, m, t_found.ptr,
+                                            t_cond.ptr, p)
             ctx.sync()
             te = time.perf_counter()
             if not plain:
-                found = t_found.cpu().numpy()
+                found = t_found.numpy()
             H.close()
             return te - ts, found, st
 
@@ -125,12 +128,24 @@ def main() -> int:
                     value=m * steps / total, ms_per_step=total / steps * 1e3,
                     walk_ms=float(np.mean(walk_ms)), edit_ms=float(np.mean(upd_ms)))
 
-    plain = run_mode("plain", max(1, min(args.steps, 3)), 1) if args.mode == "device" else None
+    if args.copy_calibration_mb:
+        nb = args.copy_calibration_mb << 20
+        a_src = DeviceArray(ctx, nb // 4, np.uint32)
+        a_dst = DeviceArray(ctx, nb // 4, np.uint32)
+        import ctypes
+
+        ctx._lib.dyg_copy_on_device(ctx.handle, ctypes.c_void_p(a_dst.ptr),
+                                    ctypes.c_void_p(a_src.ptr), nb)
+        log(f"calibration copy: {args.copy_calibration_mb} MiB device to device")
+        a_src.close()
+        a_dst.close()
+    plain = (run_mode("plain", max(1, min(args.steps, 3)), 1)
+             if args.mode == "device" and args.steps > 1 else None)
     head = run_mode(args.mode, args.steps, args.warmup)
     stats = head["stats"]
     found = head["found"]
     launches = int(stats["batches"])
-    bytes_run = algorithmic_bytes(stats, m)
+    bytes_run = algorithmic_bytes(stats, m, wl.walkers)
     walk_s = head["walk_ms"] / 1e3
     achieved = bytes_run / walk_s / 1e9 if walk_s > 0 else 0.0
     kernel = (f"k_walk_pool<W={wl.walkers}, insert, prune>" if args.mode == "device"

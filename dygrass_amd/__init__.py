@@ -7,6 +7,7 @@ no CPU implementation of any walk.
 from ._lib import DygError, WalkParams, WalkStats, RunStats, LIB_PATH, BIN_DIR  # noqa: F401
 from .core import (  # noqa: F401
     Context,
+    DeviceArray,
     DeviceGraph,
     DualGraph,
     WalkResult,
@@ -20,7 +21,7 @@ from .core import (  # noqa: F401
 )
 
 __all__ = [
-    "Context", "DeviceGraph", "DualGraph", "WalkResult", "csr_from_edges", "incremental_update", "incremental_update_dev",
+    "Context", "DeviceArray", "DeviceGraph", "DualGraph", "WalkResult", "csr_from_edges", "incremental_update", "incremental_update_dev",
     "params", "walk_decremental", "walk_heuristic", "walk_incremental", "DygError", "WalkParams",
     "WalkStats", "RunStats", "LIB_PATH", "BIN_DIR",
 ]

@@ -34,7 +34,8 @@ STATUS = {
 
 class WalkParams(C.Structure):
     _fields_ = [("walkers", C.c_uint32), ("max_steps", C.c_uint32), ("threshold", C.c_float),
-                ("flags", C.c_uint32), ("seed", C.c_uint64)]
+                ("flags", C.c_uint32), ("seed", C.c_uint64), ("block_edges", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
 
 class WalkStats(C.Structure):
@@ -62,6 +63,12 @@ _SIGS = {
     "dyg_last_error": (C.c_char_p, [vp]),
     "dyg_ctx_stream": (vp, [vp]),
     "dyg_ctx_sync": (C.c_int32, [vp]),
+    "dyg_dev_alloc": (C.c_int32, [vp, C.c_size_t, C.POINTER(vp)]),
+    "dyg_dev_free": (C.c_int32, [vp, vp]),
+    "dyg_copy_to_device": (C.c_int32, [vp, vp, vp, C.c_size_t]),
+    "dyg_copy_to_host": (C.c_int32, [vp, vp, vp, C.c_size_t]),
+    "dyg_copy_on_device": (C.c_int32, [vp, vp, vp, C.c_size_t]),
+    "dyg_graph_check": (C.c_int32, [vp, u64p]),
     "dyg_graph_upload": (C.c_int32, [vp, C.c_uint32, u64p, u32p, f32p, C.c_uint32, C.c_int,
                                      C.POINTER(vp)]),
     "dyg_graph_info": (C.c_int32, [vp, u32p, u64p, u64p]),

@@ -41,13 +41,15 @@ class WalkResult:
 
 
 def params(walkers: int, max_steps: int = 100, threshold: float = 0.0, seed: int = 0,
-           prune: bool = True, per_edge: bool = False, host_edits: bool = False) -> L.WalkParams:
+           prune: bool = True, per_edge: bool = False, host_edits: bool = False,
+           block_edges: int = 0) -> L.WalkParams:
     """Walk parameters.  per_edge: one workgroup per edge (the reference's
     launch shape) instead of the work-queue kernel; host_edits: plan each
-    batch's appends on the host (dyg_incremental_update only)."""
+    batch's appends on the host (dyg_incremental_update only); block_edges:
+    edges per workgroup of the work-queue kernel (0 = automatic)."""
     flags = ((0 if prune else L.DYG_WALK_NO_PRUNE) | (L.DYG_WALK_PER_EDGE if per_edge else 0)
              | (L.DYG_RUN_HOST_EDITS if host_edits else 0))
-    return L.WalkParams(walkers, max_steps, threshold, flags, seed)
+    return L.WalkParams(walkers, max_steps, threshold, flags, seed, block_edges, 0)
 
 
 class Context:
@@ -69,6 +71,48 @@ class Context:
     def close(self) -> None:
         if self.handle:
             self._lib.dyg_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceArray:
+    """A device buffer holding one numpy array's worth of data (dyg_dev_alloc)."""
+
+    def __init__(self, ctx: Context, n: int, dtype):
+        self.ctx = ctx
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        h = C.c_void_p()
+        check(ctx._lib.dyg_dev_alloc(ctx.handle, self.n * self.dtype.itemsize, C.byref(h)),
+              ctx.handle)
+        self.handle = h
+
+    @classmethod
+    def from_numpy(cls, ctx: Context, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(ctx, a.size, a.dtype)
+        check(ctx._lib.dyg_copy_to_device(ctx.handle, d.handle, a.ctypes.data_as(C.c_void_p),
+                                          a.nbytes), ctx.handle)
+        return d
+
+    @property
+    def ptr(self) -> int:
+        return self.handle.value
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.n, self.dtype)
+        check(self.ctx._lib.dyg_copy_to_host(self.ctx.handle, out.ctypes.data_as(C.c_void_p),
+                                             self.handle, out.nbytes), self.ctx.handle)
+        return out
+
+    def close(self) -> None:
+        if self.handle:
+            self.ctx._lib.dyg_dev_free(self.ctx.handle, self.handle)
             self.handle = None
 
     def __del__(self):
@@ -130,6 +174,12 @@ class DeviceGraph:
         check(self._lib.dyg_graph_download(self.handle, ptr(rp, L.u64p), ptr(col, L.u32p),
                                            ptr(w, L.f32p)), self.ctx.handle)
         return rp, col[:n], w[:n]
+
+    def check(self) -> int:
+        """Live slots whose copy of the neighbour's row descriptor is stale (0 = consistent)."""
+        bad = C.c_uint64()
+        check(self._lib.dyg_graph_check(self.handle, C.byref(bad)), self.ctx.handle)
+        return bad.value
 
     def reserve(self, max_inserts: int, max_batch: int) -> None:
         """Pre-allocate arena and device-edit buffers (dyg_graph_reserve)."""

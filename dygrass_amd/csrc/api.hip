@@ -107,6 +107,7 @@ dyg_status walk_host(dyg_graph g, bool insert, const uint32_t* src, const uint32
     a.max_steps = p->max_steps;
     a.thr = p->threshold;
     a.flags = p->flags;
+    a.edges_per_block = p->block_edges;
     a.seed = p->seed;
     a.found = d_found;
     a.cond = d_cond;
@@ -191,6 +192,46 @@ extern "C" dyg_status dyg_ctx_sync(dyg_ctx ctx) {
     return DYG_OK;
 }
 
+extern "C" dyg_status dyg_dev_alloc(dyg_ctx ctx, size_t bytes, void** out) {
+    if (!ctx || !out) return DYG_ERR_ARG;
+    *out = nullptr;
+    if (hipMalloc(out, bytes ? bytes : 1) != hipSuccess) {
+        set_error(ctx, "dyg_dev_alloc: device allocation failed");
+        return DYG_ERR_NOMEM;
+    }
+    return DYG_OK;
+}
+
+extern "C" dyg_status dyg_dev_free(dyg_ctx ctx, void* p) {
+    if (!ctx) return DYG_ERR_ARG;
+    if (p) {
+        DYG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        DYG_HIP(ctx, hipFree(p));
+    }
+    return DYG_OK;
+}
+
+extern "C" dyg_status dyg_copy_to_device(dyg_ctx ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return DYG_ERR_ARG;
+    DYG_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    DYG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return DYG_OK;
+}
+
+extern "C" dyg_status dyg_copy_on_device(dyg_ctx ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return DYG_ERR_ARG;
+    DYG_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    DYG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return DYG_OK;
+}
+
+extern "C" dyg_status dyg_copy_to_host(dyg_ctx ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return DYG_ERR_ARG;
+    DYG_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    DYG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return DYG_OK;
+}
+
 /* ------------------------------------------------------------------ */
 extern "C" dyg_status dyg_walk_incremental(dyg_graph H, const uint32_t* src, const uint32_t* dst,
                                            const float* w, const uint64_t* edge_id, uint32_t n,
@@ -233,6 +274,7 @@ extern "C" dyg_status dyg_walk_incremental_dev(dyg_graph H, const uint32_t* src,
     a.max_steps = p->max_steps;
     a.thr = p->threshold;
     a.flags = p->flags;
+    a.edges_per_block = p->block_edges;
     a.seed = p->seed;
     a.found = found_at;
     a.cond = cond;
@@ -262,6 +304,7 @@ extern "C" dyg_status dyg_walk_decremental_dev(dyg_graph G, const uint32_t* src,
     a.walkers = p->walkers;
     a.max_steps = p->max_steps;
     a.flags = p->flags;
+    a.edges_per_block = p->block_edges;
     a.seed = p->seed;
     a.found = found_at;
     a.cond = cond;

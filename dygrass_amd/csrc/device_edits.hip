@@ -12,6 +12,8 @@
 //   k_alloc     per row: relocate to the arena tail when the row is full
 //               (device bump allocator, order kept) and publish (start, deg)
 //   k_write     each append writes its slot at start + old degree + rank
+//   k_refresh   per touched row: every slot pointing to it (found through its
+//               neighbours' rows) gets the row's new (first slot, degree)
 // Slot order is what the walks index, and it equals the host plan's; the
 // physical position of a relocated row may differ, which no walk can see.
 #include <hipcub/hipcub.hpp>
@@ -80,7 +82,7 @@ __global__ void k_heads(const uint32_t* __restrict__ keys, const uint32_t* __res
     atomicAdd(&segcnt[r], 1u);
 }
 
-__global__ void k_alloc(uint2* __restrict__ rows, uint2* __restrict__ adj,
+__global__ void k_alloc(uint2* __restrict__ rows, uint4* __restrict__ adj,
                         uint32_t* __restrict__ cap, const uint32_t* __restrict__ keys,
                         const uint32_t* __restrict__ count, const uint32_t* __restrict__ segstart,
                         const uint32_t* __restrict__ segcnt, uint32_t* __restrict__ rowbase,
@@ -112,7 +114,7 @@ __global__ void k_alloc(uint2* __restrict__ rows, uint2* __restrict__ adj,
     rows[r] = make_uint2(row.x, d0 + k);
 }
 
-__global__ void k_write(uint2* __restrict__ adj, const uint32_t* __restrict__ keys,
+__global__ void k_write(uint4* __restrict__ adj, const uint32_t* __restrict__ keys,
                         const uint32_t* __restrict__ vals, const uint32_t* __restrict__ count,
                         uint32_t* __restrict__ segstart, uint32_t* __restrict__ segcnt,
                         const uint32_t* __restrict__ rowbase, const uint32_t* __restrict__ au,
@@ -126,9 +128,32 @@ __global__ void k_write(uint2* __restrict__ adj, const uint32_t* __restrict__ ke
         const uint32_t o = vals[p];
         const uint32_t k = o >> 1;
         const uint32_t nbr = (o & 1u) ? au[k] : av[k];
-        adj[base + (p - st)] = make_uint2(nbr, __float_as_uint(aw[k]));
+        // the neighbour's descriptor is written by k_refresh of that neighbour
+        adj[base + (p - st)] = make_uint4(nbr, __float_as_uint(aw[k]), 0u, 0u);
     }
-    if (p == st) segcnt[r] = 0;  // ready for the next batch
+}
+
+// One thread per touched row r (its first append): rewrite r's descriptor in
+// every slot that points to r.  Each such slot is written only by r's thread
+// and only its upper half (.z,.w), so threads never race; the neighbour ids
+// and the row descriptors read here are final once k_write has run.
+__global__ void k_refresh(const uint2* __restrict__ rows, uint4* __restrict__ adj,
+                          const uint32_t* __restrict__ keys, const uint32_t* __restrict__ count,
+                          uint32_t* __restrict__ segstart, uint32_t* __restrict__ segcnt) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= 2 * *count) return;
+    const uint32_t r = keys[p];
+    if (segstart[r] != p) return;
+    segcnt[r] = 0;  // ready for the next batch
+    const uint2 rr = rows[r];
+    for (uint32_t q = 0; q < rr.y; ++q) {
+        const uint32_t x = adj[rr.x + q].x;
+        const uint2 rx = rows[x];
+        for (uint32_t k = 0; k < rx.y; ++k) {
+            uint4* sl = &adj[rx.x + k];
+            if (sl->x == r) *reinterpret_cast<uint2*>(&sl->z) = rr;
+        }
+    }
 }
 
 }  // namespace
@@ -232,6 +257,8 @@ dyg_status device_edits_batch(dyg_graph g, const uint32_t* src, const uint32_t* 
                        (unsigned long long)g->cap_slots, g->slack, err);
     hipLaunchKernelGGL(k_write, dim3(blocks), dim3(256), 0, s, g->d_adj, d.keys_out, d.vals_out,
                        count, d.segstart, d.segcnt, d.rowbase, d.au, d.av, d.aw);
+    hipLaunchKernelGGL(k_refresh, dim3(blocks), dim3(256), 0, s, g->d_rows, g->d_adj, d.keys_out,
+                       count, d.segstart, d.segcnt);
     DYG_HIP(ctx, hipGetLastError());
     return DYG_OK;
 }

@@ -1,18 +1,22 @@
 // graph_store.hip -- device graph store with per-row slack and in-place edits.
 //
 // HBM layout (one graph):
-//   rows[V]        uint2 (first slot, degree)           8 B per vertex
-//   adj[capacity]  uint2 (neighbour, weight bits)        8 B per slot
+//   rows[V]        uint2 (first slot, degree)                          8 B per vertex
+//   adj[capacity]  uint4 (neighbour, weight bits, neighbour's first
+//                         slot, neighbour's degree)                    16 B per slot
 // Each row owns `cap` consecutive slots; the first `deg` are live, in the
 // reference's slot order.  Inserts append at slot deg; a full row moves to
 // the arena tail with its order kept (the reference's relocation,
 // random_walk_incremental/gpu_graph.cuh:357-384).  Deletes move the row's
 // last slot into the hole (random_walk_decremental/gpu_graph.cuh:368-399).
+// Every slot also holds the row descriptor of the vertex it points to, so a
+// walk hop is one load; an edit that changes a row therefore rewrites that
+// copy in each neighbour's slots that point back to it ("refresh").
 //
 // The host keeps a mirror of rows and slots.  An edit is planned on the host
-// in reference order and shipped as a delta (copies, slot writes, row
-// writes) that one kernel applies; the reference instead re-uploads the whole
-// arena after every batch (gpu_graph.cuh:333-335, :462-465).
+// in reference order and shipped as a delta (final slot values and row
+// descriptors) that one kernel applies; the reference instead re-uploads the
+// whole arena after every batch (gpu_graph.cuh:333-335, :462-465).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -65,7 +69,7 @@ inline uint32_t fbits(float f) {
 }
 
 // Grow the slot arena to hold at least `need` slots (device copy of the used
-// prefix; host mirror resized).
+// prefix).  The host mirror grows when the host next plans an edit.
 dyg_status grow_arena(dyg_graph g, uint64_t need) {
     if (need <= g->cap_slots) return DYG_OK;
     uint64_t ncap = std::max<uint64_t>(need, g->cap_slots + g->cap_slots / 2 + 1024);
@@ -74,22 +78,54 @@ dyg_status grow_arena(dyg_graph g, uint64_t need) {
         set_error(g->ctx, "slot arena exceeds 2^32 slots");
         return DYG_ERR_NOMEM;
     }
-    uint2* nadj = nullptr;
-    if (hipMalloc(&nadj, ncap * sizeof(uint2)) != hipSuccess) {
+    uint4* nadj = nullptr;
+    if (hipMalloc(&nadj, ncap * sizeof(uint4)) != hipSuccess) {
         set_error(g->ctx, "device allocation of the slot arena failed");
         return DYG_ERR_NOMEM;
     }
-    DYG_HIP(g->ctx, hipMemcpyAsync(nadj, g->d_adj, g->used_slots * sizeof(uint2),
+    DYG_HIP(g->ctx, hipMemcpyAsync(nadj, g->d_adj, g->used_slots * sizeof(uint4),
                                    hipMemcpyDeviceToDevice, g->ctx->stream));
     DYG_HIP(g->ctx, hipStreamSynchronize(g->ctx->stream));
     DYG_HIP(g->ctx, hipFree(g->d_adj));
     g->d_adj = nadj;
     g->cap_slots = ncap;
-    // the host mirror grows when the host next plans an edit
     return DYG_OK;
 }
 
-dyg_status ship_delta(dyg_graph g, uint32_t n_copy, uint32_t n_write, uint32_t n_row) {
+// Rewrite, in every slot that points to one of `rows_changed`, the copy of
+// that row's descriptor (host mirror), and record the slots touched.
+void refresh_neighbour_copies(dyg_graph g, const std::vector<uint32_t>& rows_changed,
+                              std::vector<uint32_t>& dirty) {
+    for (uint32_t r : rows_changed) {
+        const uint32_t sr = g->start[r], dr = g->deg[r];
+        for (uint32_t q = 0; q < dr; ++q) {
+            const uint32_t x = g->adj[sr + q].x;
+            const uint32_t sx = g->start[x], dx = g->deg[x];
+            for (uint32_t k = 0; k < dx; ++k) {
+                uint4& sl = g->adj[sx + k];
+                if (sl.x == r && (sl.z != sr || sl.w != dr)) {
+                    sl.z = sr;
+                    sl.w = dr;
+                    dirty.push_back(sx + k);
+                }
+            }
+        }
+    }
+}
+
+// Ship the final values of `dirty` slots and the descriptors of `rows`.
+dyg_status ship_delta(dyg_graph g, std::vector<uint32_t>& dirty, std::vector<uint32_t>& rows) {
+    std::sort(dirty.begin(), dirty.end());
+    dirty.erase(std::unique(dirty.begin(), dirty.end()), dirty.end());
+    std::sort(rows.begin(), rows.end());
+    rows.erase(std::unique(rows.begin(), rows.end()), rows.end());
+    g->delta.clear();
+    g->delta.reserve(dirty.size() * 5 + rows.size() * 3);
+    for (uint32_t sl : dirty) {
+        const uint4 a = g->adj[sl];
+        g->delta.insert(g->delta.end(), {sl, a.x, a.y, a.z, a.w});
+    }
+    for (uint32_t r : rows) g->delta.insert(g->delta.end(), {r, g->start[r], g->deg[r]});
     const size_t bytes = g->delta.size() * sizeof(uint32_t);
     if (bytes == 0) return DYG_OK;
     dyg_ctx ctx = g->ctx;
@@ -100,8 +136,8 @@ dyg_status ship_delta(dyg_graph g, uint32_t n_copy, uint32_t n_write, uint32_t n
     std::memcpy(ctx->stage.p, g->delta.data(), bytes);
     DYG_HIP(ctx, hipMemcpyAsync(g->d_delta.p, ctx->stage.p, bytes, hipMemcpyHostToDevice,
                                 ctx->stream));
-    DYG_HIP(ctx, launch_apply_delta(g->d_rows, g->d_adj, (const uint32_t*)g->d_delta.p, n_copy,
-                                    n_write, n_row, ctx->stream));
+    DYG_HIP(ctx, launch_apply_delta(g->d_rows, g->d_adj, (const uint32_t*)g->d_delta.p,
+                                    (uint32_t)dirty.size(), (uint32_t)rows.size(), ctx->stream));
     return DYG_OK;
 }
 
@@ -118,8 +154,8 @@ dyg_status graph_refresh_mirror(dyg_graph g) {
                            hipMemcpyDeviceToHost));
     if (g->dev.cap)
         DYG_HIP(ctx, hipMemcpy(g->cap.data(), g->dev.cap, (size_t)g->V * 4, hipMemcpyDeviceToHost));
-    if (g->adj.size() < g->used_slots) g->adj.resize(g->used_slots, make_uint2(0, 0));
-    DYG_HIP(ctx, hipMemcpy(g->adj.data(), g->d_adj, g->used_slots * sizeof(uint2),
+    if (g->adj.size() < g->used_slots) g->adj.resize(g->used_slots, make_uint4(0, 0, 0, 0));
+    DYG_HIP(ctx, hipMemcpy(g->adj.data(), g->d_adj, g->used_slots * sizeof(uint4),
                            hipMemcpyDeviceToHost));
     for (uint32_t v = 0; v < g->V; ++v) {
         g->start[v] = rows[v].x;
@@ -136,15 +172,15 @@ dyg_status graph_apply_inserts(dyg_graph g, const uint32_t* u, const uint32_t* v
         set_error(g->ctx, "insert: keyed graph edited on the device");
         return DYG_ERR_STATE;
     }
-    {
-        dyg_status rs = graph_refresh_mirror(g);
-        if (rs != DYG_OK) return rs;
-    }
     for (uint32_t i = 0; i < n; ++i)
         if (u[i] >= g->V || v[i] >= g->V) {
             set_error(g->ctx, "insert: vertex id out of range");
             return DYG_ERR_ARG;
         }
+    {
+        dyg_status rs = graph_refresh_mirror(g);
+        if (rs != DYG_OK) return rs;
+    }
     // appends in reference order: edge i gives (u->v) then (v->u)
     struct App {
         uint32_t row, seq, nbr;
@@ -170,10 +206,15 @@ dyg_status graph_apply_inserts(dyg_graph g, const uint32_t* u, const uint32_t* v
     }
     dyg_status st = grow_arena(g, g->used_slots + extra);
     if (st != DYG_OK) return st;
-    if (g->adj.size() < g->used_slots + extra) g->adj.resize(g->cap_slots, make_uint2(0, 0));
+    if (g->adj.size() < g->used_slots + extra) {
+        // grow the mirror geometrically, not to the (possibly reserved) arena size
+        const uint64_t want = std::max<uint64_t>(g->used_slots + extra, g->adj.size() * 3 / 2);
+        g->adj.resize(std::min<uint64_t>(want, g->cap_slots), make_uint4(0, 0, 0, 0));
+    }
 
-    std::vector<uint32_t> copies, writes, rowops;
+    std::vector<uint32_t> dirty, touched;
     std::vector<uint32_t> slot_of(2 * (size_t)n);
+    dirty.reserve(4 * (size_t)n);
     for (size_t p = 0; p < apps.size();) {
         size_t q = p;
         while (q < apps.size() && apps[q].row == apps[p].row) ++q;
@@ -184,28 +225,24 @@ dyg_status graph_apply_inserts(dyg_graph g, const uint32_t* u, const uint32_t* v
             const uint64_t ncap = std::max<uint64_t>({2ull * g->cap[r], need + g->slack, 4ull});
             const uint32_t ns = (uint32_t)g->used_slots;
             g->used_slots += ncap;
-            copies.push_back(g->start[r]);
-            copies.push_back(ns);
-            copies.push_back(d0);
             std::copy(g->adj.begin() + g->start[r], g->adj.begin() + g->start[r] + d0,
                       g->adj.begin() + ns);
+            for (uint32_t k = 0; k < d0; ++k) dirty.push_back(ns + k);
             g->start[r] = ns;
             g->cap[r] = (uint32_t)ncap;
         }
         for (size_t k = p; k < q; ++k) {
             const uint32_t slot = g->start[r] + d0 + (uint32_t)(k - p);
-            g->adj[slot] = make_uint2(apps[k].nbr, fbits(apps[k].w));
-            writes.push_back(slot);
-            writes.push_back(apps[k].nbr);
-            writes.push_back(fbits(apps[k].w));
+            // the neighbour's descriptor is filled by the refresh below
+            g->adj[slot] = make_uint4(apps[k].nbr, fbits(apps[k].w), 0, 0);
+            dirty.push_back(slot);
             slot_of[apps[k].seq] = (uint32_t)(d0 + (k - p));
         }
         g->deg[r] = (uint32_t)need;
-        rowops.push_back(r);
-        rowops.push_back(g->start[r]);
-        rowops.push_back(g->deg[r]);
+        touched.push_back(r);
         p = q;
     }
+    refresh_neighbour_copies(g, touched, dirty);
     g->nslots += 2ull * n;
     if (g->keyed) {
         // first occurrence wins, as std::unordered_map::insert in the reference
@@ -217,12 +254,7 @@ dyg_status graph_apply_inserts(dyg_graph g, const uint32_t* u, const uint32_t* v
             else g->key_map.insert({key, {sb, sa}});
         }
     }
-    g->delta.clear();
-    g->delta.insert(g->delta.end(), copies.begin(), copies.end());
-    g->delta.insert(g->delta.end(), writes.begin(), writes.end());
-    g->delta.insert(g->delta.end(), rowops.begin(), rowops.end());
-    return ship_delta(g, (uint32_t)(copies.size() / 3), (uint32_t)(writes.size() / 3),
-                      (uint32_t)(rowops.size() / 3));
+    return ship_delta(g, dirty, touched);
 }
 
 dyg_status graph_apply_deletes(dyg_graph g, const uint32_t* u, const uint32_t* v, uint32_t n,
@@ -231,15 +263,15 @@ dyg_status graph_apply_deletes(dyg_graph g, const uint32_t* u, const uint32_t* v
         set_error(g->ctx, "delete: graph was uploaded without its edge map");
         return DYG_ERR_ARG;
     }
-    {
-        dyg_status rs = graph_refresh_mirror(g);
-        if (rs != DYG_OK) return rs;
-    }
     for (uint32_t i = 0; i < n; ++i)
         if (u[i] >= g->V || v[i] >= g->V) {
             set_error(g->ctx, "delete: vertex id out of range");
             return DYG_ERR_ARG;
         }
+    {
+        dyg_status rs = graph_refresh_mirror(g);
+        if (rs != DYG_OK) return rs;
+    }
     // validation pass: no vertex may reach degree 0 (the reference asserts,
     // random_walk_decremental/gpu_graph.cuh:362-363); nothing changes on failure
     {
@@ -256,7 +288,7 @@ dyg_status graph_apply_deletes(dyg_graph g, const uint32_t* u, const uint32_t* v
             }
         }
     }
-    std::vector<uint32_t> dirty_slots, dirty_rows;
+    std::vector<uint32_t> dirty, touched;
     for (uint32_t i = 0; i < n; ++i) {
         long a = u[i], b = v[i];
         if (a > b) std::swap(a, b);
@@ -272,41 +304,29 @@ dyg_status graph_apply_deletes(dyg_graph g, const uint32_t* u, const uint32_t* v
         const uint32_t da = --g->deg[a];
         const uint32_t db = --g->deg[b];
         if (da != ib_a) {
-            const uint2 last = g->adj[g->start[a] + da];
+            const uint4 last = g->adj[g->start[a] + da];
             g->adj[g->start[a] + ib_a] = last;
             const long al = last.x;
             if (a < al) g->key_map.at(a * g->mult + al).first = ib_a;
             else g->key_map.at(al * g->mult + a).second = ib_a;
-            dirty_slots.push_back(g->start[a] + ib_a);
+            dirty.push_back(g->start[a] + ib_a);
         }
         if (db != ia_b) {
-            const uint2 last = g->adj[g->start[b] + db];
+            const uint4 last = g->adj[g->start[b] + db];
             g->adj[g->start[b] + ia_b] = last;
             const long bl = last.x;
             if (b < bl) g->key_map.at(b * g->mult + bl).first = ia_b;
             else g->key_map.at(bl * g->mult + b).second = ia_b;
-            dirty_slots.push_back(g->start[b] + ia_b);
+            dirty.push_back(g->start[b] + ia_b);
         }
-        dirty_rows.push_back((uint32_t)a);
-        dirty_rows.push_back((uint32_t)b);
+        touched.push_back((uint32_t)a);
+        touched.push_back((uint32_t)b);
         g->nslots -= 2;
     }
-    std::sort(dirty_slots.begin(), dirty_slots.end());
-    dirty_slots.erase(std::unique(dirty_slots.begin(), dirty_slots.end()), dirty_slots.end());
-    std::sort(dirty_rows.begin(), dirty_rows.end());
-    dirty_rows.erase(std::unique(dirty_rows.begin(), dirty_rows.end()), dirty_rows.end());
-    g->delta.clear();
-    for (uint32_t sl : dirty_slots) {
-        g->delta.push_back(sl);
-        g->delta.push_back(g->adj[sl].x);
-        g->delta.push_back(g->adj[sl].y);
-    }
-    for (uint32_t r : dirty_rows) {
-        g->delta.push_back(r);
-        g->delta.push_back(g->start[r]);
-        g->delta.push_back(g->deg[r]);
-    }
-    return ship_delta(g, 0, (uint32_t)dirty_slots.size(), (uint32_t)dirty_rows.size());
+    std::sort(touched.begin(), touched.end());
+    touched.erase(std::unique(touched.begin(), touched.end()), touched.end());
+    refresh_neighbour_copies(g, touched, dirty);
+    return ship_delta(g, dirty, touched);
 }
 
 }  // namespace dyg
@@ -316,7 +336,7 @@ using namespace dyg;
 extern "C" dyg_status dyg_graph_upload(dyg_ctx ctx, uint32_t V, const uint64_t* row_ptr,
                                        const uint32_t* col, const float* w, uint32_t slack,
                                        int keyed, dyg_graph* out) {
-    if (!ctx || !out || (!row_ptr && V) || V == 0) {
+    if (!ctx || !out || !row_ptr || V == 0) {
         set_error(ctx, "dyg_graph_upload: invalid argument");
         return DYG_ERR_ARG;
     }
@@ -331,6 +351,10 @@ extern "C" dyg_status dyg_graph_upload(dyg_ctx ctx, uint32_t V, const uint64_t* 
             return DYG_ERR_ARG;
         }
     const uint64_t E = row_ptr[V];
+    if (E && (!col || !w)) {
+        set_error(ctx, "dyg_graph_upload: invalid argument");
+        return DYG_ERR_ARG;
+    }
     for (uint64_t p = 0; p < E; ++p)
         if (col[p] >= V) {
             set_error(ctx, "dyg_graph_upload: neighbour id out of range");
@@ -350,7 +374,8 @@ extern "C" dyg_status dyg_graph_upload(dyg_ctx ctx, uint32_t V, const uint64_t* 
         g->start[v] = (uint32_t)total;
         total += g->cap[v];
     }
-    const uint64_t cap_slots = total + std::max<uint64_t>(4096, total / 16);
+    // room for relocations without an early regrow (HBM is plentiful)
+    const uint64_t cap_slots = total + std::max<uint64_t>(4096, total / 2);
     if (cap_slots > 0xFFFFFFFFull) {
         delete g;
         set_error(ctx, "dyg_graph_upload: more than 2^32 slots");
@@ -359,14 +384,16 @@ extern "C" dyg_status dyg_graph_upload(dyg_ctx ctx, uint32_t V, const uint64_t* 
     g->used_slots = total;
     g->cap_slots = cap_slots;
     g->nslots = E;
-    g->adj.assign(cap_slots, make_uint2(0, 0));
+    g->adj.assign(total + total / 4 + 64, make_uint4(0, 0, 0, 0));
     for (uint32_t v = 0; v < V; ++v)
-        for (uint32_t k = 0; k < g->deg[v]; ++k)
-            g->adj[g->start[v] + k] = make_uint2(col[row_ptr[v] + k], fbits(w[row_ptr[v] + k]));
+        for (uint32_t k = 0; k < g->deg[v]; ++k) {
+            const uint32_t c = col[row_ptr[v] + k];
+            g->adj[g->start[v] + k] = make_uint4(c, fbits(w[row_ptr[v] + k]), g->start[c], g->deg[c]);
+        }
     std::vector<uint2> rows(V);
     for (uint32_t v = 0; v < V; ++v) rows[v] = make_uint2(g->start[v], g->deg[v]);
     if (hipMalloc(&g->d_rows, (size_t)V * sizeof(uint2)) != hipSuccess ||
-        hipMalloc(&g->d_adj, cap_slots * sizeof(uint2)) != hipSuccess) {
+        hipMalloc(&g->d_adj, cap_slots * sizeof(uint4)) != hipSuccess) {
         if (g->d_rows) (void)hipFree(g->d_rows);
         if (g->d_adj) (void)hipFree(g->d_adj);
         delete g;
@@ -375,7 +402,7 @@ extern "C" dyg_status dyg_graph_upload(dyg_ctx ctx, uint32_t V, const uint64_t* 
     }
     hipError_t e1 = hipMemcpy(g->d_rows, rows.data(), (size_t)V * sizeof(uint2),
                               hipMemcpyHostToDevice);
-    hipError_t e2 = hipMemcpy(g->d_adj, g->adj.data(), cap_slots * sizeof(uint2),
+    hipError_t e2 = hipMemcpy(g->d_adj, g->adj.data(), total * sizeof(uint4),
                               hipMemcpyHostToDevice);
     if (e1 != hipSuccess || e2 != hipSuccess) {
         set_error(ctx, std::string("dyg_graph_upload: ") +
@@ -421,23 +448,46 @@ extern "C" dyg_status dyg_graph_download(dyg_graph g, uint64_t* row_ptr, uint32_
                                          float* w) {
     if (!g || !row_ptr) return DYG_ERR_ARG;
     dyg_ctx ctx = g->ctx;
-    std::vector<uint2> rows(g->V), adj(g->used_slots);
+    std::vector<uint2> rows(g->V);
+    std::vector<uint4> adj(g->used_slots);
     DYG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     DYG_HIP(ctx, hipMemcpy(rows.data(), g->d_rows, (size_t)g->V * sizeof(uint2),
                            hipMemcpyDeviceToHost));
-    DYG_HIP(ctx, hipMemcpy(adj.data(), g->d_adj, g->used_slots * sizeof(uint2),
+    DYG_HIP(ctx, hipMemcpy(adj.data(), g->d_adj, g->used_slots * sizeof(uint4),
                            hipMemcpyDeviceToHost));
     uint64_t p = 0;
     for (uint32_t v = 0; v < g->V; ++v) {
         row_ptr[v] = p;
         for (uint32_t k = 0; k < rows[v].y; ++k) {
-            const uint2 sl = adj[rows[v].x + k];
+            const uint4 sl = adj[rows[v].x + k];
             if (col) col[p] = sl.x;
             if (w) std::memcpy(&w[p], &sl.y, 4);
             ++p;
         }
     }
     row_ptr[g->V] = p;
+    return DYG_OK;
+}
+
+extern "C" dyg_status dyg_graph_check(dyg_graph g, uint64_t* bad_slots) {
+    // Every live slot's copy of its neighbour's row descriptor equals the
+    // neighbour's row (device state).
+    if (!g) return DYG_ERR_ARG;
+    dyg_ctx ctx = g->ctx;
+    std::vector<uint2> rows(g->V);
+    std::vector<uint4> adj(g->used_slots);
+    DYG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    DYG_HIP(ctx, hipMemcpy(rows.data(), g->d_rows, (size_t)g->V * sizeof(uint2),
+                           hipMemcpyDeviceToHost));
+    DYG_HIP(ctx, hipMemcpy(adj.data(), g->d_adj, g->used_slots * sizeof(uint4),
+                           hipMemcpyDeviceToHost));
+    uint64_t bad = 0;
+    for (uint32_t v = 0; v < g->V; ++v)
+        for (uint32_t k = 0; k < rows[v].y; ++k) {
+            const uint4 sl = adj[rows[v].x + k];
+            if (sl.x >= g->V || sl.z != rows[sl.x].x || sl.w != rows[sl.x].y) ++bad;
+        }
+    if (bad_slots) *bad_slots = bad;
     return DYG_OK;
 }
 

@@ -69,6 +69,7 @@ dyg_status run_incremental(dyg_graph H, const RunBufs& b, const uint32_t* h_src,
         a.max_steps = p->max_steps;
         a.thr = p->threshold;
         a.flags = p->flags;
+        a.edges_per_block = p->block_edges;
         a.seed = p->seed;
         a.found = b.found + b0;
         a.cond = b.cond + b0;

@@ -77,19 +77,21 @@ struct DevEdits {
 };
 }  // namespace dyg
 
-// One slot of a row: neighbour id and weight bits, 8 bytes, read in one load.
-// One row descriptor: first slot and degree, 8 bytes, read in one load.
+// One row descriptor: first slot and degree, 8 bytes.
+// One slot: (neighbour id, weight bits, neighbour's first slot, neighbour's
+// degree), 16 bytes read in one load -- a walk hop needs no row lookup.
+// Edits keep every slot's copy of its neighbour's row descriptor current.
 struct dyg_graph_s {
     dyg_ctx ctx = nullptr;
     uint32_t V = 0;
     uint2* d_rows = nullptr;  // [V] (start, deg)
-    uint2* d_adj = nullptr;   // [cap_slots] (col, float bits of w)
+    uint4* d_adj = nullptr;   // [cap_slots] (col, float bits of w, start(col), deg(col))
     uint64_t cap_slots = 0;   // slots allocated on the device
     uint64_t used_slots = 0;  // bump pointer of the slot arena
     uint32_t slack = 0;       // spare slots given to a relocated row
     // host mirror of the device state (same slot positions)
     std::vector<uint32_t> start, deg, cap;
-    std::vector<uint2> adj;
+    std::vector<uint4> adj;
     uint64_t nslots = 0;  // sum of degrees
     // reference edge map (random_walk_decremental/functions.cpp:243-258)
     bool keyed = false;
@@ -136,7 +138,7 @@ dyg_status device_edits_end(dyg_graph g, uint64_t* added);
 // Walk launchers (walk_kernels.hip).  All pointers are device pointers.
 struct WalkLaunch {
     const uint2* rows;
-    const uint2* adj;
+    const uint4* adj;
     const uint32_t* src;
     const uint32_t* dst;
     const float* w;          // insertion weights (nullptr for deletions)
@@ -158,11 +160,13 @@ struct WalkLaunch {
 };
 hipError_t launch_walk_incremental(const WalkLaunch& a, hipStream_t s);
 hipError_t launch_walk_decremental(const WalkLaunch& a, hipStream_t s);
-hipError_t launch_heuristic(const uint2* rows, const uint2* adj, const uint32_t* endpoint,
+hipError_t launch_heuristic(const uint2* rows, const uint4* adj, const uint32_t* endpoint,
                             const uint64_t* key, uint32_t n, uint32_t walkers, uint64_t seed,
                             uint32_t* path3, hipStream_t s);
-hipError_t launch_apply_delta(uint2* rows, uint2* adj, const uint32_t* delta, uint32_t n_copy,
-                              uint32_t n_write, uint32_t n_row, hipStream_t s);
+// Delta layout (uint32 words): writes (slot, col, wbits, nstart, ndeg), then
+// rows (v, start, deg).
+hipError_t launch_apply_delta(uint2* rows, uint4* adj, const uint32_t* delta, uint32_t n_write,
+                              uint32_t n_row, hipStream_t s);
 
 bool valid_walkers(uint32_t w);
 // Validates walkers (power of two, 1..1024) and max_steps (1..32767).

@@ -1,9 +1,11 @@
 // walk_kernels.hip -- non-backtracking random-walk kernels for CDNA4 (gfx950).
 //
-// Layout read by every kernel (device graph store, 8-byte records):
-//   rows[v] = (first slot, degree)          one 8 B load per visited vertex
-//   adj[p]  = (neighbour id, weight bits)   one 8 B load per draw
-// A walk hop is two dependent loads (row, then slot).  The slot that the
+// Layout read by every kernel (device graph store):
+//   rows[v] = (first slot, degree)                              8 B, source only
+//   adj[p]  = (neighbour, weight bits, neighbour's first slot,
+//              neighbour's degree)                              16 B per draw
+// A walk hop is one dependent 16 B load: the drawn slot already carries the
+// row descriptor of the vertex it leads to.  The slot that the
 // non-backtracking rule may redirect to (the row's last slot) is loaded in the
 // same step as the drawn slot, so a redirect costs no extra latency.
 //
@@ -74,6 +76,7 @@ struct WalkOut {
 };
 
 __device__ __forceinline__ uint2 ld(const uint2* __restrict__ p, uint32_t i) { return p[i]; }
+__device__ __forceinline__ uint4 ld(const uint4* __restrict__ p, uint32_t i) { return p[i]; }
 
 // One walker.  INSERT: distortion test R*w_e >= thr before the target test
 // (random_walk_incremental/main.cu:97-163).  Otherwise the deletion walk
@@ -83,7 +86,7 @@ __device__ __forceinline__ uint2 ld(const uint2* __restrict__ p, uint32_t i) { r
 // no longer win and the edge's winner is unchanged.
 template <bool INSERT, bool RECORD, bool PRUNE>
 __device__ __forceinline__ void walk_lane(const uint2* __restrict__ rows,
-                                          const uint2* __restrict__ adj, uint32_t s, uint32_t t,
+                                          const uint4* __restrict__ adj, uint32_t s, uint32_t t,
                                           float we, float thr, int max_steps, Draws& rng,
                                           uint32_t* best, uint32_t* __restrict__ path,
                                           WalkOut& o) {
@@ -97,7 +100,7 @@ __device__ __forceinline__ void walk_lane(const uint2* __restrict__ rows,
     if (row.y == 0) return;  // isolated source: nothing to find
     uint32_t i = rng.next() % row.y;
     o.hops = 1;
-    uint2 sl = ld(adj, row.x + i);
+    uint4 sl = ld(adj, row.x + i);
     float R = 1.0f / __uint_as_float(sl.y);
     if (RECORD) path[1] = sl.x;
     if (sl.x == t) {
@@ -107,17 +110,17 @@ __device__ __forceinline__ void walk_lane(const uint2* __restrict__ rows,
         return;
     }
     uint32_t prev = s, cur = sl.x;
+    row = make_uint2(sl.z, sl.w);
     int step = 2;
     while (step < max_steps) {
-        row = ld(rows, cur);
         if (row.y <= 1) {  // leaf (degree 0 cannot occur: deletions keep degree >= 1)
             o.leaf = 1;
             break;
         }
         i = rng.next() % (row.y - 1);
         ++o.hops;
-        const uint2 a = ld(adj, row.x + i);
-        const uint2 b = ld(adj, row.x + row.y - 1);
+        const uint4 a = ld(adj, row.x + i);
+        const uint4 b = ld(adj, row.x + row.y - 1);
         const bool back = (a.x == prev);
         o.redir += back ? 1u : 0u;
         sl = back ? b : a;
@@ -135,6 +138,7 @@ __device__ __forceinline__ void walk_lane(const uint2* __restrict__ rows,
         }
         prev = cur;
         cur = sl.x;
+        row = make_uint2(sl.z, sl.w);
         ++step;
     }
     o.R = R;
@@ -260,12 +264,13 @@ __global__ __launch_bounds__((1 << LOGW) < 64 ? 64 : (1 << LOGW)) void k_walk(Wa
 // trajectory depends only on (edge, walker) and the graph.
 // ---------------------------------------------------------------------------
 constexpr int kPoolThreads = 256;
-constexpr int kPoolMaxEdges = 64;
+constexpr int kPoolMaxEdges = kPoolThreads;  // one staging lane per edge
 
 struct Lane {
     uint32_t edge;  // local edge index
     uint32_t walker;
     uint32_t cur, prev, t;
+    uint2 row;      // descriptor of `cur`
     float R, we;
     int step;
 };
@@ -309,7 +314,14 @@ __global__ __launch_bounds__(kPoolThreads) void k_walk_pool(WalkLaunch a, uint32
     uint32_t hops = 0, redir = 0, leaf = 0;  // this lane's running counts
     uint32_t my_hops = 0;                    // current walker's draws
 
+    uint2 pending = make_uint2(0, 0);  // source row of a lane refilled last turn
     for (;;) {
+        // ---- a lane refilled last turn adopts its source row: the load was
+        // issued a full turn ago, so this read does not stall the wave ----
+        if (active && L.step == 0) {
+            L.row = pending;
+            L.step = 1;
+        }
         // ---- refill idle lanes (one LDS atomic per wave) ----
         const unsigned long long need = __ballot(!active && !drained);
         if (need) {
@@ -325,11 +337,12 @@ __global__ __launch_bounds__(kPoolThreads) void k_walk_pool(WalkLaunch a, uint32
                     L.edge = task >> LOGW;
                     L.walker = task & (W - 1u);
                     L.cur = s_src[L.edge];
+                    pending = ld(a.rows, L.cur);  // consumed next turn
                     L.t = s_dst[L.edge];
                     L.we = s_w[L.edge];
                     L.prev = 0;
                     L.R = 0.0f;
-                    L.step = 1;
+                    L.step = 0;
                     rng = Draws(a.seed, INSERT ? kTagInsert : kTagDelete, s_eid[L.edge], L.walker);
                     my_hops = 0;
                     active = true;
@@ -339,19 +352,19 @@ __global__ __launch_bounds__(kPoolThreads) void k_walk_pool(WalkLaunch a, uint32
             }
         }
         if (!__any(active)) break;
-        if (!active) continue;
+        if (!active || L.step == 0) continue;
 
         // ---- one hop ----
         bool done = false;
         int found = -1;
-        const uint2 row = ld(a.rows, L.cur);
+        const uint2 row = L.row;
         if (L.step == 1) {
             if (row.y == 0) {
                 done = true;  // isolated source
             } else {
                 const uint32_t i = rng.next() % row.y;
                 ++my_hops;
-                const uint2 sl = ld(a.adj, row.x + i);
+                const uint4 sl = ld(a.adj, row.x + i);
                 L.R = 1.0f / __uint_as_float(sl.y);
                 if (sl.x == L.t) {
                     found = 1;
@@ -359,6 +372,7 @@ __global__ __launch_bounds__(kPoolThreads) void k_walk_pool(WalkLaunch a, uint32
                 } else {
                     L.prev = L.cur;
                     L.cur = sl.x;
+                    L.row = make_uint2(sl.z, sl.w);
                     L.step = 2;
                     if (L.step >= (int)a.max_steps) done = true;
                 }
@@ -369,11 +383,11 @@ __global__ __launch_bounds__(kPoolThreads) void k_walk_pool(WalkLaunch a, uint32
         } else {
             const uint32_t i = rng.next() % (row.y - 1);
             ++my_hops;
-            const uint2 x = ld(a.adj, row.x + i);
-            const uint2 y = ld(a.adj, row.x + row.y - 1);
+            const uint4 x = ld(a.adj, row.x + i);
+            const uint4 y = ld(a.adj, row.x + row.y - 1);
             const bool back = (x.x == L.prev);
             redir += back ? 1u : 0u;
-            const uint2 sl = back ? y : x;
+            const uint4 sl = back ? y : x;
             L.R += 1.0f / __uint_as_float(sl.y);
             if (INSERT && L.R * L.we >= a.thr) {
                 done = true;
@@ -387,6 +401,7 @@ __global__ __launch_bounds__(kPoolThreads) void k_walk_pool(WalkLaunch a, uint32
                 }
                 L.prev = L.cur;
                 L.cur = sl.x;
+                L.row = make_uint2(sl.z, sl.w);
                 ++L.step;
                 if (L.step >= (int)a.max_steps) done = true;
             }
@@ -438,7 +453,7 @@ __global__ __launch_bounds__(kPoolThreads) void k_walk_pool(WalkLaunch a, uint32
 }
 
 __global__ __launch_bounds__(256) void k_heuristic(const uint2* __restrict__ rows,
-                                                   const uint2* __restrict__ adj,
+                                                   const uint4* __restrict__ adj,
                                                    const uint32_t* __restrict__ endpoint,
                                                    const uint64_t* __restrict__ key, uint32_t n,
                                                    uint32_t walkers, uint64_t seed,
@@ -451,9 +466,10 @@ __global__ __launch_bounds__(256) void k_heuristic(const uint2* __restrict__ row
     uint32_t n1 = u, n2 = u;
     const uint2 ru = rows[u];
     if (ru.y != 0) {
-        n1 = adj[ru.x + rng.next() % ru.y].x;
-        const uint2 r1 = rows[n1];
-        n2 = adj[r1.x + rng.next() % r1.y].x;  // r1.y >= 1: n1 is a neighbour of u
+        const uint4 s1 = adj[ru.x + rng.next() % ru.y];
+        n1 = s1.x;
+        const uint2 r1 = make_uint2(s1.z, s1.w);  // r1.y >= 1: n1 is a neighbour of u
+        n2 = adj[r1.x + rng.next() % r1.y].x;
         if (n2 == u) n2 = adj[r1.x + r1.y - 1].x;
     }
     path3[gid * 3 + 0] = u;
@@ -461,34 +477,20 @@ __global__ __launch_bounds__(256) void k_heuristic(const uint2* __restrict__ row
     path3[gid * 3 + 2] = n2;
 }
 
-// Graph edits: relocation copies, slot writes, row descriptor writes.  The
-// three sets touch disjoint memory (copies read abandoned regions, writes
-// land beyond the copied prefix or in rows not relocated), so one launch
-// applies them all.  Delta layout (uint32 words): copies (src, dst, len),
-// writes (slot, col, wbits), rows (v, start, deg).
+// Graph edits: final slot values and row descriptors planned on the host.
+// Every written slot and row appears once, so one launch applies them all.
 __global__ __launch_bounds__(256) void k_apply_delta(uint2* __restrict__ rows,
-                                                     uint2* __restrict__ adj,
+                                                     uint4* __restrict__ adj,
                                                      const uint32_t* __restrict__ d,
-                                                     uint32_t n_copy, uint32_t n_write,
-                                                     uint32_t n_row) {
-    const uint32_t copy_blocks = (n_copy + 3u) / 4u;
-    if (blockIdx.x < copy_blocks) {
-        const uint32_t op = blockIdx.x * 4u + (threadIdx.x >> 6);
-        const uint32_t lane = threadIdx.x & 63u;
-        if (op < n_copy) {
-            const uint32_t src = d[op * 3 + 0], dst = d[op * 3 + 1], len = d[op * 3 + 2];
-            for (uint32_t k = lane; k < len; k += 64u) adj[dst + k] = adj[src + k];
-        }
-        return;
-    }
-    const uint32_t g = (blockIdx.x - copy_blocks) * 256u + threadIdx.x;
-    const uint32_t* wr = d + (uint64_t)n_copy * 3;
-    const uint32_t* rw = wr + (uint64_t)n_write * 3;
+                                                     uint32_t n_write, uint32_t n_row) {
+    const uint32_t g = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t* rw = d + (uint64_t)n_write * 5;
     if (g < n_write) {
-        adj[wr[g * 3 + 0]] = make_uint2(wr[g * 3 + 1], wr[g * 3 + 2]);
+        const uint32_t* q = d + (uint64_t)g * 5;
+        adj[q[0]] = make_uint4(q[1], q[2], q[3], q[4]);
     } else if (g - n_write < n_row) {
-        const uint32_t q = g - n_write;
-        rows[rw[q * 3 + 0]] = make_uint2(rw[q * 3 + 1], rw[q * 3 + 2]);
+        const uint32_t k = g - n_write;
+        rows[rw[k * 3 + 0]] = make_uint2(rw[k * 3 + 1], rw[k * 3 + 2]);
     }
 }
 
@@ -559,7 +561,7 @@ hipError_t launch_walk_decremental(const WalkLaunch& a, hipStream_t s) {
     return launch_any<false>(a, s);
 }
 
-hipError_t launch_heuristic(const uint2* rows, const uint2* adj, const uint32_t* endpoint,
+hipError_t launch_heuristic(const uint2* rows, const uint4* adj, const uint32_t* endpoint,
                             const uint64_t* key, uint32_t n, uint32_t walkers, uint64_t seed,
                             uint32_t* path3, hipStream_t s) {
     const uint64_t total = (uint64_t)n * walkers;
@@ -570,14 +572,12 @@ hipError_t launch_heuristic(const uint2* rows, const uint2* adj, const uint32_t*
     return hipGetLastError();
 }
 
-hipError_t launch_apply_delta(uint2* rows, uint2* adj, const uint32_t* delta, uint32_t n_copy,
-                              uint32_t n_write, uint32_t n_row, hipStream_t s) {
-    const uint32_t copy_blocks = (n_copy + 3u) / 4u;
-    const uint32_t rest_blocks = (uint32_t)(((uint64_t)n_write + n_row + 255) / 256);
-    const uint32_t blocks = copy_blocks + rest_blocks;
+hipError_t launch_apply_delta(uint2* rows, uint4* adj, const uint32_t* delta, uint32_t n_write,
+                              uint32_t n_row, hipStream_t s) {
+    const uint32_t blocks = (uint32_t)(((uint64_t)n_write + n_row + 255) / 256);
     if (blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_apply_delta, dim3(blocks), dim3(256), 0, s, rows, adj, delta, n_copy,
-                       n_write, n_row);
+    hipLaunchKernelGGL(k_apply_delta, dim3(blocks), dim3(256), 0, s, rows, adj, delta, n_write,
+                       n_row);
     return hipGetLastError();
 }
 

@@ -57,6 +57,14 @@ const char* dyg_version(void);
 void* dyg_ctx_stream(dyg_ctx ctx);
 /* Forces all work enqueued on ctx to finish. */
 dyg_status dyg_ctx_sync(dyg_ctx ctx);
+/* Device buffers for the *_dev entry points (inputs kept resident in HBM).
+ * Copies are ordered on the context stream and complete before returning.
+ * No counterpart in the reference (its buffers live inside its classes). */
+dyg_status dyg_dev_alloc(dyg_ctx ctx, size_t bytes, void** out);
+dyg_status dyg_dev_free(dyg_ctx ctx, void* p);
+dyg_status dyg_copy_to_device(dyg_ctx ctx, void* dst, const void* src, size_t bytes);
+dyg_status dyg_copy_to_host(dyg_ctx ctx, void* dst, const void* src, size_t bytes);
+dyg_status dyg_copy_on_device(dyg_ctx ctx, void* dst, const void* src, size_t bytes);
 
 /* ------------------------------------------------------------------ */
 /* Device graph store                                                 */
@@ -92,6 +100,9 @@ dyg_status dyg_graph_insert(dyg_graph g, const uint32_t* u, const uint32_t* v, c
 dyg_status dyg_graph_delete(dyg_graph g, const uint32_t* u, const uint32_t* v, uint32_t n,
                             int8_t* hit);
 void dyg_graph_free(dyg_graph g);
+/* Consistency check of the device state: counts live slots whose copy of the
+ * neighbour's row descriptor differs from that row (0 when consistent). */
+dyg_status dyg_graph_check(dyg_graph g, uint64_t* bad_slots);
 
 /* ------------------------------------------------------------------ */
 /* Walks                                                              */
@@ -115,6 +126,9 @@ typedef struct {
     float threshold;    /* insertion distortion threshold (argv[2]) */
     uint32_t flags;     /* DYG_WALK_* */
     uint64_t seed;      /* Philox key of the draw streams */
+    uint32_t block_edges; /* work-queue kernel: edges owned by one workgroup (0 = automatic,
+                             max 256); a launch-shape knob, results do not depend on it */
+    uint32_t reserved;
 } dyg_walk_params;
 
 typedef struct {

@@ -499,6 +499,59 @@ bool parse_targets(const char* path, bool weighted, uint32_t divisor, std::vecto
     return true;
 }
 
+/* Passes 2-3 of the parsers on lines already in memory (0-based ids):
+ * random_walk_decremental/functions.cpp:149-270 (degrees, slots in line
+ * order with the self-loop layout, edge map with the first line winning).
+ * V = largest id + 1. */
+void build_from_lines(const uint32_t* u, const uint32_t* v, const float* w, uint64_t m,
+                      bool build_map, CSR& g) {
+    uint32_t v_max = 0;
+    for (uint64_t l = 0; l < m; ++l) v_max = std::max(v_max, std::max(u[l], v[l]));
+    g.lines = m;
+    g.V = m ? (uint64_t)v_max + 1 : 0;
+    g.begin.assign(g.V + 1, 0);
+    g.degree.assign(g.V, 0);
+    g.adj.assign(2 * m, 0);
+    g.weight.assign(2 * m, 0.0f);
+    g.mtx.resize(m);
+    if (build_map) {
+        g.multiplier = (long)std::pow(10, find_digits((long)g.V));
+        g.edge_map.reserve((size_t)((unsigned)m * 1.5));
+    }
+    for (uint64_t l = 0; l < m; ++l) {
+        g.degree[u[l]]++;
+        g.degree[v[l]]++;
+    }
+    for (uint64_t i = 1; i <= g.V; i++) g.begin[i] = g.begin[i - 1] + g.degree[i - 1];
+    std::fill(g.degree.begin(), g.degree.end(), 0u);
+    for (uint64_t l = 0; l < m; ++l) {
+        const uint32_t src = u[l], dst = v[l];
+        const float wt = w ? w[l] : 1.0f;
+        uint64_t pos = g.begin[src] + g.degree[src];
+        g.adj[pos] = dst;
+        if (dst == src) {
+            uint64_t p2 = g.begin[dst] + g.degree[dst] + 1;
+            if (p2 < g.adj.size()) g.adj[p2] = src;
+        } else {
+            g.adj[g.begin[dst] + g.degree[dst]] = src;
+        }
+        g.weight[g.begin[src] + g.degree[src]] = wt;
+        g.weight[g.begin[dst] + g.degree[dst]] = wt;
+        g.mtx[l] = std::make_tuple(src, dst, wt);
+        if (build_map) {
+            long a = src, b = dst;
+            uint32_t da = g.degree[a], db = g.degree[b];
+            if (a > b) {
+                std::swap(a, b);
+                std::swap(da, db);
+            }
+            g.edge_map.insert({a * g.multiplier + b, {da, db}});
+        }
+        g.degree[src]++;
+        g.degree[dst]++;
+    }
+}
+
 orc_graph* graph_from_csr(const CSR& c) {
     orc_graph* g = new orc_graph;
     g->rows.resize(c.V);
@@ -571,6 +624,130 @@ struct orc_dense {
     long mult;
     EdgeMap map;
 };
+
+namespace {
+
+/* The decremental batch loop (random_walk_decremental/main.cu:363-444):
+ * deletions (du[i], dv[i]) in file order, snapshot batches of ceil(M/10);
+ * per batch removeAndUpdateDelEdges, the walks on G, incrementalUpdateSparse
+ * and, for no-path edges, the two-hop recovery.  Writes the reference's
+ * stdout lines to `log`. */
+struct DecTotals {
+    uint64_t hops = 0, no_path = 0, filtered = 0, batches = 0;
+    double walk_s = 0, total_s = 0;
+};
+int dec_loop(orc_graph* G, EdgeMap& dense_map, long mult, std::unordered_set<long>& sparse_map,
+             std::unordered_set<long>& added_edges, const uint32_t* du, const uint32_t* dv,
+             uint32_t M, uint64_t seed, int threads, std::ostream& log, DecTotals& tot) {
+    const uint32_t n_steps = 100, n_walkers = 512, divisor = 10, n_heur = 16;
+    std::vector<uint32_t> starts = batch_starts((uint32_t)M, divisor);
+    uint32_t nb = (uint32_t)starts.size() - 1;
+    uint64_t hops_total = 0, no_path_total = 0, filtered_total = 0;
+    double t_walk = 0, t0 = now_s();
+    for (uint32_t k = 0; k < nb; ++k) {
+        uint32_t b0 = starts[k], n = starts[k + 1] - starts[k];
+        /* removeAndUpdateDelEdges */
+        std::vector<uint32_t> fa, fb;
+        std::vector<uint64_t> fid;
+        for (uint32_t i = 0; i < n; ++i) {
+            long a = du[b0 + i];
+            long b = dv[b0 + i];
+            if (a > b) std::swap(a, b);
+            long key = a * mult + b;
+            int r = dense_remove(G, dense_map, mult, a, b);
+            if (r == -1) {
+                log << "dense_map not found: a: " << a << " b: " << b << " key: " << key << "\n";
+                continue;
+            }
+            if (r == -2) {
+                std::fprintf(stderr, "oracle: deletion leaves a zero degree\n");
+                return -5; /* the reference asserts (gpu_graph.cuh:362-363) */
+            }
+            if (sparse_map.count(key) == 0) {
+                log << "sparse_map not found: a: " << a << " b: " << b << " key: " << key << "\n";
+            } else {
+                sparse_map.erase(key);
+                fa.push_back((uint32_t)a);
+                fb.push_back((uint32_t)b);
+                fid.push_back(b0 + i);
+            }
+        }
+        if (k == 0) {
+            log << "Graph copied to GPU\n";
+            t0 = now_s();
+        }
+        uint32_t nf = (uint32_t)fa.size();
+        filtered_total += nf;
+        std::vector<int32_t> f(nf);
+        std::vector<float> c(nf);
+        std::vector<uint64_t> h(nf);
+        std::vector<uint32_t> path((uint64_t)nf * n_steps);
+        double tw0 = now_s();
+        orc_walk_dec_batch(G, nf, fa.data(), fb.data(), fid.data(), n_walkers, n_steps, seed,
+                           threads, f.data(), c.data(), nullptr, h.data(), path.data());
+        t_walk += now_s() - tw0;
+        log << " result copied\n";
+        /* incrementalUpdateSparse */
+        std::vector<uint32_t> heur;
+        std::vector<uint64_t> heur_key;
+        uint64_t add_count = 0;
+        for (uint32_t i = 0; i < nf; ++i) {
+            hops_total += h[i];
+            if (f[i] != -1) {
+                long from = path[(uint64_t)i * n_steps];
+                for (int j = 0; j < f[i]; ++j) {
+                    long to = path[(uint64_t)i * n_steps + j + 1];
+                    long a = from, b = to;
+                    if (a > b) std::swap(a, b);
+                    long key = a * mult + b;
+                    if (sparse_map.count(key) == 0) {
+                        add_count++;
+                        sparse_map.insert(key);
+                        added_edges.insert(key);
+                    }
+                    from = to;
+                }
+            } else {
+                heur.push_back(fa[i]);
+                heur.push_back(fb[i]);
+                heur_key.push_back(fid[i] << 1);
+                heur_key.push_back((fid[i] << 1) | 1u);
+            }
+        }
+        no_path_total += heur.size() / 2;
+        log << "No path count: " << heur.size() / 2 << "\n";
+        log << "Added edges: " << add_count << "\n";
+        if (!heur.empty()) {
+            std::vector<uint32_t> p3((uint64_t)heur.size() * n_heur * 3);
+            for (size_t e = 0; e < heur.size(); ++e)
+                heuristic_one(G, heur[e], n_heur, seed, heur_key[e], p3.data() + e * n_heur * 3);
+            log << " result copied\n";
+            for (size_t e = 0; e < heur.size(); ++e)
+                for (uint32_t j = 0; j < n_heur; ++j) {
+                    long from = p3[(e * n_heur + j) * 3];
+                    for (int q = 1; q < 3; ++q) {
+                        long to = p3[(e * n_heur + j) * 3 + q];
+                        long a = from, b = to;
+                        if (a > b) std::swap(a, b);
+                        long key = a * mult + b;
+                        if (sparse_map.count(key) == 0) sparse_map.insert(key);
+                        from = to;
+                    }
+                }
+        }
+    }
+    if (nb == 0) /* an empty first batch still launches (main.cu:363-397) */
+        log << "Graph copied to GPU\n result copied\nNo path count: 0\nAdded edges: 0\n";
+    tot.hops = hops_total;
+    tot.no_path = no_path_total;
+    tot.filtered = filtered_total;
+    tot.batches = nb;
+    tot.walk_s = t_walk;
+    tot.total_s = now_s() - t0;
+    return 0;
+}
+
+}  // namespace
 
 /* ==================================================================== */
 /* C ABI                                                                */
@@ -688,6 +865,51 @@ void orc_walk_dec_batch(const orc_graph* G, uint32_t n, const uint32_t* src, con
         if (winner) winner[e] = win;
         if (hops) hops[e] = h;
     }
+}
+
+int orc_decremental_lines(uint32_t V, const uint32_t* g_u, const uint32_t* g_v, const float* g_w,
+                          uint64_t g_m, const uint32_t* h_u, const uint32_t* h_v, uint64_t h_m,
+                          const uint32_t* d_u, const uint32_t* d_v, uint32_t d_m, uint64_t seed,
+                          int threads, orc_run_stats* st, uint32_t* added_u, uint32_t* added_v,
+                          uint64_t added_cap) {
+    CSR dense;
+    build_from_lines(g_u, g_v, g_w, g_m, true, dense);
+    if (dense.V != V) return -4; /* V is set by the largest id, as in a file */
+    const long mult = dense.multiplier;
+    orc_graph* G = graph_from_csr(dense);
+    std::unordered_set<long> sparse_map, added_edges;
+    for (uint64_t i = 0; i < h_m; ++i) {
+        long a = h_u[i], b = h_v[i];
+        if (a > b) std::swap(a, b);
+        sparse_map.insert(a * mult + b);
+    }
+    const uint64_t before = sparse_map.size();
+    std::ostringstream log;
+    DecTotals tot;
+    int rc = dec_loop(G, dense.edge_map, mult, sparse_map, added_edges, d_u, d_v, d_m, seed,
+                      threads, log, tot);
+    delete G;
+    if (rc != 0) return rc;
+    std::vector<long> keys(added_edges.begin(), added_edges.end());
+    std::sort(keys.begin(), keys.end());
+    for (size_t i = 0; i < keys.size() && i < added_cap; ++i) {
+        added_u[i] = (uint32_t)(keys[i] / mult);
+        added_v[i] = (uint32_t)(keys[i] % mult);
+    }
+    if (st) {
+        std::memset(st, 0, sizeof(*st));
+        st->n_updates = d_m;
+        st->n_batches = tot.batches;
+        st->hops = tot.hops;
+        st->sparse_before = before;
+        st->sparse_after = sparse_map.size();
+        st->added_edges = added_edges.size();
+        st->no_path = tot.no_path;
+        st->filtered = tot.filtered;
+        st->walk_seconds = tot.walk_s;
+        st->total_seconds = tot.total_s;
+    }
+    return 0;
 }
 
 /* `ostream << float` as the reference's writer uses it
@@ -894,7 +1116,7 @@ int orc_decremental_run(const char* dir, uint64_t seed, int threads, const char*
     std::string dense_file = d + "/updated_dense.mtx";
     std::string del_file = d + "/del.mtx";
     std::string output_file = d + "/added_edges.mtx";
-    const uint32_t n_steps = 100, n_walkers = 512, divisor = 10, n_heur = 16;
+    const uint32_t divisor = 10;
     std::ostringstream log;
     log << "Read graph and sparsifier\n";
     CSR dense, sparse;
@@ -915,106 +1137,29 @@ int orc_decremental_run(const char* dir, uint64_t seed, int threads, const char*
     std::unordered_set<long> added_edges;
     const uint64_t old_edge_num = sparse_map.size();
 
-    std::vector<uint32_t> starts = batch_starts((uint32_t)M, divisor);
-    uint32_t nb = (uint32_t)starts.size() - 1;
-    uint64_t hops_total = 0, no_path_total = 0, filtered_total = 0;
-    double t_walk = 0, t0 = 0;
-    for (uint32_t k = 0; k < nb; ++k) {
-        uint32_t b0 = starts[k], n = starts[k + 1] - starts[k];
-        /* removeAndUpdateDelEdges */
-        std::vector<uint32_t> fa, fb;
-        std::vector<uint64_t> fid;
-        for (uint32_t i = 0; i < n; ++i) {
-            long a = nodes[2 * (uint64_t)b0 + i];
-            long b = nodes[2 * (uint64_t)b0 + n + i];
-            if (a > b) std::swap(a, b);
-            long key = a * mult + b;
-            int r = dense_remove(G, dense_map, mult, a, b);
-            if (r == -1) {
-                log << "dense_map not found: a: " << a << " b: " << b << " key: " << key << "\n";
-                continue;
+    /* the 10-segment layout decoded back to file order */
+    std::vector<uint32_t> du(M), dv(M);
+    {
+        std::vector<uint32_t> st0 = batch_starts((uint32_t)M, divisor);
+        for (size_t k = 0; k + 1 < st0.size(); ++k) {
+            const uint32_t b0 = st0[k], n = st0[k + 1] - st0[k];
+            for (uint32_t i = 0; i < n; ++i) {
+                du[b0 + i] = nodes[2 * (uint64_t)b0 + i];
+                dv[b0 + i] = nodes[2 * (uint64_t)b0 + n + i];
             }
-            if (r == -2) {
-                std::fprintf(stderr, "oracle: deletion leaves a zero degree\n");
-                delete G;
-                return -5; /* the reference asserts (gpu_graph.cuh:362-363) */
-            }
-            if (sparse_map.count(key) == 0) {
-                log << "sparse_map not found: a: " << a << " b: " << b << " key: " << key << "\n";
-            } else {
-                sparse_map.erase(key);
-                fa.push_back((uint32_t)a);
-                fb.push_back((uint32_t)b);
-                fid.push_back(b0 + i);
-            }
-        }
-        if (k == 0) {
-            log << "Graph copied to GPU\n";
-            t0 = now_s();
-        }
-        uint32_t nf = (uint32_t)fa.size();
-        filtered_total += nf;
-        std::vector<int32_t> f(nf);
-        std::vector<float> c(nf);
-        std::vector<uint64_t> h(nf);
-        std::vector<uint32_t> path((uint64_t)nf * n_steps);
-        double tw0 = now_s();
-        orc_walk_dec_batch(G, nf, fa.data(), fb.data(), fid.data(), n_walkers, n_steps, seed,
-                           threads, f.data(), c.data(), nullptr, h.data(), path.data());
-        t_walk += now_s() - tw0;
-        log << " result copied\n";
-        /* incrementalUpdateSparse */
-        std::vector<uint32_t> heur;
-        std::vector<uint64_t> heur_key;
-        uint64_t add_count = 0;
-        for (uint32_t i = 0; i < nf; ++i) {
-            hops_total += h[i];
-            if (f[i] != -1) {
-                long from = path[(uint64_t)i * n_steps];
-                for (int j = 0; j < f[i]; ++j) {
-                    long to = path[(uint64_t)i * n_steps + j + 1];
-                    long a = from, b = to;
-                    if (a > b) std::swap(a, b);
-                    long key = a * mult + b;
-                    if (sparse_map.count(key) == 0) {
-                        add_count++;
-                        sparse_map.insert(key);
-                        added_edges.insert(key);
-                    }
-                    from = to;
-                }
-            } else {
-                heur.push_back(fa[i]);
-                heur.push_back(fb[i]);
-                heur_key.push_back(fid[i] << 1);
-                heur_key.push_back((fid[i] << 1) | 1u);
-            }
-        }
-        no_path_total += heur.size() / 2;
-        log << "No path count: " << heur.size() / 2 << "\n";
-        log << "Added edges: " << add_count << "\n";
-        if (!heur.empty()) {
-            std::vector<uint32_t> p3((uint64_t)heur.size() * n_heur * 3);
-            for (size_t e = 0; e < heur.size(); ++e)
-                heuristic_one(G, heur[e], n_heur, seed, heur_key[e], p3.data() + e * n_heur * 3);
-            log << " result copied\n";
-            for (size_t e = 0; e < heur.size(); ++e)
-                for (uint32_t j = 0; j < n_heur; ++j) {
-                    long from = p3[(e * n_heur + j) * 3];
-                    for (int q = 1; q < 3; ++q) {
-                        long to = p3[(e * n_heur + j) * 3 + q];
-                        long a = from, b = to;
-                        if (a > b) std::swap(a, b);
-                        long key = a * mult + b;
-                        if (sparse_map.count(key) == 0) sparse_map.insert(key);
-                        from = to;
-                    }
-                }
         }
     }
-    if (nb == 0) /* an empty first batch still launches (main.cu:363-397) */
-        log << "Graph copied to GPU\n result copied\nNo path count: 0\nAdded edges: 0\n";
-    double t1 = now_s();
+    DecTotals tot;
+    int rc = dec_loop(G, dense_map, mult, sparse_map, added_edges, du.data(), dv.data(),
+                      (uint32_t)M, seed, threads, log, tot);
+    if (rc != 0) {
+        delete G;
+        return rc;
+    }
+    const uint32_t nb = (uint32_t)tot.batches;
+    const uint64_t hops_total = tot.hops, no_path_total = tot.no_path,
+                   filtered_total = tot.filtered;
+    const double t_walk = tot.walk_s;
     uint64_t new_edge_num = sparse_map.size();
     log << "Increased edges number: " << (size_t)(new_edge_num - old_edge_num) << "\n";
     log << "Density: " << (float)(size_t)(new_edge_num - (uint32_t)V) / (uint32_t)V * 100 << " %\n";
@@ -1043,7 +1188,7 @@ int orc_decremental_run(const char* dir, uint64_t seed, int threads, const char*
         st->no_path = no_path_total;
         st->filtered = filtered_total;
         st->walk_seconds = t_walk;
-        st->total_seconds = t1 - t0;
+        st->total_seconds = tot.total_s;
     }
     delete G;
     return 0;

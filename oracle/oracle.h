@@ -126,6 +126,16 @@ int orc_incremental_arrays(orc_graph* H, uint32_t m, const uint32_t* src, const 
 /* random_walk_incremental/main.cu:240-362 */
 int orc_incremental_run(const char* dir, float thr, uint32_t W, uint64_t seed, int threads,
                         const char* log_path, orc_run_stats* st);
+/* The decremental loop (random_walk_decremental/main.cu:363-444) on arrays:
+ * G as lines in file order (0-based ids; V = largest id + 1), H as an edge
+ * list, deletions in file order.  added_u/v receive the added set sorted by
+ * key (up to added_cap pairs; st->added_edges gives the full count). */
+int orc_decremental_lines(uint32_t V, const uint32_t* g_u, const uint32_t* g_v, const float* g_w,
+                          uint64_t g_m, const uint32_t* h_u, const uint32_t* h_v, uint64_t h_m,
+                          const uint32_t* d_u, const uint32_t* d_v, uint32_t d_m, uint64_t seed,
+                          int threads, orc_run_stats* st, uint32_t* added_u, uint32_t* added_v,
+                          uint64_t added_cap);
+
 /* random_walk_decremental/main.cu:318-468 */
 int orc_decremental_run(const char* dir, uint64_t seed, int threads, const char* log_path,
                         orc_run_stats* st);

@@ -56,6 +56,9 @@ _SIGS = {
                                   C.c_uint64, C.c_int, i32p, f32p, u32p, u64p, u32p]),
     "orc_batches": (C.c_uint32, [C.c_uint32, C.c_uint32, u32p]),
     "orc_format_float": (C.c_int, [C.c_float, C.c_char_p, C.c_int]),
+    "orc_decremental_lines": (C.c_int, [C.c_uint32, u32p, u32p, f32p, C.c_uint64, u32p, u32p,
+                                        C.c_uint64, u32p, u32p, C.c_uint32, C.c_uint64, C.c_int,
+                                        C.POINTER(RunStats), u32p, u32p, C.c_uint64]),
     "orc_incremental_arrays": (C.c_int, [vp, C.c_uint32, u32p, u32p, f32p, C.c_uint32, C.c_float,
                                          C.c_uint32, C.c_uint32, C.c_uint64, C.c_int, i32p,
                                          C.POINTER(RunStats)]),
@@ -258,6 +261,24 @@ class Dense:
                 self.h = None
         except Exception:
             pass
+
+
+def decremental_lines(V, gu, gv, gw, hu, hv, du, dv, seed=0, threads=1):
+    """Whole decremental loop on arrays; returns (stats, added pairs [k,2])."""
+    a = [np.ascontiguousarray(x, np.uint32) for x in (gu, gv, hu, hv, du, dv)]
+    gw = np.ascontiguousarray(gw, np.float32)
+    cap = 4 * len(a[4]) * 100 + 16
+    au = np.zeros(cap, np.uint32)
+    av = np.zeros(cap, np.uint32)
+    st = RunStats()
+    rc = lib().orc_decremental_lines(V, _p(a[0], u32p), _p(a[1], u32p), _p(gw, f32p), len(a[0]),
+                                     _p(a[2], u32p), _p(a[3], u32p), len(a[2]), _p(a[4], u32p),
+                                     _p(a[5], u32p), len(a[4]), seed, threads, C.byref(st),
+                                     _p(au, u32p), _p(av, u32p), cap)
+    if rc != 0:
+        raise RuntimeError(f"oracle decremental run failed ({rc})")
+    n = int(st.added_edges)
+    return st, np.stack([au[:n], av[:n]], axis=1)
 
 
 def incremental_run(dirpath: str, thr: float, W: int, seed: int = 0, threads: int = 1,

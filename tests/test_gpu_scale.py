@@ -46,7 +46,32 @@ def test_full_size_incremental_matches_oracle(ctx, rgg):
     orp, ocol, ow = og.export()
     assert np.array_equal(grp, orp) and np.array_equal(gcol, ocol)
     assert np.array_equal(f32bits(gw), f32bits(ow))
+    assert H.check() == 0
 
     H2 = DeviceGraph(ctx, rgg.H.V, rp, col, w, slack=0)
     found2, _ = incremental_update(H2, rgg.ins.u, rgg.ins.v, rgg.ins.w, p)
     assert np.array_equal(found, found2)
+
+
+def test_full_size_decremental_matches_oracle(ctx):
+    """2M-node triangulated mesh, 100k uniform deletions in 10 snapshot batches,
+    512 walkers: the recovered edge set, the sparsifier size and every counter
+    equal the oracle's."""
+    from dygrass_amd import DualGraph, params, synth
+    from oracle import pyoracle as po
+
+    wl = synth.config_mesh_delete(1.0, 0)
+    rp, col, w = csr(wl.G)
+    d = DualGraph(ctx, wl.G.V, rp, col, w, wl.H.u, wl.H.v)
+    before = d.sparse_size
+    log = d.run(wl.dels.u, wl.dels.v, params(512, 100, seed=0))
+    added = d.added_edges()
+    threads = min(os.cpu_count() or 1, 16)
+    st, oadded = po.decremental_lines(wl.G.V, wl.G.u, wl.G.v, wl.G.w, wl.H.u, wl.H.v,
+                                      wl.dels.u, wl.dels.v, seed=0, threads=threads)
+    assert before == st.sparse_before
+    assert d.sparse_size == st.sparse_after
+    assert np.array_equal(added, oadded)
+    assert sum(b["queued"] for b in log) == st.filtered
+    assert sum(b["no_path"] for b in log) == st.no_path
+    assert len(log) == 10 and st.filtered > 20000

@@ -99,6 +99,7 @@ def test_incremental_update_matches_oracle_run(ctx, grid_wl, per_edge, host_edit
     assert np.array_equal(grp, orp)
     assert np.array_equal(gcol, ocol)
     assert np.array_equal(f32bits(gw), f32bits(ow))
+    assert H.check() == 0
 
 
 def test_device_edits_chain_with_host_edits(ctx, grid_wl):
@@ -117,9 +118,11 @@ def test_device_edits_chain_with_host_edits(ctx, grid_wl):
                                       params(128, 100, 10.0, seed=b0))
         ofound, _ = og.incremental(u[b0:b1], v[b0:b1], x[b0:b1], 10.0, 128, seed=b0, threads=8)
         assert np.array_equal(found, ofound)
+    assert H.check() == 0
     H.insert(u[:50], v[:50], x[:50])
     for i in range(50):
         og.insert(int(u[i]), int(v[i]), float(x[i]))
+    assert H.check() == 0
     g, o = H.download(), og.export()
     assert np.array_equal(g[0], o[0]) and np.array_equal(g[1], o[1])
     assert np.array_equal(f32bits(g[2]), f32bits(o[2]))
@@ -144,6 +147,7 @@ def test_insert_relocation_keeps_slot_order(ctx):
         x = rng.uniform(0.1, 10, 25).astype(np.float32)
         a[:10] = 5  # hammer one row
         G.insert(a, b, x)
+        assert G.check() == 0
         for i in range(25):
             og.insert(int(a[i]), int(b[i]), float(x[i]))
     g = G.download()
@@ -167,6 +171,7 @@ def test_delete_swap_with_last_matches_oracle(ctx, mesh_wl):
         du = np.concatenate([du, du[:3], [0]]).astype(np.uint32)
         dv = np.concatenate([dv, dv[:3], [mesh_wl.G.V - 1]]).astype(np.uint32)
         hit = G.delete(du, dv)
+        assert G.check() == 0
         ohit = np.array([od.remove(int(a), int(b)) for a, b in zip(du, dv)])
         assert np.array_equal(hit == 1, ohit == 1)
         g = G.download()
@@ -234,11 +239,40 @@ def test_walks_are_deterministic_and_prune_invariant(ctx, grid_wl):
         assert np.array_equal(a.winner, x.winner)
 
 
+@pytest.mark.parametrize("W,block_edges", [(8, 1), (8, 7), (8, 64), (8, 256), (64, 3),
+                                           (512, 256), (1, 256)])
+@pytest.mark.parametrize("insert", [True, False])
+def test_pool_shape_does_not_change_results(ctx, grid_wl, W, block_edges, insert):
+    """Edges per workgroup is a launch-shape knob: ragged last workgroups,
+    one edge per workgroup and 256 edges per workgroup all give the
+    one-workgroup-per-edge kernel's results, hop counts included."""
+    from dygrass_amd import DeviceGraph, params, walk_decremental, walk_incremental
+
+    rp, col, w = csr(grid_wl.H)
+    H = DeviceGraph(ctx, grid_wl.H.V, rp, col, w)
+    n = 1001  # not a multiple of any block_edges above
+    s, t, we = grid_wl.ins.u[:n], grid_wl.ins.v[:n], grid_wl.ins.w[:n]
+    if insert:
+        run = lambda p: walk_incremental(H, s, t, we, p)
+    else:
+        a, b = np.minimum(s, t), np.maximum(s, t)
+        run = lambda p: walk_decremental(H, a, b, p)
+    ref = run(params(W, 100, 10.0, seed=2, prune=False, per_edge=True))
+    got = run(params(W, 100, 10.0, seed=2, prune=False, block_edges=block_edges))
+    assert np.array_equal(got.found_at, ref.found_at)
+    assert np.array_equal(f32bits(got.cond), f32bits(ref.cond))
+    assert np.array_equal(got.winner, ref.winner)
+    assert np.array_equal(got.hops, ref.hops)
+    if not insert:
+        for i in np.flatnonzero(ref.found_at > 0):
+            k = ref.found_at[i] + 1
+            assert np.array_equal(got.path[i, :k], ref.path[i, :k])
+
+
 def test_incremental_update_dev_matches_host_entry(ctx, grid_wl):
     """Device-pointer entry point (inputs resident in HBM) equals the host one."""
-    import torch
-
-    from dygrass_amd import DeviceGraph, incremental_update, incremental_update_dev, params
+    from dygrass_amd import (DeviceArray, DeviceGraph, DygError, incremental_update,
+                             incremental_update_dev, params)
 
     rp, col, w = csr(grid_wl.H)
     p = params(512, 100, 10.0, seed=4)
@@ -247,25 +281,22 @@ def test_incremental_update_dev_matches_host_entry(ctx, grid_wl):
     H2 = DeviceGraph(ctx, grid_wl.H.V, rp, col, w)
     m = len(grid_wl.ins)
     H2.reserve(m, (m + 9) // 10)
-    dev = torch.device("cuda:0")
-    ts = torch.from_numpy(grid_wl.ins.u.astype(np.int32)).to(dev)
-    td = torch.from_numpy(grid_wl.ins.v.astype(np.int32)).to(dev)
-    tw = torch.from_numpy(grid_wl.ins.w).to(dev)
-    tf = torch.empty(m, dtype=torch.int32, device=dev)
-    tc = torch.empty(m, dtype=torch.float32, device=dev)
-    torch.cuda.synchronize()
-    s2 = incremental_update_dev(H2, ts.data_ptr(), td.data_ptr(), tw.data_ptr(), m,
-                                tf.data_ptr(), tc.data_ptr(), p)
-    assert np.array_equal(tf.cpu().numpy(), f1)
-    assert s2["added"] == s1["added"] and s2["hops"] == s1["hops"]
+    ts = DeviceArray.from_numpy(ctx, grid_wl.ins.u)
+    td = DeviceArray.from_numpy(ctx, grid_wl.ins.v)
+    tw = DeviceArray.from_numpy(ctx, grid_wl.ins.w)
+    tf = DeviceArray(ctx, m, np.int32)
+    tc = DeviceArray(ctx, m, np.float32)
+    s2 = incremental_update_dev(H2, ts.ptr, td.ptr, tw.ptr, m, tf.ptr, tc.ptr, p)
+    assert np.array_equal(tf.numpy(), f1)
+    # (hop counts are not compared: with pruning they depend on when a lane
+    # sees another lane's find; the decisions do not)
+    assert s2["added"] == s1["added"]
     g1, g2 = H1.download(), H2.download()
     assert all(np.array_equal(a, b) for a, b in zip(g1, g2))
+    assert H1.check() == 0 and H2.check() == 0
     # an out-of-range id is rejected before any walk runs
-    bad = ts.clone()
-    bad[5] = grid_wl.H.V + 3
-    torch.cuda.synchronize()
-    from dygrass_amd import DygError
-
+    bad_src = grid_wl.ins.u.copy()
+    bad_src[5] = grid_wl.H.V + 3
+    tb = DeviceArray.from_numpy(ctx, bad_src)
     with pytest.raises(DygError):
-        incremental_update_dev(H2, bad.data_ptr(), td.data_ptr(), tw.data_ptr(), m,
-                               tf.data_ptr(), tc.data_ptr(), p)
+        incremental_update_dev(H2, tb.ptr, td.ptr, tw.ptr, m, tf.ptr, tc.ptr, p)

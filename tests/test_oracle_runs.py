@@ -106,3 +106,21 @@ def test_incremental_path_chords_by_distortion():
     assert st.n_added == 2
     assert out[-2:] == ["1 6 2", "1 10 3"]
     assert out[:9] == [f"{i + 1} {i + 2} 1" for i in range(9)]
+
+
+def test_decremental_lines_equals_file_run():
+    """The array entry (used at full size) equals the file run on the same data."""
+    from dygrass_amd import synth
+
+    wl = synth.config_mesh_delete(0.002, 3)
+    with tempfile.TemporaryDirectory() as root:
+        d = synth.write_dataset(root, wl)
+        st_file = po.decremental_run(d, seed=1, threads=4)
+        added_file = {tuple(int(x) - 1 for x in ln.split())
+                      for ln in open(os.path.join(d, "added_edges.mtx")) if ln.strip()}
+    st, added = po.decremental_lines(wl.G.V, wl.G.u, wl.G.v, wl.G.w, wl.H.u, wl.H.v,
+                                     wl.dels.u, wl.dels.v, seed=1, threads=4)
+    assert {tuple(x) for x in added.tolist()} == added_file
+    for k in ("added_edges", "sparse_before", "sparse_after", "no_path", "filtered", "hops"):
+        assert getattr(st, k) == getattr(st_file, k), k
+    assert st.filtered > 0
