@@ -4,8 +4,8 @@
 // H in batch order, (u,v) to u's row then (v,u) to v's row, exactly as the
 // host plan does (random_walk_incremental/gpu_graph.cuh:297-329), but without
 // a host round trip:
-//   k_compact   one workgroup: ordered compaction of the no-path edges and
-//               the list of appends keyed by row, padded to 2n
+//   k_keys      append keys by row (pad keys for edges that found a path)
+//               and the batch's append count
 //   radix sort  stable sort of appends by row (hipCUB), so each row's
 //               appends stay in batch order
 //   k_heads     first append of each row records its position; counts per row
@@ -26,57 +26,37 @@
 namespace dyg {
 namespace {
 
-constexpr int kCompactThreads = 1024;
-
-__global__ __launch_bounds__(kCompactThreads) void k_compact(
+// Sort keys of one batch: entry 2i / 2i+1 is the append to src[i] / dst[i]
+// when edge i found no path, else a pad key that sorts after every row.  The
+// values carry the entry index, so a stable sort keeps each row's appends in
+// batch order.  Also counts the batch's appends (wave-aggregated atomics)
+// and clears the other parity's counter for the next batch.
+__global__ __launch_bounds__(256) void k_keys(
     const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
-    const float* __restrict__ w, const int32_t* __restrict__ found, uint32_t n,
-    uint32_t* __restrict__ au, uint32_t* __restrict__ av, float* __restrict__ aw,
-    uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, uint32_t* __restrict__ count,
-    unsigned long long* __restrict__ added_total) {
-    __shared__ uint32_t s_scan[kCompactThreads];
-    const uint32_t tid = threadIdx.x;
-    const uint32_t chunk = (n + kCompactThreads - 1) / kCompactThreads;
-    const uint32_t b = min(n, tid * chunk), e = min(n, b + chunk);
-    uint32_t c = 0;
-    for (uint32_t i = b; i < e; ++i) c += (found[i] == -1) ? 1u : 0u;
-    s_scan[tid] = c;
-    __syncthreads();
-    // inclusive Hillis-Steele scan over the per-thread counts
-    for (uint32_t off = 1; off < kCompactThreads; off <<= 1) {
-        const uint32_t v = tid >= off ? s_scan[tid - off] : 0u;
-        __syncthreads();
-        s_scan[tid] += v;
-        __syncthreads();
+    const int32_t* __restrict__ found, uint32_t n, uint32_t* __restrict__ keys,
+    uint32_t* __restrict__ vals, uint32_t* __restrict__ count,
+    uint32_t* __restrict__ count_next, unsigned long long* __restrict__ added_total) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i == 0) *count_next = 0;
+    const bool add = i < n && found[i] == -1;
+    if (i < n) {
+        keys[2 * i] = add ? src[i] : 0xFFFFFFFFu;
+        keys[2 * i + 1] = add ? dst[i] : 0xFFFFFFFFu;
+        vals[2 * i] = 2 * i;
+        vals[2 * i + 1] = 2 * i + 1;
     }
-    uint32_t pos = s_scan[tid] - c;
-    const uint32_t total = s_scan[kCompactThreads - 1];
-    for (uint32_t i = b; i < e; ++i)
-        if (found[i] == -1) {
-            au[pos] = src[i];
-            av[pos] = dst[i];
-            aw[pos] = w[i];
-            keys[2 * pos] = src[i];
-            vals[2 * pos] = 2 * pos;
-            keys[2 * pos + 1] = dst[i];
-            vals[2 * pos + 1] = 2 * pos + 1;
-            ++pos;
-        }
-    // padding sorts after every real row
-    for (uint32_t p = 2 * total + tid; p < 2 * n; p += kCompactThreads) {
-        keys[p] = 0xFFFFFFFFu;
-        vals[p] = 0xFFFFFFFFu;
-    }
-    if (tid == 0) {
-        *count = total;
-        *added_total += total;
+    const unsigned long long b = __ballot(add);
+    if (b && __lane_id() == (uint32_t)(__ffsll((long long)b) - 1)) {
+        const uint32_t c = (uint32_t)__popcll(b);
+        atomicAdd(count, 2u * c);
+        atomicAdd(added_total, (unsigned long long)c);
     }
 }
 
 __global__ void k_heads(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ count,
                         uint32_t* __restrict__ segstart, uint32_t* __restrict__ segcnt) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= 2 * *count) return;
+    if (p >= *count) return;
     const uint32_t r = keys[p];
     if (p == 0 || keys[p - 1] != r) segstart[r] = p;
     atomicAdd(&segcnt[r], 1u);
@@ -89,7 +69,7 @@ __global__ void k_alloc(uint2* __restrict__ rows, uint4* __restrict__ adj,
                         unsigned long long* __restrict__ used, unsigned long long cap_slots,
                         uint32_t slack, uint32_t* __restrict__ err) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= 2 * *count) return;
+    if (p >= *count) return;
     const uint32_t r = keys[p];
     if (segstart[r] != p) return;
     const uint32_t k = segcnt[r];
@@ -117,19 +97,19 @@ __global__ void k_alloc(uint2* __restrict__ rows, uint4* __restrict__ adj,
 __global__ void k_write(uint4* __restrict__ adj, const uint32_t* __restrict__ keys,
                         const uint32_t* __restrict__ vals, const uint32_t* __restrict__ count,
                         uint32_t* __restrict__ segstart, uint32_t* __restrict__ segcnt,
-                        const uint32_t* __restrict__ rowbase, const uint32_t* __restrict__ au,
-                        const uint32_t* __restrict__ av, const float* __restrict__ aw) {
+                        const uint32_t* __restrict__ rowbase, const uint32_t* __restrict__ src,
+                        const uint32_t* __restrict__ dst, const float* __restrict__ w) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= 2 * *count) return;
+    if (p >= *count) return;
     const uint32_t r = keys[p];
     const uint32_t base = rowbase[r];
     const uint32_t st = segstart[r];
     if (base != 0xFFFFFFFFu) {
         const uint32_t o = vals[p];
         const uint32_t k = o >> 1;
-        const uint32_t nbr = (o & 1u) ? au[k] : av[k];
+        const uint32_t nbr = (o & 1u) ? src[k] : dst[k];
         // the neighbour's descriptor is written by k_refresh of that neighbour
-        adj[base + (p - st)] = make_uint4(nbr, __float_as_uint(aw[k]), 0u, 0u);
+        adj[base + (p - st)] = make_uint4(nbr, __float_as_uint(w[k]), 0u, 0u);
     }
 }
 
@@ -141,7 +121,7 @@ __global__ void k_refresh(const uint2* __restrict__ rows, uint4* __restrict__ ad
                           const uint32_t* __restrict__ keys, const uint32_t* __restrict__ count,
                           uint32_t* __restrict__ segstart, uint32_t* __restrict__ segcnt) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= 2 * *count) return;
+    if (p >= *count) return;
     const uint32_t r = keys[p];
     if (segstart[r] != p) return;
     segcnt[r] = 0;  // ready for the next batch
@@ -181,9 +161,6 @@ dyg_status device_edits_prepare(dyg_graph g, uint32_t max_batch, uint64_t max_in
                   hipMalloc(&d.segstart, (size_t)g->V * 4) == hipSuccess &&
                   hipMalloc(&d.segcnt, (size_t)g->V * 4) == hipSuccess &&
                   hipMalloc(&d.rowbase, (size_t)g->V * 4) == hipSuccess &&
-                  hipMalloc(&d.au, n2 / 2 * 4) == hipSuccess &&
-                  hipMalloc(&d.av, n2 / 2 * 4) == hipSuccess &&
-                  hipMalloc(&d.aw, n2 / 2 * 4) == hipSuccess &&
                   hipMalloc(&d.keys_in, n2 * 4) == hipSuccess &&
                   hipMalloc(&d.keys_out, n2 * 4) == hipSuccess &&
                   hipMalloc(&d.vals_in, n2 * 4) == hipSuccess &&
@@ -207,6 +184,13 @@ dyg_status device_edits_prepare(dyg_graph g, uint32_t max_batch, uint64_t max_in
         d.max_batch = max_batch;
         d.ready = true;
     }
+    // row capacities: uploaded when the host's are newer than the device's
+    if (!d.cap_valid) {
+        DYG_HIP(ctx, hipMemcpyAsync(d.cap, g->cap.data(), (size_t)g->V * 4,
+                                    hipMemcpyHostToDevice, ctx->stream));
+        DYG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        d.cap_valid = true;
+    }
     return DYG_OK;
 }
 
@@ -216,18 +200,15 @@ dyg_status device_edits_begin(dyg_graph g, uint32_t max_batch, uint64_t max_inse
     dyg_ctx ctx = g->ctx;
     DevEdits& d = g->dev;
     hipStream_t s = ctx->stream;
-    // capacities: the host's are current unless an earlier device run changed them
-    if (!g->mirror_stale || !d.cap_valid)
-        DYG_HIP(ctx, hipMemcpyAsync(d.cap, g->cap.data(), (size_t)g->V * 4, hipMemcpyHostToDevice,
-                                    s));
-    d.cap_valid = true;
     DYG_HIP(ctx, hipMemsetAsync(d.segcnt, 0, (size_t)g->V * 4, s));
-    // meta: [0] used slots (u64), [1] added edges (u64), [2] error flag | count (u32 x2)
+    // meta: [0] used slots (u64), [1] added edges (u64), [2] error flag (u32),
+    // [3] appends of the current batch (u32, two by batch parity)
     unsigned long long meta[4] = {g->used_slots, 0, 0, 0};
     DYG_HIP(ctx, ctx->stage.reserve(64));
     DYG_HIP(ctx, hipStreamSynchronize(s));
     std::memcpy(ctx->stage.p, meta, sizeof(meta));
     DYG_HIP(ctx, hipMemcpyAsync(d.meta, ctx->stage.p, sizeof(meta), hipMemcpyHostToDevice, s));
+    d.seq = 0;
     g->mirror_stale = true;
     return DYG_OK;
 }
@@ -241,9 +222,12 @@ dyg_status device_edits_batch(dyg_graph g, const uint32_t* src, const uint32_t* 
     unsigned long long* used = d.meta;
     unsigned long long* added = d.meta + 1;
     uint32_t* err = (uint32_t*)(d.meta + 2);
-    uint32_t* count = err + 1;
-    hipLaunchKernelGGL(k_compact, dim3(1), dim3(kCompactThreads), 0, s, src, dst, w, found, n,
-                       d.au, d.av, d.aw, d.keys_in, d.vals_in, count, added);
+    uint32_t* counts = (uint32_t*)(d.meta + 3);  // appends of this batch, by parity
+    uint32_t* count = counts + (d.seq & 1u);
+    uint32_t* count_next = counts + ((d.seq + 1u) & 1u);
+    ++d.seq;
+    hipLaunchKernelGGL(k_keys, dim3((n + 255) / 256), dim3(256), 0, s, src, dst, found, n,
+                       d.keys_in, d.vals_in, count, count_next, added);
     DYG_HIP(ctx, hipGetLastError());
     const int end_bit = std::min(32, 33 - __builtin_clz(std::max<uint32_t>(g->V, 1)));
     size_t tb = d.temp_bytes;
@@ -256,7 +240,7 @@ dyg_status device_edits_batch(dyg_graph g, const uint32_t* src, const uint32_t* 
                        d.keys_out, count, d.segstart, d.segcnt, d.rowbase, used,
                        (unsigned long long)g->cap_slots, g->slack, err);
     hipLaunchKernelGGL(k_write, dim3(blocks), dim3(256), 0, s, g->d_adj, d.keys_out, d.vals_out,
-                       count, d.segstart, d.segcnt, d.rowbase, d.au, d.av, d.aw);
+                       count, d.segstart, d.segcnt, d.rowbase, src, dst, w);
     hipLaunchKernelGGL(k_refresh, dim3(blocks), dim3(256), 0, s, g->d_rows, g->d_adj, d.keys_out,
                        count, d.segstart, d.segcnt);
     DYG_HIP(ctx, hipGetLastError());
@@ -282,13 +266,10 @@ dyg_status device_edits_end(dyg_graph g, uint64_t* added_out) {
 }
 
 void DevEdits::release() {
-    void* ps[] = {cap, segstart, segcnt, rowbase, au, av, aw, keys_in, keys_out,
-                  vals_in, vals_out, meta, temp};
+    void* ps[] = {cap, segstart, segcnt, rowbase, keys_in, keys_out, vals_in, vals_out, meta, temp};
     for (void* p : ps)
         if (p) (void)hipFree(p);
-    cap = segstart = segcnt = rowbase = au = av = keys_in = keys_out = vals_in = vals_out =
-        nullptr;
-    aw = nullptr;
+    cap = segstart = segcnt = rowbase = keys_in = keys_out = vals_in = vals_out = nullptr;
     meta = nullptr;
     temp = nullptr;
     temp_bytes = 0;

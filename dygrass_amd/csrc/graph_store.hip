@@ -212,6 +212,7 @@ dyg_status graph_apply_inserts(dyg_graph g, const uint32_t* u, const uint32_t* v
         g->adj.resize(std::min<uint64_t>(want, g->cap_slots), make_uint4(0, 0, 0, 0));
     }
 
+    g->dev.cap_valid = false;  // relocations below change row capacities on the host
     std::vector<uint32_t> dirty, touched;
     std::vector<uint32_t> slot_of(2 * (size_t)n);
     dirty.reserve(4 * (size_t)n);

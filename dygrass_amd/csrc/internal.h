@@ -66,10 +66,9 @@ struct DevEdits {
     bool cap_valid = false;  // `cap` holds the current capacities
     uint32_t max_batch = 0;
     uint32_t *cap = nullptr, *segstart = nullptr, *segcnt = nullptr, *rowbase = nullptr;
-    uint32_t *au = nullptr, *av = nullptr;
-    float* aw = nullptr;
+    uint32_t seq = 0;  // batches since device_edits_begin (selects the count slot)
     uint32_t *keys_in = nullptr, *keys_out = nullptr, *vals_in = nullptr, *vals_out = nullptr;
-    unsigned long long* meta = nullptr;  // used slots, added edges, error|count
+    unsigned long long* meta = nullptr;  // used slots, added edges, error, counts
     void* temp = nullptr;
     size_t temp_bytes = 0;
     void release();

@@ -20,6 +20,8 @@
 // Draws come from Philox4x32-10 keyed by (seed, tag, edge id, walker): each
 // 10-round block yields the next four draws of a walker, so results do not
 // depend on how walkers are mapped to lanes, workgroups or launches.
+#include <atomic>
+
 #include "internal.h"
 
 namespace dyg {
@@ -27,6 +29,31 @@ namespace {
 
 constexpr uint32_t kM0 = 0xD2511F53u, kM1 = 0xCD9E8D57u;
 constexpr uint32_t kW0 = 0x9E3779B9u, kW1 = 0xBB67AE85u;
+
+// Philox4x32-10 block `blk` of the stream (c1, c2, c3) under key (k0, k1):
+// counter (blk, c1, c2, c3).
+__device__ __forceinline__ uint4 philox_block(uint32_t blk, uint32_t c1, uint32_t c2,
+                                              uint32_t c3, uint32_t k0, uint32_t k1) {
+    uint32_t x0 = blk, x1 = c1, x2 = c2, x3 = c3, q0 = k0, q1 = k1;
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        // one 32x32->64 multiply gives both halves (a single v_mad_u64_u32)
+        const uint64_t p0 = (uint64_t)kM0 * x0, p1 = (uint64_t)kM1 * x2;
+        const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        x0 = hi1 ^ x1 ^ q0;
+        x1 = lo1;
+        x2 = hi0 ^ x3 ^ q1;
+        x3 = lo0;
+        q0 += kW0;
+        q1 += kW1;
+    }
+    return make_uint4(x0, x1, x2, x3);
+}
+
+__device__ __forceinline__ uint32_t stream_c3(uint32_t tag, uint64_t id) {
+    return (tag << 24) | ((uint32_t)(id >> 32) & 0xFFFFFFu);
+}
 
 struct Draws {
     uint32_t k0, k1, c1, c2, c3, blk;
@@ -37,25 +64,14 @@ struct Draws {
           k1((uint32_t)(seed >> 32)),
           c1(walker),
           c2((uint32_t)id),
-          c3((tag << 24) | ((uint32_t)(id >> 32) & 0xFFFFFFu)),
+          c3(stream_c3(tag, id)),
           blk(0),
           b0(0), b1(0), b2(0), b3(0),
           avail(0) {}
 
     __device__ __forceinline__ void refill() {
-        uint32_t x0 = blk, x1 = c1, x2 = c2, x3 = c3, q0 = k0, q1 = k1;
-#pragma unroll
-        for (int r = 0; r < 10; ++r) {
-            const uint32_t hi0 = __umulhi(kM0, x0), lo0 = kM0 * x0;
-            const uint32_t hi1 = __umulhi(kM1, x2), lo1 = kM1 * x2;
-            x0 = hi1 ^ x1 ^ q0;
-            x1 = lo1;
-            x2 = hi0 ^ x3 ^ q1;
-            x3 = lo0;
-            q0 += kW0;
-            q1 += kW1;
-        }
-        b0 = x0; b1 = x1; b2 = x2; b3 = x3;
+        const uint4 x = philox_block(blk, c1, c2, c3, k0, k1);
+        b0 = x.x; b1 = x.y; b2 = x.z; b3 = x.w;
         avail = 4;
         ++blk;
     }
@@ -256,15 +272,34 @@ __global__ __launch_bounds__((1 << LOGW) < 64 ? 64 : (1 << LOGW)) void k_walk(Wa
 // Work-queue form.  A workgroup of 256 lanes owns `E` consecutive edges and
 // their E*W (edge, walker) tasks, taken from an LDS counter; a lane whose
 // walker ends takes the next task at once, so a wave is never held by its
-// longest walker.  Tasks are edge-major, so the lanes that start an edge share
-// its first row.  Each lane advances one hop per loop turn; refills are
-// wave-aggregated (one LDS atomic per wave per turn).  Per-edge winner keys
-// and hop counts live in LDS (64-bit LDS max); pruning reads the edge's key.
-// Results equal the one-workgroup-per-edge kernel's bit for bit: a walker's
-// trajectory depends only on (edge, walker) and the graph.
+// longest walker.  Refills are wave-aggregated (one LDS atomic per wave per
+// turn).  Per-edge winner keys and hop counts live in LDS (64-bit LDS max);
+// pruning reads the edge's key.  Results equal the one-workgroup-per-edge
+// kernel's bit for bit: a walker's trajectory depends only on (edge, walker)
+// and the graph.
+//
+// The kernel is bound by instruction issue, not by memory, so the loop keeps
+// every lane on one code path:
+//  * first and later hops share the draw, the load phase and the tests;
+//  * each edge's source row is staged in LDS (a refill reads no global memory);
+//  * draws come from an 8-entry ring of two Philox blocks.  Block 0 of every
+//    task is computed up front by the whole workgroup into LDS; later blocks
+//    are computed every fourth turn by all active lanes at once, into the
+//    half of the ring already consumed (a walker draws exactly once per
+//    turn, so the half not holding the next draw is always spent then).  The
+//    Philox rounds thus run once per four turns for the whole wave, instead
+//    of whenever any one lane runs dry.
 // ---------------------------------------------------------------------------
 constexpr int kPoolThreads = 256;
-constexpr int kPoolMaxEdges = kPoolThreads;  // one staging lane per edge
+constexpr float kPruneSlack = 1.0f + 0x1p-20f;
+constexpr int kPoolMaxEdges = 64;     // edges owned by one workgroup, at most
+constexpr int kPoolMaxTasks = 1024;   // and (edge, walker) tasks, at most
+
+__host__ __device__ constexpr uint32_t pool_edge_cap(int logw) {
+    return (uint32_t)((kPoolMaxTasks >> logw) < 1 ? 1
+                      : (kPoolMaxTasks >> logw) > kPoolMaxEdges ? kPoolMaxEdges
+                                                                : (kPoolMaxTasks >> logw));
+}
 
 struct Lane {
     uint32_t edge;  // local edge index
@@ -275,23 +310,63 @@ struct Lane {
     int step;
 };
 
+// Draw ring: r[d & 7] holds draw d of the walker; block b sits in half b & 1.
+struct Ring {
+    uint32_t r[8];
+    uint32_t q;    // index of the next draw (mod 8)
+    uint32_t blk;  // next block to compute
+    uint32_t c2, c3;
+
+    __device__ __forceinline__ void start(uint4 b0, uint64_t id, uint32_t tag) {
+        r[0] = b0.x; r[1] = b0.y; r[2] = b0.z; r[3] = b0.w;
+        q = 0;
+        blk = 1;
+        c2 = (uint32_t)id;
+        c3 = stream_c3(tag, id);
+    }
+    __device__ __forceinline__ void append(uint32_t walker, uint32_t k0, uint32_t k1) {
+        const uint4 x = philox_block(blk, walker, c2, c3, k0, k1);
+        const bool hi = (blk & 1u) != 0u;
+        r[0] = hi ? r[0] : x.x; r[1] = hi ? r[1] : x.y;
+        r[2] = hi ? r[2] : x.z; r[3] = hi ? r[3] : x.w;
+        r[4] = hi ? x.x : r[4]; r[5] = hi ? x.y : r[5];
+        r[6] = hi ? x.z : r[6]; r[7] = hi ? x.w : r[7];
+        ++blk;
+    }
+    __device__ __forceinline__ uint32_t next() {
+        const bool b0 = (q & 1u) != 0u, b1 = (q & 2u) != 0u, b2 = (q & 4u) != 0u;
+        const uint32_t a0 = b0 ? r[1] : r[0], a1 = b0 ? r[3] : r[2];
+        const uint32_t a2 = b0 ? r[5] : r[4], a3 = b0 ? r[7] : r[6];
+        const uint32_t l = b1 ? a1 : a0, h = b1 ? a3 : a2;
+        q = (q + 1u) & 7u;
+        return b2 ? h : l;
+    }
+};
+
 template <int LOGW, bool INSERT, bool PRUNE>
 __global__ __launch_bounds__(kPoolThreads) void k_walk_pool(WalkLaunch a, uint32_t E) {
     constexpr uint32_t W = 1u << LOGW;
-    __shared__ unsigned long long s_key[kPoolMaxEdges];
-    __shared__ uint32_t s_hops[kPoolMaxEdges];
-    __shared__ uint32_t s_src[kPoolMaxEdges], s_dst[kPoolMaxEdges];
-    __shared__ float s_w[kPoolMaxEdges];
-    __shared__ unsigned long long s_eid[kPoolMaxEdges];
+    constexpr uint32_t kE = pool_edge_cap(LOGW);
+    constexpr uint32_t kT = kE * W;
+    constexpr uint32_t kTag = INSERT ? kTagInsert : kTagDelete;
+    __shared__ unsigned long long s_key[kE];
+    __shared__ uint32_t s_hops[kE];
+    __shared__ uint32_t s_src[kE], s_dst[kE];
+    __shared__ float s_w[kE];
+    __shared__ unsigned long long s_eid[kE];
+    __shared__ uint2 s_row[kE];  // source rows: every walker of an edge starts there
+    __shared__ uint4 s_blk0[kT];  // draw block 0 of every task
     __shared__ uint32_t s_next;
     __shared__ uint32_t s_cnt[3];
 
     const uint32_t e0 = blockIdx.x * E;
     const uint32_t ne = min(E, a.n - e0);
     const uint32_t tid = threadIdx.x;
+    const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
     if (tid < ne) {
         const uint32_t e = e0 + tid;
         s_src[tid] = a.src[e];
+        s_row[tid] = ld(a.rows, a.src[e]);
         s_dst[tid] = a.dst[e];
         s_w[tid] = INSERT ? a.w[e] : 0.0f;
         s_eid[tid] = a.eid ? a.eid[e] : a.eid_base + e;
@@ -304,24 +379,22 @@ __global__ __launch_bounds__(kPoolThreads) void k_walk_pool(WalkLaunch a, uint32
         s_cnt[0] = s_cnt[1] = s_cnt[2] = 0;
     }
     __syncthreads();
-
     const uint32_t total = ne * W;
+    for (uint32_t t = tid; t < total; t += kPoolThreads) {
+        const uint64_t id = s_eid[t >> LOGW];
+        s_blk0[t] = philox_block(0u, t & (W - 1u), (uint32_t)id, stream_c3(kTag, id), k0, k1);
+    }
+    __syncthreads();
+
     const uint32_t lane = __lane_id();
     bool active = false;
     bool drained = false;
     Lane L{};
-    Draws rng(0, 0, 0, 0);
+    Ring rng{};
     uint32_t hops = 0, redir = 0, leaf = 0;  // this lane's running counts
     uint32_t my_hops = 0;                    // current walker's draws
 
-    uint2 pending = make_uint2(0, 0);  // source row of a lane refilled last turn
-    for (;;) {
-        // ---- a lane refilled last turn adopts its source row: the load was
-        // issued a full turn ago, so this read does not stall the wave ----
-        if (active && L.step == 0) {
-            L.row = pending;
-            L.step = 1;
-        }
+    for (uint32_t turn = 0;; ++turn) {
         // ---- refill idle lanes (one LDS atomic per wave) ----
         const unsigned long long need = __ballot(!active && !drained);
         if (need) {
@@ -337,13 +410,13 @@ __global__ __launch_bounds__(kPoolThreads) void k_walk_pool(WalkLaunch a, uint32
                     L.edge = task >> LOGW;
                     L.walker = task & (W - 1u);
                     L.cur = s_src[L.edge];
-                    pending = ld(a.rows, L.cur);  // consumed next turn
+                    L.row = s_row[L.edge];
                     L.t = s_dst[L.edge];
                     L.we = s_w[L.edge];
                     L.prev = 0;
                     L.R = 0.0f;
-                    L.step = 0;
-                    rng = Draws(a.seed, INSERT ? kTagInsert : kTagDelete, s_eid[L.edge], L.walker);
+                    L.step = 1;
+                    rng.start(s_blk0[task], s_eid[L.edge], kTag);
                     my_hops = 0;
                     active = true;
                 } else {
@@ -352,52 +425,43 @@ __global__ __launch_bounds__(kPoolThreads) void k_walk_pool(WalkLaunch a, uint32
             }
         }
         if (!__any(active)) break;
-        if (!active || L.step == 0) continue;
+        // ---- every fourth turn: the next draw block of every active lane ----
+        if ((turn & 3u) == 0u && active) rng.append(L.walker, k0, k1);
+        if (!active) continue;
 
-        // ---- one hop ----
+        // ---- one hop: first and later hops share one path ----
         bool done = false;
         int found = -1;
         const uint2 row = L.row;
-        if (L.step == 1) {
-            if (row.y == 0) {
-                done = true;  // isolated source
-            } else {
-                const uint32_t i = rng.next() % row.y;
-                ++my_hops;
-                const uint4 sl = ld(a.adj, row.x + i);
-                L.R = 1.0f / __uint_as_float(sl.y);
-                if (sl.x == L.t) {
-                    found = 1;
-                    done = true;
-                } else {
-                    L.prev = L.cur;
-                    L.cur = sl.x;
-                    L.row = make_uint2(sl.z, sl.w);
-                    L.step = 2;
-                    if (L.step >= (int)a.max_steps) done = true;
-                }
-            }
-        } else if (row.y <= 1) {
-            ++leaf;
+        const bool first = (L.step == 1);
+        // first hop: any slot of the source row; later hops: the first deg-1
+        // slots, with the previous vertex redirected to the last slot
+        const uint32_t span = first ? row.y : row.y - 1u;
+        if (row.y == 0u || span == 0u) {
+            leaf += first ? 0u : 1u;  // isolated source, or a leaf
             done = true;
         } else {
-            const uint32_t i = rng.next() % (row.y - 1);
+            const uint32_t i = rng.next() % span;
             ++my_hops;
             const uint4 x = ld(a.adj, row.x + i);
-            const uint4 y = ld(a.adj, row.x + row.y - 1);
-            const bool back = (x.x == L.prev);
+            const uint4 y = ld(a.adj, row.x + row.y - 1u);
+            const bool back = !first && (x.x == L.prev);
             redir += back ? 1u : 0u;
             const uint4 sl = back ? y : x;
-            L.R += 1.0f / __uint_as_float(sl.y);
-            if (INSERT && L.R * L.we >= a.thr) {
+            L.R += 1.0f / __uint_as_float(sl.y);  // L.R starts at 0: first hop gives 1/w
+            if (INSERT && !first && L.R * L.we >= a.thr) {
                 done = true;
             } else if (sl.x == L.t) {
                 found = L.step;
                 done = true;
             } else {
-                if (PRUNE) {
+                if (PRUNE && !first) {
+                    // conservative: prune only when fl(1/R) < c is certain;
+                    // rcp is within 1 ulp of 1/R, the margin covers 2^-20
                     const uint32_t bb = (uint32_t)(s_key[L.edge] >> 32);
-                    if (bb != 0u && __float_as_uint(1.0f / L.R) + 1u < bb) done = true;
+                    if (bb != 0u &&
+                        __builtin_amdgcn_rcpf(L.R) * kPruneSlack < __uint_as_float(bb - 1u))
+                        done = true;
                 }
                 L.prev = L.cur;
                 L.cur = sl.x;
@@ -513,18 +577,59 @@ hipError_t launch_walk_t(const WalkLaunch& a, hipStream_t s) {
     return hipGetLastError();
 }
 
-// Edges per workgroup of the work-queue kernel: about 16 walkers per lane.
-inline uint32_t pool_edges(uint32_t W, uint32_t requested) {
-    uint32_t E = requested ? requested : (uint32_t)(kPoolThreads * 16) / W;
+// Workgroups of the work-queue kernel that the device holds at once (CUs x
+// resident workgroups per CU), computed once per device.
+template <int LOGW, bool INSERT, bool PRUNE>
+uint32_t pool_slots() {
+    static std::atomic<uint32_t> cached[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 2048;
+    uint32_t v = cached[dev].load(std::memory_order_relaxed);
+    if (v) return v;
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void*>(&k_walk_pool<LOGW, INSERT, PRUNE>),
+            kPoolThreads, 0) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        per_cu <= 0 || cus <= 0)
+        return 2048;
+    v = (uint32_t)(per_cu * cus);
+    cached[dev].store(v, std::memory_order_relaxed);
+    return v;
+}
+
+// Edges per workgroup: by default the fewest that keep the launch within one
+// round of resident workgroups (so no workgroup starts late and adds its own
+// tail), and at least one task per lane.
+inline uint32_t pool_edges(uint32_t W, uint32_t requested, uint32_t n, uint32_t slots) {
+    uint32_t cap = (uint32_t)kPoolMaxTasks / W;
+    if (cap < 1) cap = 1;
+    if (cap > (uint32_t)kPoolMaxEdges) cap = kPoolMaxEdges;
+    uint32_t E = requested;
+    if (!E) {
+        E = (uint32_t)(((uint64_t)n + slots - 1) / slots);
+        const uint32_t min_e = (kPoolThreads + W - 1) / W;
+        if (E < min_e) E = min_e;
+    }
     if (E < 1) E = 1;
-    if (E > (uint32_t)kPoolMaxEdges) E = kPoolMaxEdges;
+    if (E > cap) E = cap;
     return E;
 }
 
 template <bool INSERT, bool PRUNE>
 hipError_t launch_pool_t(const WalkLaunch& a, hipStream_t s) {
     if (a.n == 0) return hipSuccess;
-    const uint32_t E = pool_edges(a.walkers, a.edges_per_block);
+    uint32_t slots = 2048;
+    switch (a.walkers) {
+#define DYG_CASE(LW) \
+    case (1u << LW): slots = pool_slots<LW, INSERT, PRUNE>(); break;
+        DYG_CASE(0) DYG_CASE(1) DYG_CASE(2) DYG_CASE(3) DYG_CASE(4) DYG_CASE(5)
+        DYG_CASE(6) DYG_CASE(7) DYG_CASE(8) DYG_CASE(9) DYG_CASE(10)
+#undef DYG_CASE
+        default:
+            return hipErrorInvalidValue;
+    }
+    const uint32_t E = pool_edges(a.walkers, a.edges_per_block, a.n, slots);
     const dim3 grid((a.n + E - 1) / E);
     switch (a.walkers) {
 #define DYG_CASE(LW)                                                                        \
