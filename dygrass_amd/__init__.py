@@ -10,18 +10,20 @@ from .core import (  # noqa: F401
     DeviceArray,
     DeviceGraph,
     DualGraph,
+    Stream,
     WalkResult,
     csr_from_edges,
     incremental_update,
     incremental_update_dev,
     params,
+    stream_params,
     walk_decremental,
     walk_heuristic,
     walk_incremental,
 )
 
 __all__ = [
-    "Context", "DeviceArray", "DeviceGraph", "DualGraph", "WalkResult", "csr_from_edges", "incremental_update", "incremental_update_dev",
+    "Context", "DeviceArray", "DeviceGraph", "DualGraph", "Stream", "stream_params", "WalkResult", "csr_from_edges", "incremental_update", "incremental_update_dev",
     "params", "walk_decremental", "walk_heuristic", "walk_incremental", "DygError", "WalkParams",
     "WalkStats", "RunStats", "LIB_PATH", "BIN_DIR",
 ]

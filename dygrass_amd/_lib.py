@@ -49,6 +49,24 @@ class RunStats(C.Structure):
                 ("walk_ms", C.c_double), ("update_ms", C.c_double), ("total_ms", C.c_double)]
 
 
+class StreamParams(C.Structure):
+    _fields_ = [("ins", WalkParams), ("del_", WalkParams), ("heur_walkers", C.c_uint32),
+                ("max_batch", C.c_uint32), ("max_inserts", C.c_uint64), ("slack", C.c_uint32),
+                ("flags", C.c_uint32)]
+
+
+class StreamStats(C.Structure):
+    _fields_ = [(k, C.c_uint64) for k in ("updates", "weight_updates", "found", "added",
+                                          "skipped", "g_only", "recovered", "heuristic",
+                                          "invalid", "candidates", "hops")] + [
+        ("walk_ms", C.c_double), ("total_ms", C.c_double)]
+
+
+DYG_STREAM_INSERT, DYG_STREAM_DELETE = 0, 1
+OUTCOMES = {1: "weight", 2: "found", 3: "added", 4: "skip", 5: "g_only", 6: "recovered",
+            7: "heuristic", 8: "invalid"}
+
+
 class DygError(RuntimeError):
     def __init__(self, status: int, msg: str):
         super().__init__(f"{STATUS.get(status, status)}: {msg}")
@@ -105,6 +123,13 @@ _SIGS = {
     "dyg_dual_added_edges": (C.c_int32, [vp, u32p, u32p]),
     "dyg_dual_dense": (vp, [vp]),
     "dyg_dual_free": (None, [vp]),
+    "dyg_stream_create": (C.c_int32, [vp, C.c_uint32, u64p, u32p, f32p, u64p, u32p, f32p,
+                                      C.POINTER(StreamParams), C.POINTER(vp)]),
+    "dyg_stream_apply": (C.c_int32, [vp, C.c_int, u32p, u32p, f32p, C.c_uint32, i32p,
+                                     C.POINTER(StreamStats)]),
+    "dyg_stream_graph": (vp, [vp, C.c_int]),
+    "dyg_stream_position": (C.c_uint64, [vp]),
+    "dyg_stream_free": (None, [vp]),
     "dyg_mtx_read_graph": (C.c_int32, [C.c_char_p, C.c_int, C.c_int, C.POINTER(vp)]),
     "dyg_mtx_read_edges": (C.c_int32, [C.c_char_p, C.c_int, C.POINTER(vp)]),
     "dyg_mtx_nv": (C.c_uint32, [vp]),

@@ -199,9 +199,17 @@ class DeviceGraph:
                                          ptr(hit, L.i8p)), self.ctx.handle)
         return hit[: len(u)]
 
+    @classmethod
+    def borrowed(cls, ctx: Context, handle, V: int, owner) -> "DeviceGraph":
+        """A view of a graph owned by another object (kept alive by `owner`)."""
+        g = cls.__new__(cls)
+        g.ctx, g._lib, g.V, g.handle, g._owner = ctx, ctx._lib, int(V), C.c_void_p(handle), owner
+        return g
+
     def close(self) -> None:
         if self.handle:
-            self._lib.dyg_graph_free(self.handle)
+            if getattr(self, "_owner", None) is None:
+                self._lib.dyg_graph_free(self.handle)
             self.handle = None
 
     def __del__(self):
@@ -360,6 +368,64 @@ class DualGraph:
     def close(self) -> None:
         if self.handle:
             self._lib.dyg_dual_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def stream_params(w_ins: int = 256, w_del: int = 512, threshold: float = 100.0, seed: int = 0,
+                  max_batch: int = 50000, max_inserts: int = 0, heur_walkers: int = 16,
+                  slack: int = 2, max_steps: int = 100) -> L.StreamParams:
+    sp = L.StreamParams()
+    sp.ins = params(w_ins, max_steps, threshold, seed)
+    sp.del_ = params(w_del, max_steps, 0.0, seed)
+    sp.heur_walkers, sp.max_batch, sp.max_inserts = heur_walkers, max_batch, max_inserts
+    sp.slack, sp.flags = slack, 0
+    return sp
+
+
+class Stream:
+    """Mixed insertion/deletion update stream on device-resident G and H
+    (dyg_stream; dynamic_update/main.cu:92-124 batch loop semantics)."""
+
+    def __init__(self, ctx: Context, V: int, g_edges, h_edges, sp: L.StreamParams):
+        self.ctx = ctx
+        self._lib = ctx._lib
+        self.V = int(V)
+        grp, gcol, gw = csr_from_edges(V, *g_edges)
+        hrp, hcol, hw = csr_from_edges(V, *h_edges)
+        h = C.c_void_p()
+        check(self._lib.dyg_stream_create(ctx.handle, self.V, ptr(grp, L.u64p), ptr(gcol, L.u32p),
+                                          ptr(gw, L.f32p), ptr(hrp, L.u64p), ptr(hcol, L.u32p),
+                                          ptr(hw, L.f32p), C.byref(sp), C.byref(h)), ctx.handle)
+        self.handle = h
+        self.G = DeviceGraph.borrowed(ctx, self._lib.dyg_stream_graph(h, 0), V, self)
+        self.H = DeviceGraph.borrowed(ctx, self._lib.dyg_stream_graph(h, 1), V, self)
+
+    @property
+    def position(self) -> int:
+        return self._lib.dyg_stream_position(self.handle)
+
+    def apply(self, op: str, u, v, w=None) -> tuple[np.ndarray, dict]:
+        """One batch: op "ins" or "del".  Returns per-update outcome codes and stats."""
+        u, v = _u32(u), _u32(v)
+        n = len(u)
+        ww = _f32(w) if w is not None else None
+        out = np.zeros(max(n, 1), np.int32)
+        st = L.StreamStats()
+        code = L.DYG_STREAM_INSERT if op == "ins" else L.DYG_STREAM_DELETE
+        check(self._lib.dyg_stream_apply(self.handle, code, ptr(u, L.u32p), ptr(v, L.u32p),
+                                         ptr(ww, L.f32p), n, ptr(out, L.i32p), C.byref(st)),
+              self.ctx.handle)
+        return out[:n], {k: getattr(st, k) for k, _ in L.StreamStats._fields_}
+
+    def close(self) -> None:
+        if self.handle:
+            self._lib.dyg_stream_free(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -16,6 +16,13 @@
 //               neighbours' rows) gets the row's new (first slot, degree)
 // Slot order is what the walks index, and it equals the host plan's; the
 // physical position of a relocated row may differ, which no walk can see.
+//
+// Deletions (random_walk_decremental/gpu_graph.cuh:338-400 semantics) use the
+// same keys, sort and heads; then one thread per touched row removes that
+// row's entries in batch order, each by swap-with-last, and k_refresh
+// publishes the new degrees.  Edge lists may carry their length in device
+// memory (n_dev), so a batch whose size is decided on the device needs no
+// host round trip.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -26,19 +33,22 @@
 namespace dyg {
 namespace {
 
-// Sort keys of one batch: entry 2i / 2i+1 is the append to src[i] / dst[i]
-// when edge i found no path, else a pad key that sorts after every row.  The
+// Sort keys of one batch: entry 2i / 2i+1 is the edit of src[i]'s / dst[i]'s
+// row when found[i] == match (-1: no path, append), else a pad key that sorts
+// after every row.  The
 // values carry the entry index, so a stable sort keeps each row's appends in
 // batch order.  Also counts the batch's appends (wave-aggregated atomics)
 // and clears the other parity's counter for the next batch.
 __global__ __launch_bounds__(256) void k_keys(
     const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
-    const int32_t* __restrict__ found, uint32_t n, uint32_t* __restrict__ keys,
+    const int32_t* __restrict__ found, int32_t match, uint32_t n,
+    const uint32_t* __restrict__ n_dev, uint32_t* __restrict__ keys,
     uint32_t* __restrict__ vals, uint32_t* __restrict__ count,
     uint32_t* __restrict__ count_next, unsigned long long* __restrict__ added_total) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i == 0) *count_next = 0;
-    const bool add = i < n && found[i] == -1;
+    const uint32_t nn = n_dev ? min(*n_dev, n) : n;
+    const bool add = i < nn && found[i] == match;
     if (i < n) {
         keys[2 * i] = add ? src[i] : 0xFFFFFFFFu;
         keys[2 * i + 1] = add ? dst[i] : 0xFFFFFFFFu;
@@ -111,6 +121,37 @@ __global__ void k_write(uint4* __restrict__ adj, const uint32_t* __restrict__ ke
         // the neighbour's descriptor is written by k_refresh of that neighbour
         adj[base + (p - st)] = make_uint4(nbr, __float_as_uint(w[k]), 0u, 0u);
     }
+}
+
+// One thread per touched row: remove the row's entries in batch order, each
+// by swap-with-last (the last live slot moves into the hole; its copy of its
+// neighbour's row descriptor moves with it).
+__global__ void k_del_rows(uint2* __restrict__ rows, uint4* __restrict__ adj,
+                           const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                           const uint32_t* __restrict__ count,
+                           const uint32_t* __restrict__ segstart,
+                           const uint32_t* __restrict__ segcnt, const uint32_t* __restrict__ src,
+                           const uint32_t* __restrict__ dst, uint32_t* __restrict__ err) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= *count) return;
+    const uint32_t r = keys[p];
+    if (segstart[r] != p) return;
+    const uint32_t k = segcnt[r];
+    uint2 row = rows[r];
+    for (uint32_t e = p; e < p + k; ++e) {
+        const uint32_t o = vals[e];
+        const uint32_t nbr = (o & 1u) ? src[o >> 1] : dst[o >> 1];
+        uint32_t q = 0;
+        while (q < row.y && adj[row.x + q].x != nbr) ++q;
+        if (q == row.y) {  // not present: the caller's lists were inconsistent
+            atomicOr(err, 2u);
+            continue;
+        }
+        const uint32_t last = row.y - 1u;
+        if (q != last) adj[row.x + q] = adj[row.x + last];
+        row.y = last;
+    }
+    rows[r] = row;
 }
 
 // One thread per touched row r (its first append): rewrite r's descriptor in
@@ -201,33 +242,39 @@ dyg_status device_edits_begin(dyg_graph g, uint32_t max_batch, uint64_t max_inse
     DevEdits& d = g->dev;
     hipStream_t s = ctx->stream;
     DYG_HIP(ctx, hipMemsetAsync(d.segcnt, 0, (size_t)g->V * 4, s));
-    // meta: [0] used slots (u64), [1] added edges (u64), [2] error flag (u32),
-    // [3] appends of the current batch (u32, two by batch parity)
-    unsigned long long meta[4] = {g->used_slots, 0, 0, 0};
+    // meta: [0] used slots (u64), [1] appended edges (u64), [2] error flags
+    // (u32), [3] entries of the current batch (u32, two by batch parity),
+    // [4] deleted edges (u64)
+    unsigned long long meta[5] = {g->used_slots, 0, 0, 0, 0};
     DYG_HIP(ctx, ctx->stage.reserve(64));
     DYG_HIP(ctx, hipStreamSynchronize(s));
     std::memcpy(ctx->stage.p, meta, sizeof(meta));
     DYG_HIP(ctx, hipMemcpyAsync(d.meta, ctx->stage.p, sizeof(meta), hipMemcpyHostToDevice, s));
     d.seq = 0;
+    d.base_nslots = g->nslots;
     g->mirror_stale = true;
     return DYG_OK;
 }
 
-dyg_status device_edits_batch(dyg_graph g, const uint32_t* src, const uint32_t* dst,
-                              const float* w, const int32_t* found, uint32_t n) {
-    if (n == 0) return DYG_OK;
+namespace {
+// keys, sort and heads of one batch of row edits; returns the count pointer
+dyg_status edit_keys(dyg_graph g, const uint32_t* src, const uint32_t* dst, const int32_t* found,
+                     int32_t match, uint32_t n, const uint32_t* n_dev, uint32_t** count_out) {
     dyg_ctx ctx = g->ctx;
     DevEdits& d = g->dev;
     hipStream_t s = ctx->stream;
-    unsigned long long* used = d.meta;
+    if (n > d.max_batch) {
+        set_error(ctx, "device edits: batch larger than reserved");
+        return DYG_ERR_ARG;
+    }
     unsigned long long* added = d.meta + 1;
-    uint32_t* err = (uint32_t*)(d.meta + 2);
-    uint32_t* counts = (uint32_t*)(d.meta + 3);  // appends of this batch, by parity
+    uint32_t* counts = (uint32_t*)(d.meta + 3);  // entries of this batch, by parity
     uint32_t* count = counts + (d.seq & 1u);
     uint32_t* count_next = counts + ((d.seq + 1u) & 1u);
     ++d.seq;
-    hipLaunchKernelGGL(k_keys, dim3((n + 255) / 256), dim3(256), 0, s, src, dst, found, n,
-                       d.keys_in, d.vals_in, count, count_next, added);
+    hipLaunchKernelGGL(k_keys, dim3((n + 255) / 256), dim3(256), 0, s, src, dst, found, match, n,
+                       n_dev, d.keys_in, d.vals_in, count, count_next,
+                       match == -1 ? added : d.meta + 4);
     DYG_HIP(ctx, hipGetLastError());
     const int end_bit = std::min(32, 33 - __builtin_clz(std::max<uint32_t>(g->V, 1)));
     size_t tb = d.temp_bytes;
@@ -236,6 +283,44 @@ dyg_status device_edits_batch(dyg_graph g, const uint32_t* src, const uint32_t* 
     const uint32_t blocks = (2 * n + 255) / 256;
     hipLaunchKernelGGL(k_heads, dim3(blocks), dim3(256), 0, s, d.keys_out, count, d.segstart,
                        d.segcnt);
+    DYG_HIP(ctx, hipGetLastError());
+    *count_out = count;
+    return DYG_OK;
+}
+}  // namespace
+
+dyg_status device_deletes_batch(dyg_graph g, const uint32_t* src, const uint32_t* dst,
+                                const int32_t* flag, uint32_t n, const uint32_t* n_dev) {
+    if (n == 0) return DYG_OK;
+    dyg_ctx ctx = g->ctx;
+    DevEdits& d = g->dev;
+    hipStream_t s = ctx->stream;
+    uint32_t* count = nullptr;
+    dyg_status st = edit_keys(g, src, dst, flag, 1, n, n_dev, &count);
+    if (st != DYG_OK) return st;
+    uint32_t* err = (uint32_t*)(d.meta + 2);
+    const uint32_t blocks = (2 * n + 255) / 256;
+    hipLaunchKernelGGL(k_del_rows, dim3(blocks), dim3(256), 0, s, g->d_rows, g->d_adj, d.keys_out,
+                       d.vals_out, count, d.segstart, d.segcnt, src, dst, err);
+    hipLaunchKernelGGL(k_refresh, dim3(blocks), dim3(256), 0, s, g->d_rows, g->d_adj, d.keys_out,
+                       count, d.segstart, d.segcnt);
+    DYG_HIP(ctx, hipGetLastError());
+    return DYG_OK;
+}
+
+dyg_status device_edits_batch(dyg_graph g, const uint32_t* src, const uint32_t* dst,
+                              const float* w, const int32_t* found, uint32_t n,
+                              const uint32_t* n_dev) {
+    if (n == 0) return DYG_OK;
+    dyg_ctx ctx = g->ctx;
+    DevEdits& d = g->dev;
+    hipStream_t s = ctx->stream;
+    unsigned long long* used = d.meta;
+    uint32_t* err = (uint32_t*)(d.meta + 2);
+    uint32_t* count = nullptr;
+    dyg_status st = edit_keys(g, src, dst, found, -1, n, n_dev, &count);
+    if (st != DYG_OK) return st;
+    const uint32_t blocks = (2 * n + 255) / 256;
     hipLaunchKernelGGL(k_alloc, dim3(blocks), dim3(256), 0, s, g->d_rows, g->d_adj, d.cap,
                        d.keys_out, count, d.segstart, d.segcnt, d.rowbase, used,
                        (unsigned long long)g->cap_slots, g->slack, err);
@@ -250,17 +335,21 @@ dyg_status device_edits_batch(dyg_graph g, const uint32_t* src, const uint32_t* 
 // Waits for the stream, then brings the host's view of the arena up to date.
 dyg_status device_edits_end(dyg_graph g, uint64_t* added_out) {
     dyg_ctx ctx = g->ctx;
-    unsigned long long meta[4];
+    unsigned long long meta[5];
     DYG_HIP(ctx, hipMemcpyAsync(meta, g->dev.meta, sizeof(meta), hipMemcpyDeviceToHost,
                                 ctx->stream));
     DYG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     const uint32_t err = (uint32_t)meta[2];
     g->used_slots = meta[0];
-    g->nslots += 2 * meta[1];
+    g->nslots = g->dev.base_nslots + 2 * meta[1] - 2 * meta[4];
     if (added_out) *added_out = meta[1];
-    if (err) {
+    if (err & 1u) {
         set_error(ctx, "device edits: slot arena exhausted");
         return DYG_ERR_NOMEM;
+    }
+    if (err & 2u) {
+        set_error(ctx, "device edits: deletion of an edge the graph does not hold");
+        return DYG_ERR_STATE;
     }
     return DYG_OK;
 }

@@ -67,6 +67,7 @@ struct DevEdits {
     uint32_t max_batch = 0;
     uint32_t *cap = nullptr, *segstart = nullptr, *segcnt = nullptr, *rowbase = nullptr;
     uint32_t seq = 0;  // batches since device_edits_begin (selects the count slot)
+    uint64_t base_nslots = 0;  // live slots at device_edits_begin
     uint32_t *keys_in = nullptr, *keys_out = nullptr, *vals_in = nullptr, *vals_out = nullptr;
     unsigned long long* meta = nullptr;  // used slots, added edges, error, counts
     void* temp = nullptr;
@@ -131,7 +132,14 @@ dyg_status device_edits_begin(dyg_graph g, uint32_t max_batch, uint64_t max_inse
 // timed or latency-critical run.
 dyg_status device_edits_prepare(dyg_graph g, uint32_t max_batch, uint64_t max_inserts);
 dyg_status device_edits_batch(dyg_graph g, const uint32_t* src, const uint32_t* dst,
-                              const float* w, const int32_t* found, uint32_t n);
+                              const float* w, const int32_t* found, uint32_t n,
+                              const uint32_t* n_dev = nullptr);
+// Swap-with-last removal of (src[i], dst[i]) for every i < n (or < *n_dev)
+// with flag[i] == 1, in batch order per row.  Each edge must be present.
+dyg_status device_deletes_batch(dyg_graph g, const uint32_t* src, const uint32_t* dst,
+                                const int32_t* flag, uint32_t n, const uint32_t* n_dev);
+// Waits for the stream; updates the host's slot counts; reports the edges
+// appended since begin().  The pipeline stays usable afterwards.
 dyg_status device_edits_end(dyg_graph g, uint64_t* added);
 
 // Walk launchers (walk_kernels.hip).  All pointers are device pointers.
@@ -156,12 +164,13 @@ struct WalkLaunch {
     uint32_t* hops;          // optional, per edge
     uint32_t* path;          // deletions: [n*max_steps]
     unsigned long long* stats;  // optional [3]: hops, redirects, leaf stops
+    const uint32_t* n_dev;   // optional: list length in device memory (<= n; grid sized by n)
 };
 hipError_t launch_walk_incremental(const WalkLaunch& a, hipStream_t s);
 hipError_t launch_walk_decremental(const WalkLaunch& a, hipStream_t s);
 hipError_t launch_heuristic(const uint2* rows, const uint4* adj, const uint32_t* endpoint,
                             const uint64_t* key, uint32_t n, uint32_t walkers, uint64_t seed,
-                            uint32_t* path3, hipStream_t s);
+                            uint32_t* path3, hipStream_t s, const uint32_t* n_dev = nullptr);
 // Delta layout (uint32 words): writes (slot, col, wbits, nstart, ndeg), then
 // rows (v, start, deg).
 hipError_t launch_apply_delta(uint2* rows, uint4* adj, const uint32_t* delta, uint32_t n_write,

@@ -1,0 +1,668 @@
+// stream.hip -- mixed insertion/deletion update stream over a device-resident
+// dense graph G and its sparsifier H (C ABI dyg_stream_*, include/dygrass.h).
+//
+// Semantics follow dynamic_update/gpu_graph.cuh:361-439 (preprocessStreamEdges),
+// :451-465 (weight update) and :620-700 (updateSparsiferFromResult), with the
+// walks, block reduction and heuristic of the plain updaters
+// (random_walk_incremental/main.cu:70-211, random_walk_decremental/main.cu:
+// 77-290, random_walk_decremental/gpu_graph.cuh:509-615), per batch:
+//
+// insertion batch   (a,b) in G: weight += w in G and in H when there;
+//                   else appended to G and kept; kept edges walk in H (state
+//                   at batch start, after the batch's weight updates); those
+//                   without a path are appended to H in batch order.
+// deletion batch    (a,b) not in G: skipped; else removed from G and, when in
+//                   H, from H and kept; kept edges walk in G after all of the
+//                   batch's deletions, from min(a,b) to max(a,b); every hop of a
+//                   found path that H lacks, then every heuristic edge (both
+//                   endpoints, 16 walkers, two hops) that H lacks, is appended
+//                   to H with G's weight, in that order.
+// Duplicates inside a batch act in batch order (a second insertion of a new
+// edge is a weight update of it; a second deletion is skipped).  Self-loops
+// are rejected per update.  G and H must be simple graphs with H inside G.
+//
+// Every step runs on the device: the batch's keys are sorted (stable, 64-bit
+// (min,max) keys) so that one thread per distinct edge sees its occurrences
+// in batch order; membership and positions come from scanning the smaller
+// row; row edits use the append and swap-with-last pipelines of
+// device_edits.hip; the walks use the work-queue kernels with the list
+// length in device memory.  An insertion batch never waits for the host
+// before its end; a deletion batch reads three counts once (after its walks)
+// to size the recovery step.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <vector>
+
+#include "internal.h"
+
+namespace dyg {
+namespace {
+
+constexpr int32_t kOutWeight = 1, kOutFound = 2, kOutAdded = 3, kOutSkip = 4, kOutGOnly = 5,
+                  kOutRecovered = 6, kOutHeuristic = 7, kOutInvalid = 8;
+constexpr unsigned long long kPadKey = ~0ull;
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(
+               std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+__device__ __forceinline__ int row_find(const uint2* __restrict__ rows,
+                                        const uint4* __restrict__ adj, uint32_t a, uint32_t b) {
+    const uint2 r = rows[a];
+    for (uint32_t q = 0; q < r.y; ++q)
+        if (adj[r.x + q].x == b) return (int)q;
+    return -1;
+}
+
+// membership through the shorter of the two rows
+__device__ __forceinline__ bool has_edge(const uint2* __restrict__ rows,
+                                         const uint4* __restrict__ adj, uint32_t a, uint32_t b) {
+    return rows[a].y <= rows[b].y ? row_find(rows, adj, a, b) >= 0
+                                  : row_find(rows, adj, b, a) >= 0;
+}
+
+__global__ void k_batch_keys(const uint32_t* __restrict__ u, const uint32_t* __restrict__ v,
+                             uint32_t n, unsigned long long* __restrict__ key,
+                             uint32_t* __restrict__ idx) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t a = min(u[i], v[i]), b = max(u[i], v[i]);
+    key[i] = a == b ? kPadKey : ((unsigned long long)a << 32) | b;
+    idx[i] = i;
+}
+
+// One thread per distinct key (first position of its run in the sorted order).
+// mode[p] at the head: 0 nothing more, 1 existing edge (weight updates for
+// all occurrences; bit 2 set when H holds it too), 2 new edge with further
+// occurrences (weight updates after the append).
+__global__ void k_ins_groups(const unsigned long long* __restrict__ key,
+                             const uint32_t* __restrict__ idx, uint32_t n,
+                             const uint2* __restrict__ grows, const uint4* __restrict__ gadj,
+                             const uint2* __restrict__ hrows, const uint4* __restrict__ hadj,
+                             int32_t* __restrict__ cls, int32_t* __restrict__ flag,
+                             uint8_t* __restrict__ keep, uint8_t* __restrict__ mode) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const unsigned long long k = key[p];
+    if (p > 0 && key[p - 1] == k) return;
+    uint32_t q = p + 1;
+    while (q < n && key[q] == k) ++q;
+    uint8_t md = 0;
+    if (k == kPadKey) {
+        for (uint32_t e = p; e < q; ++e) {
+            cls[idx[e]] = kOutInvalid;
+            flag[idx[e]] = 0;
+            keep[idx[e]] = 0;
+        }
+    } else {
+        const uint32_t a = (uint32_t)(k >> 32), b = (uint32_t)k;
+        const bool in_g = has_edge(grows, gadj, a, b);
+        for (uint32_t e = p; e < q; ++e) {
+            const bool first_new = !in_g && e == p;
+            cls[idx[e]] = first_new ? 0 : kOutWeight;
+            flag[idx[e]] = first_new ? -1 : 0;
+            keep[idx[e]] = first_new ? 1 : 0;
+        }
+        if (in_g) md = 1 | (has_edge(hrows, hadj, a, b) ? 4 : 0);
+        else if (q - p > 1) md = 2;
+    }
+    mode[p] = md;
+}
+
+// Weight updates of one edge, in batch order, on both of its slots.
+__device__ void add_weights(const uint2* __restrict__ rows, uint4* __restrict__ adj, uint32_t a,
+                            uint32_t b, const uint32_t* __restrict__ idx, uint32_t e0,
+                            uint32_t e1, const float* __restrict__ w) {
+    const int pa = row_find(rows, adj, a, b), pb = row_find(rows, adj, b, a);
+    if (pa < 0 || pb < 0) return;
+    uint4* sa = &adj[rows[a].x + pa];
+    uint4* sb = &adj[rows[b].x + pb];
+    float x = __uint_as_float(sa->y);
+    for (uint32_t e = e0; e < e1; ++e) x = x + w[idx[e]];
+    sa->y = __float_as_uint(x);
+    sb->y = __float_as_uint(x);
+}
+
+__global__ void k_ins_weights(const unsigned long long* __restrict__ key,
+                              const uint32_t* __restrict__ idx, uint32_t n,
+                              const uint8_t* __restrict__ mode, const float* __restrict__ w,
+                              const uint2* __restrict__ grows, uint4* __restrict__ gadj,
+                              const uint2* __restrict__ hrows, uint4* __restrict__ hadj) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const unsigned long long k = key[p];
+    if (p > 0 && key[p - 1] == k) return;  // not a head: mode[p] is not written
+    const uint8_t md = mode[p];
+    if (md == 0) return;
+    uint32_t q = p + 1;
+    while (q < n && key[q] == k) ++q;
+    const uint32_t a = (uint32_t)(k >> 32), b = (uint32_t)k;
+    const uint32_t e0 = (md & 1) ? p : p + 1;  // a new edge already holds its first weight
+    add_weights(grows, gadj, a, b, idx, e0, q, w);
+    if (md & 4) add_weights(hrows, hadj, a, b, idx, e0, q, w);
+}
+
+__global__ void k_del_groups(const unsigned long long* __restrict__ key,
+                             const uint32_t* __restrict__ idx, uint32_t n,
+                             const uint2* __restrict__ grows, const uint4* __restrict__ gadj,
+                             const uint2* __restrict__ hrows, const uint4* __restrict__ hadj,
+                             int32_t* __restrict__ cls, int32_t* __restrict__ flag_g,
+                             int32_t* __restrict__ flag_h, uint8_t* __restrict__ keep) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const unsigned long long k = key[p];
+    if (p > 0 && key[p - 1] == k) return;
+    uint32_t q = p + 1;
+    while (q < n && key[q] == k) ++q;
+    bool in_g = false, in_h = false;
+    if (k != kPadKey) {
+        const uint32_t a = (uint32_t)(k >> 32), b = (uint32_t)k;
+        in_g = has_edge(grows, gadj, a, b);
+        in_h = in_g && has_edge(hrows, hadj, a, b);
+    }
+    for (uint32_t e = p; e < q; ++e) {
+        const uint32_t i = idx[e];
+        const bool first = (e == p) && in_g;
+        cls[i] = k == kPadKey ? kOutInvalid : !first ? kOutSkip : in_h ? 0 : kOutGOnly;
+        flag_g[i] = first ? 1 : 0;
+        flag_h[i] = first && in_h ? 1 : 0;
+        keep[i] = first && in_h ? 1 : 0;
+    }
+}
+
+__global__ void k_gather(const uint32_t* __restrict__ sel, const uint32_t* __restrict__ nsel,
+                         const uint32_t* __restrict__ u, const uint32_t* __restrict__ v,
+                         const float* __restrict__ w, uint64_t base, bool ordered,
+                         uint32_t* __restrict__ cu, uint32_t* __restrict__ cv,
+                         float* __restrict__ cw, uint64_t* __restrict__ ceid) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= *nsel) return;
+    const uint32_t i = sel[j];
+    const uint32_t a = u[i], b = v[i];
+    cu[j] = ordered ? min(a, b) : a;
+    cv[j] = ordered ? max(a, b) : b;
+    if (w) cw[j] = w[i];
+    ceid[j] = base + i;
+}
+
+__global__ void k_outcome(const uint32_t* __restrict__ sel, const uint32_t* __restrict__ nsel,
+                          const int32_t* __restrict__ found, int32_t* __restrict__ cls,
+                          int32_t if_found, int32_t if_not) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= *nsel) return;
+    cls[sel[j]] = found[j] == -1 ? if_not : if_found;
+}
+
+// Per kept deletion: path hop count (found walks) and the no-path flag.
+__global__ void k_path_counts(const int32_t* __restrict__ found, const uint32_t* __restrict__ nsel,
+                              uint32_t n, uint32_t* __restrict__ pc, uint8_t* __restrict__ nopath) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const bool live = j < *nsel;
+    pc[j] = live && found[j] > 0 ? (uint32_t)found[j] : 0u;
+    nopath[j] = live && found[j] == -1 ? 1 : 0;
+}
+
+// cnt[2] = heuristic endpoints, cnt[3] = total path hops
+__global__ void k_recovery_sizes(const uint32_t* __restrict__ poff, const uint32_t* __restrict__ pc,
+                                 uint32_t n, uint32_t* __restrict__ cnt) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        cnt[2] = 2u * cnt[1];
+        cnt[3] = n ? poff[n - 1] + pc[n - 1] : 0u;
+    }
+}
+
+__global__ void k_heur_endpoints(const uint32_t* __restrict__ npsel,
+                                 const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ ca,
+                                 const uint32_t* __restrict__ cb, const uint64_t* __restrict__ ceid,
+                                 uint32_t* __restrict__ ep, uint64_t* __restrict__ ekey) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= cnt[1]) return;
+    const uint32_t j = npsel[k];
+    ep[2 * k] = ca[j];
+    ep[2 * k + 1] = cb[j];
+    ekey[2 * k] = ceid[j] << 1;
+    ekey[2 * k + 1] = (ceid[j] << 1) | 1ull;
+}
+
+// Candidate list, part one: the hops of every found path, kept-edge order.
+__global__ void k_cand_paths(const int32_t* __restrict__ found, const uint32_t* __restrict__ nsel,
+                             const uint32_t* __restrict__ path, uint32_t max_steps,
+                             const uint32_t* __restrict__ poff, uint32_t* __restrict__ xu,
+                             uint32_t* __restrict__ xv) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= *nsel || found[j] <= 0) return;
+    const uint32_t* pj = path + (uint64_t)j * max_steps;
+    const uint32_t o = poff[j];
+    for (int q = 1; q <= found[j]; ++q) {
+        xu[o + q - 1] = pj[q - 1];
+        xv[o + q - 1] = pj[q];
+    }
+}
+
+// Part two, after all path hops: per endpoint and walker, (u,n1) then (n1,n2).
+__global__ void k_cand_heur(const uint32_t* __restrict__ path3, uint32_t n3, uint32_t base,
+                            uint32_t* __restrict__ xu, uint32_t* __restrict__ xv) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n3) return;
+    const uint32_t* t = path3 + (uint64_t)g * 3;
+    xu[base + 2 * g] = t[0];
+    xv[base + 2 * g] = t[1];
+    xu[base + 2 * g + 1] = t[1];
+    xv[base + 2 * g + 1] = t[2];
+}
+
+// First occurrence of each candidate edge that H lacks: append, with G's weight.
+__global__ void k_cand_groups(const unsigned long long* __restrict__ key,
+                              const uint32_t* __restrict__ idx, uint32_t n,
+                              const uint2* __restrict__ grows, const uint4* __restrict__ gadj,
+                              const uint2* __restrict__ hrows, const uint4* __restrict__ hadj,
+                              int32_t* __restrict__ flag, float* __restrict__ xw) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const unsigned long long k = key[p];
+    const bool head = p == 0 || key[p - 1] != k;
+    bool add = false;
+    if (head && k != kPadKey) {
+        const uint32_t a = (uint32_t)(k >> 32), b = (uint32_t)k;
+        if (!has_edge(hrows, hadj, a, b)) {
+            const int q = row_find(grows, gadj, a, b);
+            if (q >= 0) {  // always: path and heuristic hops are edges of G
+                add = true;
+                xw[idx[p]] = __uint_as_float(gadj[grows[a].x + q].y);
+            }
+        }
+    }
+    flag[idx[p]] = add ? -1 : 0;
+}
+
+// Device buffers of one stream, carved from one allocation.
+struct Arena {
+    char* base = nullptr;
+    size_t off = 0, cap = 0;
+    template <class T>
+    T* take(size_t n) {
+        off = (off + 255) & ~(size_t)255;
+        T* p = (T*)(base + off);
+        off += std::max<size_t>(n, 1) * sizeof(T);
+        return p;
+    }
+};
+
+}  // namespace
+}  // namespace dyg
+
+using namespace dyg;
+
+struct dyg_stream_s {
+    dyg_ctx ctx = nullptr;
+    dyg_graph G = nullptr, H = nullptr;
+    dyg_stream_params sp{};
+    uint64_t base = 0;  // stream position of the next update
+    uint32_t cmax = 0;  // recovery candidates per deletion batch, at most
+    void* mem = nullptr;
+    // per-batch buffers (max_batch entries unless noted)
+    uint32_t *u, *v;
+    float* w;
+    int32_t* cls;
+    unsigned long long *k_in, *k_out;
+    uint32_t *i_in, *i_out;
+    int32_t *flag_g, *flag_h;
+    uint8_t *keep, *mode;
+    uint32_t* sel;
+    uint32_t* cnt;  // [8] device counters: kept, no-path, endpoints, path hops
+    uint32_t *cu, *cv;
+    float *cw, *ccond;
+    uint64_t* ceid;
+    int32_t* cfound;
+    uint32_t* path;  // [max_batch * max_steps]
+    uint32_t *pc, *poff, *npsel;
+    uint8_t* nopath;
+    uint32_t* ep;     // [2 * max_batch]
+    uint64_t* ekey;   // [2 * max_batch]
+    uint32_t* path3;  // [2 * max_batch * heur_walkers * 3]
+    uint32_t *xu, *xv;  // [cmax] recovery candidates
+    float* xw;
+    int32_t* xflag;
+    unsigned long long *xk_in, *xk_out;
+    uint32_t *xi_in, *xi_out;
+    unsigned long long* stats;  // [3] walk counters
+    void* temp = nullptr;
+    size_t temp_bytes = 0;
+    std::vector<int32_t> h_cls;
+    ~dyg_stream_s() {
+        if (mem) (void)hipFree(mem);
+        if (temp) (void)hipFree(temp);
+        if (G) dyg_graph_free(G);
+        if (H) dyg_graph_free(H);
+    }
+};
+
+namespace {
+
+dyg_status stream_alloc(dyg_stream s) {
+    const uint32_t n = s->sp.max_batch;
+    const uint32_t ms = s->sp.del.max_steps;
+    const uint32_t hw = s->sp.heur_walkers;
+    s->cmax = (uint32_t)std::min<uint64_t>((uint64_t)n * (ms - 1) + 4ull * n * hw, 0x7FFFFFFFull);
+    const uint64_t cm = s->cmax;
+    // size pass then carve pass
+    Arena ar;
+    for (int pass = 0; pass < 2; ++pass) {
+        ar.off = 0;
+        s->u = ar.take<uint32_t>(n);
+        s->v = ar.take<uint32_t>(n);
+        s->w = ar.take<float>(n);
+        s->cls = ar.take<int32_t>(n);
+        s->k_in = ar.take<unsigned long long>(n);
+        s->k_out = ar.take<unsigned long long>(n);
+        s->i_in = ar.take<uint32_t>(n);
+        s->i_out = ar.take<uint32_t>(n);
+        s->flag_g = ar.take<int32_t>(n);
+        s->flag_h = ar.take<int32_t>(n);
+        s->keep = ar.take<uint8_t>(n);
+        s->mode = ar.take<uint8_t>(n);
+        s->sel = ar.take<uint32_t>(n);
+        s->cnt = ar.take<uint32_t>(8);
+        s->cu = ar.take<uint32_t>(n);
+        s->cv = ar.take<uint32_t>(n);
+        s->cw = ar.take<float>(n);
+        s->ccond = ar.take<float>(n);
+        s->ceid = ar.take<uint64_t>(n);
+        s->cfound = ar.take<int32_t>(n);
+        s->path = ar.take<uint32_t>((size_t)n * ms);
+        s->pc = ar.take<uint32_t>(n);
+        s->poff = ar.take<uint32_t>(n);
+        s->npsel = ar.take<uint32_t>(n);
+        s->nopath = ar.take<uint8_t>(n);
+        s->ep = ar.take<uint32_t>(2 * (size_t)n);
+        s->ekey = ar.take<uint64_t>(2 * (size_t)n);
+        s->path3 = ar.take<uint32_t>(2 * (size_t)n * hw * 3);
+        s->xu = ar.take<uint32_t>(cm);
+        s->xv = ar.take<uint32_t>(cm);
+        s->xw = ar.take<float>(cm);
+        s->xflag = ar.take<int32_t>(cm);
+        s->xk_in = ar.take<unsigned long long>(cm);
+        s->xk_out = ar.take<unsigned long long>(cm);
+        s->xi_in = ar.take<uint32_t>(cm);
+        s->xi_out = ar.take<uint32_t>(cm);
+        s->stats = ar.take<unsigned long long>(4);
+        if (pass == 0) {
+            ar.cap = ar.off + 256;
+            if (hipMalloc(&s->mem, ar.cap) != hipSuccess) {
+                s->mem = nullptr;
+                set_error(s->ctx, "stream: device allocation failed");
+                return DYG_ERR_NOMEM;
+            }
+            ar.base = (char*)s->mem;
+        }
+    }
+    // hipCUB scratch: the largest of the sorts, selections and the scan
+    size_t t = 0, b = 0;
+    hipcub::CountingInputIterator<uint32_t> it(0);
+    DYG_HIP(s->ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, b, s->xk_in, s->xk_out, s->xi_in,
+                                                       s->xi_out, (int)std::max<uint64_t>(cm, n),
+                                                       0, 64, s->ctx->stream));
+    t = std::max(t, b);
+    DYG_HIP(s->ctx, hipcub::DeviceSelect::Flagged(nullptr, b, it, s->keep, s->sel, s->cnt, (int)n,
+                                                  s->ctx->stream));
+    t = std::max(t, b);
+    DYG_HIP(s->ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, b, s->pc, s->poff, (int)n,
+                                                     s->ctx->stream));
+    t = std::max(t, b);
+    if (hipMalloc(&s->temp, t) != hipSuccess) {
+        s->temp = nullptr;
+        set_error(s->ctx, "stream: device allocation failed");
+        return DYG_ERR_NOMEM;
+    }
+    s->temp_bytes = t;
+    return DYG_OK;
+}
+
+int key_bits(uint32_t V) { return std::min(64, 32 + 33 - __builtin_clz(std::max<uint32_t>(V, 1))); }
+
+dyg_status sort_keys(dyg_stream s, unsigned long long* kin, unsigned long long* kout,
+                     uint32_t* iin, uint32_t* iout, uint32_t n) {
+    size_t tb = s->temp_bytes;
+    DYG_HIP(s->ctx, hipcub::DeviceRadixSort::SortPairs(s->temp, tb, kin, kout, iin, iout, (int)n,
+                                                       0, 64, s->ctx->stream));
+    return DYG_OK;
+}
+
+dyg_status select_flagged(dyg_stream s, const uint8_t* flags, uint32_t* out, uint32_t* count,
+                          uint32_t n) {
+    size_t tb = s->temp_bytes;
+    hipcub::CountingInputIterator<uint32_t> it(0);
+    DYG_HIP(s->ctx, hipcub::DeviceSelect::Flagged(s->temp, tb, it, flags, out, count, (int)n,
+                                                  s->ctx->stream));
+    return DYG_OK;
+}
+
+WalkLaunch walk_args(dyg_stream s, dyg_graph g, const dyg_walk_params& p, uint32_t n) {
+    WalkLaunch a{};
+    a.rows = g->d_rows;
+    a.adj = g->d_adj;
+    a.src = s->cu;
+    a.dst = s->cv;
+    a.w = s->cw;
+    a.eid = s->ceid;
+    a.n = n;
+    a.n_dev = s->cnt;
+    a.walkers = p.walkers;
+    a.max_steps = p.max_steps;
+    a.thr = p.threshold;
+    a.flags = p.flags;
+    a.edges_per_block = p.block_edges;
+    a.seed = p.seed;
+    a.found = s->cfound;
+    a.cond = s->ccond;
+    a.stats = s->stats;
+    return a;
+}
+
+#define LAUNCH_1D(kernel, n, ...)                                                              \
+    do {                                                                                       \
+        const uint32_t _n = (n);                                                               \
+        if (_n) hipLaunchKernelGGL(kernel, dim3((_n + 255) / 256), dim3(256), 0, st, __VA_ARGS__); \
+    } while (0)
+
+dyg_status apply_insert(dyg_stream s, uint32_t n, dyg_stream_stats* out) {
+    hipStream_t st = s->ctx->stream;
+    dyg_graph G = s->G, H = s->H;
+    LAUNCH_1D(k_batch_keys, n, s->u, s->v, n, s->k_in, s->i_in);
+    dyg_status r = sort_keys(s, s->k_in, s->k_out, s->i_in, s->i_out, n);
+    if (r != DYG_OK) return r;
+    LAUNCH_1D(k_ins_groups, n, s->k_out, s->i_out, n, G->d_rows, G->d_adj, H->d_rows, H->d_adj,
+              s->cls, s->flag_g, s->keep, s->mode);
+    r = device_edits_batch(G, s->u, s->v, s->w, s->flag_g, n);
+    if (r != DYG_OK) return r;
+    LAUNCH_1D(k_ins_weights, n, s->k_out, s->i_out, n, s->mode, s->w, G->d_rows, G->d_adj,
+              H->d_rows, H->d_adj);
+    r = select_flagged(s, s->keep, s->sel, s->cnt, n);
+    if (r != DYG_OK) return r;
+    LAUNCH_1D(k_gather, n, s->sel, s->cnt, s->u, s->v, s->w, s->base, false, s->cu, s->cv, s->cw,
+              s->ceid);
+    DYG_HIP(s->ctx, hipGetLastError());
+    hipEvent_t e0 = s->ctx->ev0, e1 = s->ctx->ev1;
+    DYG_HIP(s->ctx, hipEventRecord(e0, st));
+    DYG_HIP(s->ctx, launch_walk_incremental(walk_args(s, H, s->sp.ins, n), st));
+    DYG_HIP(s->ctx, hipEventRecord(e1, st));
+    r = device_edits_batch(H, s->cu, s->cv, s->cw, s->cfound, n, s->cnt);
+    if (r != DYG_OK) return r;
+    LAUNCH_1D(k_outcome, n, s->sel, s->cnt, s->cfound, s->cls, kOutFound, kOutAdded);
+    DYG_HIP(s->ctx, hipGetLastError());
+    (void)out;
+    return DYG_OK;
+}
+
+dyg_status apply_delete(dyg_stream s, uint32_t n, dyg_stream_stats* out) {
+    hipStream_t st = s->ctx->stream;
+    dyg_graph G = s->G, H = s->H;
+    const uint32_t ms = s->sp.del.max_steps, hw = s->sp.heur_walkers;
+    LAUNCH_1D(k_batch_keys, n, s->u, s->v, n, s->k_in, s->i_in);
+    dyg_status r = sort_keys(s, s->k_in, s->k_out, s->i_in, s->i_out, n);
+    if (r != DYG_OK) return r;
+    LAUNCH_1D(k_del_groups, n, s->k_out, s->i_out, n, G->d_rows, G->d_adj, H->d_rows, H->d_adj,
+              s->cls, s->flag_g, s->flag_h, s->keep);
+    r = device_deletes_batch(G, s->u, s->v, s->flag_g, n, nullptr);
+    if (r != DYG_OK) return r;
+    r = device_deletes_batch(H, s->u, s->v, s->flag_h, n, nullptr);
+    if (r != DYG_OK) return r;
+    r = select_flagged(s, s->keep, s->sel, s->cnt, n);
+    if (r != DYG_OK) return r;
+    LAUNCH_1D(k_gather, n, s->sel, s->cnt, s->u, s->v, (const float*)nullptr, s->base, true, s->cu,
+              s->cv, (float*)nullptr, s->ceid);
+    DYG_HIP(s->ctx, hipGetLastError());
+    WalkLaunch a = walk_args(s, G, s->sp.del, n);
+    a.w = nullptr;
+    a.path = s->path;
+    DYG_HIP(s->ctx, hipEventRecord(s->ctx->ev0, st));
+    DYG_HIP(s->ctx, launch_walk_decremental(a, st));
+    DYG_HIP(s->ctx, hipEventRecord(s->ctx->ev1, st));
+    LAUNCH_1D(k_path_counts, n, s->cfound, s->cnt, n, s->pc, s->nopath);
+    size_t tb = s->temp_bytes;
+    DYG_HIP(s->ctx, hipcub::DeviceScan::ExclusiveSum(s->temp, tb, s->pc, s->poff, (int)n, st));
+    r = select_flagged(s, s->nopath, s->npsel, s->cnt + 1, n);
+    if (r != DYG_OK) return r;
+    hipLaunchKernelGGL(k_recovery_sizes, dim3(1), dim3(64), 0, st, s->poff, s->pc, n, s->cnt);
+    LAUNCH_1D(k_heur_endpoints, n, s->npsel, s->cnt, s->cu, s->cv, s->ceid, s->ep, s->ekey);
+    DYG_HIP(s->ctx, launch_heuristic(G->d_rows, G->d_adj, s->ep, s->ekey, 2 * n, hw,
+                                     s->sp.del.seed, s->path3, st, s->cnt + 2));
+    LAUNCH_1D(k_cand_paths, n, s->cfound, s->cnt, s->path, ms, s->poff, s->xu, s->xv);
+    DYG_HIP(s->ctx, hipGetLastError());
+    // the one read-back of a deletion batch: recovery list sizes
+    uint32_t hc[4];
+    DYG_HIP(s->ctx, hipMemcpyAsync(hc, s->cnt, sizeof(hc), hipMemcpyDeviceToHost, st));
+    DYG_HIP(s->ctx, hipStreamSynchronize(st));
+    const uint32_t npath = hc[3], n3 = hc[2] * hw;
+    const uint32_t ncand = npath + 2 * n3;
+    if (ncand > s->cmax) {
+        set_error(s->ctx, "stream: recovery list exceeds its reservation");
+        return DYG_ERR_STATE;
+    }
+    LAUNCH_1D(k_cand_heur, n3, s->path3, n3, npath, s->xu, s->xv);
+    LAUNCH_1D(k_batch_keys, ncand, s->xu, s->xv, ncand, s->xk_in, s->xi_in);
+    if (ncand) {
+        r = sort_keys(s, s->xk_in, s->xk_out, s->xi_in, s->xi_out, ncand);
+        if (r != DYG_OK) return r;
+    }
+    LAUNCH_1D(k_cand_groups, ncand, s->xk_out, s->xi_out, ncand, G->d_rows, G->d_adj, H->d_rows,
+              H->d_adj, s->xflag, s->xw);
+    r = device_edits_batch(H, s->xu, s->xv, s->xw, s->xflag, ncand);
+    if (r != DYG_OK) return r;
+    LAUNCH_1D(k_outcome, n, s->sel, s->cnt, s->cfound, s->cls, kOutRecovered, kOutHeuristic);
+    DYG_HIP(s->ctx, hipGetLastError());
+    if (out) out->candidates = ncand;
+    return DYG_OK;
+}
+
+}  // namespace
+
+extern "C" dyg_status dyg_stream_create(dyg_ctx ctx, uint32_t V, const uint64_t* g_row_ptr,
+                                        const uint32_t* g_col, const float* g_w,
+                                        const uint64_t* h_row_ptr, const uint32_t* h_col,
+                                        const float* h_w, const dyg_stream_params* sp,
+                                        dyg_stream* out) {
+    if (!ctx || !sp || !out || V == 0) return DYG_ERR_ARG;
+    *out = nullptr;
+    dyg_status r = check_params(ctx, &sp->ins);
+    if (r == DYG_OK) r = check_params(ctx, &sp->del);
+    if (r != DYG_OK) return r;
+    if (sp->max_batch == 0 || sp->heur_walkers == 0 || sp->heur_walkers > 1024) {
+        set_error(ctx, "stream: max_batch must be positive, heur_walkers in [1,1024]");
+        return DYG_ERR_ARG;
+    }
+    auto* s = new dyg_stream_s;
+    s->ctx = ctx;
+    s->sp = *sp;
+    r = dyg_graph_upload(ctx, V, g_row_ptr, g_col, g_w, sp->slack, 0, &s->G);
+    if (r == DYG_OK) r = dyg_graph_upload(ctx, V, h_row_ptr, h_col, h_w, sp->slack, 0, &s->H);
+    if (r == DYG_OK) r = stream_alloc(s);
+    if (r == DYG_OK) r = device_edits_begin(s->G, sp->max_batch, sp->max_inserts);
+    if (r == DYG_OK) r = device_edits_begin(s->H, s->cmax, sp->max_inserts);
+    if (r != DYG_OK) {
+        delete s;
+        return r;
+    }
+    *out = s;
+    return DYG_OK;
+}
+
+extern "C" dyg_status dyg_stream_apply(dyg_stream s, int op, const uint32_t* u, const uint32_t* v,
+                                       const float* w, uint32_t n, int32_t* outcome,
+                                       dyg_stream_stats* st) {
+    if (!s) return DYG_ERR_ARG;
+    dyg_ctx ctx = s->ctx;
+    if ((op != DYG_STREAM_INSERT && op != DYG_STREAM_DELETE) || (n && (!u || !v)) ||
+        (n && op == DYG_STREAM_INSERT && !w)) {
+        set_error(ctx, "stream apply: bad operation or missing array");
+        return DYG_ERR_ARG;
+    }
+    if (n > s->sp.max_batch) {
+        set_error(ctx, "stream apply: batch larger than max_batch");
+        return DYG_ERR_ARG;
+    }
+    for (uint32_t i = 0; i < n; ++i)
+        if (u[i] >= s->G->V || v[i] >= s->G->V) {
+            set_error(ctx, "stream apply: vertex id out of range");
+            return DYG_ERR_ARG;
+        }
+    const double t0 = now_ms();
+    dyg_stream_stats rs{};
+    rs.updates = n;
+    if (n) {
+        hipStream_t str = ctx->stream;
+        DYG_HIP(ctx, hipMemcpyAsync(s->u, u, n * 4ull, hipMemcpyHostToDevice, str));
+        DYG_HIP(ctx, hipMemcpyAsync(s->v, v, n * 4ull, hipMemcpyHostToDevice, str));
+        if (op == DYG_STREAM_INSERT)
+            DYG_HIP(ctx, hipMemcpyAsync(s->w, w, n * 4ull, hipMemcpyHostToDevice, str));
+        DYG_HIP(ctx, hipMemsetAsync(s->stats, 0, 3 * sizeof(unsigned long long), str));
+        dyg_status r = op == DYG_STREAM_INSERT ? apply_insert(s, n, &rs) : apply_delete(s, n, &rs);
+        if (r != DYG_OK) return r;
+        s->h_cls.resize(n);
+        unsigned long long hst[3];
+        DYG_HIP(ctx, hipMemcpyAsync(s->h_cls.data(), s->cls, n * 4ull, hipMemcpyDeviceToHost, str));
+        DYG_HIP(ctx, hipMemcpyAsync(hst, s->stats, sizeof(hst), hipMemcpyDeviceToHost, str));
+        r = device_edits_end(s->G, nullptr);  // waits for the stream
+        if (r == DYG_OK) r = device_edits_end(s->H, nullptr);
+        if (r != DYG_OK) return r;
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess) rs.walk_ms = ms;
+        rs.hops = hst[0];
+        for (int32_t c : s->h_cls) {
+            switch (c) {
+                case kOutWeight: rs.weight_updates++; break;
+                case kOutFound: rs.found++; break;
+                case kOutAdded: rs.added++; break;
+                case kOutSkip: rs.skipped++; break;
+                case kOutGOnly: rs.g_only++; break;
+                case kOutRecovered: rs.recovered++; break;
+                case kOutHeuristic: rs.heuristic++; break;
+                default: rs.invalid++; break;
+            }
+        }
+        if (outcome) std::memcpy(outcome, s->h_cls.data(), n * 4ull);
+        // keep a batch's worth of arena headroom for the next batch
+        r = device_edits_prepare(s->G, s->sp.max_batch, s->sp.max_batch);
+        if (r == DYG_OK) r = device_edits_prepare(s->H, s->cmax, s->cmax);
+        if (r != DYG_OK) return r;
+    }
+    s->base += n;
+    rs.total_ms = now_ms() - t0;
+    if (st) *st = rs;
+    return DYG_OK;
+}
+
+extern "C" dyg_graph dyg_stream_graph(dyg_stream s, int which) {
+    if (!s) return nullptr;
+    return which == 0 ? s->G : s->H;
+}
+
+extern "C" uint64_t dyg_stream_position(dyg_stream s) { return s ? s->base : 0; }
+
+extern "C" void dyg_stream_free(dyg_stream s) { delete s; }

@@ -204,6 +204,7 @@ __global__ __launch_bounds__((1 << LOGW) < 64 ? 64 : (1 << LOGW)) void k_walk(Wa
     __shared__ uint32_t s_best;
 
     const uint32_t e = blockIdx.x;
+    if (a.n_dev && e >= *a.n_dev) return;  // list shorter than the grid (uniform per block)
     const uint32_t j = threadIdx.x;
     const uint32_t lane = j & 63u, wave = j >> 6;
     if (j == 0) s_best = 0u;
@@ -359,8 +360,10 @@ __global__ __launch_bounds__(kPoolThreads) void k_walk_pool(WalkLaunch a, uint32
     __shared__ uint32_t s_next;
     __shared__ uint32_t s_cnt[3];
 
+    const uint32_t n = a.n_dev ? min(*a.n_dev, a.n) : a.n;  // list length (device or host)
     const uint32_t e0 = blockIdx.x * E;
-    const uint32_t ne = min(E, a.n - e0);
+    if (e0 >= n) return;  // uniform per workgroup
+    const uint32_t ne = min(E, n - e0);
     const uint32_t tid = threadIdx.x;
     const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
     if (tid < ne) {
@@ -521,8 +524,10 @@ __global__ __launch_bounds__(256) void k_heuristic(const uint2* __restrict__ row
                                                    const uint32_t* __restrict__ endpoint,
                                                    const uint64_t* __restrict__ key, uint32_t n,
                                                    uint32_t walkers, uint64_t seed,
-                                                   uint32_t* __restrict__ path3) {
+                                                   uint32_t* __restrict__ path3,
+                                                   const uint32_t* __restrict__ n_dev) {
     const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n_dev && *n_dev < n) n = *n_dev;
     if (gid >= (uint64_t)n * walkers) return;
     const uint32_t i = (uint32_t)(gid / walkers), j = (uint32_t)(gid % walkers);
     const uint32_t u = endpoint[i];
@@ -668,12 +673,12 @@ hipError_t launch_walk_decremental(const WalkLaunch& a, hipStream_t s) {
 
 hipError_t launch_heuristic(const uint2* rows, const uint4* adj, const uint32_t* endpoint,
                             const uint64_t* key, uint32_t n, uint32_t walkers, uint64_t seed,
-                            uint32_t* path3, hipStream_t s) {
+                            uint32_t* path3, hipStream_t s, const uint32_t* n_dev) {
     const uint64_t total = (uint64_t)n * walkers;
     if (total == 0) return hipSuccess;
     const uint32_t blocks = (uint32_t)((total + 255) / 256);
     hipLaunchKernelGGL(k_heuristic, dim3(blocks), dim3(256), 0, s, rows, adj, endpoint, key, n,
-                       walkers, seed, path3);
+                       walkers, seed, path3, n_dev);
     return hipGetLastError();
 }
 

@@ -40,6 +40,7 @@ typedef int32_t dyg_status;
 typedef struct dyg_ctx_s* dyg_ctx;
 typedef struct dyg_graph_s* dyg_graph;
 typedef struct dyg_dual_s* dyg_dual;
+typedef struct dyg_stream_s* dyg_stream;
 
 /* ------------------------------------------------------------------ */
 /* Context                                                            */
@@ -244,6 +245,67 @@ uint64_t dyg_dual_added_count(dyg_dual d);
 dyg_status dyg_dual_added_edges(dyg_dual d, uint32_t* u, uint32_t* v);
 dyg_graph dyg_dual_dense(dyg_dual d);
 void dyg_dual_free(dyg_dual d);
+
+/* ------------------------------------------------------------------ */
+/* Mixed insertion/deletion update stream (device-resident G and H)   */
+/* ------------------------------------------------------------------ */
+/* The stream form of dynamic_update (dynamic_update/main.cu:92-124 batch
+ * loop; preprocessStreamEdges dynamic_update/gpu_graph.cuh:361-439, weight
+ * update :451-465 with the weight, not the neighbour id, updated;
+ * updateSparsiferFromResult :620-700), with the plain updaters' walks,
+ * reduction and heuristic.  One batch is one operation kind and one snapshot:
+ *   insertion: an edge already in G gets weight += w in G (and in H when
+ *   there); a new edge is appended to G and walks in H; without a path it is
+ *   appended to H.
+ *   deletion: an edge not in G is skipped; otherwise it leaves G and, when in
+ *   H, leaves H and walks in G (after all of the batch's deletions) from
+ *   min(a,b) to max(a,b); found paths' hops, then the heuristic edges of
+ *   no-path deletions, join H (when absent) with G's weight.
+ * Duplicates in a batch act in batch order.  G and H must be simple graphs,
+ * H inside G.  Edge ids of the draw streams: the update's position in the
+ * whole stream (dyg_stream_position before the batch + index). */
+#define DYG_STREAM_INSERT 0
+#define DYG_STREAM_DELETE 1
+/* per-update outcome codes */
+#define DYG_OUT_WEIGHT 1     /* insertion of an existing edge: weight update */
+#define DYG_OUT_FOUND 2      /* insertion: path found, H unchanged */
+#define DYG_OUT_ADDED 3      /* insertion: no path, appended to H */
+#define DYG_OUT_SKIP 4       /* deletion of an edge not in G (or repeated) */
+#define DYG_OUT_G_ONLY 5     /* deletion from G only */
+#define DYG_OUT_RECOVERED 6  /* deletion from G and H, path hops joined H */
+#define DYG_OUT_HEURISTIC 7  /* deletion from G and H, no path: heuristic edges joined H */
+#define DYG_OUT_INVALID 8    /* self-loop */
+typedef struct {
+    dyg_walk_params ins;    /* insertion walks (threshold used) */
+    dyg_walk_params del;    /* deletion walks (threshold ignored) */
+    uint32_t heur_walkers;  /* heuristic walkers per endpoint (the reference: 16) */
+    uint32_t max_batch;     /* largest batch apply() accepts */
+    uint64_t max_inserts;   /* arena reservation hint: appends expected per graph */
+    uint32_t slack;         /* spare slots given to a relocated row */
+    uint32_t flags;         /* reserved, 0 */
+} dyg_stream_params;
+typedef struct {
+    uint64_t updates, weight_updates, found, added, skipped, g_only, recovered, heuristic,
+        invalid;
+    uint64_t candidates;  /* deletion batches: recovery edges examined */
+    uint64_t hops;        /* neighbour draws of the batch's walks */
+    double walk_ms;       /* walk kernel (HIP events) */
+    double total_ms;      /* whole apply() call, host clock */
+} dyg_stream_stats;
+/* G and H in CSR (slot order as given), uploaded and owned by the stream. */
+dyg_status dyg_stream_create(dyg_ctx ctx, uint32_t V, const uint64_t* g_row_ptr,
+                             const uint32_t* g_col, const float* g_w, const uint64_t* h_row_ptr,
+                             const uint32_t* h_col, const float* h_w,
+                             const dyg_stream_params* params, dyg_stream* out);
+/* Apply one batch (host arrays; w ignored for deletions).  outcome[i]
+ * (optional) receives DYG_OUT_* of update i.  Returns when the batch is done. */
+dyg_status dyg_stream_apply(dyg_stream s, int op, const uint32_t* u, const uint32_t* v,
+                            const float* w, uint32_t n, int32_t* outcome, dyg_stream_stats* st);
+/* 0: G, 1: H (owned by the stream; use with dyg_graph_info/download/check). */
+dyg_graph dyg_stream_graph(dyg_stream s, int which);
+/* Updates applied so far (the edge id of the next update). */
+uint64_t dyg_stream_position(dyg_stream s);
+void dyg_stream_free(dyg_stream s);
 
 /* ------------------------------------------------------------------ */
 /* .mtx I/O with the reference's parsing rules                        */

@@ -1,0 +1,101 @@
+"""GPU parity of the mixed insertion/deletion stream against the oracle.
+
+Each batch's per-update outcome codes must be equal, and after each batch G
+and H must be equal slot for slot (neighbour order within every row, weight
+bits), which pins the walks, the duplicate handling, the weight updates, the
+swap-with-last deletions and the recovery order together.
+"""
+import numpy as np
+import pytest
+
+from conftest import f32bits
+
+pytestmark = pytest.mark.gpu
+
+
+def _mixed_batches(wl, seed, n_ins=300, n_del=200):
+    """Insertions (new edges, existing G edges, in-batch repeats, a self-loop)
+    and deletions (G edges, H edges, non-edges, repeats), alternating."""
+    rng = np.random.default_rng(seed)
+    G = wl.G
+    out = []
+    k = 0
+    for b in range(4):
+        if b % 2 == 0:
+            u = list(wl.ins.u[k:k + n_ins])
+            v = list(wl.ins.v[k:k + n_ins])
+            w = list(wl.ins.w[k:k + n_ins])
+            k += n_ins
+            ex = rng.choice(len(G), 25, replace=False)  # existing edges: weight updates
+            u += list(G.v[ex]); v += list(G.u[ex]); w += list(rng.uniform(0.1, 2, 25))
+            rep = rng.choice(n_ins, 6, replace=False)  # repeats inside the batch
+            u += [u[i] for i in rep]; v += [v[i] for i in rep]; w += list(rng.uniform(0.1, 2, 6))
+            u.append(5); v.append(5); w.append(1.0)  # self-loop
+            perm = rng.permutation(len(u))
+            out.append(("ins", np.array(u)[perm], np.array(v)[perm],
+                        np.array(w, np.float32)[perm]))
+        else:
+            idx = rng.choice(len(G), n_del, replace=False)
+            hidx = rng.choice(len(wl.H), 30, replace=False)
+            u = list(G.u[idx]) + list(wl.H.u[hidx]) + list(rng.integers(0, G.V, 10))
+            v = list(G.v[idx]) + list(wl.H.v[hidx]) + list(rng.integers(0, G.V, 10))
+            u += u[:5]; v += v[:5]  # repeats
+            # a previously inserted edge, deleted again
+            u.append(int(wl.ins.u[0])); v.append(int(wl.ins.v[0]))
+            perm = rng.permutation(len(u))
+            out.append(("del", np.array(u)[perm], np.array(v)[perm], None))
+    return out
+
+
+def _assert_same_graph(dev_graph, rows):
+    rp, col, w = dev_graph.download()
+    orp, ocol, ow = rows.csr()
+    assert np.array_equal(rp, orp)
+    assert np.array_equal(col, ocol)
+    assert np.array_equal(f32bits(w), f32bits(ow))
+    assert dev_graph.check() == 0
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_stream_matches_oracle(ctx, grid_wl, seed):
+    from dygrass_amd import Stream, stream_params
+    from oracle import stream as S
+
+    wl = grid_wl
+    sp = stream_params(w_ins=64, w_del=64, threshold=10.0, seed=seed, max_batch=1024,
+                       max_inserts=4096)
+    st = Stream(ctx, wl.G.V, (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w), sp)
+    orc = S.Stream(wl.G.V, (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w), thr=10.0,
+                   w_ins=64, w_del=64, seed=seed)
+    seen = set()
+    for op, u, v, w in _mixed_batches(wl, seed):
+        got, stats = st.apply(op, u, v, w)
+        want = orc.apply(op, u, v, w)
+        assert np.array_equal(got, want), (op, np.flatnonzero(got != want)[:10])
+        assert stats["updates"] == len(u)
+        seen.update(np.unique(got).tolist())
+        _assert_same_graph(st.G, orc.G)
+        _assert_same_graph(st.H, orc.H)
+    assert st.position == orc.base
+    # the batches exercise every outcome
+    assert seen >= {S.WEIGHT, S.FOUND, S.ADDED, S.SKIP, S.G_ONLY, S.RECOVERED, S.INVALID}
+
+
+def test_stream_heuristic_recovery(ctx):
+    """Deleting a bridge of G that H holds leaves no path: the heuristic runs."""
+    from dygrass_amd import Stream, stream_params
+    from oracle import stream as S
+
+    # two triangles joined by the bridge (2,3); H = a spanning tree with it
+    gu = np.array([0, 1, 2, 3, 4, 5, 2], np.uint32)
+    gv = np.array([1, 2, 0, 4, 5, 3, 3], np.uint32)
+    gw = np.array([1, 2, 3, 4, 5, 6, 7], np.float32)
+    hu, hv, hw = gu[[0, 1, 3, 4, 6]], gv[[0, 1, 3, 4, 6]], gw[[0, 1, 3, 4, 6]]
+    sp = stream_params(w_ins=8, w_del=8, threshold=10.0, seed=3, max_batch=16, max_inserts=64)
+    st = Stream(ctx, 6, (gu, gv, gw), (hu, hv, hw), sp)
+    orc = S.Stream(6, (gu, gv, gw), (hu, hv, hw), thr=10.0, w_ins=8, w_del=8, seed=3)
+    got, _ = st.apply("del", [2], [3])
+    want = orc.apply("del", [2], [3])
+    assert got[0] == want[0] == S.HEURISTIC
+    _assert_same_graph(st.G, orc.G)
+    _assert_same_graph(st.H, orc.H)
