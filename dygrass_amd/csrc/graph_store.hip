@@ -259,7 +259,7 @@ dyg_status graph_apply_inserts(dyg_graph g, const uint32_t* u, const uint32_t* v
 }
 
 dyg_status graph_apply_deletes(dyg_graph g, const uint32_t* u, const uint32_t* v, uint32_t n,
-                               int8_t* hit) {
+                               int8_t* hit, bool allow_zero_degree) {
     if (!g->keyed) {
         set_error(g->ctx, "delete: graph was uploaded without its edge map");
         return DYG_ERR_ARG;
@@ -275,7 +275,7 @@ dyg_status graph_apply_deletes(dyg_graph g, const uint32_t* u, const uint32_t* v
     }
     // validation pass: no vertex may reach degree 0 (the reference asserts,
     // random_walk_decremental/gpu_graph.cuh:362-363); nothing changes on failure
-    {
+    if (!allow_zero_degree) {
         std::unordered_map<long, char> gone;
         std::unordered_map<uint32_t, uint32_t> dec;
         for (uint32_t i = 0; i < n; ++i) {
@@ -328,6 +328,37 @@ dyg_status graph_apply_deletes(dyg_graph g, const uint32_t* u, const uint32_t* v
     touched.erase(std::unique(touched.begin(), touched.end()), touched.end());
     refresh_neighbour_copies(g, touched, dirty);
     return ship_delta(g, dirty, touched);
+}
+
+dyg_status graph_add_weights(dyg_graph g, const uint32_t* u, const uint32_t* v, const float* dw,
+                             uint32_t n) {
+    if (!g->keyed) {
+        set_error(g->ctx, "weight update: graph was uploaded without its edge map");
+        return DYG_ERR_ARG;
+    }
+    {
+        dyg_status rs = graph_refresh_mirror(g);
+        if (rs != DYG_OK) return rs;
+    }
+    std::vector<uint32_t> dirty, rows;
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t a = std::min(u[i], v[i]), b = std::max(u[i], v[i]);
+        auto it = g->key_map.find(key_of(a, b, g->mult));
+        if (it == g->key_map.end()) {
+            set_error(g->ctx, "weight update: edge not in the graph");
+            return DYG_ERR_STATE;
+        }
+        uint4& sa = g->adj[g->start[a] + it->second.first];
+        uint4& sb = g->adj[g->start[b] + it->second.second];
+        float x;
+        std::memcpy(&x, &sa.y, 4);
+        x = x + dw[i];
+        sa.y = fbits(x);
+        sb.y = fbits(x);
+        dirty.push_back(g->start[a] + it->second.first);
+        dirty.push_back(g->start[b] + it->second.second);
+    }
+    return ship_delta(g, dirty, rows);
 }
 
 }  // namespace dyg

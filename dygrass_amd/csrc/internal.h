@@ -116,7 +116,10 @@ long key_multiplier(uint64_t V);
 dyg_status graph_apply_inserts(dyg_graph g, const uint32_t* u, const uint32_t* v, const float* w,
                                uint32_t n);
 dyg_status graph_apply_deletes(dyg_graph g, const uint32_t* u, const uint32_t* v, uint32_t n,
-                               int8_t* hit);
+                               int8_t* hit, bool allow_zero_degree = false);
+// weight(u[i],v[i]) += dw[i] in order, both slots (keyed graphs).
+dyg_status graph_add_weights(dyg_graph g, const uint32_t* u, const uint32_t* v, const float* dw,
+                             uint32_t n);
 // Grow the slot arena to at least `need` slots.
 dyg_status graph_reserve_slots(dyg_graph g, uint64_t need);
 // Re-read rows, capacities and slots from the device after device edits.

@@ -34,6 +34,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <unordered_set>
 #include <vector>
 
 #include "internal.h"
@@ -561,6 +562,157 @@ dyg_status apply_delete(dyg_stream s, uint32_t n, dyg_stream_stats* out) {
     return DYG_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Plain form: the reference's shape.  The host keeps both edge maps, decides
+// every update, plans the row edits (shipped to the device as deltas) and
+// launches one walk per batch with one workgroup per edge, waiting for each
+// result before going on.  Same semantics and results as the device form.
+// ---------------------------------------------------------------------------
+dyg_status plain_insert(dyg_stream s, const uint32_t* u, const uint32_t* v, const float* w,
+                        uint32_t n, int32_t* cls, dyg_stream_stats* rs) {
+    dyg_graph G = s->G, H = s->H;
+    std::vector<uint32_t> nu, nv, ku, kv, kidx, gu, gv, hu, hv;
+    std::vector<float> nw, kw, gw, hw;
+    std::vector<uint64_t> keid;
+    std::unordered_set<long> fresh;
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t a = std::min(u[i], v[i]), b = std::max(u[i], v[i]);
+        if (a == b) {
+            cls[i] = kOutInvalid;
+            continue;
+        }
+        const long key = key_of(a, b, G->mult);
+        if (G->key_map.count(key) || fresh.count(key)) {
+            cls[i] = kOutWeight;
+            gu.push_back(a); gv.push_back(b); gw.push_back(w[i]);
+            if (H->key_map.count(key)) {
+                hu.push_back(a); hv.push_back(b); hw.push_back(w[i]);
+            }
+        } else {
+            fresh.insert(key);
+            nu.push_back(u[i]); nv.push_back(v[i]); nw.push_back(w[i]);
+            ku.push_back(u[i]); kv.push_back(v[i]); kw.push_back(w[i]);
+            keid.push_back(s->base + i);
+            kidx.push_back(i);
+        }
+    }
+    dyg_status r = graph_apply_inserts(G, nu.data(), nv.data(), nw.data(), (uint32_t)nu.size());
+    if (r == DYG_OK) r = graph_add_weights(G, gu.data(), gv.data(), gw.data(), (uint32_t)gu.size());
+    if (r == DYG_OK) r = graph_add_weights(H, hu.data(), hv.data(), hw.data(), (uint32_t)hu.size());
+    if (r != DYG_OK) return r;
+    const uint32_t nk = (uint32_t)ku.size();
+    std::vector<int32_t> found(nk);
+    std::vector<float> cond(nk);
+    dyg_walk_params p = s->sp.ins;
+    p.flags |= DYG_WALK_PER_EDGE;
+    dyg_walk_stats ws{};
+    if (nk) {
+        r = dyg_walk_incremental(H, ku.data(), kv.data(), kw.data(), keid.data(), nk, &p,
+                                 found.data(), cond.data(), nullptr, nullptr, &ws);
+        if (r != DYG_OK) return r;
+    }
+    rs->walk_ms += ws.kernel_ms;
+    rs->hops += ws.hops;
+    std::vector<uint32_t> au, av;
+    std::vector<float> aw;
+    for (uint32_t j = 0; j < nk; ++j) {
+        cls[kidx[j]] = found[j] == -1 ? kOutAdded : kOutFound;
+        if (found[j] == -1) {
+            au.push_back(ku[j]); av.push_back(kv[j]); aw.push_back(kw[j]);
+        }
+    }
+    return graph_apply_inserts(H, au.data(), av.data(), aw.data(), (uint32_t)au.size());
+}
+
+dyg_status plain_delete(dyg_stream s, const uint32_t* u, const uint32_t* v, uint32_t n,
+                        int32_t* cls, dyg_stream_stats* rs) {
+    dyg_graph G = s->G, H = s->H;
+    std::vector<uint32_t> gu, gv, hu, hv, ka, kb, kidx;
+    std::vector<uint64_t> keid;
+    std::unordered_set<long> gone;
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t a = std::min(u[i], v[i]), b = std::max(u[i], v[i]);
+        if (a == b) {
+            cls[i] = kOutInvalid;
+            continue;
+        }
+        const long key = key_of(a, b, G->mult);
+        if (!G->key_map.count(key) || gone.count(key)) {
+            cls[i] = kOutSkip;
+            continue;
+        }
+        gone.insert(key);
+        gu.push_back(u[i]); gv.push_back(v[i]);
+        if (H->key_map.count(key)) {
+            hu.push_back(u[i]); hv.push_back(v[i]);
+            ka.push_back(a); kb.push_back(b);
+            keid.push_back(s->base + i);
+            kidx.push_back(i);
+        } else {
+            cls[i] = kOutGOnly;
+        }
+    }
+    dyg_status r = graph_apply_deletes(G, gu.data(), gv.data(), (uint32_t)gu.size(), nullptr, true);
+    if (r == DYG_OK)
+        r = graph_apply_deletes(H, hu.data(), hv.data(), (uint32_t)hu.size(), nullptr, true);
+    if (r != DYG_OK) return r;
+    const uint32_t nk = (uint32_t)ka.size();
+    if (nk == 0) return DYG_OK;
+    const uint32_t ms = s->sp.del.max_steps, hwk = s->sp.heur_walkers;
+    std::vector<int32_t> found(nk);
+    std::vector<float> cond(nk);
+    std::vector<uint32_t> path((size_t)nk * ms);
+    dyg_walk_params p = s->sp.del;
+    p.flags |= DYG_WALK_PER_EDGE;
+    dyg_walk_stats ws{};
+    r = dyg_walk_decremental(G, ka.data(), kb.data(), keid.data(), nk, &p, found.data(),
+                             cond.data(), nullptr, nullptr, path.data(), &ws);
+    if (r != DYG_OK) return r;
+    rs->walk_ms += ws.kernel_ms;
+    rs->hops += ws.hops;
+    std::vector<uint32_t> xu, xv, ep;
+    std::vector<uint64_t> ek;
+    for (uint32_t j = 0; j < nk; ++j) {
+        cls[kidx[j]] = found[j] > 0 ? kOutRecovered : kOutHeuristic;
+        if (found[j] > 0) {
+            for (int q = 1; q <= found[j]; ++q) {
+                xu.push_back(path[(size_t)j * ms + q - 1]);
+                xv.push_back(path[(size_t)j * ms + q]);
+            }
+        } else {
+            ep.push_back(ka[j]); ek.push_back(keid[j] << 1);
+            ep.push_back(kb[j]); ek.push_back((keid[j] << 1) | 1ull);
+        }
+    }
+    if (!ep.empty()) {
+        std::vector<uint32_t> p3(ep.size() * hwk * 3);
+        r = dyg_walk_heuristic(G, ep.data(), ek.data(), (uint32_t)ep.size(), hwk, s->sp.del.seed,
+                               p3.data());
+        if (r != DYG_OK) return r;
+        for (size_t g = 0; g < ep.size() * hwk; ++g) {
+            xu.push_back(p3[g * 3]); xv.push_back(p3[g * 3 + 1]);
+            xu.push_back(p3[g * 3 + 1]); xv.push_back(p3[g * 3 + 2]);
+        }
+    }
+    rs->candidates = xu.size();
+    std::vector<uint32_t> au, av;
+    std::vector<float> aw;
+    std::unordered_set<long> added;
+    for (size_t c = 0; c < xu.size(); ++c) {
+        const uint32_t a = std::min(xu[c], xv[c]), b = std::max(xu[c], xv[c]);
+        if (a == b) continue;
+        const long key = key_of(a, b, G->mult);
+        if (H->key_map.count(key) || added.count(key)) continue;
+        auto it = G->key_map.find(key);
+        if (it == G->key_map.end()) continue;  // unreachable: hops are edges of G
+        added.insert(key);
+        float x;
+        std::memcpy(&x, &G->adj[G->start[a] + it->second.first].y, 4);
+        au.push_back(xu[c]); av.push_back(xv[c]); aw.push_back(x);
+    }
+    return graph_apply_inserts(H, au.data(), av.data(), aw.data(), (uint32_t)au.size());
+}
+
 }  // namespace
 
 extern "C" dyg_status dyg_stream_create(dyg_ctx ctx, uint32_t V, const uint64_t* g_row_ptr,
@@ -580,11 +732,14 @@ extern "C" dyg_status dyg_stream_create(dyg_ctx ctx, uint32_t V, const uint64_t*
     auto* s = new dyg_stream_s;
     s->ctx = ctx;
     s->sp = *sp;
-    r = dyg_graph_upload(ctx, V, g_row_ptr, g_col, g_w, sp->slack, 0, &s->G);
-    if (r == DYG_OK) r = dyg_graph_upload(ctx, V, h_row_ptr, h_col, h_w, sp->slack, 0, &s->H);
-    if (r == DYG_OK) r = stream_alloc(s);
-    if (r == DYG_OK) r = device_edits_begin(s->G, sp->max_batch, sp->max_inserts);
-    if (r == DYG_OK) r = device_edits_begin(s->H, s->cmax, sp->max_inserts);
+    const bool plain = (sp->flags & DYG_STREAM_PLAIN) != 0;
+    r = dyg_graph_upload(ctx, V, g_row_ptr, g_col, g_w, sp->slack, plain, &s->G);
+    if (r == DYG_OK) r = dyg_graph_upload(ctx, V, h_row_ptr, h_col, h_w, sp->slack, plain, &s->H);
+    if (!plain) {
+        if (r == DYG_OK) r = stream_alloc(s);
+        if (r == DYG_OK) r = device_edits_begin(s->G, sp->max_batch, sp->max_inserts);
+        if (r == DYG_OK) r = device_edits_begin(s->H, s->cmax, sp->max_inserts);
+    }
     if (r != DYG_OK) {
         delete s;
         return r;
@@ -615,7 +770,13 @@ extern "C" dyg_status dyg_stream_apply(dyg_stream s, int op, const uint32_t* u, 
     const double t0 = now_ms();
     dyg_stream_stats rs{};
     rs.updates = n;
-    if (n) {
+    if (n && (s->sp.flags & DYG_STREAM_PLAIN)) {
+        s->h_cls.assign(n, 0);
+        dyg_status r = op == DYG_STREAM_INSERT ? plain_insert(s, u, v, w, n, s->h_cls.data(), &rs)
+                                               : plain_delete(s, u, v, n, s->h_cls.data(), &rs);
+        if (r != DYG_OK) return r;
+        DYG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    } else if (n) {
         hipStream_t str = ctx->stream;
         DYG_HIP(ctx, hipMemcpyAsync(s->u, u, n * 4ull, hipMemcpyHostToDevice, str));
         DYG_HIP(ctx, hipMemcpyAsync(s->v, v, n * 4ull, hipMemcpyHostToDevice, str));
@@ -634,6 +795,12 @@ extern "C" dyg_status dyg_stream_apply(dyg_stream s, int op, const uint32_t* u, 
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess) rs.walk_ms = ms;
         rs.hops = hst[0];
+        // keep a batch's worth of arena headroom for the next batch
+        r = device_edits_prepare(s->G, s->sp.max_batch, s->sp.max_batch);
+        if (r == DYG_OK) r = device_edits_prepare(s->H, s->cmax, s->cmax);
+        if (r != DYG_OK) return r;
+    }
+    if (n) {
         for (int32_t c : s->h_cls) {
             switch (c) {
                 case kOutWeight: rs.weight_updates++; break;
@@ -647,10 +814,6 @@ extern "C" dyg_status dyg_stream_apply(dyg_stream s, int op, const uint32_t* u, 
             }
         }
         if (outcome) std::memcpy(outcome, s->h_cls.data(), n * 4ull);
-        // keep a batch's worth of arena headroom for the next batch
-        r = device_edits_prepare(s->G, s->sp.max_batch, s->sp.max_batch);
-        if (r == DYG_OK) r = device_edits_prepare(s->H, s->cmax, s->cmax);
-        if (r != DYG_OK) return r;
     }
     s->base += n;
     rs.total_ms = now_ms() - t0;

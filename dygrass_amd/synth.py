@@ -163,6 +163,61 @@ def config_grid3d(scale: float = 1.0, seed: int = 0) -> Workload:
     return Workload(f"grid3d{n}-sc", G, H, walkers=256, threshold=100.0)
 
 
+@dataclass
+class StreamWorkload:
+    """G, H and a list of batches (op, u, v, w) with op "ins" or "del"."""
+    name: str
+    G: Edges
+    H: Edges
+    batches: list
+    threshold: float = 100.0
+    w_ins: int = 256
+    w_del: int = 512
+
+    @property
+    def updates(self) -> int:
+        return sum(len(b[1]) for b in self.batches)
+
+
+def stream_batches(G: Edges, nbatch: int, max_size: int, frac_ins: float, seed: int,
+                   geometric: bool, wlo: float = 0.0, whi: float = 0.0) -> list:
+    """`nbatch` batches with sizes floor(exp(U[0, ln max_size])), each an
+    insertion batch with probability frac_ins, else a deletion batch.
+    Insertions: new edges within 2..5 hops of G (weights as the graph's);
+    deletions: distinct edges of G drawn uniformly, consumed in order."""
+    rng = np.random.default_rng(seed)
+    sizes = np.floor(np.exp(rng.uniform(0.0, math.log(max_size), nbatch))).astype(np.int64)
+    sizes = np.maximum(sizes, 1)
+    ops = np.where(rng.uniform(size=nbatch) < frac_ins, "ins", "del")
+    n_ins = int(sizes[ops == "ins"].sum())
+    n_del = int(sizes[ops == "del"].sum())
+    ins = insertions(G, max(n_ins, 1), 2, 5, wlo, whi, seed ^ 0x51, geometric=geometric)
+    dels = deletions(G, max(n_del, 1), seed ^ 0xD7)
+    out, ki, kd = [], 0, 0
+    for op, k in zip(ops, sizes):
+        k = int(k)
+        if op == "ins":
+            k = min(k, len(ins) - ki)
+            out.append(("ins", ins.u[ki:ki + k], ins.v[ki:ki + k], ins.w[ki:ki + k]))
+            ki += k
+        else:
+            k = min(k, len(dels) - kd)
+            out.append(("del", dels.u[kd:kd + k], dels.v[kd:kd + k], None))
+            kd += k
+    return out
+
+
+def config_batched(scale: float = 1.0, seed: int = 0) -> StreamWorkload:
+    """4M-point 12-NN random geometric graph (w = 1/dist), H = tree + 2% V;
+    64 batches of log-uniform size in [1, 50000], 70% insertions (within 5
+    hops, w = 1/dist) and 30% deletions (uniform over G); threshold 100,
+    256 insertion walkers, 512 deletion walkers."""
+    wl = config_rgg12(scale, seed)
+    mx = max(2, int(50_000 * min(1.0, scale * 4)))
+    b = stream_batches(wl.G, 64, mx, 0.7, _seed(4, 2, seed), geometric=True)
+    return StreamWorkload(f"rgg{wl.G.V}-knn12-stream", wl.G, wl.H, b, 100.0, 256, 512)
+
+
 CONFIGS = {
     "grid-insert": config_grid_insert,
     "rgg-insert": config_rgg_insert,

@@ -56,14 +56,15 @@ def _assert_same_graph(dev_graph, rows):
     assert dev_graph.check() == 0
 
 
-@pytest.mark.parametrize("seed", [0, 1])
-def test_stream_matches_oracle(ctx, grid_wl, seed):
+@pytest.mark.parametrize("seed,plain", [(0, False), (1, False), (0, True)])
+def test_stream_matches_oracle(ctx, grid_wl, seed, plain):
+    """Device-resident form and the plain (host bookkeeping) form."""
     from dygrass_amd import Stream, stream_params
     from oracle import stream as S
 
     wl = grid_wl
     sp = stream_params(w_ins=64, w_del=64, threshold=10.0, seed=seed, max_batch=1024,
-                       max_inserts=4096)
+                       max_inserts=4096, plain=plain)
     st = Stream(ctx, wl.G.V, (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w), sp)
     orc = S.Stream(wl.G.V, (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w), thr=10.0,
                    w_ins=64, w_del=64, seed=seed)
@@ -81,7 +82,8 @@ def test_stream_matches_oracle(ctx, grid_wl, seed):
     assert seen >= {S.WEIGHT, S.FOUND, S.ADDED, S.SKIP, S.G_ONLY, S.RECOVERED, S.INVALID}
 
 
-def test_stream_heuristic_recovery(ctx):
+@pytest.mark.parametrize("plain", [False, True])
+def test_stream_heuristic_recovery(ctx, plain):
     """Deleting a bridge of G that H holds leaves no path: the heuristic runs."""
     from dygrass_amd import Stream, stream_params
     from oracle import stream as S
@@ -91,7 +93,8 @@ def test_stream_heuristic_recovery(ctx):
     gv = np.array([1, 2, 0, 4, 5, 3, 3], np.uint32)
     gw = np.array([1, 2, 3, 4, 5, 6, 7], np.float32)
     hu, hv, hw = gu[[0, 1, 3, 4, 6]], gv[[0, 1, 3, 4, 6]], gw[[0, 1, 3, 4, 6]]
-    sp = stream_params(w_ins=8, w_del=8, threshold=10.0, seed=3, max_batch=16, max_inserts=64)
+    sp = stream_params(w_ins=8, w_del=8, threshold=10.0, seed=3, max_batch=16, max_inserts=64,
+                       plain=plain)
     st = Stream(ctx, 6, (gu, gv, gw), (hu, hv, hw), sp)
     orc = S.Stream(6, (gu, gv, gw), (hu, hv, hw), thr=10.0, w_ins=8, w_del=8, seed=3)
     got, _ = st.apply("del", [2], [3])
