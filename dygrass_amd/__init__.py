@@ -10,6 +10,7 @@ from .core import (  # noqa: F401
     DeviceArray,
     DeviceGraph,
     DualGraph,
+    Server,
     Stream,
     WalkResult,
     csr_from_edges,
@@ -23,7 +24,7 @@ from .core import (  # noqa: F401
 )
 
 __all__ = [
-    "Context", "DeviceArray", "DeviceGraph", "DualGraph", "Stream", "stream_params", "WalkResult", "csr_from_edges", "incremental_update", "incremental_update_dev",
+    "Context", "DeviceArray", "DeviceGraph", "DualGraph", "Server", "Stream", "stream_params", "WalkResult", "csr_from_edges", "incremental_update", "incremental_update_dev",
     "params", "walk_decremental", "walk_heuristic", "walk_incremental", "DygError", "WalkParams",
     "WalkStats", "RunStats", "LIB_PATH", "BIN_DIR",
 ]

@@ -62,6 +62,11 @@ class StreamStats(C.Structure):
         ("walk_ms", C.c_double), ("total_ms", C.c_double)]
 
 
+class ServeResult(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("queue_ms", C.c_double), ("latency_ms", C.c_double),
+                ("hops", C.c_uint64)]
+
+
 DYG_STREAM_INSERT, DYG_STREAM_DELETE = 0, 1
 DYG_STREAM_PLAIN = 1
 OUTCOMES = {1: "weight", 2: "found", 3: "added", 4: "skip", 5: "g_only", 6: "recovered",
@@ -131,6 +136,11 @@ _SIGS = {
     "dyg_stream_graph": (vp, [vp, C.c_int]),
     "dyg_stream_position": (C.c_uint64, [vp]),
     "dyg_stream_free": (None, [vp]),
+    "dyg_serve_open": (C.c_int32, [vp, C.c_uint32, C.c_uint32, C.c_uint64, C.POINTER(vp)]),
+    "dyg_serve_submit": (C.c_int32, [vp, C.c_int, u32p, u32p, f32p, C.c_uint32,
+                                     C.POINTER(C.c_uint64)]),
+    "dyg_serve_wait": (C.c_int32, [vp, C.c_uint64, i32p, C.POINTER(ServeResult)]),
+    "dyg_serve_close": (None, [vp]),
     "dyg_mtx_read_graph": (C.c_int32, [C.c_char_p, C.c_int, C.c_int, C.POINTER(vp)]),
     "dyg_mtx_read_edges": (C.c_int32, [C.c_char_p, C.c_int, C.POINTER(vp)]),
     "dyg_mtx_nv": (C.c_uint32, [vp]),
@@ -161,6 +171,9 @@ _SYNTH_SIGS = {
                                   C.c_uint64]),
     "dyg_synth_insertions": (vp, [C.c_uint32, u32p, u32p, C.c_uint64, f32p, C.c_uint64,
                                   C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.c_uint64]),
+    "dyg_synth_insertions_walk": (vp, [C.c_uint32, u32p, u32p, C.c_uint64, C.c_uint64,
+                                       C.c_uint32, C.c_uint32, C.c_float, C.c_float,
+                                       C.c_uint64]),
     "dyg_synth_deletions": (vp, [C.c_uint32, u32p, u32p, f32p, C.c_uint64, C.c_uint64,
                                  C.c_uint64]),
     "dyg_synth_write_mtx": (C.c_int, [C.c_char_p, u32p, u32p, f32p, C.c_uint64, C.c_int]),

@@ -436,3 +436,47 @@ class Stream:
             self.close()
         except Exception:
             pass
+
+
+class Server:
+    """Request loop over a Stream (dyg_serve): requests are admitted into
+    slots as slots and the memory budget allow, their inputs are copied while
+    earlier requests run, and they are applied in admission order."""
+
+    def __init__(self, stream: Stream, slots: int = 4, max_request: int = 10000,
+                 budget_bytes: int = 0):
+        self.stream = stream
+        self._lib = stream._lib
+        h = C.c_void_p()
+        check(self._lib.dyg_serve_open(stream.handle, slots, max_request, budget_bytes,
+                                       C.byref(h)), stream.ctx.handle)
+        self.handle = h
+
+    def submit(self, op: str, u, v, w=None) -> int:
+        """Blocks until admitted; returns the ticket."""
+        u, v = _u32(u), _u32(v)
+        ww = _f32(w) if w is not None else None
+        t = C.c_uint64()
+        code = L.DYG_STREAM_INSERT if op == "ins" else L.DYG_STREAM_DELETE
+        check(self._lib.dyg_serve_submit(self.handle, code, ptr(u, L.u32p), ptr(v, L.u32p),
+                                         ptr(ww, L.f32p), len(u), C.byref(t)),
+              self.stream.ctx.handle)
+        return t.value
+
+    def wait(self, ticket: int, n: int) -> tuple[np.ndarray, dict]:
+        out = np.zeros(max(n, 1), np.int32)
+        res = L.ServeResult()
+        check(self._lib.dyg_serve_wait(self.handle, ticket, ptr(out, L.i32p), C.byref(res)),
+              self.stream.ctx.handle)
+        return out[:n], {k: getattr(res, k) for k, _ in L.ServeResult._fields_}
+
+    def close(self) -> None:
+        if self.handle:
+            self._lib.dyg_serve_close(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

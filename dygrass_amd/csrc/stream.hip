@@ -33,7 +33,12 @@
 
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
 #include <cstring>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
 #include <unordered_set>
 #include <vector>
 
@@ -472,59 +477,65 @@ WalkLaunch walk_args(dyg_stream s, dyg_graph g, const dyg_walk_params& p, uint32
         if (_n) hipLaunchKernelGGL(kernel, dim3((_n + 255) / 256), dim3(256), 0, st, __VA_ARGS__); \
     } while (0)
 
-dyg_status apply_insert(dyg_stream s, uint32_t n, dyg_stream_stats* out) {
+dyg_status apply_insert(dyg_stream s, const uint32_t* du, const uint32_t* dv, const float* dw,
+                        uint32_t n, int32_t* dcls, unsigned long long* dstats, bool timed,
+                        dyg_stream_stats* out) {
     hipStream_t st = s->ctx->stream;
     dyg_graph G = s->G, H = s->H;
-    LAUNCH_1D(k_batch_keys, n, s->u, s->v, n, s->k_in, s->i_in);
+    LAUNCH_1D(k_batch_keys, n, du, dv, n, s->k_in, s->i_in);
     dyg_status r = sort_keys(s, s->k_in, s->k_out, s->i_in, s->i_out, n);
     if (r != DYG_OK) return r;
     LAUNCH_1D(k_ins_groups, n, s->k_out, s->i_out, n, G->d_rows, G->d_adj, H->d_rows, H->d_adj,
-              s->cls, s->flag_g, s->keep, s->mode);
-    r = device_edits_batch(G, s->u, s->v, s->w, s->flag_g, n);
+              dcls, s->flag_g, s->keep, s->mode);
+    r = device_edits_batch(G, du, dv, dw, s->flag_g, n);
     if (r != DYG_OK) return r;
-    LAUNCH_1D(k_ins_weights, n, s->k_out, s->i_out, n, s->mode, s->w, G->d_rows, G->d_adj,
+    LAUNCH_1D(k_ins_weights, n, s->k_out, s->i_out, n, s->mode, dw, G->d_rows, G->d_adj,
               H->d_rows, H->d_adj);
     r = select_flagged(s, s->keep, s->sel, s->cnt, n);
     if (r != DYG_OK) return r;
-    LAUNCH_1D(k_gather, n, s->sel, s->cnt, s->u, s->v, s->w, s->base, false, s->cu, s->cv, s->cw,
+    LAUNCH_1D(k_gather, n, s->sel, s->cnt, du, dv, dw, s->base, false, s->cu, s->cv, s->cw,
               s->ceid);
     DYG_HIP(s->ctx, hipGetLastError());
-    hipEvent_t e0 = s->ctx->ev0, e1 = s->ctx->ev1;
-    DYG_HIP(s->ctx, hipEventRecord(e0, st));
-    DYG_HIP(s->ctx, launch_walk_incremental(walk_args(s, H, s->sp.ins, n), st));
-    DYG_HIP(s->ctx, hipEventRecord(e1, st));
+    if (timed) DYG_HIP(s->ctx, hipEventRecord(s->ctx->ev0, st));
+    WalkLaunch a = walk_args(s, H, s->sp.ins, n);
+    a.stats = dstats;
+    DYG_HIP(s->ctx, launch_walk_incremental(a, st));
+    if (timed) DYG_HIP(s->ctx, hipEventRecord(s->ctx->ev1, st));
     r = device_edits_batch(H, s->cu, s->cv, s->cw, s->cfound, n, s->cnt);
     if (r != DYG_OK) return r;
-    LAUNCH_1D(k_outcome, n, s->sel, s->cnt, s->cfound, s->cls, kOutFound, kOutAdded);
+    LAUNCH_1D(k_outcome, n, s->sel, s->cnt, s->cfound, dcls, kOutFound, kOutAdded);
     DYG_HIP(s->ctx, hipGetLastError());
     (void)out;
     return DYG_OK;
 }
 
-dyg_status apply_delete(dyg_stream s, uint32_t n, dyg_stream_stats* out) {
+dyg_status apply_delete(dyg_stream s, const uint32_t* du, const uint32_t* dv, uint32_t n,
+                        int32_t* dcls, unsigned long long* dstats, bool timed,
+                        dyg_stream_stats* out) {
     hipStream_t st = s->ctx->stream;
     dyg_graph G = s->G, H = s->H;
     const uint32_t ms = s->sp.del.max_steps, hw = s->sp.heur_walkers;
-    LAUNCH_1D(k_batch_keys, n, s->u, s->v, n, s->k_in, s->i_in);
+    LAUNCH_1D(k_batch_keys, n, du, dv, n, s->k_in, s->i_in);
     dyg_status r = sort_keys(s, s->k_in, s->k_out, s->i_in, s->i_out, n);
     if (r != DYG_OK) return r;
     LAUNCH_1D(k_del_groups, n, s->k_out, s->i_out, n, G->d_rows, G->d_adj, H->d_rows, H->d_adj,
-              s->cls, s->flag_g, s->flag_h, s->keep);
-    r = device_deletes_batch(G, s->u, s->v, s->flag_g, n, nullptr);
+              dcls, s->flag_g, s->flag_h, s->keep);
+    r = device_deletes_batch(G, du, dv, s->flag_g, n, nullptr);
     if (r != DYG_OK) return r;
-    r = device_deletes_batch(H, s->u, s->v, s->flag_h, n, nullptr);
+    r = device_deletes_batch(H, du, dv, s->flag_h, n, nullptr);
     if (r != DYG_OK) return r;
     r = select_flagged(s, s->keep, s->sel, s->cnt, n);
     if (r != DYG_OK) return r;
-    LAUNCH_1D(k_gather, n, s->sel, s->cnt, s->u, s->v, (const float*)nullptr, s->base, true, s->cu,
+    LAUNCH_1D(k_gather, n, s->sel, s->cnt, du, dv, (const float*)nullptr, s->base, true, s->cu,
               s->cv, (float*)nullptr, s->ceid);
     DYG_HIP(s->ctx, hipGetLastError());
     WalkLaunch a = walk_args(s, G, s->sp.del, n);
     a.w = nullptr;
     a.path = s->path;
-    DYG_HIP(s->ctx, hipEventRecord(s->ctx->ev0, st));
+    a.stats = dstats;
+    if (timed) DYG_HIP(s->ctx, hipEventRecord(s->ctx->ev0, st));
     DYG_HIP(s->ctx, launch_walk_decremental(a, st));
-    DYG_HIP(s->ctx, hipEventRecord(s->ctx->ev1, st));
+    if (timed) DYG_HIP(s->ctx, hipEventRecord(s->ctx->ev1, st));
     LAUNCH_1D(k_path_counts, n, s->cfound, s->cnt, n, s->pc, s->nopath);
     size_t tb = s->temp_bytes;
     DYG_HIP(s->ctx, hipcub::DeviceScan::ExclusiveSum(s->temp, tb, s->pc, s->poff, (int)n, st));
@@ -556,7 +567,7 @@ dyg_status apply_delete(dyg_stream s, uint32_t n, dyg_stream_stats* out) {
               H->d_adj, s->xflag, s->xw);
     r = device_edits_batch(H, s->xu, s->xv, s->xw, s->xflag, ncand);
     if (r != DYG_OK) return r;
-    LAUNCH_1D(k_outcome, n, s->sel, s->cnt, s->cfound, s->cls, kOutRecovered, kOutHeuristic);
+    LAUNCH_1D(k_outcome, n, s->sel, s->cnt, s->cfound, dcls, kOutRecovered, kOutHeuristic);
     DYG_HIP(s->ctx, hipGetLastError());
     if (out) out->candidates = ncand;
     return DYG_OK;
@@ -783,7 +794,9 @@ extern "C" dyg_status dyg_stream_apply(dyg_stream s, int op, const uint32_t* u, 
         if (op == DYG_STREAM_INSERT)
             DYG_HIP(ctx, hipMemcpyAsync(s->w, w, n * 4ull, hipMemcpyHostToDevice, str));
         DYG_HIP(ctx, hipMemsetAsync(s->stats, 0, 3 * sizeof(unsigned long long), str));
-        dyg_status r = op == DYG_STREAM_INSERT ? apply_insert(s, n, &rs) : apply_delete(s, n, &rs);
+        dyg_status r = op == DYG_STREAM_INSERT
+                           ? apply_insert(s, s->u, s->v, s->w, n, s->cls, s->stats, true, &rs)
+                           : apply_delete(s, s->u, s->v, n, s->cls, s->stats, true, &rs);
         if (r != DYG_OK) return r;
         s->h_cls.resize(n);
         unsigned long long hst[3];
@@ -829,3 +842,391 @@ extern "C" dyg_graph dyg_stream_graph(dyg_stream s, int which) {
 extern "C" uint64_t dyg_stream_position(dyg_stream s) { return s ? s->base : 0; }
 
 extern "C" void dyg_stream_free(dyg_stream s) { delete s; }
+
+// ---------------------------------------------------------------------------
+// Served form: a request loop over one stream.  Requests are admitted into
+// slots (pinned host + device buffers of one request) while slots and the
+// memory budget allow; an admitted request's input is copied to the device
+// at once on a copy stream, so it overlaps the requests ahead of it; a worker
+// thread launches requests in arrival order on the compute stream without
+// waiting for the previous one to finish (deletions wait once, inside, for
+// their walks); a completion thread copies results back on the copy stream
+// and timestamps them.  Order of application = order of admission, so every
+// result equals the one the same request gets from dyg_stream_apply in the
+// same sequence.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct ServeSlot {
+    uint32_t *du = nullptr, *dv = nullptr;
+    float* dw = nullptr;
+    int32_t* dcls = nullptr;
+    unsigned long long* dstats = nullptr;
+    void* host = nullptr;  // pinned: u, v, w, cls, stats
+    uint32_t *hu = nullptr, *hv = nullptr;
+    float* hw = nullptr;
+    int32_t* hcls = nullptr;
+    unsigned long long* hstats = nullptr;
+    hipEvent_t e_in = nullptr, e_done = nullptr, e_out = nullptr;
+    bool busy = false;
+};
+
+struct ServeReq {
+    uint64_t ticket = 0;
+    int op = 0;
+    uint32_t n = 0, slot = 0;
+    uint64_t need = 0;  // admitted bytes
+    double t_submit = 0, t_start = 0, t_done = 0;
+    dyg_status status = DYG_OK;
+    std::string error;
+    std::vector<int32_t> outcome;
+    uint64_t hops = 0;
+    bool done = false;
+};
+
+}  // namespace
+
+struct dyg_serve_s {
+    dyg_stream s = nullptr;
+    hipStream_t copy = nullptr, copy_out = nullptr;  // inputs in, results out
+    uint32_t slot_cap = 0;
+    std::vector<ServeSlot> slots;
+    uint64_t budget = 0, resident = 0, admitted = 0;
+    uint64_t used_est_g = 0, used_est_h = 0;  // arena bump pointers, upper estimates
+    std::mutex mu;
+    std::condition_variable cv_work, cv_inflight, cv_done, cv_admit;
+    std::deque<ServeReq*> queue, inflight;
+    std::unordered_map<uint64_t, ServeReq*> reqs;
+    std::thread worker, completer;
+    bool stop = false, worker_done = false;
+    dyg_status failed = DYG_OK;
+    std::string fail_msg;
+    uint64_t next_ticket = 0;
+};
+
+namespace {
+
+uint64_t request_need(const dyg_serve_s* v, int op, uint32_t n) {
+    // device + pinned buffers are preallocated per slot; what a request adds
+    // is arena growth: 2 slots per appended edge per graph, doubled for row
+    // relocation, plus the recovery appends of a deletion batch
+    const uint64_t rec = op == DYG_STREAM_DELETE
+                             ? (uint64_t)n * (v->s->sp.del.max_steps - 1 + 4 * v->s->sp.heur_walkers)
+                             : (uint64_t)n;
+    return 16ull * 4 * (2ull * n + 2 * rec) + 32ull * n;
+}
+
+// Grows the arenas (and resyncs the host's counts) when the headroom left
+// could not hold the next request's worst case.
+dyg_status ensure_headroom(dyg_serve_s* v, const ServeReq& r) {
+    dyg_stream s = v->s;
+    const uint64_t add = 4ull * (2ull * r.n) * (s->sp.slack + 4) + 8ull * r.n;
+    const uint64_t add_h = r.op == DYG_STREAM_DELETE ? add + 8ull * s->cmax : add;
+    if (v->used_est_g + add <= s->G->cap_slots && v->used_est_h + add_h <= s->H->cap_slots) {
+        v->used_est_g += add;
+        v->used_est_h += add_h;
+        return DYG_OK;
+    }
+    dyg_status st = device_edits_end(s->G, nullptr);  // waits for the compute stream
+    if (st == DYG_OK) st = device_edits_end(s->H, nullptr);
+    if (st == DYG_OK) st = device_edits_prepare(s->G, s->sp.max_batch, 2ull * r.n + s->sp.max_batch);
+    if (st == DYG_OK) st = device_edits_prepare(s->H, s->cmax, 2ull * r.n + s->cmax);
+    if (st != DYG_OK) return st;
+    v->used_est_g = s->G->used_slots + add;
+    v->used_est_h = s->H->used_slots + add_h;
+    size_t mfree = 0, mtotal = 0;  // arena growth is resident state from now on
+    if (hipMemGetInfo(&mfree, &mtotal) == hipSuccess) {
+        std::lock_guard<std::mutex> lk(v->mu);
+        v->resident = std::max<uint64_t>(v->resident, mtotal - mfree - v->admitted);
+    }
+    if (v->used_est_g > s->G->cap_slots || v->used_est_h > s->H->cap_slots) {
+        set_error(s->ctx, "serve: arena headroom could not be reserved");
+        return DYG_ERR_NOMEM;
+    }
+    return DYG_OK;
+}
+
+void serve_worker(dyg_serve_s* v) {
+    dyg_stream s = v->s;
+    hipStream_t cs = s->ctx->stream;
+    (void)hipSetDevice(s->ctx->device);
+    for (;;) {
+        ServeReq* r = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(v->mu);
+            v->cv_work.wait(lk, [&] { return v->stop || !v->queue.empty(); });
+            if (v->queue.empty()) {  // stop requested and drained
+                v->worker_done = true;
+                v->cv_inflight.notify_all();
+                return;
+            }
+            r = v->queue.front();
+            v->queue.pop_front();
+        }
+        ServeSlot& sl = v->slots[r->slot];
+        r->t_start = now_ms();
+        dyg_status st = v->failed;
+        if (st == DYG_OK) st = ensure_headroom(v, *r);
+        if (st == DYG_OK && hipStreamWaitEvent(cs, sl.e_in, 0) != hipSuccess) st = DYG_ERR_HIP;
+        if (st == DYG_OK && hipMemsetAsync(sl.dstats, 0, 3 * sizeof(unsigned long long), cs) !=
+                                hipSuccess)
+            st = DYG_ERR_HIP;
+        dyg_stream_stats rs{};
+        if (st == DYG_OK)
+            st = r->op == DYG_STREAM_INSERT
+                     ? apply_insert(s, sl.du, sl.dv, sl.dw, r->n, sl.dcls, sl.dstats, false, &rs)
+                     : apply_delete(s, sl.du, sl.dv, r->n, sl.dcls, sl.dstats, false, &rs);
+        if (st == DYG_OK) {
+            s->base += r->n;
+            bool ok = hipEventRecord(sl.e_done, cs) == hipSuccess &&
+                      hipStreamWaitEvent(v->copy_out, sl.e_done, 0) == hipSuccess &&
+                      hipMemcpyAsync(sl.hcls, sl.dcls, r->n * 4ull, hipMemcpyDeviceToHost,
+                                     v->copy_out) == hipSuccess &&
+                      hipMemcpyAsync(sl.hstats, sl.dstats, 3 * sizeof(unsigned long long),
+                                     hipMemcpyDeviceToHost, v->copy_out) == hipSuccess &&
+                      hipEventRecord(sl.e_out, v->copy_out) == hipSuccess;
+            if (!ok) st = DYG_ERR_HIP;
+        }
+        std::lock_guard<std::mutex> lk(v->mu);
+        if (st != DYG_OK) {
+            r->status = st;
+            r->error = dyg_last_error(s->ctx);
+            if (v->failed == DYG_OK) {
+                v->failed = st;
+                v->fail_msg = r->error;
+            }
+        }
+        v->inflight.push_back(r);
+        v->cv_inflight.notify_one();
+    }
+}
+
+void serve_completer(dyg_serve_s* v) {
+    (void)hipSetDevice(v->s->ctx->device);
+    for (;;) {
+        ServeReq* r = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(v->mu);
+            v->cv_inflight.wait(lk, [&] { return v->worker_done || !v->inflight.empty(); });
+            if (v->inflight.empty()) return;  // the worker has finished and all is delivered
+            r = v->inflight.front();
+            v->inflight.pop_front();
+        }
+        ServeSlot& sl = v->slots[r->slot];
+        if (r->status == DYG_OK) {
+            if (hipEventSynchronize(sl.e_out) != hipSuccess) {
+                r->status = DYG_ERR_HIP;
+                r->error = "serve: result copy failed";
+            } else {
+                r->outcome.assign(sl.hcls, sl.hcls + r->n);
+                r->hops = sl.hstats[0];
+            }
+        }
+        r->t_done = now_ms();
+        std::lock_guard<std::mutex> lk(v->mu);
+        sl.busy = false;
+        v->admitted -= r->need;
+        r->done = true;
+        v->cv_done.notify_all();
+        v->cv_admit.notify_all();
+    }
+}
+
+}  // namespace
+
+extern "C" dyg_status dyg_serve_open(dyg_stream s, uint32_t slots, uint32_t max_request,
+                                     uint64_t budget_bytes, dyg_serve* out) {
+    if (!s || !out || slots == 0 || max_request == 0) return DYG_ERR_ARG;
+    *out = nullptr;
+    dyg_ctx ctx = s->ctx;
+    if (s->sp.flags & DYG_STREAM_PLAIN) {
+        set_error(ctx, "serve: the plain form is served by dyg_stream_apply");
+        return DYG_ERR_ARG;
+    }
+    if (max_request > s->sp.max_batch) {
+        set_error(ctx, "serve: max_request exceeds the stream's max_batch");
+        return DYG_ERR_ARG;
+    }
+    auto* v = new dyg_serve_s;
+    v->s = s;
+    v->slot_cap = max_request;
+    v->slots.resize(slots);
+    bool ok = hipStreamCreateWithFlags(&v->copy, hipStreamNonBlocking) == hipSuccess &&
+              hipStreamCreateWithFlags(&v->copy_out, hipStreamNonBlocking) == hipSuccess;
+    for (ServeSlot& sl : v->slots) {
+        if (!ok) break;
+        const size_t n = max_request;
+        ok = hipMalloc(&sl.du, n * 4) == hipSuccess && hipMalloc(&sl.dv, n * 4) == hipSuccess &&
+             hipMalloc(&sl.dw, n * 4) == hipSuccess && hipMalloc(&sl.dcls, n * 4) == hipSuccess &&
+             hipMalloc(&sl.dstats, 32) == hipSuccess &&
+             hipHostMalloc(&sl.host, n * 16 + 64, hipHostMallocDefault) == hipSuccess &&
+             hipEventCreateWithFlags(&sl.e_in, hipEventDisableTiming) == hipSuccess &&
+             hipEventCreateWithFlags(&sl.e_done, hipEventDisableTiming) == hipSuccess &&
+             hipEventCreateWithFlags(&sl.e_out, hipEventDisableTiming) == hipSuccess;
+        if (ok) {
+            char* h = (char*)sl.host;
+            sl.hu = (uint32_t*)h;
+            sl.hv = (uint32_t*)(h + n * 4);
+            sl.hw = (float*)(h + n * 8);
+            sl.hcls = (int32_t*)(h + n * 12);
+            sl.hstats = (unsigned long long*)(h + n * 16);
+        }
+    }
+    size_t mfree = 0, mtotal = 0;
+    if (ok) ok = hipMemGetInfo(&mfree, &mtotal) == hipSuccess;
+    if (!ok) {
+        set_error(ctx, "serve: slot allocation failed");
+        dyg_serve_close(v);
+        return DYG_ERR_NOMEM;
+    }
+    v->budget = budget_bytes ? std::min<uint64_t>(budget_bytes, mtotal) : mtotal;
+    v->resident = mtotal - mfree;  // graphs, stream buffers and slots already in place
+    if (v->resident >= v->budget) {
+        set_error(ctx, "serve: resident state already exceeds the memory budget");
+        dyg_serve_close(v);
+        return DYG_ERR_ADMISSION;
+    }
+    v->used_est_g = s->G->used_slots;
+    v->used_est_h = s->H->used_slots;
+    v->worker = std::thread(serve_worker, v);
+    v->completer = std::thread(serve_completer, v);
+    *out = v;
+    return DYG_OK;
+}
+
+extern "C" dyg_status dyg_serve_submit(dyg_serve v, int op, const uint32_t* u, const uint32_t* w_v,
+                                       const float* w, uint32_t n, uint64_t* ticket) {
+    if (!v || !ticket || (op != DYG_STREAM_INSERT && op != DYG_STREAM_DELETE) ||
+        (n && (!u || !w_v || (op == DYG_STREAM_INSERT && !w))))
+        return DYG_ERR_ARG;
+    dyg_ctx ctx = v->s->ctx;
+    const double t0 = now_ms();
+    if (n == 0 || n > v->slot_cap) {
+        set_error(ctx, "serve submit: request size must be in [1, max_request]");
+        return DYG_ERR_ARG;
+    }
+    const uint32_t V = v->s->G->V;
+    for (uint32_t i = 0; i < n; ++i)
+        if (u[i] >= V || w_v[i] >= V) {
+            set_error(ctx, "serve submit: vertex id out of range");
+            return DYG_ERR_ARG;
+        }
+    const uint64_t need = request_need(v, op, n);
+    auto* r = new ServeReq;
+    r->op = op;
+    r->n = n;
+    r->need = need;
+    r->t_submit = t0;
+    uint32_t slot = 0;
+    {
+        std::unique_lock<std::mutex> lk(v->mu);
+        if (v->failed != DYG_OK) {
+            set_error(ctx, "serve: stopped after an error: " + v->fail_msg);
+            delete r;
+            return v->failed;
+        }
+        if (v->resident + need > v->budget) {
+            set_error(ctx, "serve submit: request cannot fit the memory budget");
+            delete r;
+            return DYG_ERR_ADMISSION;
+        }
+        // admission: a free slot and room in the budget; tickets (and the
+        // order of application) follow the order of admission
+        v->cv_admit.wait(lk, [&] {
+            if (v->resident + v->admitted + need > v->budget) return false;
+            for (const ServeSlot& sl : v->slots)
+                if (!sl.busy) return true;
+            return false;
+        });
+        for (slot = 0; v->slots[slot].busy; ++slot) {
+        }
+        v->slots[slot].busy = true;
+        v->admitted += need;
+        r->slot = slot;
+        r->ticket = v->next_ticket++;
+        v->reqs[r->ticket] = r;
+        // the input goes to the device now, ahead of the requests before it
+        ServeSlot& sl = v->slots[slot];
+        std::memcpy(sl.hu, u, n * 4ull);
+        std::memcpy(sl.hv, w_v, n * 4ull);
+        if (op == DYG_STREAM_INSERT) std::memcpy(sl.hw, w, n * 4ull);
+        bool ok = hipMemcpyAsync(sl.du, sl.hu, n * 4ull, hipMemcpyHostToDevice, v->copy) ==
+                      hipSuccess &&
+                  hipMemcpyAsync(sl.dv, sl.hv, n * 4ull, hipMemcpyHostToDevice, v->copy) ==
+                      hipSuccess &&
+                  (op != DYG_STREAM_INSERT ||
+                   hipMemcpyAsync(sl.dw, sl.hw, n * 4ull, hipMemcpyHostToDevice, v->copy) ==
+                       hipSuccess) &&
+                  hipEventRecord(sl.e_in, v->copy) == hipSuccess;
+        if (!ok) {
+            r->status = DYG_ERR_HIP;
+            r->error = "serve submit: input copy failed";
+        }
+        v->queue.push_back(r);
+        *ticket = r->ticket;
+    }
+    v->cv_work.notify_one();
+    return DYG_OK;
+}
+
+extern "C" dyg_status dyg_serve_wait(dyg_serve v, uint64_t ticket, int32_t* outcome,
+                                     dyg_serve_result* res) {
+    if (!v) return DYG_ERR_ARG;
+    ServeReq* r = nullptr;
+    {
+        std::unique_lock<std::mutex> lk(v->mu);
+        auto it = v->reqs.find(ticket);
+        if (it == v->reqs.end()) {
+            set_error(v->s->ctx, "serve wait: unknown ticket");
+            return DYG_ERR_ARG;
+        }
+        r = it->second;
+        v->cv_done.wait(lk, [&] { return r->done; });
+        v->reqs.erase(it);
+    }
+    const dyg_status st = r->status;
+    if (st != DYG_OK) set_error(v->s->ctx, r->error);
+    if (st == DYG_OK && outcome) std::memcpy(outcome, r->outcome.data(), r->n * 4ull);
+    if (res) {
+        res->n = r->n;
+        res->queue_ms = r->t_start - r->t_submit;
+        res->latency_ms = r->t_done - r->t_submit;
+        res->hops = r->hops;
+    }
+    delete r;
+    return st;
+}
+
+extern "C" void dyg_serve_close(dyg_serve v) {
+    if (!v) return;
+    {
+        std::lock_guard<std::mutex> lk(v->mu);
+        v->stop = true;
+    }
+    v->cv_work.notify_all();
+    v->cv_inflight.notify_all();
+    if (v->worker.joinable()) v->worker.join();
+    if (v->completer.joinable()) v->completer.join();
+    if (v->s) {
+        (void)hipStreamSynchronize(v->s->ctx->stream);
+        (void)device_edits_end(v->s->G, nullptr);  // host counts current again
+        (void)device_edits_end(v->s->H, nullptr);
+    }
+    if (v->copy) (void)hipStreamSynchronize(v->copy);
+    if (v->copy_out) (void)hipStreamSynchronize(v->copy_out);
+    for (ServeSlot& sl : v->slots) {
+        void* ps[] = {sl.du, sl.dv, sl.dw, sl.dcls, sl.dstats};
+        for (void* p : ps)
+            if (p) (void)hipFree(p);
+        if (sl.host) (void)hipHostFree(sl.host);
+        if (sl.e_in) (void)hipEventDestroy(sl.e_in);
+        if (sl.e_done) (void)hipEventDestroy(sl.e_done);
+        if (sl.e_out) (void)hipEventDestroy(sl.e_out);
+    }
+    for (auto& kv : v->reqs) delete kv.second;
+    for (ServeReq* r : v->queue) delete r;
+    for (ServeReq* r : v->inflight) delete r;
+    if (v->copy) (void)hipStreamDestroy(v->copy);
+    if (v->copy_out) (void)hipStreamDestroy(v->copy_out);
+    delete v;
+}

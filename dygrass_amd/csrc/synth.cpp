@@ -406,6 +406,53 @@ dyg_synth dyg_synth_insertions(uint32_t V, const uint32_t* u, const uint32_t* v,
     return o;
 }
 
+// Insertions for graphs whose balls grow too fast for the BFS generator
+// (shortcut graphs): the endpoint is where a non-backtracking random walk of
+// d in [dlo,dhi] hops from a uniform source stops, so it lies within d hops;
+// pairs that are already edges of G, or already drawn, are rejected.
+dyg_synth dyg_synth_insertions_walk(uint32_t V, const uint32_t* u, const uint32_t* v, uint64_t m,
+                                    uint64_t count, uint32_t dlo, uint32_t dhi, float wlo,
+                                    float whi, uint64_t seed) {
+    auto* o = new Out;
+    o->V = V;
+    EdgeList g;
+    g.u.assign(u, u + m);
+    g.v.assign(v, v + m);
+    std::vector<uint64_t> rp;
+    std::vector<uint32_t> col;
+    csr_of(V, g, rp, col);
+    std::unordered_set<uint64_t> used;
+    used.reserve(count * 2);
+    Rng r(seed);
+    uint64_t tries = 0;
+    while (o->e.u.size() < count && tries < count * 50) {
+        ++tries;
+        const uint32_t s = (uint32_t)r.below(V);
+        const uint32_t d = dlo + (uint32_t)r.below(dhi - dlo + 1);
+        uint32_t cur = s, prev = 0xFFFFFFFFu;
+        bool ok = true;
+        for (uint32_t k = 0; k < d && ok; ++k) {
+            const uint64_t deg = rp[cur + 1] - rp[cur];
+            if (deg == 0 || (deg == 1 && col[rp[cur]] == prev)) {
+                ok = false;
+                break;
+            }
+            uint32_t nxt;
+            do {
+                nxt = col[rp[cur] + r.below(deg)];
+            } while (nxt == prev);
+            prev = cur;
+            cur = nxt;
+        }
+        if (!ok || cur == s) continue;
+        bool adjacent = false;
+        for (uint64_t p = rp[s]; p < rp[s + 1] && !adjacent; ++p) adjacent = col[p] == cur;
+        if (adjacent || !used.insert(pkey(s, cur)).second) continue;
+        o->e.add(s, cur, r.loguni(wlo, whi));
+    }
+    return o;
+}
+
 // Deletions: uniform edges of G without replacement, skipping those that
 // would leave an endpoint with degree 0.  Returned edges keep G's weight.
 dyg_synth dyg_synth_deletions(uint32_t V, const uint32_t* u, const uint32_t* v, const float* w,

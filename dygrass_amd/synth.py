@@ -76,6 +76,14 @@ def insertions(G: Edges, count: int, dlo: int, dhi: int, wlo: float, whi: float,
     return _take(h)
 
 
+def insertions_walk(G: Edges, count: int, dlo: int, dhi: int, wlo: float, whi: float,
+                    seed: int) -> Edges:
+    """New edges to where a non-backtracking walk of dlo..dhi hops stops."""
+    h = L.synth_lib().dyg_synth_insertions_walk(G.V, ptr(G.u, L.u32p), ptr(G.v, L.u32p), len(G),
+                                                 count, dlo, dhi, wlo, whi, seed)
+    return _take(h)
+
+
 def deletions(G: Edges, count: int, seed: int) -> Edges:
     h = L.synth_lib().dyg_synth_deletions(G.V, ptr(G.u, L.u32p), ptr(G.v, L.u32p),
                                            ptr(G.w, L.f32p), len(G), count, seed)
@@ -216,6 +224,54 @@ def config_batched(scale: float = 1.0, seed: int = 0) -> StreamWorkload:
     mx = max(2, int(50_000 * min(1.0, scale * 4)))
     b = stream_batches(wl.G, 64, mx, 0.7, _seed(4, 2, seed), geometric=True)
     return StreamWorkload(f"rgg{wl.G.V}-knn12-stream", wl.G, wl.H, b, 100.0, 256, 512)
+
+
+@dataclass
+class ServedWorkload:
+    """G, H and a request trace: arrival times (s) and (op, u, v, w)."""
+    name: str
+    G: Edges
+    H: Edges
+    arrivals: np.ndarray
+    requests: list
+    threshold: float = 100.0
+    w_ins: int = 256
+    w_del: int = 512
+
+    @property
+    def updates(self) -> int:
+        return sum(len(r[1]) for r in self.requests)
+
+
+def config_served(scale: float = 1.0, seed: int = 0, count: int = 1500,
+                  rate: float = 1000.0) -> ServedWorkload:
+    """8M-vertex 3D grid plus shortcuts (w log-uniform [0.1,10]), H = tree +
+    ~2M edges; `count` requests with Poisson arrivals at `rate` per second,
+    sizes log-uniform in [1, 10000], 60% insertions (endpoints 2..5 walk hops
+    apart, weights log-uniform [0.1,10]) and 40% deletions (uniform over G)."""
+    wl = config_grid3d(scale, seed)
+    rng = np.random.default_rng(_seed(5, 2, seed))
+    mx = max(2, int(10_000 * min(1.0, scale * 8)))
+    sizes = np.maximum(np.floor(np.exp(rng.uniform(0.0, math.log(mx), count))), 1).astype(int)
+    ops = np.where(rng.uniform(size=count) < 0.6, "ins", "del")
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, count))
+    ins = insertions_walk(wl.G, max(1, int(sizes[ops == "ins"].sum())), 2, 5, 0.1, 10.0,
+                          _seed(5, 3, seed))
+    dels = deletions(wl.G, max(1, int(sizes[ops == "del"].sum())), _seed(5, 4, seed))
+    reqs, ki, kd = [], 0, 0
+    for op, k in zip(ops, sizes):
+        k = int(k)
+        if op == "ins":
+            k = min(k, len(ins) - ki)
+            reqs.append(("ins", ins.u[ki:ki + k], ins.v[ki:ki + k], ins.w[ki:ki + k]))
+            ki += k
+        else:
+            k = min(k, len(dels) - kd)
+            reqs.append(("del", dels.u[kd:kd + k], dels.v[kd:kd + k], None))
+            kd += k
+    keep = [i for i, r in enumerate(reqs) if len(r[1]) > 0]
+    return ServedWorkload(f"{wl.name}-served", wl.G, wl.H, arrivals[keep],
+                          [reqs[i] for i in keep], 100.0, 256, 512)
 
 
 CONFIGS = {

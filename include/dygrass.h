@@ -41,6 +41,7 @@ typedef struct dyg_ctx_s* dyg_ctx;
 typedef struct dyg_graph_s* dyg_graph;
 typedef struct dyg_dual_s* dyg_dual;
 typedef struct dyg_stream_s* dyg_stream;
+typedef struct dyg_serve_s* dyg_serve;
 
 /* ------------------------------------------------------------------ */
 /* Context                                                            */
@@ -310,6 +311,35 @@ dyg_graph dyg_stream_graph(dyg_stream s, int which);
 /* Updates applied so far (the edge id of the next update). */
 uint64_t dyg_stream_position(dyg_stream s);
 void dyg_stream_free(dyg_stream s);
+
+/* ------------------------------------------------------------------ */
+/* Served form: request loop over a stream (no reference counterpart;  */
+/* the reference runs one batch at a time, dynamic_update/main.cu:92)  */
+/* ------------------------------------------------------------------ */
+/* `slots` requests of at most `max_request` updates may be admitted at
+ * once; budget_bytes (0 = device memory size) bounds resident state plus the
+ * worst-case growth of the admitted requests.  Requests are applied in
+ * admission order with the stream's per-batch semantics, so each result
+ * equals what dyg_stream_apply gives the same sequence.  While a server is
+ * open, do not call dyg_stream_apply on its stream. */
+typedef struct {
+    uint32_t n;          /* updates in the request */
+    double queue_ms;     /* submit -> launch */
+    double latency_ms;   /* submit -> results on the host */
+    uint64_t hops;       /* neighbour draws of its walks */
+} dyg_serve_result;
+dyg_status dyg_serve_open(dyg_stream s, uint32_t slots, uint32_t max_request,
+                          uint64_t budget_bytes, dyg_serve* out);
+/* Blocks until the request is admitted (a slot and memory free); returns
+ * DYG_ERR_ADMISSION at once when the request could never fit the budget.
+ * The arrays are copied before the call returns. */
+dyg_status dyg_serve_submit(dyg_serve v, int op, const uint32_t* u, const uint32_t* v_,
+                            const float* w, uint32_t n, uint64_t* ticket);
+/* Waits for one request; outcome[i] (optional) receives DYG_OUT_* codes. */
+dyg_status dyg_serve_wait(dyg_serve v, uint64_t ticket, int32_t* outcome,
+                          dyg_serve_result* res);
+/* Drains the queue and releases the slots (the stream stays usable). */
+void dyg_serve_close(dyg_serve v);
 
 /* ------------------------------------------------------------------ */
 /* .mtx I/O with the reference's parsing rules                        */
