@@ -1,18 +1,26 @@
 #!/usr/bin/env python3
-"""Benchmark of the insertion updater on one MI355X.
+"""Benchmark of the random-walk sparsifier updaters on one MI355X.
 
-Default workload (the plain-at-scale configuration): a 1M-point 8-NN random
-geometric graph (weights 1/distance, ~4.6M edges), sparsifier = maximum-weight
-spanning tree + 2% off-tree edges, 200k insertions between vertices 2..5 hops
-apart, processed as 10 snapshot batches of 20k; threshold 100, 256 walkers,
-100 steps, seed 0.
+Three workloads, all seeded synthetic graphs, every result checked against a
+second implementation in the same run:
 
-A step is one whole run over the 200k insertions starting from the original
-sparsifier (the sparsifier is restored before each step, outside the timed
-window).  Each timed step is bracketed by device synchronisation; inputs are
-resident in HBM when a step starts.  Prints one JSON line.
+batched (the headline `value`): a 4M-point 12-NN random geometric graph
+  (w = 1/distance, ~27M edges) and its sparsifier (maximum-weight spanning
+  tree + 2%); 64 batches of log-uniform size 1..50000 (~380k updates), 70%
+  insertion batches, 30% deletion batches, applied as one stream with
+  per-batch snapshot semantics.  A step is the whole stream from the
+  original graphs (rebuilt outside the timed window).  Compared with the
+  plain form (host edge maps and edit plans, one workgroup per edge, a wait
+  per batch) on the same batches.
+served: an 8M-vertex 3D grid plus 16M shortcuts (~40M edges) and a ~10M-edge
+  sparsifier; 1500 requests with Poisson arrivals (1000/s), sizes log-uniform
+  1..10000, 60/40 insert/delete; admission into slots by memory; p50/p99
+  latency, plus the capacity with every request arriving at once.
+incremental: a 1M-point 8-NN random geometric graph, 200k insertions as 10
+  snapshot batches (the plain-at-scale configuration); the walk kernel's
+  roofline comes from this leg.
 
-  python bench.py [--steps K] [--warmup W] [--mode plain|device] [--no-cpu]
+  python bench.py [--steps K] [--warmup W] [--workload all|batched|served|incremental]
 """
 from __future__ import annotations
 
@@ -20,6 +28,7 @@ import argparse
 import json
 import os
 import sys
+import threading
 import time
 
 import numpy as np
@@ -29,10 +38,15 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "edge updates/s + p50/p99 request latency, served vs plain, at equal kappa"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
 
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def pct(a, q) -> float:
+    return float(np.percentile(np.asarray(a), q)) if len(a) else 0.0
 
 
 def algorithmic_bytes(stats: dict, updates: int, walkers: int) -> float:
@@ -46,7 +60,117 @@ def algorithmic_bytes(stats: dict, updates: int, walkers: int) -> float:
             + 20.0 * updates)
 
 
-def run_stream(ctx, wl, plain: bool, seed: int, block_edges: int = 0):
+def measured_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE, calibrated), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    e = t.get(kernel, {}).get(workload)
+    return None if e is None else float(e["bytes_per_launch"])
+
+
+# ---------------------------------------------------------------------------
+def leg_incremental(args, ctx) -> dict:
+    from dygrass_amd import (DeviceArray, DeviceGraph, csr_from_edges, incremental_update,
+                             incremental_update_dev, params, synth)
+
+    t0 = time.time()
+    wl = synth.config_rgg_insert(args.scale, args.seed)
+    m = len(wl.ins)
+    log(f"incremental: {wl.name} V={wl.G.V} E_H={len(wl.H)} updates={m} "
+        f"({time.time() - t0:.1f}s)")
+    rp, col, w = csr_from_edges(wl.H.V, wl.H.u, wl.H.v, wl.H.w)
+    I = (m + 9) // 10
+
+    def run_mode(plain: bool, steps: int, warmup: int) -> dict:
+        p = params(wl.walkers, 100, wl.threshold, seed=args.seed, per_edge=plain,
+                   host_edits=plain, block_edges=args.block_edges)
+        if not plain:  # inputs resident in HBM before the timed window
+            d = [DeviceArray.from_numpy(ctx, a) for a in (wl.ins.u, wl.ins.v, wl.ins.w)]
+            d_found, d_cond = DeviceArray(ctx, m, np.int32), DeviceArray(ctx, m, np.float32)
+
+        def one_step():
+            H = DeviceGraph(ctx, wl.H.V, rp, col, w, slack=2)  # original sparsifier (untimed)
+            if not plain:
+                H.reserve(m, I)
+            ctx.sync()
+            ts = time.perf_counter()
+            if plain:
+                found, st = incremental_update(H, wl.ins.u, wl.ins.v, wl.ins.w, p)
+            else:
+                st = incremental_update_dev(H, d[0].ptr, d[1].ptr, d[2].ptr, m, d_found.ptr,
+                                            d_cond.ptr, p)
+            ctx.sync()
+            te = time.perf_counter()
+            if not plain:
+                found = d_found.numpy()
+            H.close()
+            return te - ts, found, st
+
+        for _ in range(warmup):
+            one_step()
+        times, walk_ms, upd_ms = [], [], []
+        for i in range(steps):
+            dt, found, st = one_step()
+            times.append(dt)
+            walk_ms.append(st["walk_ms"])
+            upd_ms.append(st["update_ms"])
+            log(f"incremental {'plain' if plain else 'device'} step {i}: {dt * 1e3:.2f} ms "
+                f"(walk kernels {st['walk_ms']:.2f} ms, graph edits {st['update_ms']:.2f} ms)")
+        total = float(np.sum(times))
+        return dict(found=found, stats=st, total=total, steps=steps, value=m * steps / total,
+                    ms_per_step=total / steps * 1e3, walk_ms=float(np.mean(walk_ms)),
+                    edit_ms=float(np.mean(upd_ms)))
+
+    head = run_mode(False, args.steps, args.warmup)
+    stats = head["stats"]
+    launches = int(stats["batches"])
+    bytes_run = algorithmic_bytes(stats, m, wl.walkers)
+    achieved = bytes_run / (head["walk_ms"] / 1e3) / 1e9
+    traffic = measured_traffic("k_walk_pool<8,true,true>", wl.name)
+    out = {
+        "value": head["value"], "ms_per_step": head["ms_per_step"],
+        "workload": f"{wl.name}: {launches} snapshot batches of {I} insertions, threshold "
+                    f"{wl.threshold:g}, {wl.walkers} walkers, 100 steps (work-queue walk "
+                    "kernel, graph edits on the device)",
+        "V": int(wl.G.V), "E_H": int(len(wl.H)), "updates": int(m),
+        "walk_kernel_ms_per_step": head["walk_ms"], "graph_edit_ms_per_step": head["edit_ms"],
+        "walk_steps_per_s": stats["hops"] * head["steps"] / head["total"],
+        "no_path_fraction": float(stats["added"]) / m,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": f"k_walk_pool<W={wl.walkers}, insert, prune>",
+                     "workload": wl.name, "bytes_per_launch": bytes_run / launches,
+                     "avg_launch_ms": head["walk_ms"] / launches},
+    }
+    if not args.no_plain:
+        plain = run_mode(True, max(1, min(args.steps, 3)), 1)
+        out["plain"] = {"value": plain["value"], "ms_per_step": plain["ms_per_step"],
+                        "walk_kernel_ms_per_step": plain["walk_ms"],
+                        "graph_edit_ms_per_step": plain["edit_ms"],
+                        "shape": "one workgroup per edge, graph edits planned on the host"}
+        out["speedup_vs_plain"] = head["value"] / plain["value"]
+        out["plain_decisions_identical"] = bool(np.array_equal(plain["found"], head["found"]))
+    if not args.no_cpu:
+        from oracle import pyoracle as po
+
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        og = po.Graph(wl.H.V, rp, col, w)
+        ts = time.perf_counter()
+        cfound, _ = og.incremental(wl.ins.u, wl.ins.v, wl.ins.w, wl.threshold, wl.walkers,
+                                   seed=args.seed, threads=threads)
+        cdt = time.perf_counter() - ts
+        out["cpu"] = {"value": m / cdt, "cores": threads,
+                      "decisions_identical": bool(np.array_equal(head["found"], cfound))}
+        log(f"incremental cpu: {m / cdt:.0f} edge-updates/s on {threads} threads")
+    return out
+
+
+# ---------------------------------------------------------------------------
+def run_stream(ctx, wl, plain: bool, seed: int, block_edges: int = 0) -> dict:
     """One pass of the whole batch list through a fresh stream; the stream is
     built (graphs uploaded, buffers reserved) outside the timed window."""
     from dygrass_amd import Stream, stream_params
@@ -73,24 +197,20 @@ def run_stream(ctx, wl, plain: bool, seed: int, block_edges: int = 0):
     dt = time.perf_counter() - ts
     _, nh, _ = st.H.info()
     st.close()
-    return dict(seconds=dt, outcomes=np.concatenate(outs), latencies=np.array(lat),
+    return dict(seconds=dt, outcomes=np.concatenate(outs), latencies=np.array(lat) * 1e3,
                 walk_ms=walk_ms, hops=hops, counts=agg, h_slots=nh)
 
 
-def leg_batched(args) -> dict:
-    """Mixed insertion/deletion stream (64 uneven batches) on the 4M-point
-    12-NN graph: device-resident form vs the plain form, same results."""
-    from dygrass_amd import Context, synth
+def leg_batched(args, ctx) -> dict:
+    from dygrass_amd import synth
 
     t0 = time.time()
     wl = synth.config_batched(args.scale, args.seed)
     m = wl.updates
-    log(f"workload {wl.name}: V={wl.G.V} E_G={len(wl.G)} E_H={len(wl.H)} batches="
+    log(f"batched: {wl.name} V={wl.G.V} E_G={len(wl.G)} E_H={len(wl.H)} batches="
         f"{len(wl.batches)} updates={m} ({time.time() - t0:.1f}s)")
-    ctx = Context(0)
-    for i in range(args.warmup):
-        r = run_stream(ctx, wl, False, args.seed, args.block_edges)
-        log(f"batched warmup {i}: {r['seconds'] * 1e3:.1f} ms")
+    for _ in range(args.warmup):
+        run_stream(ctx, wl, False, args.seed, args.block_edges)
     runs = []
     for i in range(args.steps):
         r = run_stream(ctx, wl, False, args.seed, args.block_edges)
@@ -99,104 +219,128 @@ def leg_batched(args) -> dict:
     total = sum(r["seconds"] for r in runs)
     head = runs[-1]
     out = {
-        "value": m * len(runs) / total,
-        "ms_per_step": total / len(runs) * 1e3,
+        "value": m * len(runs) / total, "ms_per_step": total / len(runs) * 1e3,
         "workload": f"{wl.name}: {len(wl.batches)} batches of log-uniform size 1..50000 "
                     f"({m} updates, 70% insertion batches), threshold {wl.threshold:g}, "
                     f"{wl.w_ins} insertion / {wl.w_del} deletion walkers, 100 steps "
                     "(device-resident stream: work-queue walk kernels, device edits)",
         "V": int(wl.G.V), "E_G": int(len(wl.G)), "E_H": int(len(wl.H)), "updates": int(m),
-        "batches": len(wl.batches),
-        "walk_kernel_ms_per_step": head["walk_ms"],
+        "batches": len(wl.batches), "walk_kernel_ms_per_step": head["walk_ms"],
         "walk_steps_per_s": head["hops"] / head["seconds"],
-        "batch_latency_ms": {"p50": float(np.percentile(head["latencies"], 50) * 1e3),
-                             "p99": float(np.percentile(head["latencies"], 99) * 1e3)},
+        "batch_latency_ms": {"p50": pct(head["latencies"], 50),
+                             "p99": pct(head["latencies"], 99)},
         "outcomes": head["counts"],
     }
     if not args.no_plain:
         pr = run_stream(ctx, wl, True, args.seed)
-        log(f"batched plain: {pr['seconds'] * 1e3:.1f} ms (walk kernels {pr['walk_ms']:.1f} ms)")
+        log(f"batched plain: {pr['seconds'] * 1e3:.1f} ms")
         out["plain"] = {"value": m / pr["seconds"], "ms_per_step": pr["seconds"] * 1e3,
                         "walk_kernel_ms_per_step": pr["walk_ms"],
-                        "batch_latency_ms": {
-                            "p50": float(np.percentile(pr["latencies"], 50) * 1e3),
-                            "p99": float(np.percentile(pr["latencies"], 99) * 1e3)},
+                        "batch_latency_ms": {"p50": pct(pr["latencies"], 50),
+                                             "p99": pct(pr["latencies"], 99)},
                         "shape": "host edge maps and edit plans, one workgroup per edge, "
                                  "a wait per batch"}
         out["speedup_vs_plain"] = out["value"] / out["plain"]["value"]
         out["plain_outcomes_identical"] = bool(np.array_equal(pr["outcomes"], head["outcomes"]))
         out["plain_h_slots_identical"] = pr["h_slots"] == head["h_slots"]
-    ctx.close()
+    if not args.no_cpu:
+        out["cpu"] = cpu_sample_batched(wl, args)
     return out
 
 
-def leg_served(args) -> dict:
-    """Request loop on the 8M-vertex 3D grid + shortcuts: Poisson arrivals,
-    admission by memory into slots, p50/p99 latency.  The plain form serves
-    the first requests of the same trace one at a time (FIFO)."""
-    import threading
+def cpu_sample_batched(wl, args) -> dict:
+    """The oracle's insertion walks on H for the stream's first insertion
+    batches (about 20k updates), OpenMP over each batch's edges."""
+    from dygrass_amd import csr_from_edges
+    from oracle import pyoracle as po
 
-    from dygrass_amd import Context, Server, Stream, stream_params, synth
+    threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+    rp, col, w = csr_from_edges(wl.G.V, wl.H.u, wl.H.v, wl.H.w)
+    og = po.Graph(wl.G.V, rp, col, w)
+    n, base, secs = 0, 0, 0.0
+    for op, u, v, ww in wl.batches:
+        if op == "ins" and n < 20000:
+            ts = time.perf_counter()
+            og.walk_inc_batch(u, v, ww, wl.threshold, wl.w_ins, seed=args.seed,
+                              edge_id=np.arange(base, base + len(u), dtype=np.uint64),
+                              threads=threads)
+            secs += time.perf_counter() - ts
+            n += len(u)
+        base += len(u)
+    log(f"batched cpu sample: {n / secs:.0f} walked updates/s on {threads} threads")
+    return {"value": n / secs, "unit": "edge-updates/s", "cores": threads, "kind": "port",
+            "sample": f"insertion walks of the stream's first insertion batches ({n} updates) "
+                      "on the original sparsifier, through the oracle, OpenMP over edges; "
+                      "graph bookkeeping not included"}
+
+
+# ---------------------------------------------------------------------------
+def leg_served(args, ctx) -> dict:
+    from dygrass_amd import Server, Stream, stream_params, synth
 
     t0 = time.time()
     wl = synth.config_served(args.scale, args.seed, args.requests, args.rate)
     m = wl.updates
-    log(f"workload {wl.name}: V={wl.G.V} E_G={len(wl.G)} E_H={len(wl.H)} requests="
-        f"{len(wl.requests)} updates={m} rate={args.rate:g}/s ({time.time() - t0:.1f}s)")
-    ctx = Context(0)
+    n = len(wl.requests)
+    log(f"served: {wl.name} V={wl.G.V} E_G={len(wl.G)} E_H={len(wl.H)} requests={n} "
+        f"updates={m} rate={args.rate:g}/s ({time.time() - t0:.1f}s)")
     maxr = max(len(r[1]) for r in wl.requests)
+    sizes = np.array([len(r[1]) for r in wl.requests])
+    g_edges, h_edges = (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w)
 
-    def params(plain):
+    def sp(plain):
         return stream_params(w_ins=wl.w_ins, w_del=wl.w_del, threshold=wl.threshold,
                              seed=args.seed, max_batch=maxr, max_inserts=m, plain=plain)
 
-    g_edges, h_edges = (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w)
-    st = Stream(ctx, wl.G.V, g_edges, h_edges, params(False))
-    srv = Server(st, slots=args.slots, max_request=maxr)
-    n = len(wl.requests)
-    tickets, lat, done_at = [], np.zeros(n), np.zeros(n)
+    def serve(arrivals):
+        st = Stream(ctx, wl.G.V, g_edges, h_edges, sp(False))
+        srv = Server(st, slots=args.slots, max_request=maxr)
+        tickets, lat, done_at = [], np.zeros(n), np.zeros(n)
 
-    def collect():
-        for i in range(n):
-            while len(tickets) <= i:
-                time.sleep(0.0002)
-            _, res = srv.wait(tickets[i], len(wl.requests[i][1]))
-            lat[i] = res["latency_ms"]
-            done_at[i] = time.perf_counter()
+        def collect():
+            for i in range(n):
+                while len(tickets) <= i:
+                    time.sleep(0.0002)
+                _, res = srv.wait(tickets[i], int(sizes[i]))
+                lat[i] = res["latency_ms"]
+                done_at[i] = time.perf_counter()
 
-    th = threading.Thread(target=collect)
-    th.start()
-    start = time.perf_counter()
-    for i, (op, u, v, w) in enumerate(wl.requests):
-        delay = start + wl.arrivals[i] - time.perf_counter()
-        if delay > 0:
-            time.sleep(delay)
-        tickets.append(srv.submit(op, u, v, w))
-    th.join()
-    span = done_at[-1] - start
-    srv.close()
-    st.close()
-    sizes = np.array([len(r[1]) for r in wl.requests])
+        th = threading.Thread(target=collect)
+        th.start()
+        start = time.perf_counter()
+        for i, (op, u, v, w) in enumerate(wl.requests):
+            delay = start + arrivals[i] - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            tickets.append(srv.submit(op, u, v, w))
+        th.join()
+        span = done_at[-1] - start
+        srv.close()
+        st.close()
+        return span, lat
+
+    span, lat = serve(wl.arrivals)
     small = sizes <= 1000
     out = {
         "value": m / span,
         "workload": f"{wl.name}: {n} requests, Poisson arrivals at {args.rate:g}/s, sizes "
                     f"log-uniform 1..10000 ({m} updates, 60% insertions), threshold "
                     f"{wl.threshold:g}, {wl.w_ins}/{wl.w_del} walkers, {args.slots} slots",
-        "V": int(wl.G.V), "E_G": int(len(wl.G)), "E_H": int(len(wl.H)),
-        "requests": n, "updates": int(m), "span_s": span,
-        "latency_ms": {"p50": float(np.percentile(lat, 50)),
-                       "p99": float(np.percentile(lat, 99))},
-        "latency_ms_le_1k": {"p50": float(np.percentile(lat[small], 50)),
-                             "p99": float(np.percentile(lat[small], 99)),
+        "V": int(wl.G.V), "E_G": int(len(wl.G)), "E_H": int(len(wl.H)), "requests": n,
+        "updates": int(m), "span_s": span,
+        "latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99)},
+        "latency_ms_le_1k": {"p50": pct(lat[small], 50), "p99": pct(lat[small], 99),
                              "requests": int(small.sum())},
     }
-    log(f"served: {out['value']:.0f} updates/s, p50 {out['latency_ms']['p50']:.2f} ms, "
-        f"p99 {out['latency_ms']['p99']:.2f} ms (<=1k: p99 "
-        f"{out['latency_ms_le_1k']['p99']:.2f} ms)")
+    log(f"served at {args.rate:g}/s: {out['value']:.0f} updates/s, p50 {pct(lat, 50):.2f} ms, "
+        f"p99 {pct(lat, 99):.2f} ms (<=1k edges: p99 {pct(lat[small], 99):.2f} ms)")
+    cspan, _ = serve(np.zeros(n))  # every request waiting at t=0: capacity
+    out["capacity"] = {"value": m / cspan, "span_s": cspan,
+                       "note": "same trace, all requests arriving at once"}
+    log(f"served capacity: {m / cspan:.0f} updates/s")
     if not args.no_plain and args.plain_requests > 0:
         k = min(n, args.plain_requests)
-        pst = Stream(ctx, wl.G.V, g_edges, h_edges, params(True))
+        pst = Stream(ctx, wl.G.V, g_edges, h_edges, sp(True))
         plat, busy = [], 0.0
         start = time.perf_counter()
         for i in range(k):
@@ -213,19 +357,18 @@ def leg_served(args) -> dict:
         pst.close()
         pm = int(sizes[:k].sum())
         out["plain"] = {"value": pm / busy, "requests": k, "updates": pm,
-                        "latency_ms": {"p50": float(np.percentile(plat, 50)),
-                                       "p99": float(np.percentile(plat, 99))},
+                        "latency_ms": {"p50": pct(plat, 50), "p99": pct(plat, 99)},
                         "shape": "requests applied one at a time in arrival order, host edge "
                                  "maps and edit plans, one workgroup per edge",
                         "note": "value = updates / busy time over the first requests of the "
                                 "same trace"}
-        out["speedup_vs_plain"] = out["value"] / out["plain"]["value"]
-        log(f"served plain ({k} requests): {out['plain']['value']:.0f} updates/s busy, "
-            f"p99 {out['plain']['latency_ms']['p99']:.1f} ms")
-    ctx.close()
+        out["speedup_vs_plain"] = out["capacity"]["value"] / out["plain"]["value"]
+        log(f"served plain ({k} requests): {pm / busy:.0f} updates/s busy, "
+            f"p99 {pct(plat, 99):.1f} ms")
     return out
 
 
+# ---------------------------------------------------------------------------
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -233,19 +376,18 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--scale", type=float, default=1.0, help="workload scale (tests only)")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--mode", choices=["device", "plain"], default="device")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--workload", choices=["all", "batched", "served", "incremental"],
+                    default="all")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparisons")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores (max 16)")
     ap.add_argument("--block-edges", type=int, default=0,
                     help="edges per workgroup of the work-queue walk kernel (0 = automatic)")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores")
-    ap.add_argument("--workload", choices=["incremental", "batched", "served"],
-                    default="incremental")
     ap.add_argument("--requests", type=int, default=1500, help="served: requests in the trace")
     ap.add_argument("--rate", type=float, default=1000.0, help="served: arrivals per second")
     ap.add_argument("--slots", type=int, default=8, help="served: requests admitted at once")
-    ap.add_argument("--plain-requests", type=int, default=60,
+    ap.add_argument("--plain-requests", type=int, default=40,
                     help="served: requests of the trace the plain form serves")
-    ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run one device-to-device copy of this size (a known-size "
                          "dispatch for scaling hardware byte counters)")
@@ -257,178 +399,56 @@ def main() -> int:
               "this build does not implement", file=sys.stderr)
         return 2
 
-    if args.workload == "served":
-        b = leg_served(args)
-        print(json.dumps({"metric": METRIC, "value": b["value"], "unit": "edge-updates/s",
-                          "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": b["span_s"] * 1e3,
-                          "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-                          "dtype": "f32", "data": "synthetic",
-                          "config": {"workload": b.pop("workload")}, "served": b}), flush=True)
-        return 0
-    if args.workload == "batched":
-        b = leg_batched(args)
-        print(json.dumps({"metric": METRIC, "value": b["value"], "unit": "edge-updates/s",
-                          "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-                          "ms_per_step": b["ms_per_step"], "higher_is_better": True,
-                          "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-                          "data": "synthetic", "config": {"workload": b.pop("workload")},
-                          "batched": b}), flush=True)
-        return 0
+    from dygrass_amd import Context, DeviceArray
 
-    from dygrass_amd import (Context, DeviceArray, DeviceGraph, csr_from_edges,
-                             incremental_update, incremental_update_dev, params)
-    from dygrass_amd import synth
-
-    t0 = time.time()
-    wl = synth.config_rgg_insert(args.scale, args.seed)
-    log(f"workload {wl.name}: V={wl.G.V} E_G={len(wl.G)} E_H={len(wl.H)} "
-        f"updates={len(wl.ins)} ({time.time() - t0:.1f}s)")
-    rp, col, w = csr_from_edges(wl.H.V, wl.H.u, wl.H.v, wl.H.w)
     ctx = Context(0)
-    m = len(wl.ins)
-
-    def run_mode(mode: str, steps: int, warmup: int) -> dict:
-        plain = mode == "plain"
-        p = params(wl.walkers, 100, wl.threshold, seed=args.seed, per_edge=plain,
-                   host_edits=plain, block_edges=args.block_edges)
-        I = (m + 9) // 10
-        if not plain:
-            # inputs resident in HBM before the timed window
-            t_src = DeviceArray.from_numpy(ctx, wl.ins.u)
-            t_dst = DeviceArray.from_numpy(ctx, wl.ins.v)
-            t_w = DeviceArray.from_numpy(ctx, wl.ins.w)
-            t_found = DeviceArray(ctx, m, np.int32)
-            t_cond = DeviceArray(ctx, m, np.float32)
-
-        def one_step():
-            H = DeviceGraph(ctx, wl.H.V, rp, col, w, slack=2)  # restore (untimed)
-            if not plain:
-                H.reserve(m, I)
-            ctx.sync()
-            ts = time.perf_counter()
-            if plain:
-                found, st = incremental_update(H, wl.ins.u, wl.ins.v, wl.ins.w, p)
-            else:
-                st = incremental_update_dev(H, t_src.ptr, t_dst.ptr, t_w.ptr, m, t_found.ptr,
-                                            t_cond.ptr, p)
-            ctx.sync()
-            te = time.perf_counter()
-            if not plain:
-                found = t_found.numpy()
-            H.close()
-            return te - ts, found, st
-
-        for i in range(warmup):
-            dt, _, _ = one_step()
-            log(f"{mode} warmup {i}: {dt * 1e3:.2f} ms")
-        times, walk_ms, upd_ms, st, found = [], [], [], None, None
-        for i in range(steps):
-            dt, found, st = one_step()
-            times.append(dt)
-            walk_ms.append(st["walk_ms"])
-            upd_ms.append(st["update_ms"])
-            log(f"{mode} step {i}: {dt * 1e3:.2f} ms (walk kernels {st['walk_ms']:.2f} ms, "
-                f"graph edits {st['update_ms']:.2f} ms, added {st['added']})")
-        total = float(np.sum(times))
-        return dict(found=found, stats=st, total=total, steps=steps,
-                    value=m * steps / total, ms_per_step=total / steps * 1e3,
-                    walk_ms=float(np.mean(walk_ms)), edit_ms=float(np.mean(upd_ms)))
-
     if args.copy_calibration_mb:
-        nb = args.copy_calibration_mb << 20
-        a_src = DeviceArray(ctx, nb // 4, np.uint32)
-        a_dst = DeviceArray(ctx, nb // 4, np.uint32)
         import ctypes
 
-        ctx._lib.dyg_copy_on_device(ctx.handle, ctypes.c_void_p(a_dst.ptr),
-                                    ctypes.c_void_p(a_src.ptr), nb)
+        nb = args.copy_calibration_mb << 20
+        a, b = DeviceArray(ctx, nb // 4, np.uint32), DeviceArray(ctx, nb // 4, np.uint32)
+        ctx._lib.dyg_copy_on_device(ctx.handle, ctypes.c_void_p(b.ptr), ctypes.c_void_p(a.ptr),
+                                    nb)
+        ctx.sync()
         log(f"calibration copy: {args.copy_calibration_mb} MiB device to device")
-        a_src.close()
-        a_dst.close()
-    plain = (run_mode("plain", max(1, min(args.steps, 3)), 1)
-             if args.mode == "device" and args.steps > 1 else None)
-    head = run_mode(args.mode, args.steps, args.warmup)
-    stats = head["stats"]
-    found = head["found"]
-    launches = int(stats["batches"])
-    bytes_run = algorithmic_bytes(stats, m, wl.walkers)
-    walk_s = head["walk_ms"] / 1e3
-    achieved = bytes_run / walk_s / 1e9 if walk_s > 0 else 0.0
-    kernel = (f"k_walk_pool<W={wl.walkers}, insert, prune>" if args.mode == "device"
-              else f"k_walk<W={wl.walkers}, insert, prune>")
-    shape = ("work-queue walk kernel, graph edits on the device, no host round trip between "
-             "batches" if args.mode == "device"
-             else "one workgroup per edge, graph edits planned on the host between batches")
-
+        a.close()
+        b.close()
+    want = (["batched", "served", "incremental"] if args.workload == "all"
+            else [args.workload])
+    legs = {}
+    for name in want:
+        legs[name] = {"batched": leg_batched, "served": leg_served,
+                      "incremental": leg_incremental}[name](args, ctx)
+    head_name = want[0]
+    head = legs[head_name]
+    inc = legs.get("incremental")
+    cpu = head.pop("cpu", None)
+    if head_name == "incremental" and cpu is not None:
+        cpu = {"value": cpu["value"], "unit": "edge-updates/s", "cores": cpu["cores"],
+               "kind": "port", "sample": "the whole workload through the oracle, OpenMP over "
+                                         "each batch's edges",
+               "decisions_identical": cpu["decisions_identical"]}
     out = {
         "metric": METRIC,
         "value": head["value"],
         "unit": "edge-updates/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": head["ms_per_step"],
+        "steps": args.steps if head_name != "served" else 1,
+        "warmup": args.warmup if head_name != "served" else 0,
+        "ms_per_step": head.get("ms_per_step", head.get("span_s", 0.0) * 1e3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {
-            "workload": f"{wl.name}: incremental update, {launches} snapshot batches of "
-                        f"{(m + 9) // 10} insertions, threshold {wl.threshold:g}, "
-                        f"{wl.walkers} walkers, 100 steps ({shape})",
-            "V": int(wl.G.V), "E_G": int(len(wl.G)), "E_H": int(len(wl.H)), "updates": int(m),
-            "walkers": int(wl.walkers), "threshold": float(wl.threshold), "mode": args.mode,
-            "seed": args.seed, "scale": args.scale,
-        },
-        "walk_steps_per_s": stats["hops"] * head["steps"] / head["total"],
-        "no_path_fraction": float(stats["added"]) / m,
-        "walk_kernel_ms_per_step": head["walk_ms"],
-        "graph_edit_ms_per_step": head["edit_ms"],
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "kernel": kernel,
-            "bytes_per_launch": bytes_run / launches,
-            "avg_launch_ms": head["walk_ms"] / launches,
-        },
-        "cpu_baseline": None,
+        "config": {"workload": head.pop("workload"), "form": head_name},
+        "roofline": inc.pop("roofline") if inc else None,
+        "cpu_baseline": cpu,
     }
-    if plain is not None:
-        out["plain"] = {"value": plain["value"], "ms_per_step": plain["ms_per_step"],
-                        "walk_kernel_ms_per_step": plain["walk_ms"],
-                        "graph_edit_ms_per_step": plain["edit_ms"],
-                        "steps": plain["steps"],
-                        "shape": "one workgroup per edge, graph edits planned on the host"}
-        out["speedup_vs_plain"] = head["value"] / plain["value"]
-        out["plain_decisions_identical"] = bool(np.array_equal(plain["found"], found))
-
-    if not args.no_cpu:
-        from oracle import pyoracle as po
-
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
-        og = po.Graph(wl.H.V, rp, col, w)
-        ts = time.perf_counter()
-        cfound, cst = og.incremental(wl.ins.u, wl.ins.v, wl.ins.w, wl.threshold, wl.walkers,
-                                     seed=args.seed, threads=threads)
-        cdt = time.perf_counter() - ts
-        out["cpu_baseline"] = {
-            "value": m / cdt,
-            "unit": "edge-updates/s",
-            "cores": threads,
-            "kind": "port",
-            "sample": f"the whole workload ({m} insertions, {int(cst.n_batches)} snapshot "
-                      f"batches) through the oracle, OpenMP over each batch's edges",
-            "walk_seconds": cst.walk_seconds,
-        }
-        out["parity_with_cpu"] = bool(np.array_equal(found, cfound))
-        log(f"cpu baseline: {m / cdt:.0f} edge-updates/s on {threads} threads; "
-            f"decisions identical: {out['parity_with_cpu']}")
+    for name, leg in legs.items():
+        out[name] = leg
     print(json.dumps(out), flush=True)
+    ctx.close()
     return 0
 
 
