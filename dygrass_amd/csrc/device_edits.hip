@@ -158,16 +158,21 @@ __global__ void k_del_rows(uint2* __restrict__ rows, uint4* __restrict__ adj,
 // every slot that points to r.  Each such slot is written only by r's thread
 // and only its upper half (.z,.w), so threads never race; the neighbour ids
 // and the row descriptors read here are final once k_write has run.
-__global__ void k_refresh(const uint2* __restrict__ rows, uint4* __restrict__ adj,
-                          const uint32_t* __restrict__ keys, const uint32_t* __restrict__ count,
-                          uint32_t* __restrict__ segstart, uint32_t* __restrict__ segcnt) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_refresh(
+    const uint2* __restrict__ rows, uint4* __restrict__ adj, const uint32_t* __restrict__ keys,
+    const uint32_t* __restrict__ count, uint32_t* __restrict__ segstart,
+    uint32_t* __restrict__ segcnt) {
+    // one wave per sorted entry; only a row's first entry does the work, its
+    // lanes taking the row's neighbours (so a lane's chain of dependent loads
+    // is one neighbour's row, not all of them)
+    const uint32_t p = blockIdx.x * (blockDim.x / 64u) + (threadIdx.x >> 6);
+    const uint32_t lane = __lane_id();
     if (p >= *count) return;
     const uint32_t r = keys[p];
     if (segstart[r] != p) return;
-    segcnt[r] = 0;  // ready for the next batch
+    if (lane == 0) segcnt[r] = 0;  // ready for the next batch
     const uint2 rr = rows[r];
-    for (uint32_t q = 0; q < rr.y; ++q) {
+    for (uint32_t q = lane; q < rr.y; q += 64u) {
         const uint32_t x = adj[rr.x + q].x;
         const uint2 rx = rows[x];
         for (uint32_t k = 0; k < rx.y; ++k) {
@@ -302,8 +307,8 @@ dyg_status device_deletes_batch(dyg_graph g, const uint32_t* src, const uint32_t
     const uint32_t blocks = (2 * n + 255) / 256;
     hipLaunchKernelGGL(k_del_rows, dim3(blocks), dim3(256), 0, s, g->d_rows, g->d_adj, d.keys_out,
                        d.vals_out, count, d.segstart, d.segcnt, src, dst, err);
-    hipLaunchKernelGGL(k_refresh, dim3(blocks), dim3(256), 0, s, g->d_rows, g->d_adj, d.keys_out,
-                       count, d.segstart, d.segcnt);
+    hipLaunchKernelGGL(k_refresh, dim3((2 * n + 3) / 4), dim3(256), 0, s, g->d_rows, g->d_adj,
+                       d.keys_out, count, d.segstart, d.segcnt);
     DYG_HIP(ctx, hipGetLastError());
     return DYG_OK;
 }
@@ -326,9 +331,25 @@ dyg_status device_edits_batch(dyg_graph g, const uint32_t* src, const uint32_t* 
                        (unsigned long long)g->cap_slots, g->slack, err);
     hipLaunchKernelGGL(k_write, dim3(blocks), dim3(256), 0, s, g->d_adj, d.keys_out, d.vals_out,
                        count, d.segstart, d.segcnt, d.rowbase, src, dst, w);
-    hipLaunchKernelGGL(k_refresh, dim3(blocks), dim3(256), 0, s, g->d_rows, g->d_adj, d.keys_out,
-                       count, d.segstart, d.segcnt);
+    hipLaunchKernelGGL(k_refresh, dim3((2 * n + 3) / 4), dim3(256), 0, s, g->d_rows, g->d_adj,
+                       d.keys_out, count, d.segstart, d.segcnt);
     DYG_HIP(ctx, hipGetLastError());
+    return DYG_OK;
+}
+
+dyg_status device_edits_absorb(dyg_graph g, const unsigned long long* meta, uint64_t* added_out) {
+    const uint32_t err = (uint32_t)meta[2];
+    g->used_slots = meta[0];
+    g->nslots = g->dev.base_nslots + 2 * meta[1] - 2 * meta[4];
+    if (added_out) *added_out = meta[1];
+    if (err & 1u) {
+        set_error(g->ctx, "device edits: slot arena exhausted");
+        return DYG_ERR_NOMEM;
+    }
+    if (err & 2u) {
+        set_error(g->ctx, "device edits: deletion of an edge the graph does not hold");
+        return DYG_ERR_STATE;
+    }
     return DYG_OK;
 }
 
@@ -339,19 +360,7 @@ dyg_status device_edits_end(dyg_graph g, uint64_t* added_out) {
     DYG_HIP(ctx, hipMemcpyAsync(meta, g->dev.meta, sizeof(meta), hipMemcpyDeviceToHost,
                                 ctx->stream));
     DYG_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    const uint32_t err = (uint32_t)meta[2];
-    g->used_slots = meta[0];
-    g->nslots = g->dev.base_nslots + 2 * meta[1] - 2 * meta[4];
-    if (added_out) *added_out = meta[1];
-    if (err & 1u) {
-        set_error(ctx, "device edits: slot arena exhausted");
-        return DYG_ERR_NOMEM;
-    }
-    if (err & 2u) {
-        set_error(ctx, "device edits: deletion of an edge the graph does not hold");
-        return DYG_ERR_STATE;
-    }
-    return DYG_OK;
+    return device_edits_absorb(g, meta, added_out);
 }
 
 void DevEdits::release() {

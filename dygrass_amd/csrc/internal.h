@@ -144,6 +144,9 @@ dyg_status device_deletes_batch(dyg_graph g, const uint32_t* src, const uint32_t
 // Waits for the stream; updates the host's slot counts; reports the edges
 // appended since begin().  The pipeline stays usable afterwards.
 dyg_status device_edits_end(dyg_graph g, uint64_t* added);
+// The host-side half of end() for a copy of the 5 meta words already taken
+// (after the caller's own wait on the stream).
+dyg_status device_edits_absorb(dyg_graph g, const unsigned long long* meta, uint64_t* added);
 
 // Walk launchers (walk_kernels.hip).  All pointers are device pointers.
 struct WalkLaunch {

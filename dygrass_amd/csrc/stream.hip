@@ -338,10 +338,12 @@ struct dyg_stream_s {
     unsigned long long *xk_in, *xk_out;
     uint32_t *xi_in, *xi_out;
     unsigned long long* stats;  // [3] walk counters
+    void* pin = nullptr;        // pinned: inputs, outcomes, counters, both graphs' meta
     void* temp = nullptr;
     size_t temp_bytes = 0;
     std::vector<int32_t> h_cls;
     ~dyg_stream_s() {
+        if (pin) (void)hipHostFree(pin);
         if (mem) (void)hipFree(mem);
         if (temp) (void)hipFree(temp);
         if (G) dyg_graph_free(G);
@@ -427,6 +429,11 @@ dyg_status stream_alloc(dyg_stream s) {
         return DYG_ERR_NOMEM;
     }
     s->temp_bytes = t;
+    if (hipHostMalloc(&s->pin, (size_t)n * 16 + 256, hipHostMallocDefault) != hipSuccess) {
+        s->pin = nullptr;
+        set_error(s->ctx, "stream: pinned allocation failed");
+        return DYG_ERR_NOMEM;
+    }
     return DYG_OK;
 }
 
@@ -434,9 +441,10 @@ int key_bits(uint32_t V) { return std::min(64, 32 + 33 - __builtin_clz(std::max<
 
 dyg_status sort_keys(dyg_stream s, unsigned long long* kin, unsigned long long* kout,
                      uint32_t* iin, uint32_t* iout, uint32_t n) {
+    // keys are (min << 32 | max) or all ones: bits 32 + bits(V) order them
     size_t tb = s->temp_bytes;
     DYG_HIP(s->ctx, hipcub::DeviceRadixSort::SortPairs(s->temp, tb, kin, kout, iin, iout, (int)n,
-                                                       0, 64, s->ctx->stream));
+                                                       0, key_bits(s->G->V), s->ctx->stream));
     return DYG_OK;
 }
 
@@ -788,23 +796,37 @@ extern "C" dyg_status dyg_stream_apply(dyg_stream s, int op, const uint32_t* u, 
         if (r != DYG_OK) return r;
         DYG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     } else if (n) {
+        // pinned staging: true async copies, and one wait for the whole batch
         hipStream_t str = ctx->stream;
-        DYG_HIP(ctx, hipMemcpyAsync(s->u, u, n * 4ull, hipMemcpyHostToDevice, str));
-        DYG_HIP(ctx, hipMemcpyAsync(s->v, v, n * 4ull, hipMemcpyHostToDevice, str));
+        char* pin = (char*)s->pin;
+        uint32_t* pu = (uint32_t*)pin;
+        uint32_t* pv = (uint32_t*)(pin + n * 4ull);
+        float* pw = (float*)(pin + n * 8ull);
+        int32_t* pcls = (int32_t*)(pin + n * 12ull);
+        unsigned long long* hst = (unsigned long long*)(pin + n * 16ull);  // 4 words
+        unsigned long long* mg = hst + 4;                                   // 5 words
+        unsigned long long* mh = mg + 5;                                    // 5 words
+        std::memcpy(pu, u, n * 4ull);
+        std::memcpy(pv, v, n * 4ull);
+        if (op == DYG_STREAM_INSERT) std::memcpy(pw, w, n * 4ull);
+        DYG_HIP(ctx, hipMemcpyAsync(s->u, pu, n * 4ull, hipMemcpyHostToDevice, str));
+        DYG_HIP(ctx, hipMemcpyAsync(s->v, pv, n * 4ull, hipMemcpyHostToDevice, str));
         if (op == DYG_STREAM_INSERT)
-            DYG_HIP(ctx, hipMemcpyAsync(s->w, w, n * 4ull, hipMemcpyHostToDevice, str));
+            DYG_HIP(ctx, hipMemcpyAsync(s->w, pw, n * 4ull, hipMemcpyHostToDevice, str));
         DYG_HIP(ctx, hipMemsetAsync(s->stats, 0, 3 * sizeof(unsigned long long), str));
         dyg_status r = op == DYG_STREAM_INSERT
                            ? apply_insert(s, s->u, s->v, s->w, n, s->cls, s->stats, true, &rs)
                            : apply_delete(s, s->u, s->v, n, s->cls, s->stats, true, &rs);
         if (r != DYG_OK) return r;
-        s->h_cls.resize(n);
-        unsigned long long hst[3];
-        DYG_HIP(ctx, hipMemcpyAsync(s->h_cls.data(), s->cls, n * 4ull, hipMemcpyDeviceToHost, str));
-        DYG_HIP(ctx, hipMemcpyAsync(hst, s->stats, sizeof(hst), hipMemcpyDeviceToHost, str));
-        r = device_edits_end(s->G, nullptr);  // waits for the stream
-        if (r == DYG_OK) r = device_edits_end(s->H, nullptr);
+        DYG_HIP(ctx, hipMemcpyAsync(pcls, s->cls, n * 4ull, hipMemcpyDeviceToHost, str));
+        DYG_HIP(ctx, hipMemcpyAsync(hst, s->stats, 3 * 8ull, hipMemcpyDeviceToHost, str));
+        DYG_HIP(ctx, hipMemcpyAsync(mg, s->G->dev.meta, 5 * 8ull, hipMemcpyDeviceToHost, str));
+        DYG_HIP(ctx, hipMemcpyAsync(mh, s->H->dev.meta, 5 * 8ull, hipMemcpyDeviceToHost, str));
+        DYG_HIP(ctx, hipStreamSynchronize(str));
+        r = device_edits_absorb(s->G, mg, nullptr);
+        if (r == DYG_OK) r = device_edits_absorb(s->H, mh, nullptr);
         if (r != DYG_OK) return r;
+        s->h_cls.assign(pcls, pcls + n);
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess) rs.walk_ms = ms;
         rs.hops = hst[0];
