@@ -18,9 +18,13 @@ served: an 8M-vertex 3D grid plus 16M shortcuts (~40M edges) and a ~10M-edge
   latency, plus the capacity with every request arriving at once.
 incremental: a 1M-point 8-NN random geometric graph, 200k insertions as 10
   snapshot batches (the plain-at-scale configuration); the walk kernel's
-  roofline comes from this leg.
+  roofline and the FP64 kappa check come from this leg.
+decremental: a 2M-node triangulated mesh, 100k deletions as 10 snapshot
+  batches (the plain-decremental configuration); deletion-only stream vs
+  the dual graph.
 
-  python bench.py [--steps K] [--warmup W] [--workload all|batched|served|incremental]
+  python bench.py [--steps K] [--warmup W]
+                  [--workload all|batched|served|incremental|decremental]
 """
 from __future__ import annotations
 
@@ -92,6 +96,8 @@ def leg_incremental(args, ctx) -> dict:
             d = [DeviceArray.from_numpy(ctx, a) for a in (wl.ins.u, wl.ins.v, wl.ins.w)]
             d_found, d_cond = DeviceArray(ctx, m, np.int32), DeviceArray(ctx, m, np.float32)
 
+        final = {}
+
         def one_step():
             H = DeviceGraph(ctx, wl.H.V, rp, col, w, slack=2)  # original sparsifier (untimed)
             if not plain:
@@ -107,6 +113,7 @@ def leg_incremental(args, ctx) -> dict:
             te = time.perf_counter()
             if not plain:
                 found = d_found.numpy()
+            final["H"] = H.download()  # updated sparsifier (untimed)
             H.close()
             return te - ts, found, st
 
@@ -123,7 +130,7 @@ def leg_incremental(args, ctx) -> dict:
         total = float(np.sum(times))
         return dict(found=found, stats=st, total=total, steps=steps, value=m * steps / total,
                     ms_per_step=total / steps * 1e3, walk_ms=float(np.mean(walk_ms)),
-                    edit_ms=float(np.mean(upd_ms)))
+                    edit_ms=float(np.mean(upd_ms)), H=final["H"])
 
     head = run_mode(False, args.steps, args.warmup)
     stats = head["stats"]
@@ -154,6 +161,28 @@ def leg_incremental(args, ctx) -> dict:
                         "shape": "one workgroup per edge, graph edits planned on the host"}
         out["speedup_vs_plain"] = head["value"] / plain["value"]
         out["plain_decisions_identical"] = bool(np.array_equal(plain["found"], head["found"]))
+        out["plain_sparsifier_identical"] = all(
+            np.array_equal(a.view(np.uint32) if a.dtype == np.float32 else a,
+                           b.view(np.uint32) if b.dtype == np.float32 else b)
+            for a, b in zip(plain["H"], head["H"]))
+    if not args.no_kappa:
+        from dygrass_amd.kappa import kappa
+
+        hrp, hcol, hw = head["H"]
+        rows = np.repeat(np.arange(wl.H.V, dtype=np.int64), np.diff(hrp).astype(np.int64))
+        once = hcol > rows  # each undirected edge once
+        dense = (np.concatenate([wl.G.u, wl.ins.u]), np.concatenate([wl.G.v, wl.ins.v]),
+                 np.concatenate([wl.G.w, wl.ins.w]))
+        ts = time.perf_counter()
+        k_after = kappa(wl.G.V, dense, (rows[once], hcol[once], hw[once]), tol=1e-8)
+        k_before = kappa(wl.G.V, dense, (wl.H.u, wl.H.v, wl.H.w), tol=1e-8)
+        out["kappa"] = {"before": k_before, "after": k_after,
+                        "edges_H_after": int(once.sum()),
+                        "definition": "lambda_max(L_G+ins + 1e-6 I, L_H + 1e-6 I), FP64 "
+                                      "Lanczos on the host; plain and device H identical, so "
+                                      "kappa is equal by construction",
+                        "seconds": time.perf_counter() - ts}
+        log(f"incremental kappa: {k_before:.1f} -> {k_after:.1f}")
     if not args.no_cpu:
         from oracle import pyoracle as po
 
@@ -195,10 +224,10 @@ def run_stream(ctx, wl, plain: bool, seed: int, block_edges: int = 0) -> dict:
             agg[k] = agg.get(k, 0) + stt[k]
     ctx.sync()
     dt = time.perf_counter() - ts
-    _, nh, _ = st.H.info()
+    H = st.H.download()  # final sparsifier (untimed)
     st.close()
     return dict(seconds=dt, outcomes=np.concatenate(outs), latencies=np.array(lat) * 1e3,
-                walk_ms=walk_ms, hops=hops, counts=agg, h_slots=nh)
+                walk_ms=walk_ms, hops=hops, counts=agg, H=H)
 
 
 def leg_batched(args, ctx) -> dict:
@@ -242,7 +271,10 @@ def leg_batched(args, ctx) -> dict:
                                  "a wait per batch"}
         out["speedup_vs_plain"] = out["value"] / out["plain"]["value"]
         out["plain_outcomes_identical"] = bool(np.array_equal(pr["outcomes"], head["outcomes"]))
-        out["plain_h_slots_identical"] = pr["h_slots"] == head["h_slots"]
+        out["plain_sparsifier_identical"] = all(
+            np.array_equal(a.view(np.uint32) if a.dtype == np.float32 else a,
+                           b.view(np.uint32) if b.dtype == np.float32 else b)
+            for a, b in zip(pr["H"], head["H"]))
     if not args.no_cpu:
         out["cpu"] = cpu_sample_batched(wl, args)
     return out
@@ -369,6 +401,71 @@ def leg_served(args, ctx) -> dict:
 
 
 # ---------------------------------------------------------------------------
+def leg_decremental(args, ctx) -> dict:
+    """The plain-decremental configuration: 2M-node triangulated mesh, 100k
+    deletions in 10 snapshot batches, 512 walkers.  Device form: a
+    deletion-only stream (G and H on the device); plain form: the dual graph
+    (G with the host edge map, H as a host key set, one workgroup per edge)."""
+    from dygrass_amd import DualGraph, Stream, csr_from_edges, params, stream_params, synth
+
+    t0 = time.time()
+    wl = synth.config_mesh_delete(args.scale, args.seed)
+    G, H, D = wl.G, wl.H, wl.dels
+    M = len(D)
+    I = (M + 9) // 10
+    log(f"decremental: {wl.name} V={G.V} E_G={len(G)} E_H={len(H)} deletions={M} "
+        f"({time.time() - t0:.1f}s)")
+    sp = stream_params(w_ins=512, w_del=512, threshold=0.0, seed=args.seed, max_batch=I,
+                       max_inserts=4 * M)
+    times, counts = [], None
+    for i in range(args.warmup + args.steps):
+        st = Stream(ctx, G.V, (G.u, G.v, G.w), (H.u, H.v, H.w), sp)
+        ctx.sync()
+        ts = time.perf_counter()
+        agg = {}
+        for b0 in range(0, M, I):
+            _, stt = st.apply("del", D.u[b0:b0 + I], D.v[b0:b0 + I])
+            for k in ("skipped", "g_only", "recovered", "heuristic"):
+                agg[k] = agg.get(k, 0) + stt[k]
+        ctx.sync()
+        dt = time.perf_counter() - ts
+        if i >= args.warmup:
+            times.append(dt)
+            log(f"decremental device step {i - args.warmup}: {dt * 1e3:.1f} ms")
+        rp, col, _ = st.H.download()
+        st.close()
+        counts = agg
+    rows = np.repeat(np.arange(G.V, dtype=np.int64), np.diff(rp).astype(np.int64))
+    h_dev = set(zip(rows[col > rows].tolist(), col[col > rows].tolist()))
+    total = float(np.sum(times))
+    out = {"value": M * len(times) / total, "ms_per_step": total / len(times) * 1e3,
+           "workload": f"{wl.name}: {M} deletions in 10 snapshot batches, 512 walkers, 100 "
+                       "steps (deletion-only stream, G and H on the device)",
+           "V": int(G.V), "E_G": int(len(G)), "E_H": int(len(H)), "deletions": M,
+           "outcomes": counts, "H_after": len(h_dev)}
+    if not args.no_plain:
+        grp, gcol, gw = csr_from_edges(G.V, G.u, G.v, G.w)
+        dual = DualGraph(ctx, G.V, grp, gcol, gw, H.u, H.v)
+        p = params(512, 100, 0.0, seed=args.seed, per_edge=True)
+        ts = time.perf_counter()
+        dual.run(D.u, D.v, p)
+        ctx.sync()
+        dt = time.perf_counter() - ts
+        added = dual.added_edges()
+        out["plain"] = {"value": M / dt, "ms_per_step": dt * 1e3,
+                        "shape": "dual graph: host edge map and key set, host-planned deletes, "
+                                 "one workgroup per edge"}
+        out["speedup_vs_plain"] = out["value"] / out["plain"]["value"]
+        out["plain_same_sparsifier_size"] = int(dual.sparse_size) == len(h_dev)
+        gone = set(zip(np.minimum(D.u, D.v).tolist(), np.maximum(D.u, D.v).tolist()))
+        out["plain_recovered_edges_agree"] = all(
+            ((min(a, b), max(a, b)) in h_dev) == ((min(a, b), max(a, b)) not in gone)
+            for a, b in added.tolist())
+        dual.close()
+        log(f"decremental plain: {dt * 1e3:.1f} ms")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -376,10 +473,11 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--scale", type=float, default=1.0, help="workload scale (tests only)")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--workload", choices=["all", "batched", "served", "incremental"],
-                    default="all")
+    ap.add_argument("--workload", choices=["all", "batched", "served", "incremental",
+                                           "decremental"], default="all")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparisons")
+    ap.add_argument("--no-kappa", action="store_true", help="skip the FP64 kappa check")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores (max 16)")
     ap.add_argument("--block-edges", type=int, default=0,
                     help="edges per workgroup of the work-queue walk kernel (0 = automatic)")
@@ -413,12 +511,13 @@ def main() -> int:
         log(f"calibration copy: {args.copy_calibration_mb} MiB device to device")
         a.close()
         b.close()
-    want = (["batched", "served", "incremental"] if args.workload == "all"
+    want = (["batched", "served", "incremental", "decremental"] if args.workload == "all"
             else [args.workload])
     legs = {}
     for name in want:
         legs[name] = {"batched": leg_batched, "served": leg_served,
-                      "incremental": leg_incremental}[name](args, ctx)
+                      "incremental": leg_incremental,
+                      "decremental": leg_decremental}[name](args, ctx)
     head_name = want[0]
     head = legs[head_name]
     inc = legs.get("incremental")

@@ -102,3 +102,33 @@ def test_stream_heuristic_recovery(ctx, plain):
     assert got[0] == want[0] == S.HEURISTIC
     _assert_same_graph(st.G, orc.G)
     _assert_same_graph(st.H, orc.H)
+
+
+def test_deletion_stream_equals_decremental_updater(ctx, mesh_wl):
+    """A deletion-only stream fed the decremental updater's snapshot batches
+    recovers what the decremental oracle (random_walk_decremental restated)
+    recovers: same sparsifier size; every recovered path edge is in the final
+    H unless a later batch deleted it (the reference's added set keeps it)."""
+    from dygrass_amd import Stream, stream_params
+    from oracle import pyoracle as po
+
+    wl = mesh_wl
+    G, H, D = wl.G, wl.H, wl.dels
+    st_o, added = po.decremental_lines(G.V, G.u, G.v, G.w, H.u, H.v, D.u, D.v, seed=0,
+                                       threads=8)
+    sp = stream_params(w_ins=512, w_del=512, threshold=0.0, seed=0, max_batch=len(D),
+                       max_inserts=4 * len(D))
+    st = Stream(ctx, G.V, (G.u, G.v, G.w), (H.u, H.v, H.w), sp)
+    M = len(D)
+    I = (M + 9) // 10
+    for b0 in range(0, M, I):
+        st.apply("del", D.u[b0:b0 + I], D.v[b0:b0 + I])
+    rp, col, _ = st.H.download()
+    rows = np.repeat(np.arange(G.V), np.diff(rp).astype(np.int64))
+    h = set(zip(rows[col > rows].tolist(), col[col > rows].tolist()))
+    assert len(h) == int(st_o.sparse_after)
+    gone = set(zip(np.minimum(D.u, D.v).tolist(), np.maximum(D.u, D.v).tolist()))
+    for a, b in added.tolist():
+        e = (min(a, b), max(a, b))
+        assert (e in h) == (e not in gone), e
+    assert st.G.check() == 0 and st.H.check() == 0

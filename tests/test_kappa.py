@@ -1,0 +1,86 @@
+"""FP64 kappa tool: dense generalized eigenvalues, closed forms (CPU only)."""
+import numpy as np
+import pytest
+import scipy.linalg as la
+
+from dygrass_amd.kappa import SHIFT, kappa
+
+
+def _dense_lap(V, u, v, w):
+    L = np.zeros((V, V))
+    for a, b, x in zip(u, v, w):
+        L[a, a] += x
+        L[b, b] += x
+        L[a, b] -= x
+        L[b, a] -= x
+    return L + SHIFT * np.eye(V)
+
+
+def _cycle(n, w):
+    u = np.arange(n)
+    return u, (u + 1) % n, np.asarray(w, np.float64)
+
+
+def _thomas(lo, di, up, rhs):
+    """Tridiagonal solve (sub-, main-, super-diagonal), no pivoting."""
+    n = len(di)
+    c, d = np.zeros(n), np.zeros(n)
+    c[0], d[0] = up[0] / di[0], rhs[0] / di[0]
+    for i in range(1, n):
+        m = di[i] - lo[i - 1] * c[i - 1]
+        c[i] = up[i] / m if i < n - 1 else 0.0
+        d[i] = (rhs[i] - lo[i - 1] * d[i - 1]) / m
+    x = np.zeros(n)
+    x[-1] = d[-1]
+    for i in range(n - 2, -1, -1):
+        x[i] = d[i] - c[i] * x[i + 1]
+    return x
+
+
+@pytest.mark.parametrize("n", [60, 2500])
+def test_cycle_over_spanning_path_rank_one(n):
+    # L_C = L_P + w_e b b^T with b = e_0 - e_{n-1}, so the pencil (L_C + eps I,
+    # L_P + eps I) has eigenvalues 1 and 1 + w_e b^T (L_P + eps I)^-1 b;
+    # the path matrix is tridiagonal and solved here by the Thomas algorithm
+    rng = np.random.default_rng(n)
+    w = rng.uniform(0.5, 2.0, n)
+    u, v, ww = _cycle(n, w)
+    wp = w[:-1]
+    di = np.zeros(n)
+    di[:-1] += wp
+    di[1:] += wp
+    di += SHIFT
+    b = np.zeros(n)
+    b[0], b[-1] = 1.0, -1.0
+    x = _thomas(-wp, di, -wp, b)
+    want = 1.0 + w[-1] * b @ x
+    got = kappa(n, (u, v, ww), (u[:-1], v[:-1], ww[:-1]))
+    assert got == pytest.approx(want, rel=1e-8)
+    if n == 60:  # the shift is negligible here: 1 + w_e * effective resistance of the path
+        assert got == pytest.approx(1.0 + w[-1] * np.sum(1.0 / wp), rel=1e-3)
+
+
+def test_identical_graphs_give_one():
+    u, v, w = _cycle(40, np.ones(40))
+    assert kappa(40, (u, v, w), (u, v, w)) == pytest.approx(1.0, abs=1e-9)
+
+
+@pytest.mark.parametrize("V", [200, 2100])
+def test_matches_dense_generalized_eigenproblem(V):
+    """Random connected graph vs its spanning tree plus a few edges; the dense
+    reference is built entry by entry here and solved with LAPACK."""
+    rng = np.random.default_rng(V)
+    tu = np.arange(1, V)
+    tv = rng.integers(0, tu)  # random recursive tree
+    m = 3 * V
+    eu, ev = rng.integers(0, V, m), rng.integers(0, V, m)
+    keep = eu != ev
+    gu = np.concatenate([tu, eu[keep]])
+    gv = np.concatenate([tv, ev[keep]])
+    gw = rng.uniform(0.1, 10.0, len(gu))
+    extra = len(tu) + np.arange(0, keep.sum(), 20)
+    hi = np.concatenate([np.arange(len(tu)), extra])
+    G = (gu, gv, gw)
+    H = (gu[hi], gv[hi], gw[hi])
+    want = la.eigh(_dense_lap(V, *G), _dense_lap(V, *H), eigvals_only=True)[-1]
+    assert kappa(V, G, H) == pytest.approx(want, rel=1e-7)
