@@ -17,6 +17,7 @@ from .core import (  # noqa: F401
     incremental_update,
     incremental_update_dev,
     params,
+    read_mtx_auto,
     stream_params,
     walk_decremental,
     walk_heuristic,
@@ -24,7 +25,7 @@ from .core import (  # noqa: F401
 )
 
 __all__ = [
-    "Context", "DeviceArray", "DeviceGraph", "DualGraph", "Server", "Stream", "stream_params", "WalkResult", "csr_from_edges", "incremental_update", "incremental_update_dev",
+    "Context", "DeviceArray", "DeviceGraph", "DualGraph", "Server", "Stream", "stream_params", "read_mtx_auto", "WalkResult", "csr_from_edges", "incremental_update", "incremental_update_dev",
     "params", "walk_decremental", "walk_heuristic", "walk_incremental", "DygError", "WalkParams",
     "WalkStats", "RunStats", "LIB_PATH", "BIN_DIR",
 ]

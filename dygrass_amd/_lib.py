@@ -67,6 +67,11 @@ class ServeResult(C.Structure):
                 ("hops", C.c_uint64)]
 
 
+class MtxInfo(C.Structure):
+    _fields_ = [(k, C.c_int) for k in ("has_header", "has_dimensions", "weighted", "laplacian",
+                                       "triangular", "base")] + [("entries", C.c_uint64)]
+
+
 DYG_STREAM_INSERT, DYG_STREAM_DELETE = 0, 1
 DYG_STREAM_PLAIN = 1
 OUTCOMES = {1: "weight", 2: "found", 3: "added", 4: "skip", 5: "g_only", 6: "recovered",
@@ -143,6 +148,7 @@ _SIGS = {
     "dyg_serve_close": (None, [vp]),
     "dyg_mtx_read_graph": (C.c_int32, [C.c_char_p, C.c_int, C.c_int, C.POINTER(vp)]),
     "dyg_mtx_read_edges": (C.c_int32, [C.c_char_p, C.c_int, C.POINTER(vp)]),
+    "dyg_mtx_read_auto": (C.c_int32, [C.c_char_p, C.POINTER(MtxInfo), C.POINTER(vp)]),
     "dyg_mtx_nv": (C.c_uint32, [vp]),
     "dyg_mtx_nlines": (C.c_uint64, [vp]),
     "dyg_mtx_row_ptr": (u64p, [vp]),

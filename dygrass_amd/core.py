@@ -480,3 +480,25 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def read_mtx_auto(path: str):
+    """Tolerant .mtx reader (dyg_mtx_read_auto): returns (info dict, V, u, v, w)
+    with 0-based ids, self-loops and mirrored entries dropped."""
+    lib = L.lib()
+    info = L.MtxInfo()
+    h = C.c_void_p()
+    check(lib.dyg_mtx_read_auto(path.encode(), C.byref(info), C.byref(h)))
+    try:
+        n = lib.dyg_mtx_nlines(h)
+        V = lib.dyg_mtx_nv(h)
+        if n:
+            u = np.ctypeslib.as_array(lib.dyg_mtx_line_u(h), shape=(n,)).copy()
+            v = np.ctypeslib.as_array(lib.dyg_mtx_line_v(h), shape=(n,)).copy()
+            w = np.ctypeslib.as_array(lib.dyg_mtx_line_w(h), shape=(n,)).copy()
+        else:
+            u = v = np.zeros(0, np.uint32)
+            w = np.zeros(0, np.float32)
+    finally:
+        lib.dyg_mtx_free(h)
+    return {k: getattr(info, k) for k, _ in L.MtxInfo._fields_}, V, u, v, w

@@ -9,6 +9,8 @@
 // and that slot's weight stays 0 (functions.cpp:186-191).
 // Writing follows gpu_graph.cuh:427-454: "u+1 v+1 w" with w printed as C++
 // `ostream << float` prints it (precision 6, %g style).
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -138,6 +140,162 @@ extern "C" dyg_status dyg_mtx_read_graph(const char* path, int base, int weighte
         m->w[m->row_ptr[b] + deg[b]] = wt;
         deg[a]++;
         deg[b]++;
+    }
+    *out = m;
+    return DYG_OK;
+}
+
+namespace {
+
+struct Entry {
+    double a, b, w;
+    int n;  // numbers on the line (2 or 3)
+};
+
+// Lines of a file as (first, second[, third]) numbers; '\r' and trailing
+// blanks ignored; a leading run of '%' lines is the header.
+dyg_status scan_entries(const std::string& s, std::vector<Entry>& out, bool& header,
+                        bool& pattern, bool& symmetric) {
+    header = pattern = symmetric = false;
+    size_t i = 0;
+    const size_t n = s.size();
+    bool in_head = true;
+    while (i < n) {
+        size_t e = s.find('\n', i);
+        if (e == std::string::npos) e = n;
+        std::string line = s.substr(i, e - i);
+        i = e + 1;
+        while (!line.empty() && (line.back() == '\r' || line.back() == ' ' || line.back() == '\t'))
+            line.pop_back();
+        size_t k = 0;
+        while (k < line.size() && (line[k] == ' ' || line[k] == '\t')) ++k;
+        if (k == line.size()) continue;  // blank line
+        if (line[k] == '%') {
+            if (!in_head) return DYG_ERR_FORMAT;  // comments only before the data
+            header = true;
+            if (line.compare(k, 14, "%%MatrixMarket") == 0) {
+                pattern = line.find("pattern") != std::string::npos;
+                symmetric = line.find("symmetric") != std::string::npos;
+            }
+            continue;
+        }
+        in_head = false;
+        Entry en{0, 0, 1.0, 0};
+        const char* c = line.c_str() + k;
+        char* end = nullptr;
+        double v[3];
+        int got = 0;
+        while (got < 3) {
+            v[got] = std::strtod(c, &end);
+            if (end == c) break;
+            c = end;
+            ++got;
+        }
+        while (*c == ' ' || *c == '\t') ++c;
+        if (got < 2 || *c != '\0') return DYG_ERR_FORMAT;
+        en.a = v[0];
+        en.b = v[1];
+        en.n = got;
+        if (got == 3) en.w = v[2];
+        out.push_back(en);
+    }
+    return DYG_OK;
+}
+
+}  // namespace
+
+extern "C" dyg_status dyg_mtx_read_auto(const char* path, dyg_mtx_info* info, dyg_mtx* out) {
+    if (!path || !out) return DYG_ERR_ARG;
+    *out = nullptr;
+    std::string buf;
+    if (!slurp(path, buf)) return DYG_ERR_IO;
+    std::vector<Entry> en;
+    bool header = false, pattern = false, symmetric = false;
+    dyg_status st = scan_entries(buf, en, header, pattern, symmetric);
+    if (st != DYG_OK) return st;
+    if (en.empty()) return DYG_ERR_FORMAT;
+    // a first line "n n k" with integral k >= 1 is a dimensions line if the
+    // data that follows agrees with n as the vertex count
+    size_t first = 0;
+    double dims = 0;
+    const Entry& e0 = en[0];
+    const bool dims_shape = e0.n == 3 && e0.a == e0.b && e0.w == std::floor(e0.w) && e0.w >= 1 &&
+                            e0.a == std::floor(e0.a) && e0.a >= 1;
+    double vmin_rest = 1e300, vmax_rest = -1;
+    for (size_t i = 1; i < en.size(); ++i) {
+        vmin_rest = std::min({vmin_rest, en[i].a, en[i].b});
+        vmax_rest = std::max({vmax_rest, en[i].a, en[i].b});
+    }
+    if (dims_shape && en.size() > 1 &&
+        ((vmin_rest == 1 && vmax_rest <= e0.a) || (vmin_rest == 0 && vmax_rest + 1 <= e0.a))) {
+        first = 1;
+        dims = e0.a;
+    }
+    double vmin = 1e300, vmax = -1;
+    bool weighted = !pattern, laplacian = false;
+    for (size_t i = first; i < en.size(); ++i) {
+        const Entry& x = en[i];
+        if (x.a != std::floor(x.a) || x.b != std::floor(x.b) || x.a < 0 || x.b < 0)
+            return DYG_ERR_FORMAT;
+        vmin = std::min({vmin, x.a, x.b});
+        vmax = std::max({vmax, x.a, x.b});
+        if (x.n == 2) weighted = false;
+        if (x.n == 3 && x.w < 0) laplacian = true;
+    }
+    if (vmin != 0 && vmin != 1) return DYG_ERR_FORMAT;  // ids must start at 0 or 1
+    const int base = (int)vmin;
+    if (vmax - base > 4294967294.0) return DYG_ERR_FORMAT;
+    // full (both triangles stored) when the first off-diagonal entry also
+    // appears transposed; then only the entries above the diagonal are kept
+    bool full = false;
+    for (size_t i = first; i < en.size() && !symmetric; ++i) {
+        if (en[i].a == en[i].b) continue;
+        for (size_t j = first; j < en.size(); ++j)
+            if (en[j].a == en[i].b && en[j].b == en[i].a) {
+                full = true;
+                break;
+            }
+        break;
+    }
+    auto* m = new dyg_mtx_s;
+    for (size_t i = first; i < en.size(); ++i) {
+        const Entry& x = en[i];
+        if (x.a == x.b) continue;               // diagonal / self-loop: no edge
+        if (full && x.a > x.b) continue;        // the mirror of a kept entry
+        float w = weighted && x.n == 3 ? (float)x.w : 1.0f;
+        if (laplacian) w = -w;                  // off-diagonal -w_ij
+        m->lu.push_back((uint32_t)(x.a - base));
+        m->lv.push_back((uint32_t)(x.b - base));
+        m->lw.push_back(w);
+    }
+    m->lines = m->lu.size();
+    const uint32_t nv = dims > 0 ? (uint32_t)dims : (uint32_t)(vmax - base + 1);
+    m->V = nv;
+    std::vector<uint32_t> deg(m->V, 0);
+    for (uint64_t l = 0; l < m->lines; ++l) {
+        deg[m->lu[l]]++;
+        deg[m->lv[l]]++;
+    }
+    m->row_ptr.assign((size_t)m->V + 1, 0);
+    for (uint32_t v = 0; v < m->V; ++v) m->row_ptr[v + 1] = m->row_ptr[v] + deg[v];
+    std::fill(deg.begin(), deg.end(), 0u);
+    m->col.assign(2 * m->lines, 0);
+    m->w.assign(2 * m->lines, 0.0f);
+    for (uint64_t l = 0; l < m->lines; ++l) {
+        const uint32_t a = m->lu[l], b = m->lv[l];
+        m->col[m->row_ptr[a] + deg[a]] = b;
+        m->w[m->row_ptr[a] + deg[a]++] = m->lw[l];
+        m->col[m->row_ptr[b] + deg[b]] = a;
+        m->w[m->row_ptr[b] + deg[b]++] = m->lw[l];
+    }
+    if (info) {
+        info->has_header = header;
+        info->has_dimensions = first == 1;
+        info->weighted = weighted;
+        info->laplacian = laplacian;
+        info->triangular = !full;
+        info->base = base;
+        info->entries = (uint64_t)(en.size() - first);
     }
     *out = m;
     return DYG_OK;

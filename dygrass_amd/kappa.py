@@ -10,7 +10,10 @@ Lanczos driver (scipy.sparse.linalg.eigsh, largest algebraic eigenvalue) on
 L_G x = lambda L_H x, with L_H^-1 applied through a sparse LU factorisation
 (L_H is a spanning tree plus a few percent of edges, so its fill is small).
 
-    python -m dygrass_amd.kappa dense.mtx sparse.mtx [--base 1]
+    python -m dygrass_amd.kappa dense.mtx sparse.mtx
+
+(files read with the tolerant reader: header, dimensions line and id base
+are detected; both files must use the same id base).
 """
 from __future__ import annotations
 
@@ -53,35 +56,21 @@ def kappa(V: int, g_edges, h_edges, tol: float = 1e-10) -> float:
     return float(vals[-1])
 
 
-def _read(path: str, base: int):
-    from ._lib import lib, check
-    import ctypes as C
+def _read(path: str):
+    from .core import read_mtx_auto
 
-    L = lib()
-    h = C.c_void_p()
-    check(L.dyg_mtx_read_edges(path.encode(), 1, C.byref(h)))
-    try:
-        n = L.dyg_mtx_nlines(h)
-        u = np.ctypeslib.as_array(L.dyg_mtx_line_u(h), shape=(n,)).copy()
-        v = np.ctypeslib.as_array(L.dyg_mtx_line_v(h), shape=(n,)).copy()
-        w = np.ctypeslib.as_array(L.dyg_mtx_line_w(h), shape=(n,)).astype(np.float64)
-    finally:
-        L.dyg_mtx_free(h)
-    if base == 0:  # the reader subtracts one from every id (uint32, so 0 wraps)
-        u, v = u + np.uint32(1), v + np.uint32(1)
-    return u.astype(np.int64), v.astype(np.int64), w
+    _, V, u, v, w = read_mtx_auto(path)  # header, dimensions and id base detected
+    return V, (u.astype(np.int64), v.astype(np.int64), w.astype(np.float64))
 
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description="kappa(L_G, L_H) = lambda_max of the pencil")
     ap.add_argument("dense")
     ap.add_argument("sparse")
-    ap.add_argument("--base", type=int, default=1, help="id base of both files (0 or 1)")
     a = ap.parse_args(argv)
-    g = _read(a.dense, a.base)
-    h = _read(a.sparse, a.base)
-    V = int(max(g[0].max(), g[1].max(), h[0].max(), h[1].max())) + 1
-    print(f"CND: {kappa(V, g, h):.6f}")
+    vg, g = _read(a.dense)
+    vh, h = _read(a.sparse)
+    print(f"CND: {kappa(max(vg, vh), g, h):.6f}")
     return 0
 
 

@@ -353,6 +353,20 @@ dyg_status dyg_mtx_read_graph(const char* path, int base, int weighted, dyg_mtx*
 /* Edge list in file order: 1-based ids made 0-based
  * (random_walk_incremental/functions.cpp:261-370). */
 dyg_status dyg_mtx_read_edges(const char* path, int weighted, dyg_mtx* out);
+/* Tolerant reader (the loader of dynamic_update/functions.cpp:35-291,
+ * restated; the reference-exact readers above stay the default): leading
+ * '%' header lines (a MatrixMarket banner's "pattern" / "symmetric" are
+ * honoured), an optional "n n k" dimensions line (taken as one when the data
+ * that follows fits n vertices), 0- or 1-based ids (from the smallest id),
+ * weighted or pattern lines, Laplacian input (any negative weight: diagonal
+ * dropped, edge weight = -value), and full matrices (the first off-diagonal
+ * entry also stored transposed: only entries above the diagonal kept).
+ * Self-loops are dropped.  Gives 0-based edges and a CSR in file order. */
+typedef struct {
+    int has_header, has_dimensions, weighted, laplacian, triangular, base;
+    uint64_t entries;  /* data lines read (after the dimensions line) */
+} dyg_mtx_info;
+dyg_status dyg_mtx_read_auto(const char* path, dyg_mtx_info* info, dyg_mtx* out);
 uint32_t dyg_mtx_nv(dyg_mtx m);
 uint64_t dyg_mtx_nlines(dyg_mtx m);
 const uint64_t* dyg_mtx_row_ptr(dyg_mtx m); /* [V+1], graphs only */

@@ -84,3 +84,18 @@ def test_matches_dense_generalized_eigenproblem(V):
     H = (gu[hi], gv[hi], gw[hi])
     want = la.eigh(_dense_lap(V, *G), _dense_lap(V, *H), eigvals_only=True)[-1]
     assert kappa(V, G, H) == pytest.approx(want, rel=1e-7)
+
+
+def test_kappa_cli_reads_both_files(tmp_path, capsys):
+    from dygrass_amd.kappa import main
+
+    n = 30
+    u = np.arange(n)
+    v = (u + 1) % n
+    (tmp_path / "g.mtx").write_text("".join(f"{a + 1} {b + 1} 1\n" for a, b in zip(u, v)))
+    (tmp_path / "h.mtx").write_text("%%MatrixMarket matrix coordinate real symmetric\n"
+                                    f"{n} {n} {n - 1}\n" +
+                                    "".join(f"{a + 1} {b + 1} 1\n" for a, b in zip(u[:-1], v[:-1])))
+    assert main([str(tmp_path / "g.mtx"), str(tmp_path / "h.mtx")]) == 0
+    got = float(capsys.readouterr().out.split()[-1])
+    assert got == pytest.approx(n, rel=1e-3)  # 1 + resistance of the 29-edge path
