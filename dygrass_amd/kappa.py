@@ -34,6 +34,10 @@ def laplacian(V: int, u, v, w, shift: float = SHIFT) -> sp.csc_matrix:
     w = np.asarray(w, np.float64)
     keep = u != v  # a self-loop adds nothing to a Laplacian
     u, v, w = u[keep], v[keep], w[keep]
+    # canonical edge order: the same graph gives the same sums, bit for bit
+    a, b = np.minimum(u, v), np.maximum(u, v)
+    order = np.lexsort((w, b, a))
+    u, v, w = a[order], b[order], w[order]
     A = sp.coo_matrix((np.concatenate([w, w]), (np.concatenate([u, v]), np.concatenate([v, u]))),
                       shape=(V, V)).tocsr()
     A.sum_duplicates()
@@ -51,7 +55,8 @@ def kappa(V: int, g_edges, h_edges, tol: float = 1e-10) -> float:
         return float(la.eigh(LG.toarray(), LH.toarray(), eigvals_only=True)[-1])
     lu = sla.splu(LH, permc_spec="COLAMD")
     Minv = sla.LinearOperator((V, V), matvec=lu.solve, dtype=np.float64)
-    vals = sla.eigsh(LG, k=1, M=LH, Minv=Minv, which="LA", tol=tol,
+    v0 = np.random.default_rng(0).uniform(0.5, 1.5, V)  # fixed start: reproducible value
+    vals = sla.eigsh(LG, k=1, M=LH, Minv=Minv, which="LA", tol=tol, v0=v0,
                      return_eigenvectors=False)
     return float(vals[-1])
 

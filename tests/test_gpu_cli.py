@@ -80,3 +80,37 @@ def test_decremental_cli_matches_oracle(mesh_wl):
         exp_log = open(log).read().replace(ref, os.path.join("..", "dataset", mesh_wl.name))
         assert _lines(out) == _lines(exp_log)
         assert st.filtered > 0
+
+
+def test_harness_pipeline_end_to_end(grid_wl):
+    """The harness's whole flow (dyGRASS.jl:408-427, 467-549) on one dataset:
+    incremental CLI, updated_dense.mtx, deletions, decremental CLI, final
+    graphs and kappa -- the same files, graphs and kappa as the oracle's run
+    of the same flow."""
+    import numpy as np
+
+    from dygrass_amd import dataset
+    from dygrass_amd.kappa import kappa
+    from oracle import pyoracle as po
+
+    wl = grid_wl
+    dels = synth.deletions(wl.G, 300, 11)
+    with tempfile.TemporaryDirectory() as root:
+        d = synth.write_dataset(root, wl)
+        ref = os.path.join(root, "oracle_copy")
+        shutil.copytree(d, ref)
+        _run_cli(root, ["incremental_update", wl.name, "10", "128"])
+        po.incremental_run(ref, 10.0, 128, seed=0, threads=8)
+        for x in (d, ref):
+            dataset.prepare_decremental(x)
+            synth.write_mtx(os.path.join(x, "del.mtx"), dels.u, dels.v, None, base=1)
+        _run_cli(root, ["decremental_update", wl.name])
+        po.decremental_run(ref, seed=0, threads=8)
+        V, G, H = dataset.final_graphs(d)
+        V2, G2, H2 = dataset.final_graphs(ref)
+        assert V == V2
+        key = lambda E: sorted(zip(E[0].tolist(), E[1].tolist(), E[2].tolist()))  # noqa: E731
+        assert key(G) == key(G2) and key(H) == key(H2)
+        k = kappa(V, G, H)
+        assert k == kappa(V2, G2, H2)
+        assert 1.0 <= k < np.inf
