@@ -74,6 +74,7 @@ class MtxInfo(C.Structure):
 
 DYG_STREAM_INSERT, DYG_STREAM_DELETE = 0, 1
 DYG_STREAM_PLAIN = 1
+DYG_STREAM_GENERAL_ONLY = 2
 OUTCOMES = {1: "weight", 2: "found", 3: "added", 4: "skip", 5: "g_only", 6: "recovered",
             7: "heuristic", 8: "invalid"}
 

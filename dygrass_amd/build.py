@@ -47,7 +47,7 @@ HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall"]
 
 LIB_SOURCES = [
     "walk_kernels.hip", "graph_store.hip", "device_edits.hip", "incremental.hip", "api.hip",
-    "stream.hip",
+    "stream.hip", "stream_small.hip",
     "mtx_io.cpp",
 ]
 CLIS = {"incremental_update": "cli_incremental.cpp", "decremental_update": "cli_decremental.cpp"}

@@ -380,14 +380,17 @@ class DualGraph:
 def stream_params(w_ins: int = 256, w_del: int = 512, threshold: float = 100.0, seed: int = 0,
                   max_batch: int = 50000, max_inserts: int = 0, heur_walkers: int = 16,
                   slack: int = 2, max_steps: int = 100, plain: bool = False,
-                  block_edges: int = 0) -> L.StreamParams:
+                  block_edges: int = 0, general_only: bool = False) -> L.StreamParams:
     """Stream parameters; plain=True selects the reference-shaped form (host
-    edge maps and edit plans, one workgroup per edge, a wait per step)."""
+    edge maps and edit plans, one workgroup per edge, a wait per step);
+    general_only=True keeps small insertion batches on the general pipeline."""
     sp = L.StreamParams()
     sp.ins = params(w_ins, max_steps, threshold, seed, block_edges=block_edges)
     sp.del_ = params(w_del, max_steps, 0.0, seed, block_edges=block_edges)
     sp.heur_walkers, sp.max_batch, sp.max_inserts = heur_walkers, max_batch, max_inserts
-    sp.slack, sp.flags = slack, (L.DYG_STREAM_PLAIN if plain else 0)
+    sp.slack = slack
+    sp.flags = ((L.DYG_STREAM_PLAIN if plain else 0)
+                | (L.DYG_STREAM_GENERAL_ONLY if general_only else 0))
     return sp
 
 

@@ -182,6 +182,58 @@ hipError_t launch_heuristic(const uint2* rows, const uint4* adj, const uint32_t*
 hipError_t launch_apply_delta(uint2* rows, uint4* adj, const uint32_t* delta, uint32_t n_write,
                               uint32_t n_row, hipStream_t s);
 
+// One graph's device state for edits made inside a single workgroup
+// (stream_small.hip): same arena, capacities and counters as the edit pipeline.
+struct BlockGraph {
+    uint2* rows;
+    uint4* adj;
+    uint32_t* cap;
+    unsigned long long* meta;  // DevEdits::meta of the graph
+    unsigned long long cap_slots;
+    uint32_t slack;
+};
+// A batch's results, written by its last kernel straight into pinned host
+// memory (device-visible addresses): the outcome codes and 14 counter words,
+// [0..3] the walk counters (zeroed once read, ready for the next batch),
+// [4..8] G's and [9..13] H's DevEdits::meta.
+constexpr int kReportWords = 14;
+struct Report {
+    int32_t* cls;                  // [n] outcome codes
+    unsigned long long* words;     // [kReportWords]
+    const unsigned long long* mg;  // device meta words of G (5)
+    const unsigned long long* mh;  // device meta words of H (5)
+};
+// Copies cls[0..n) and the counters out as described above.
+hipError_t launch_report(const Report& r, const int32_t* cls, uint32_t n,
+                         unsigned long long* stats, hipStream_t s);
+
+// An insertion batch of at most kSmallBatch edges in two one-workgroup
+// kernels around the walk (device pointers; u, v and w may point into
+// pinned host memory).
+constexpr uint32_t kSmallBatch = 1024;
+struct SmallInsArgs {
+    BlockGraph G, H;
+    const uint32_t* u;
+    const uint32_t* v;
+    const float* w;
+    uint32_t n;
+    uint64_t base;   // stream position of the batch's first update
+    int key_bits;    // significant bits of the (min << 32 | max) keys
+    int32_t* cls;    // outcome per update
+    uint32_t* sel;   // batch position of each kept edge
+    uint32_t* cnt;   // [0]: kept edges, [4]: touched rows
+    uint32_t* touched;  // [2 * kSmallBatch] rows touched by the last append
+    uint32_t* cu;
+    uint32_t* cv;
+    float* cw;
+    uint64_t* ceid;
+    const int32_t* found;  // walk result per kept edge (post kernel)
+    unsigned long long* stats;  // walk counters, reported and zeroed by the post kernel
+    Report rep;
+};
+hipError_t launch_small_ins_pre(const SmallInsArgs& a, hipStream_t s);
+hipError_t launch_small_ins_post(const SmallInsArgs& a, hipStream_t s);
+
 bool valid_walkers(uint32_t w);
 // Validates walkers (power of two, 1..1024) and max_steps (1..32767).
 dyg_status check_params(dyg_ctx ctx, const dyg_walk_params* p);

@@ -214,12 +214,14 @@ __global__ void k_path_counts(const int32_t* __restrict__ found, const uint32_t*
     nopath[j] = live && found[j] == -1 ? 1 : 0;
 }
 
-// cnt[2] = heuristic endpoints, cnt[3] = total path hops
+// cnt[2] = heuristic endpoints, cnt[3] = total path hops; cnt[0..3] also to
+// `out` (pinned host memory)
 __global__ void k_recovery_sizes(const uint32_t* __restrict__ poff, const uint32_t* __restrict__ pc,
-                                 uint32_t n, uint32_t* __restrict__ cnt) {
+                                 uint32_t n, uint32_t* __restrict__ cnt, uint32_t* __restrict__ out) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         cnt[2] = 2u * cnt[1];
         cnt[3] = n ? poff[n - 1] + pc[n - 1] : 0u;
+        for (int i = 0; i < 4; ++i) out[i] = cnt[i];
     }
 }
 
@@ -313,8 +315,7 @@ struct dyg_stream_s {
     uint32_t cmax = 0;  // recovery candidates per deletion batch, at most
     void* mem = nullptr;
     // per-batch buffers (max_batch entries unless noted)
-    uint32_t *u, *v;
-    float* w;
+    uint32_t* in;  // [3 * max_batch]: a batch's u, v, w at stride n (one copy in)
     int32_t* cls;
     unsigned long long *k_in, *k_out;
     uint32_t *i_in, *i_out;
@@ -337,12 +338,17 @@ struct dyg_stream_s {
     int32_t* xflag;
     unsigned long long *xk_in, *xk_out;
     uint32_t *xi_in, *xi_out;
-    unsigned long long* stats;  // [3] walk counters
-    void* pin = nullptr;        // pinned: inputs, outcomes, counters, both graphs' meta
+    unsigned long long* stats;  // [4] walk counters
+    // pinned: inputs (u, v, w), outcomes, report words, recovery sizes; the
+    // device reaches it at pin_dev
+    void* pin = nullptr;
+    char* pin_dev = nullptr;
+    hipEvent_t ev_sync = nullptr;  // polled by wait_spin
     void* temp = nullptr;
     size_t temp_bytes = 0;
     std::vector<int32_t> h_cls;
     ~dyg_stream_s() {
+        if (ev_sync) (void)hipEventDestroy(ev_sync);
         if (pin) (void)hipHostFree(pin);
         if (mem) (void)hipFree(mem);
         if (temp) (void)hipFree(temp);
@@ -352,6 +358,30 @@ struct dyg_stream_s {
 };
 
 namespace {
+
+// Pinned staging: report words at 0, recovery sizes of a deletion batch at
+// 128, then one batch of n (<= max_batch) updates from 256: u, v, w at
+// stride n and the outcome codes.
+constexpr size_t kPinHead = 256, kPinSizes = 128;
+size_t pin_bytes(uint32_t max_batch) { return kPinHead + (size_t)max_batch * 16; }
+struct PinLayout {
+    uint32_t *u, *v;
+    float* w;
+    int32_t* cls;
+    unsigned long long* words;  // kReportWords
+};
+PinLayout pin_layout(char* base, uint32_t n) {
+    PinLayout l;
+    l.words = (unsigned long long*)base;
+    char* b = base + kPinHead;
+    l.u = (uint32_t*)b;
+    l.v = (uint32_t*)(b + n * 4ull);
+    l.w = (float*)(b + n * 8ull);
+    l.cls = (int32_t*)(b + n * 12ull);
+    return l;
+}
+
+int key_bits(uint32_t V) { return std::min(64, 32 + 33 - __builtin_clz(std::max<uint32_t>(V, 1))); }
 
 dyg_status stream_alloc(dyg_stream s) {
     const uint32_t n = s->sp.max_batch;
@@ -363,9 +393,7 @@ dyg_status stream_alloc(dyg_stream s) {
     Arena ar;
     for (int pass = 0; pass < 2; ++pass) {
         ar.off = 0;
-        s->u = ar.take<uint32_t>(n);
-        s->v = ar.take<uint32_t>(n);
-        s->w = ar.take<float>(n);
+        s->in = ar.take<uint32_t>(3 * (size_t)n);
         s->cls = ar.take<int32_t>(n);
         s->k_in = ar.take<unsigned long long>(n);
         s->k_out = ar.take<unsigned long long>(n);
@@ -429,15 +457,40 @@ dyg_status stream_alloc(dyg_stream s) {
         return DYG_ERR_NOMEM;
     }
     s->temp_bytes = t;
-    if (hipHostMalloc(&s->pin, (size_t)n * 16 + 256, hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc(&s->pin, pin_bytes(n), hipHostMallocDefault) != hipSuccess) {
         s->pin = nullptr;
         set_error(s->ctx, "stream: pinned allocation failed");
         return DYG_ERR_NOMEM;
     }
+    void* pd = nullptr;
+    DYG_HIP(s->ctx, hipHostGetDevicePointer(&pd, s->pin, 0));
+    s->pin_dev = (char*)pd;
+    DYG_HIP(s->ctx, hipEventCreateWithFlags(&s->ev_sync, hipEventDisableTiming));
+    DYG_HIP(s->ctx, hipMemsetAsync(s->stats, 0, 4 * sizeof(unsigned long long), s->ctx->stream));
+    DYG_HIP(s->ctx, hipStreamSynchronize(s->ctx->stream));
     return DYG_OK;
 }
 
-int key_bits(uint32_t V) { return std::min(64, 32 + 33 - __builtin_clz(std::max<uint32_t>(V, 1))); }
+// Waits for a recorded event.  Polls it for up to 2 ms before blocking:
+// waking from a blocking wait costs tens of microseconds, about what a batch
+// of a few hundred updates takes.
+hipError_t spin_event(hipEvent_t ev) {
+    const double t0 = now_ms();
+    for (;;) {
+        const hipError_t e = hipEventQuery(ev);
+        if (e != hipErrorNotReady) return e;
+        if (now_ms() - t0 > 2.0) break;
+    }
+    return hipEventSynchronize(ev);
+}
+
+// Waits for everything enqueued on `st` so far (see spin_event).
+dyg_status wait_spin(dyg_ctx ctx, hipStream_t st, hipEvent_t ev) {
+    DYG_HIP(ctx, hipEventRecord(ev, st));
+    DYG_HIP(ctx, spin_event(ev));
+    return DYG_OK;
+}
+
 
 dyg_status sort_keys(dyg_stream s, unsigned long long* kin, unsigned long long* kout,
                      uint32_t* iin, uint32_t* iout, uint32_t n) {
@@ -485,9 +538,53 @@ WalkLaunch walk_args(dyg_stream s, dyg_graph g, const dyg_walk_params& p, uint32
         if (_n) hipLaunchKernelGGL(kernel, dim3((_n + 255) / 256), dim3(256), 0, st, __VA_ARGS__); \
     } while (0)
 
+BlockGraph block_graph(dyg_graph g) {
+    return BlockGraph{g->d_rows, g->d_adj, g->dev.cap, g->dev.meta,
+                      (unsigned long long)g->cap_slots, g->slack};
+}
+
+// Small insertion batch: two one-workgroup kernels around the walk.
+dyg_status apply_insert_small(dyg_stream s, const uint32_t* du, const uint32_t* dv,
+                              const float* dw, uint32_t n, int32_t* dcls,
+                              unsigned long long* dstats, bool timed, const Report& rep) {
+    hipStream_t st = s->ctx->stream;
+    SmallInsArgs a{};
+    a.G = block_graph(s->G);
+    a.H = block_graph(s->H);
+    a.u = du;
+    a.v = dv;
+    a.w = dw;
+    a.n = n;
+    a.base = s->base;
+    a.key_bits = key_bits(s->G->V);
+    a.cls = dcls;
+    a.sel = s->sel;
+    a.cnt = s->cnt;
+    a.cu = s->cu;
+    a.cv = s->cv;
+    a.cw = s->cw;
+    a.ceid = s->ceid;
+    a.found = s->cfound;
+    a.touched = s->ep;  // 2 * max_batch entries, free during insertion batches
+    a.stats = dstats;
+    a.rep = rep;
+    DYG_HIP(s->ctx, launch_small_ins_pre(a, st));
+    if (timed) DYG_HIP(s->ctx, hipEventRecord(s->ctx->ev0, st));
+    WalkLaunch w = walk_args(s, s->H, s->sp.ins, n);
+    w.stats = dstats;
+    DYG_HIP(s->ctx, launch_walk_incremental(w, st));
+    if (timed) DYG_HIP(s->ctx, hipEventRecord(s->ctx->ev1, st));
+    DYG_HIP(s->ctx, launch_small_ins_post(a, st));
+    return DYG_OK;
+}
+
+// Both apply_* enqueue a whole batch (no wait except a deletion batch's one
+// read-back of its recovery sizes) ending with the report `rep`.
 dyg_status apply_insert(dyg_stream s, const uint32_t* du, const uint32_t* dv, const float* dw,
                         uint32_t n, int32_t* dcls, unsigned long long* dstats, bool timed,
-                        dyg_stream_stats* out) {
+                        dyg_stream_stats* out, const Report& rep) {
+    if (n <= std::min(kSmallBatch, s->sp.max_batch) && !(s->sp.flags & DYG_STREAM_GENERAL_ONLY))
+        return apply_insert_small(s, du, dv, dw, n, dcls, dstats, timed, rep);
     hipStream_t st = s->ctx->stream;
     dyg_graph G = s->G, H = s->H;
     LAUNCH_1D(k_batch_keys, n, du, dv, n, s->k_in, s->i_in);
@@ -512,14 +609,14 @@ dyg_status apply_insert(dyg_stream s, const uint32_t* du, const uint32_t* dv, co
     r = device_edits_batch(H, s->cu, s->cv, s->cw, s->cfound, n, s->cnt);
     if (r != DYG_OK) return r;
     LAUNCH_1D(k_outcome, n, s->sel, s->cnt, s->cfound, dcls, kOutFound, kOutAdded);
-    DYG_HIP(s->ctx, hipGetLastError());
+    DYG_HIP(s->ctx, launch_report(rep, dcls, n, dstats, st));
     (void)out;
     return DYG_OK;
 }
 
 dyg_status apply_delete(dyg_stream s, const uint32_t* du, const uint32_t* dv, uint32_t n,
                         int32_t* dcls, unsigned long long* dstats, bool timed,
-                        dyg_stream_stats* out) {
+                        dyg_stream_stats* out, const Report& rep) {
     hipStream_t st = s->ctx->stream;
     dyg_graph G = s->G, H = s->H;
     const uint32_t ms = s->sp.del.max_steps, hw = s->sp.heur_walkers;
@@ -549,16 +646,17 @@ dyg_status apply_delete(dyg_stream s, const uint32_t* du, const uint32_t* dv, ui
     DYG_HIP(s->ctx, hipcub::DeviceScan::ExclusiveSum(s->temp, tb, s->pc, s->poff, (int)n, st));
     r = select_flagged(s, s->nopath, s->npsel, s->cnt + 1, n);
     if (r != DYG_OK) return r;
-    hipLaunchKernelGGL(k_recovery_sizes, dim3(1), dim3(64), 0, st, s->poff, s->pc, n, s->cnt);
+    uint32_t* hc = (uint32_t*)((char*)s->pin + kPinSizes);
+    hipLaunchKernelGGL(k_recovery_sizes, dim3(1), dim3(64), 0, st, s->poff, s->pc, n, s->cnt,
+                       (uint32_t*)(s->pin_dev + kPinSizes));
     LAUNCH_1D(k_heur_endpoints, n, s->npsel, s->cnt, s->cu, s->cv, s->ceid, s->ep, s->ekey);
     DYG_HIP(s->ctx, launch_heuristic(G->d_rows, G->d_adj, s->ep, s->ekey, 2 * n, hw,
                                      s->sp.del.seed, s->path3, st, s->cnt + 2));
     LAUNCH_1D(k_cand_paths, n, s->cfound, s->cnt, s->path, ms, s->poff, s->xu, s->xv);
     DYG_HIP(s->ctx, hipGetLastError());
-    // the one read-back of a deletion batch: recovery list sizes
-    uint32_t hc[4];
-    DYG_HIP(s->ctx, hipMemcpyAsync(hc, s->cnt, sizeof(hc), hipMemcpyDeviceToHost, st));
-    DYG_HIP(s->ctx, hipStreamSynchronize(st));
+    // the one wait of a deletion batch: recovery list sizes
+    r = wait_spin(s->ctx, st, s->ev_sync);
+    if (r != DYG_OK) return r;
     const uint32_t npath = hc[3], n3 = hc[2] * hw;
     const uint32_t ncand = npath + 2 * n3;
     if (ncand > s->cmax) {
@@ -576,7 +674,7 @@ dyg_status apply_delete(dyg_stream s, const uint32_t* du, const uint32_t* dv, ui
     r = device_edits_batch(H, s->xu, s->xv, s->xw, s->xflag, ncand);
     if (r != DYG_OK) return r;
     LAUNCH_1D(k_outcome, n, s->sel, s->cnt, s->cfound, dcls, kOutRecovered, kOutHeuristic);
-    DYG_HIP(s->ctx, hipGetLastError());
+    DYG_HIP(s->ctx, launch_report(rep, dcls, n, dstats, st));
     if (out) out->candidates = ncand;
     return DYG_OK;
 }
@@ -796,37 +894,37 @@ extern "C" dyg_status dyg_stream_apply(dyg_stream s, int op, const uint32_t* u, 
         if (r != DYG_OK) return r;
         DYG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     } else if (n) {
-        // pinned staging: true async copies, and one wait for the whole batch
+        // pinned staging: the inputs go in with one copy (a small insertion
+        // batch reads them in place), the last kernel writes the results
+        // back, and the batch has one wait
         hipStream_t str = ctx->stream;
-        char* pin = (char*)s->pin;
-        uint32_t* pu = (uint32_t*)pin;
-        uint32_t* pv = (uint32_t*)(pin + n * 4ull);
-        float* pw = (float*)(pin + n * 8ull);
-        int32_t* pcls = (int32_t*)(pin + n * 12ull);
-        unsigned long long* hst = (unsigned long long*)(pin + n * 16ull);  // 4 words
-        unsigned long long* mg = hst + 4;                                   // 5 words
-        unsigned long long* mh = mg + 5;                                    // 5 words
-        std::memcpy(pu, u, n * 4ull);
-        std::memcpy(pv, v, n * 4ull);
-        if (op == DYG_STREAM_INSERT) std::memcpy(pw, w, n * 4ull);
-        DYG_HIP(ctx, hipMemcpyAsync(s->u, pu, n * 4ull, hipMemcpyHostToDevice, str));
-        DYG_HIP(ctx, hipMemcpyAsync(s->v, pv, n * 4ull, hipMemcpyHostToDevice, str));
-        if (op == DYG_STREAM_INSERT)
-            DYG_HIP(ctx, hipMemcpyAsync(s->w, pw, n * 4ull, hipMemcpyHostToDevice, str));
-        DYG_HIP(ctx, hipMemsetAsync(s->stats, 0, 3 * sizeof(unsigned long long), str));
-        dyg_status r = op == DYG_STREAM_INSERT
-                           ? apply_insert(s, s->u, s->v, s->w, n, s->cls, s->stats, true, &rs)
-                           : apply_delete(s, s->u, s->v, n, s->cls, s->stats, true, &rs);
+        const PinLayout hp = pin_layout((char*)s->pin, n), dp = pin_layout(s->pin_dev, n);
+        const bool ins = op == DYG_STREAM_INSERT;
+        std::memcpy(hp.u, u, n * 4ull);
+        std::memcpy(hp.v, v, n * 4ull);
+        if (ins) std::memcpy(hp.w, w, n * 4ull);
+        const uint32_t *du = s->in, *dv = s->in + n;
+        const float* dw = (const float*)(s->in + 2ull * n);
+        if (ins && n <= std::min(kSmallBatch, s->sp.max_batch) &&
+            !(s->sp.flags & DYG_STREAM_GENERAL_ONLY)) {
+            du = dp.u;
+            dv = dp.v;
+            dw = dp.w;
+        } else {
+            DYG_HIP(ctx, hipMemcpyAsync(s->in, hp.u, n * (ins ? 12ull : 8ull),
+                                        hipMemcpyHostToDevice, str));
+        }
+        const Report rep{dp.cls, dp.words, s->G->dev.meta, s->H->dev.meta};
+        dyg_status r = ins ? apply_insert(s, du, dv, dw, n, s->cls, s->stats, true, &rs, rep)
+                           : apply_delete(s, du, dv, n, s->cls, s->stats, true, &rs, rep);
         if (r != DYG_OK) return r;
-        DYG_HIP(ctx, hipMemcpyAsync(pcls, s->cls, n * 4ull, hipMemcpyDeviceToHost, str));
-        DYG_HIP(ctx, hipMemcpyAsync(hst, s->stats, 3 * 8ull, hipMemcpyDeviceToHost, str));
-        DYG_HIP(ctx, hipMemcpyAsync(mg, s->G->dev.meta, 5 * 8ull, hipMemcpyDeviceToHost, str));
-        DYG_HIP(ctx, hipMemcpyAsync(mh, s->H->dev.meta, 5 * 8ull, hipMemcpyDeviceToHost, str));
-        DYG_HIP(ctx, hipStreamSynchronize(str));
-        r = device_edits_absorb(s->G, mg, nullptr);
-        if (r == DYG_OK) r = device_edits_absorb(s->H, mh, nullptr);
+        r = wait_spin(ctx, str, s->ev_sync);
         if (r != DYG_OK) return r;
-        s->h_cls.assign(pcls, pcls + n);
+        const unsigned long long* hst = hp.words;
+        r = device_edits_absorb(s->G, hp.words + 4, nullptr);
+        if (r == DYG_OK) r = device_edits_absorb(s->H, hp.words + 9, nullptr);
+        if (r != DYG_OK) return r;
+        s->h_cls.assign(hp.cls, hp.cls + n);
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess) rs.walk_ms = ms;
         rs.hops = hst[0];
@@ -869,27 +967,25 @@ extern "C" void dyg_stream_free(dyg_stream s) { delete s; }
 // Served form: a request loop over one stream.  Requests are admitted into
 // slots (pinned host + device buffers of one request) while slots and the
 // memory budget allow; an admitted request's input is copied to the device
-// at once on a copy stream, so it overlaps the requests ahead of it; a worker
+// at once on a copy stream, so it overlaps the requests ahead of it (a small
+// insertion request is read in place from pinned memory instead); a worker
 // thread launches requests in arrival order on the compute stream without
 // waiting for the previous one to finish (deletions wait once, inside, for
-// their walks); a completion thread copies results back on the copy stream
-// and timestamps them.  Order of application = order of admission, so every
+// their walks); each request's last kernel writes its results into the
+// slot's pinned memory, and a completion thread waits for them and
+// timestamps them.  Order of application = order of admission, so every
 // result equals the one the same request gets from dyg_stream_apply in the
 // same sequence.
 // ---------------------------------------------------------------------------
 namespace {
 
 struct ServeSlot {
-    uint32_t *du = nullptr, *dv = nullptr;
-    float* dw = nullptr;
+    uint32_t* din = nullptr;  // u, v, w of the request at stride n
     int32_t* dcls = nullptr;
     unsigned long long* dstats = nullptr;
-    void* host = nullptr;  // pinned: u, v, w, cls, stats
-    uint32_t *hu = nullptr, *hv = nullptr;
-    float* hw = nullptr;
-    int32_t* hcls = nullptr;
-    unsigned long long* hstats = nullptr;
-    hipEvent_t e_in = nullptr, e_done = nullptr, e_out = nullptr;
+    void* host = nullptr;     // pinned, laid out by pin_layout; device address hdev
+    char* hdev = nullptr;
+    hipEvent_t e_in = nullptr, e_done = nullptr;
     bool busy = false;
 };
 
@@ -903,6 +999,7 @@ struct ServeReq {
     std::string error;
     std::vector<int32_t> outcome;
     uint64_t hops = 0;
+    bool copied = false;  // input copied to the slot's device buffer (else read in place)
     bool done = false;
 };
 
@@ -910,7 +1007,7 @@ struct ServeReq {
 
 struct dyg_serve_s {
     dyg_stream s = nullptr;
-    hipStream_t copy = nullptr, copy_out = nullptr;  // inputs in, results out
+    hipStream_t copy = nullptr;  // inputs in
     uint32_t slot_cap = 0;
     std::vector<ServeSlot> slots;
     uint64_t budget = 0, resident = 0, admitted = 0;
@@ -989,25 +1086,26 @@ void serve_worker(dyg_serve_s* v) {
         r->t_start = now_ms();
         dyg_status st = v->failed;
         if (st == DYG_OK) st = ensure_headroom(v, *r);
-        if (st == DYG_OK && hipStreamWaitEvent(cs, sl.e_in, 0) != hipSuccess) st = DYG_ERR_HIP;
-        if (st == DYG_OK && hipMemsetAsync(sl.dstats, 0, 3 * sizeof(unsigned long long), cs) !=
-                                hipSuccess)
+        if (st == DYG_OK && r->copied && hipStreamWaitEvent(cs, sl.e_in, 0) != hipSuccess)
             st = DYG_ERR_HIP;
         dyg_stream_stats rs{};
+        const uint32_t n = r->n;
+        const PinLayout dp = pin_layout(sl.hdev, n);
+        const uint32_t *du = dp.u, *dv = dp.v;
+        const float* dw = dp.w;
+        if (r->copied) {
+            du = sl.din;
+            dv = sl.din + n;
+            dw = (const float*)(sl.din + 2ull * n);
+        }
+        const Report rep{dp.cls, dp.words, s->G->dev.meta, s->H->dev.meta};
         if (st == DYG_OK)
             st = r->op == DYG_STREAM_INSERT
-                     ? apply_insert(s, sl.du, sl.dv, sl.dw, r->n, sl.dcls, sl.dstats, false, &rs)
-                     : apply_delete(s, sl.du, sl.dv, r->n, sl.dcls, sl.dstats, false, &rs);
+                     ? apply_insert(s, du, dv, dw, n, sl.dcls, sl.dstats, false, &rs, rep)
+                     : apply_delete(s, du, dv, n, sl.dcls, sl.dstats, false, &rs, rep);
         if (st == DYG_OK) {
-            s->base += r->n;
-            bool ok = hipEventRecord(sl.e_done, cs) == hipSuccess &&
-                      hipStreamWaitEvent(v->copy_out, sl.e_done, 0) == hipSuccess &&
-                      hipMemcpyAsync(sl.hcls, sl.dcls, r->n * 4ull, hipMemcpyDeviceToHost,
-                                     v->copy_out) == hipSuccess &&
-                      hipMemcpyAsync(sl.hstats, sl.dstats, 3 * sizeof(unsigned long long),
-                                     hipMemcpyDeviceToHost, v->copy_out) == hipSuccess &&
-                      hipEventRecord(sl.e_out, v->copy_out) == hipSuccess;
-            if (!ok) st = DYG_ERR_HIP;
+            s->base += n;
+            if (hipEventRecord(sl.e_done, cs) != hipSuccess) st = DYG_ERR_HIP;
         }
         std::lock_guard<std::mutex> lk(v->mu);
         if (st != DYG_OK) {
@@ -1036,12 +1134,13 @@ void serve_completer(dyg_serve_s* v) {
         }
         ServeSlot& sl = v->slots[r->slot];
         if (r->status == DYG_OK) {
-            if (hipEventSynchronize(sl.e_out) != hipSuccess) {
+            if (spin_event(sl.e_done) != hipSuccess) {
                 r->status = DYG_ERR_HIP;
-                r->error = "serve: result copy failed";
+                r->error = "serve: request failed on the device";
             } else {
-                r->outcome.assign(sl.hcls, sl.hcls + r->n);
-                r->hops = sl.hstats[0];
+                const PinLayout hp = pin_layout((char*)sl.host, r->n);
+                r->outcome.assign(hp.cls, hp.cls + r->n);
+                r->hops = hp.words[0];
             }
         }
         r->t_done = now_ms();
@@ -1073,27 +1172,21 @@ extern "C" dyg_status dyg_serve_open(dyg_stream s, uint32_t slots, uint32_t max_
     v->s = s;
     v->slot_cap = max_request;
     v->slots.resize(slots);
-    bool ok = hipStreamCreateWithFlags(&v->copy, hipStreamNonBlocking) == hipSuccess &&
-              hipStreamCreateWithFlags(&v->copy_out, hipStreamNonBlocking) == hipSuccess;
+    bool ok = hipStreamCreateWithFlags(&v->copy, hipStreamNonBlocking) == hipSuccess;
     for (ServeSlot& sl : v->slots) {
         if (!ok) break;
         const size_t n = max_request;
-        ok = hipMalloc(&sl.du, n * 4) == hipSuccess && hipMalloc(&sl.dv, n * 4) == hipSuccess &&
-             hipMalloc(&sl.dw, n * 4) == hipSuccess && hipMalloc(&sl.dcls, n * 4) == hipSuccess &&
+        void* hd = nullptr;
+        ok = hipMalloc(&sl.din, n * 12) == hipSuccess && hipMalloc(&sl.dcls, n * 4) == hipSuccess &&
              hipMalloc(&sl.dstats, 32) == hipSuccess &&
-             hipHostMalloc(&sl.host, n * 16 + 64, hipHostMallocDefault) == hipSuccess &&
+             hipMemsetAsync(sl.dstats, 0, 32, ctx->stream) == hipSuccess &&
+             hipHostMalloc(&sl.host, pin_bytes(max_request), hipHostMallocDefault) == hipSuccess &&
+             hipHostGetDevicePointer(&hd, sl.host, 0) == hipSuccess &&
              hipEventCreateWithFlags(&sl.e_in, hipEventDisableTiming) == hipSuccess &&
-             hipEventCreateWithFlags(&sl.e_done, hipEventDisableTiming) == hipSuccess &&
-             hipEventCreateWithFlags(&sl.e_out, hipEventDisableTiming) == hipSuccess;
-        if (ok) {
-            char* h = (char*)sl.host;
-            sl.hu = (uint32_t*)h;
-            sl.hv = (uint32_t*)(h + n * 4);
-            sl.hw = (float*)(h + n * 8);
-            sl.hcls = (int32_t*)(h + n * 12);
-            sl.hstats = (unsigned long long*)(h + n * 16);
-        }
+             hipEventCreateWithFlags(&sl.e_done, hipEventDisableTiming) == hipSuccess;
+        sl.hdev = (char*)hd;
     }
+    if (ok) ok = hipStreamSynchronize(ctx->stream) == hipSuccess;
     size_t mfree = 0, mtotal = 0;
     if (ok) ok = hipMemGetInfo(&mfree, &mtotal) == hipSuccess;
     if (!ok) {
@@ -1168,19 +1261,18 @@ extern "C" dyg_status dyg_serve_submit(dyg_serve v, int op, const uint32_t* u, c
         r->ticket = v->next_ticket++;
         v->reqs[r->ticket] = r;
         // the input goes to the device now, ahead of the requests before it
+        // (a small insertion request is read in place from pinned memory)
         ServeSlot& sl = v->slots[slot];
-        std::memcpy(sl.hu, u, n * 4ull);
-        std::memcpy(sl.hv, w_v, n * 4ull);
-        if (op == DYG_STREAM_INSERT) std::memcpy(sl.hw, w, n * 4ull);
-        bool ok = hipMemcpyAsync(sl.du, sl.hu, n * 4ull, hipMemcpyHostToDevice, v->copy) ==
-                      hipSuccess &&
-                  hipMemcpyAsync(sl.dv, sl.hv, n * 4ull, hipMemcpyHostToDevice, v->copy) ==
-                      hipSuccess &&
-                  (op != DYG_STREAM_INSERT ||
-                   hipMemcpyAsync(sl.dw, sl.hw, n * 4ull, hipMemcpyHostToDevice, v->copy) ==
-                       hipSuccess) &&
-                  hipEventRecord(sl.e_in, v->copy) == hipSuccess;
-        if (!ok) {
+        const PinLayout hp = pin_layout((char*)sl.host, n);
+        const bool ins = op == DYG_STREAM_INSERT;
+        std::memcpy(hp.u, u, n * 4ull);
+        std::memcpy(hp.v, w_v, n * 4ull);
+        if (ins) std::memcpy(hp.w, w, n * 4ull);
+        r->copied = !(ins && n <= kSmallBatch && !(v->s->sp.flags & DYG_STREAM_GENERAL_ONLY));
+        if (r->copied &&
+            !(hipMemcpyAsync(sl.din, hp.u, n * (ins ? 12ull : 8ull), hipMemcpyHostToDevice,
+                             v->copy) == hipSuccess &&
+              hipEventRecord(sl.e_in, v->copy) == hipSuccess)) {
             r->status = DYG_ERR_HIP;
             r->error = "serve submit: input copy failed";
         }
@@ -1235,20 +1327,17 @@ extern "C" void dyg_serve_close(dyg_serve v) {
         (void)device_edits_end(v->s->H, nullptr);
     }
     if (v->copy) (void)hipStreamSynchronize(v->copy);
-    if (v->copy_out) (void)hipStreamSynchronize(v->copy_out);
     for (ServeSlot& sl : v->slots) {
-        void* ps[] = {sl.du, sl.dv, sl.dw, sl.dcls, sl.dstats};
+        void* ps[] = {sl.din, sl.dcls, sl.dstats};
         for (void* p : ps)
             if (p) (void)hipFree(p);
         if (sl.host) (void)hipHostFree(sl.host);
         if (sl.e_in) (void)hipEventDestroy(sl.e_in);
         if (sl.e_done) (void)hipEventDestroy(sl.e_done);
-        if (sl.e_out) (void)hipEventDestroy(sl.e_out);
     }
     for (auto& kv : v->reqs) delete kv.second;
     for (ServeReq* r : v->queue) delete r;
     for (ServeReq* r : v->inflight) delete r;
     if (v->copy) (void)hipStreamDestroy(v->copy);
-    if (v->copy_out) (void)hipStreamDestroy(v->copy_out);
     delete v;
 }

@@ -283,12 +283,15 @@ typedef struct {
     uint32_t max_batch;     /* largest batch apply() accepts */
     uint64_t max_inserts;   /* arena reservation hint: appends expected per graph */
     uint32_t slack;         /* spare slots given to a relocated row */
-    uint32_t flags;         /* DYG_STREAM_PLAIN or 0 */
+    uint32_t flags;         /* DYG_STREAM_PLAIN, DYG_STREAM_GENERAL_ONLY or 0 */
 } dyg_stream_params;
 /* The reference's shape: host edge maps and host-planned edits, one
  * one-workgroup-per-edge walk launch per batch, waiting for every result
  * (same results as the default device-resident form). */
 #define DYG_STREAM_PLAIN 1u
+/* Device form only: always use the general per-batch pipeline (by default
+ * insertion batches of at most 1024 edges take a two-kernel path). */
+#define DYG_STREAM_GENERAL_ONLY 2u
 typedef struct {
     uint64_t updates, weight_updates, found, added, skipped, g_only, recovered, heuristic,
         invalid;
