@@ -219,6 +219,7 @@ struct SmallInsArgs {
     uint32_t n;
     uint64_t base;   // stream position of the batch's first update
     int key_bits;    // significant bits of the (min << 32 | max) keys
+    int row_bits;    // significant bits of a vertex id (and of the pad above them)
     int32_t* cls;    // outcome per update
     uint32_t* sel;   // batch position of each kept edge
     uint32_t* cnt;   // [0]: kept edges, [4]: touched rows
@@ -233,6 +234,44 @@ struct SmallInsArgs {
 };
 hipError_t launch_small_ins_pre(const SmallInsArgs& a, hipStream_t s);
 hipError_t launch_small_ins_post(const SmallInsArgs& a, hipStream_t s);
+
+// A deletion batch of at most kSmallBatch edges: one-workgroup kernels
+// before and after the walk on G; a recovery of more than kSmallCand
+// candidate edges is left to the general kernels (flag sizes[4]).
+constexpr uint32_t kSmallCand = 1024;
+struct SmallDelArgs {
+    BlockGraph G, H;
+    const uint32_t* u;
+    const uint32_t* v;
+    uint32_t n;
+    uint64_t base;
+    int key_bits;
+    int row_bits;
+    int32_t* cls;
+    uint32_t* sel;
+    uint32_t* cnt;  // [0] kept, [1] no-path, [2] endpoints, [3] path hops,
+                    // [4], [5], [6] rows listed in touched_g, touched_h, touched_x
+    uint32_t* touched_g;  // [2n] rows of G touched by the deletions
+    uint32_t* touched_h;  // [2n] rows of H touched by the deletions
+    uint32_t* touched_x;  // [2 * candidates] rows of H touched by the recovery
+    uint32_t* cu;
+    uint32_t* cv;
+    uint64_t* ceid;
+    const int32_t* found;  // walk results (post kernel)
+    const uint32_t* path;  // [kept * max_steps]
+    uint32_t max_steps;
+    uint32_t hw;           // heuristic walkers per endpoint
+    uint64_t seed;         // of the heuristic walks
+    uint32_t* pc;          // general recovery's lists (written when it takes over)
+    uint32_t* poff;
+    uint32_t* ep;
+    uint64_t* ekey;
+    uint32_t* sizes;       // pinned: kept, no-path, endpoints, path hops, taken-over flag
+    unsigned long long* stats;
+    Report rep;
+};
+hipError_t launch_small_del_pre(const SmallDelArgs& a, hipStream_t s);
+hipError_t launch_small_del_post(const SmallDelArgs& a, hipStream_t s);
 
 bool valid_walkers(uint32_t w);
 // Validates walkers (power of two, 1..1024) and max_steps (1..32767).

@@ -382,6 +382,7 @@ PinLayout pin_layout(char* base, uint32_t n) {
 }
 
 int key_bits(uint32_t V) { return std::min(64, 32 + 33 - __builtin_clz(std::max<uint32_t>(V, 1))); }
+int row_bits(uint32_t V) { return std::min(32, 33 - __builtin_clz(std::max<uint32_t>(V, 1))); }
 
 dyg_status stream_alloc(dyg_stream s) {
     const uint32_t n = s->sp.max_batch;
@@ -423,7 +424,8 @@ dyg_status stream_alloc(dyg_stream s) {
         s->xv = ar.take<uint32_t>(cm);
         s->xw = ar.take<float>(cm);
         s->xflag = ar.take<int32_t>(cm);
-        s->xk_in = ar.take<unsigned long long>(cm);
+        // also the rows a small deletion batch's recovery touches (2 per candidate)
+        s->xk_in = ar.take<unsigned long long>(std::max<uint64_t>(cm, kSmallCand));
         s->xk_out = ar.take<unsigned long long>(cm);
         s->xi_in = ar.take<uint32_t>(cm);
         s->xi_out = ar.take<uint32_t>(cm);
@@ -557,6 +559,7 @@ dyg_status apply_insert_small(dyg_stream s, const uint32_t* du, const uint32_t* 
     a.n = n;
     a.base = s->base;
     a.key_bits = key_bits(s->G->V);
+    a.row_bits = row_bits(s->G->V);
     a.cls = dcls;
     a.sel = s->sel;
     a.cnt = s->cnt;
@@ -614,9 +617,99 @@ dyg_status apply_insert(dyg_stream s, const uint32_t* du, const uint32_t* dv, co
     return DYG_OK;
 }
 
+// Recovery of a deletion batch once the walks' candidate list sizes are in
+// `hc` (kept, no-path, endpoints, path hops) and the path hops are in xu/xv:
+// heuristic hops, first occurrences H lacks, appended with G's weight.
+dyg_status recover_general(dyg_stream s, uint32_t n, const uint32_t* hc, int32_t* dcls,
+                           unsigned long long* dstats, dyg_stream_stats* out, const Report& rep) {
+    hipStream_t st = s->ctx->stream;
+    dyg_graph G = s->G, H = s->H;
+    const uint32_t hw = s->sp.heur_walkers;
+    const uint32_t npath = hc[3], n3 = hc[2] * hw;
+    const uint32_t ncand = npath + 2 * n3;
+    if (ncand > s->cmax) {
+        set_error(s->ctx, "stream: recovery list exceeds its reservation");
+        return DYG_ERR_STATE;
+    }
+    LAUNCH_1D(k_cand_heur, n3, s->path3, n3, npath, s->xu, s->xv);
+    LAUNCH_1D(k_batch_keys, ncand, s->xu, s->xv, ncand, s->xk_in, s->xi_in);
+    if (ncand) {
+        dyg_status r = sort_keys(s, s->xk_in, s->xk_out, s->xi_in, s->xi_out, ncand);
+        if (r != DYG_OK) return r;
+    }
+    LAUNCH_1D(k_cand_groups, ncand, s->xk_out, s->xi_out, ncand, G->d_rows, G->d_adj, H->d_rows,
+              H->d_adj, s->xflag, s->xw);
+    dyg_status r = device_edits_batch(H, s->xu, s->xv, s->xw, s->xflag, ncand);
+    if (r != DYG_OK) return r;
+    LAUNCH_1D(k_outcome, n, s->sel, s->cnt, s->cfound, dcls, kOutRecovered, kOutHeuristic);
+    DYG_HIP(s->ctx, launch_report(rep, dcls, n, dstats, st));
+    if (out) out->candidates = ncand;
+    return DYG_OK;
+}
+
+// Small deletion batch: one-workgroup kernels around the walk; waits only
+// when the recovery could outgrow the post kernel (then reads its flag).
+dyg_status apply_delete_small(dyg_stream s, const uint32_t* du, const uint32_t* dv, uint32_t n,
+                              int32_t* dcls, unsigned long long* dstats, bool timed,
+                              dyg_stream_stats* out, const Report& rep) {
+    hipStream_t st = s->ctx->stream;
+    const uint32_t ms = s->sp.del.max_steps, hw = s->sp.heur_walkers;
+    SmallDelArgs a{};
+    a.G = block_graph(s->G);
+    a.H = block_graph(s->H);
+    a.u = du;
+    a.v = dv;
+    a.n = n;
+    a.base = s->base;
+    a.key_bits = key_bits(s->G->V);
+    a.row_bits = row_bits(s->G->V);
+    a.cls = dcls;
+    a.sel = s->sel;
+    a.cnt = s->cnt;
+    a.touched_g = s->ep;                  // 2 * max_batch entries
+    a.touched_h = (uint32_t*)s->ekey;     // 4 * max_batch entries
+    a.touched_x = (uint32_t*)s->xk_in;    // 2 * cmax entries >= 2 * candidates
+    a.cu = s->cu;
+    a.cv = s->cv;
+    a.ceid = s->ceid;
+    a.found = s->cfound;
+    a.path = s->path;
+    a.max_steps = ms;
+    a.hw = hw;
+    a.seed = s->sp.del.seed;
+    a.pc = s->pc;
+    a.poff = s->poff;
+    a.ep = s->ep;
+    a.ekey = s->ekey;
+    a.sizes = (uint32_t*)(s->pin_dev + kPinSizes);
+    a.stats = dstats;
+    a.rep = rep;
+    DYG_HIP(s->ctx, launch_small_del_pre(a, st));
+    WalkLaunch w = walk_args(s, s->G, s->sp.del, n);
+    w.w = nullptr;
+    w.path = s->path;
+    w.stats = dstats;
+    if (timed) DYG_HIP(s->ctx, hipEventRecord(s->ctx->ev0, st));
+    DYG_HIP(s->ctx, launch_walk_decremental(w, st));
+    if (timed) DYG_HIP(s->ctx, hipEventRecord(s->ctx->ev1, st));
+    DYG_HIP(s->ctx, launch_small_del_post(a, st));
+    if ((uint64_t)n * (ms - 1) + 4ull * n * hw <= kSmallCand) return DYG_OK;
+    dyg_status r = wait_spin(s->ctx, st, s->ev_sync);
+    if (r != DYG_OK) return r;
+    const uint32_t* hc = (const uint32_t*)((char*)s->pin + kPinSizes);
+    if (hc[4] == 0) return DYG_OK;  // the post kernel finished the batch
+    DYG_HIP(s->ctx, launch_heuristic(s->G->d_rows, s->G->d_adj, s->ep, s->ekey, 2 * n, hw,
+                                     s->sp.del.seed, s->path3, st, s->cnt + 2));
+    LAUNCH_1D(k_cand_paths, n, s->cfound, s->cnt, s->path, ms, s->poff, s->xu, s->xv);
+    DYG_HIP(s->ctx, hipGetLastError());
+    return recover_general(s, n, hc, dcls, dstats, out, rep);
+}
+
 dyg_status apply_delete(dyg_stream s, const uint32_t* du, const uint32_t* dv, uint32_t n,
                         int32_t* dcls, unsigned long long* dstats, bool timed,
                         dyg_stream_stats* out, const Report& rep) {
+    if (n <= std::min(kSmallBatch, s->sp.max_batch) && !(s->sp.flags & DYG_STREAM_GENERAL_ONLY))
+        return apply_delete_small(s, du, dv, n, dcls, dstats, timed, out, rep);
     hipStream_t st = s->ctx->stream;
     dyg_graph G = s->G, H = s->H;
     const uint32_t ms = s->sp.del.max_steps, hw = s->sp.heur_walkers;
@@ -657,26 +750,7 @@ dyg_status apply_delete(dyg_stream s, const uint32_t* du, const uint32_t* dv, ui
     // the one wait of a deletion batch: recovery list sizes
     r = wait_spin(s->ctx, st, s->ev_sync);
     if (r != DYG_OK) return r;
-    const uint32_t npath = hc[3], n3 = hc[2] * hw;
-    const uint32_t ncand = npath + 2 * n3;
-    if (ncand > s->cmax) {
-        set_error(s->ctx, "stream: recovery list exceeds its reservation");
-        return DYG_ERR_STATE;
-    }
-    LAUNCH_1D(k_cand_heur, n3, s->path3, n3, npath, s->xu, s->xv);
-    LAUNCH_1D(k_batch_keys, ncand, s->xu, s->xv, ncand, s->xk_in, s->xi_in);
-    if (ncand) {
-        r = sort_keys(s, s->xk_in, s->xk_out, s->xi_in, s->xi_out, ncand);
-        if (r != DYG_OK) return r;
-    }
-    LAUNCH_1D(k_cand_groups, ncand, s->xk_out, s->xi_out, ncand, G->d_rows, G->d_adj, H->d_rows,
-              H->d_adj, s->xflag, s->xw);
-    r = device_edits_batch(H, s->xu, s->xv, s->xw, s->xflag, ncand);
-    if (r != DYG_OK) return r;
-    LAUNCH_1D(k_outcome, n, s->sel, s->cnt, s->cfound, dcls, kOutRecovered, kOutHeuristic);
-    DYG_HIP(s->ctx, launch_report(rep, dcls, n, dstats, st));
-    if (out) out->candidates = ncand;
-    return DYG_OK;
+    return recover_general(s, n, hc, dcls, dstats, out, rep);
 }
 
 // ---------------------------------------------------------------------------
@@ -928,6 +1002,7 @@ extern "C" dyg_status dyg_stream_apply(dyg_stream s, int op, const uint32_t* u, 
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess) rs.walk_ms = ms;
         rs.hops = hst[0];
+        if (hst[3]) rs.candidates = hst[3];  // counted on the device (small deletion batch)
         // keep a batch's worth of arena headroom for the next batch
         r = device_edits_prepare(s->G, s->sp.max_batch, s->sp.max_batch);
         if (r == DYG_OK) r = device_edits_prepare(s->H, s->cmax, s->cmax);

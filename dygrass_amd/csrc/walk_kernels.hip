@@ -23,37 +23,10 @@
 #include <atomic>
 
 #include "internal.h"
+#include "philox.h"
 
 namespace dyg {
 namespace {
-
-constexpr uint32_t kM0 = 0xD2511F53u, kM1 = 0xCD9E8D57u;
-constexpr uint32_t kW0 = 0x9E3779B9u, kW1 = 0xBB67AE85u;
-
-// Philox4x32-10 block `blk` of the stream (c1, c2, c3) under key (k0, k1):
-// counter (blk, c1, c2, c3).
-__device__ __forceinline__ uint4 philox_block(uint32_t blk, uint32_t c1, uint32_t c2,
-                                              uint32_t c3, uint32_t k0, uint32_t k1) {
-    uint32_t x0 = blk, x1 = c1, x2 = c2, x3 = c3, q0 = k0, q1 = k1;
-#pragma unroll
-    for (int r = 0; r < 10; ++r) {
-        // one 32x32->64 multiply gives both halves (a single v_mad_u64_u32)
-        const uint64_t p0 = (uint64_t)kM0 * x0, p1 = (uint64_t)kM1 * x2;
-        const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
-        const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
-        x0 = hi1 ^ x1 ^ q0;
-        x1 = lo1;
-        x2 = hi0 ^ x3 ^ q1;
-        x3 = lo0;
-        q0 += kW0;
-        q1 += kW1;
-    }
-    return make_uint4(x0, x1, x2, x3);
-}
-
-__device__ __forceinline__ uint32_t stream_c3(uint32_t tag, uint64_t id) {
-    return (tag << 24) | ((uint32_t)(id >> 32) & 0xFFFFFFu);
-}
 
 struct Draws {
     uint32_t k0, k1, c1, c2, c3, blk;
@@ -530,20 +503,10 @@ __global__ __launch_bounds__(256) void k_heuristic(const uint2* __restrict__ row
     if (n_dev && *n_dev < n) n = *n_dev;
     if (gid >= (uint64_t)n * walkers) return;
     const uint32_t i = (uint32_t)(gid / walkers), j = (uint32_t)(gid % walkers);
-    const uint32_t u = endpoint[i];
-    Draws rng(seed, kTagHeuristic, key[i], j);
-    uint32_t n1 = u, n2 = u;
-    const uint2 ru = rows[u];
-    if (ru.y != 0) {
-        const uint4 s1 = adj[ru.x + rng.next() % ru.y];
-        n1 = s1.x;
-        const uint2 r1 = make_uint2(s1.z, s1.w);  // r1.y >= 1: n1 is a neighbour of u
-        n2 = adj[r1.x + rng.next() % r1.y].x;
-        if (n2 == u) n2 = adj[r1.x + r1.y - 1].x;
-    }
-    path3[gid * 3 + 0] = u;
-    path3[gid * 3 + 1] = n1;
-    path3[gid * 3 + 2] = n2;
+    const uint3 t = heuristic_walk(rows, adj, endpoint[i], key[i], j, seed);
+    path3[gid * 3 + 0] = t.x;
+    path3[gid * 3 + 1] = t.y;
+    path3[gid * 3 + 2] = t.z;
 }
 
 // Graph edits: final slot values and row descriptors planned on the host.

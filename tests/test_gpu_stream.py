@@ -134,3 +134,32 @@ def test_deletion_stream_equals_decremental_updater(ctx, mesh_wl):
         e = (min(a, b), max(a, b))
         assert (e in h) == (e not in gone), e
     assert st.G.check() == 0 and st.H.check() == 0
+
+
+@pytest.mark.parametrize("n_del", [40, 800])
+def test_small_deletion_batch_recovery_sizes(ctx, grid_wl, n_del):
+    """A deletion batch of H edges within the one-workgroup size: a short
+    candidate list is recovered inside the post kernel, a long one (more
+    than 2048 candidates) is handed to the general recovery; both match the
+    oracle, and so does the insertion batch after them."""
+    from dygrass_amd import Stream, stream_params
+    from oracle import stream as S
+
+    wl = grid_wl
+    rng = np.random.default_rng(n_del)
+    sp = stream_params(w_ins=64, w_del=64, threshold=10.0, seed=5, max_batch=1024,
+                       max_inserts=4096)
+    st = Stream(ctx, wl.G.V, (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w), sp)
+    orc = S.Stream(wl.G.V, (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w), thr=10.0,
+                   w_ins=64, w_del=64, seed=5)
+    idx = rng.choice(len(wl.H), n_del, replace=False)
+    batches = [("del", wl.H.u[idx], wl.H.v[idx], None),
+               ("ins", wl.ins.u[:500], wl.ins.v[:500], wl.ins.w[:500])]
+    for op, u, v, w in batches:
+        got, stats = st.apply(op, u, v, w)
+        want = orc.apply(op, u, v, w)
+        assert np.array_equal(got, want), (op, np.flatnonzero(got != want)[:10])
+        if op == "del":
+            assert (stats["candidates"] > 2048) == (n_del == 800), stats["candidates"]
+        _assert_same_graph(st.G, orc.G)
+        _assert_same_graph(st.H, orc.H)
