@@ -72,10 +72,13 @@ __device__ __forceinline__ bool has_edge(const uint2* __restrict__ rows,
                                   : row_find(rows, adj, b, a) >= 0;
 }
 
+// Also clears bcnt[block] (when given): the per-block counts a later
+// k_compact over the same n reads.
 __global__ void k_batch_keys(const uint32_t* __restrict__ u, const uint32_t* __restrict__ v,
                              uint32_t n, unsigned long long* __restrict__ key,
-                             uint32_t* __restrict__ idx) {
+                             uint32_t* __restrict__ idx, uint32_t* __restrict__ bcnt) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (bcnt && threadIdx.x == 0) bcnt[blockIdx.x] = 0;
     if (i >= n) return;
     const uint32_t a = min(u[i], v[i]), b = max(u[i], v[i]);
     key[i] = a == b ? kPadKey : ((unsigned long long)a << 32) | b;
@@ -91,7 +94,8 @@ __global__ void k_ins_groups(const unsigned long long* __restrict__ key,
                              const uint2* __restrict__ grows, const uint4* __restrict__ gadj,
                              const uint2* __restrict__ hrows, const uint4* __restrict__ hadj,
                              int32_t* __restrict__ cls, int32_t* __restrict__ flag,
-                             uint8_t* __restrict__ keep, uint8_t* __restrict__ mode) {
+                             uint8_t* __restrict__ keep, uint8_t* __restrict__ mode,
+                             uint32_t* __restrict__ bcnt) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const unsigned long long k = key[p];
@@ -114,6 +118,7 @@ __global__ void k_ins_groups(const unsigned long long* __restrict__ key,
             flag[idx[e]] = first_new ? -1 : 0;
             keep[idx[e]] = first_new ? 1 : 0;
         }
+        if (!in_g) atomicAdd(&bcnt[idx[p] >> 8], 1u);  // the kept first occurrence
         if (in_g) md = 1 | (has_edge(hrows, hadj, a, b) ? 4 : 0);
         else if (q - p > 1) md = 2;
     }
@@ -158,7 +163,8 @@ __global__ void k_del_groups(const unsigned long long* __restrict__ key,
                              const uint2* __restrict__ grows, const uint4* __restrict__ gadj,
                              const uint2* __restrict__ hrows, const uint4* __restrict__ hadj,
                              int32_t* __restrict__ cls, int32_t* __restrict__ flag_g,
-                             int32_t* __restrict__ flag_h, uint8_t* __restrict__ keep) {
+                             int32_t* __restrict__ flag_h, uint8_t* __restrict__ keep,
+                             uint32_t* __restrict__ bcnt) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const unsigned long long k = key[p];
@@ -179,21 +185,46 @@ __global__ void k_del_groups(const unsigned long long* __restrict__ key,
         flag_h[i] = first && in_h ? 1 : 0;
         keep[i] = first && in_h ? 1 : 0;
     }
+    if (in_h) atomicAdd(&bcnt[idx[p] >> 8], 1u);  // the kept first occurrence
 }
 
-__global__ void k_gather(const uint32_t* __restrict__ sel, const uint32_t* __restrict__ nsel,
-                         const uint32_t* __restrict__ u, const uint32_t* __restrict__ v,
-                         const float* __restrict__ w, uint64_t base, bool ordered,
-                         uint32_t* __restrict__ cu, uint32_t* __restrict__ cv,
-                         float* __restrict__ cw, uint64_t* __restrict__ ceid) {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= *nsel) return;
-    const uint32_t i = sel[j];
-    const uint32_t a = u[i], b = v[i];
-    cu[j] = ordered ? min(a, b) : a;
-    cv[j] = ordered ? max(a, b) : b;
-    if (w) cw[j] = w[i];
-    ceid[j] = base + i;
+// Stable compaction of the kept updates (keep[i], batch order) in one pass:
+// block b's output offset is the sum of the kept counts of blocks before it
+// (bcnt, counted by the classification kernel), plus its own block scan.
+// Gathers each kept update: (u, v) or, `ordered`, (min, max); its weight;
+// its stream position.  cnt[0] = kept total.  256 threads per block.
+__global__ __launch_bounds__(256) void k_compact(
+    const uint8_t* __restrict__ keep, uint32_t n, const uint32_t* __restrict__ bcnt,
+    const uint32_t* __restrict__ u, const uint32_t* __restrict__ v, const float* __restrict__ w,
+    uint64_t base, bool ordered, uint32_t* __restrict__ cu, uint32_t* __restrict__ cv,
+    float* __restrict__ cw, uint64_t* __restrict__ ceid, uint32_t* __restrict__ sel,
+    uint32_t* __restrict__ cnt) {
+    using BScan = hipcub::BlockScan<uint32_t, 256>;
+    using BReduce = hipcub::BlockReduce<uint32_t, 256>;
+    __shared__ union {
+        typename BScan::TempStorage scan;
+        typename BReduce::TempStorage reduce;
+    } tmp;
+    __shared__ uint32_t s_prefix;
+    const uint32_t b = blockIdx.x, t = threadIdx.x, i = b * 256u + t;
+    uint32_t part = 0;
+    for (uint32_t q = t; q < b; q += 256u) part += bcnt[q];
+    const uint32_t prefix = BReduce(tmp.reduce).Sum(part);
+    if (t == 0) s_prefix = prefix;
+    __syncthreads();
+    const bool k = i < n && keep[i];
+    uint32_t pos = 0, total = 0;
+    BScan(tmp.scan).ExclusiveSum(k ? 1u : 0u, pos, total);
+    pos += s_prefix;
+    if (k) {
+        const uint32_t a = u[i], c = v[i];
+        cu[pos] = ordered ? min(a, c) : a;
+        cv[pos] = ordered ? max(a, c) : c;
+        if (w) cw[pos] = w[i];
+        ceid[pos] = base + i;
+        sel[pos] = i;
+    }
+    if (b == gridDim.x - 1 && t == 0) cnt[0] = s_prefix + total;
 }
 
 __global__ void k_outcome(const uint32_t* __restrict__ sel, const uint32_t* __restrict__ nsel,
@@ -204,38 +235,73 @@ __global__ void k_outcome(const uint32_t* __restrict__ sel, const uint32_t* __re
     cls[sel[j]] = found[j] == -1 ? if_not : if_found;
 }
 
-// Per kept deletion: path hop count (found walks) and the no-path flag.
-__global__ void k_path_counts(const int32_t* __restrict__ found, const uint32_t* __restrict__ nsel,
-                              uint32_t n, uint32_t* __restrict__ pc, uint8_t* __restrict__ nopath) {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    const bool live = j < *nsel;
-    pc[j] = live && found[j] > 0 ? (uint32_t)found[j] : 0u;
-    nopath[j] = live && found[j] == -1 ? 1 : 0;
-}
-
-// cnt[2] = heuristic endpoints, cnt[3] = total path hops; cnt[0..3] also to
-// `out` (pinned host memory)
-__global__ void k_recovery_sizes(const uint32_t* __restrict__ poff, const uint32_t* __restrict__ pc,
-                                 uint32_t n, uint32_t* __restrict__ cnt, uint32_t* __restrict__ out) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        cnt[2] = 2u * cnt[1];
-        cnt[3] = n ? poff[n - 1] + pc[n - 1] : 0u;
-        for (int i = 0; i < 4; ++i) out[i] = cnt[i];
+// Per kept deletion j: path hop count pc[j] (found walks) and the no-path
+// flag; per block of 256, their sums packed as (hops << 32 | no-path) in
+// bsum[block].
+__global__ __launch_bounds__(256) void k_path_counts(
+    const int32_t* __restrict__ found, const uint32_t* __restrict__ nsel, uint32_t n,
+    uint32_t* __restrict__ pc, uint8_t* __restrict__ nopath,
+    unsigned long long* __restrict__ bsum) {
+    using BReduce = hipcub::BlockReduce<unsigned long long, 256>;
+    __shared__ typename BReduce::TempStorage tmp;
+    const uint32_t j = blockIdx.x * 256u + threadIdx.x;
+    const bool live = j < n && j < *nsel;
+    const int32_t f = live ? found[j] : 0;
+    const uint32_t c = f > 0 ? (uint32_t)f : 0u;
+    const bool np = f == -1;
+    if (j < n) {
+        pc[j] = c;
+        nopath[j] = np ? 1 : 0;
     }
+    const unsigned long long sum = BReduce(tmp).Sum(((unsigned long long)c << 32) | (np ? 1u : 0u));
+    if (threadIdx.x == 0) bsum[blockIdx.x] = sum;
 }
 
-__global__ void k_heur_endpoints(const uint32_t* __restrict__ npsel,
-                                 const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ ca,
-                                 const uint32_t* __restrict__ cb, const uint64_t* __restrict__ ceid,
-                                 uint32_t* __restrict__ ep, uint64_t* __restrict__ ekey) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= cnt[1]) return;
-    const uint32_t j = npsel[k];
-    ep[2 * k] = ca[j];
-    ep[2 * k + 1] = cb[j];
-    ekey[2 * k] = ceid[j] << 1;
-    ekey[2 * k + 1] = (ceid[j] << 1) | 1ull;
+// The recovery lists in one pass over the kept deletions (block prefixes
+// from bsum): poff[j] = exclusive prefix of the hop counts; the no-path
+// edges' endpoints ep[2k], ep[2k+1] (min, max) with their heuristic stream
+// keys (position << 1 | side), k the no-path rank.  The last block writes
+// cnt[1] = no-path count, cnt[2] = endpoints, cnt[3] = path hops and copies
+// cnt[0..3] to `out` (pinned host memory).
+__global__ __launch_bounds__(256) void k_recovery_lists(
+    const uint32_t* __restrict__ pc, const uint8_t* __restrict__ nopath, uint32_t n,
+    const unsigned long long* __restrict__ bsum, const uint32_t* __restrict__ ca,
+    const uint32_t* __restrict__ cb, const uint64_t* __restrict__ ceid,
+    uint32_t* __restrict__ poff, uint32_t* __restrict__ ep, uint64_t* __restrict__ ekey,
+    uint32_t* __restrict__ cnt, uint32_t* __restrict__ out) {
+    using BScan = hipcub::BlockScan<unsigned long long, 256>;
+    using BReduce = hipcub::BlockReduce<unsigned long long, 256>;
+    __shared__ union {
+        typename BScan::TempStorage scan;
+        typename BReduce::TempStorage reduce;
+    } tmp;
+    __shared__ unsigned long long s_prefix;
+    const uint32_t b = blockIdx.x, t = threadIdx.x, j = b * 256u + t;
+    unsigned long long part = 0;
+    for (uint32_t q = t; q < b; q += 256u) part += bsum[q];
+    const unsigned long long prefix = BReduce(tmp.reduce).Sum(part);
+    if (t == 0) s_prefix = prefix;
+    __syncthreads();
+    const uint32_t c = j < n ? pc[j] : 0u;
+    const bool np = j < n && nopath[j];
+    unsigned long long off = 0, total = 0;
+    BScan(tmp.scan).ExclusiveSum(((unsigned long long)c << 32) | (np ? 1u : 0u), off, total);
+    off += s_prefix;
+    if (j < n) poff[j] = (uint32_t)(off >> 32);
+    if (np) {
+        const uint32_t k = (uint32_t)off;
+        ep[2 * k] = ca[j];
+        ep[2 * k + 1] = cb[j];
+        ekey[2 * k] = ceid[j] << 1;
+        ekey[2 * k + 1] = (ceid[j] << 1) | 1ull;
+    }
+    if (b == gridDim.x - 1 && t == 0) {
+        const unsigned long long all = s_prefix + total;
+        cnt[1] = (uint32_t)all;
+        cnt[2] = 2u * (uint32_t)all;
+        cnt[3] = (uint32_t)(all >> 32);
+        for (int q = 0; q < 4; ++q) out[q] = cnt[q];
+    }
 }
 
 // Candidate list, part one: the hops of every found path, kept-edge order.
@@ -328,8 +394,10 @@ struct dyg_stream_s {
     uint64_t* ceid;
     int32_t* cfound;
     uint32_t* path;  // [max_batch * max_steps]
-    uint32_t *pc, *poff, *npsel;
+    uint32_t *pc, *poff;
     uint8_t* nopath;
+    uint32_t* bcnt;             // [blocks of 256] kept counts for k_compact
+    unsigned long long* bsum;   // [blocks of 256] path hop and no-path sums
     uint32_t* ep;     // [2 * max_batch]
     uint64_t* ekey;   // [2 * max_batch]
     uint32_t* path3;  // [2 * max_batch * heur_walkers * 3]
@@ -415,8 +483,9 @@ dyg_status stream_alloc(dyg_stream s) {
         s->path = ar.take<uint32_t>((size_t)n * ms);
         s->pc = ar.take<uint32_t>(n);
         s->poff = ar.take<uint32_t>(n);
-        s->npsel = ar.take<uint32_t>(n);
         s->nopath = ar.take<uint8_t>(n);
+        s->bcnt = ar.take<uint32_t>(((size_t)n + 255) / 256);
+        s->bsum = ar.take<unsigned long long>(((size_t)n + 255) / 256);
         s->ep = ar.take<uint32_t>(2 * (size_t)n);
         s->ekey = ar.take<uint64_t>(2 * (size_t)n);
         s->path3 = ar.take<uint32_t>(2 * (size_t)n * hw * 3);
@@ -440,19 +509,11 @@ dyg_status stream_alloc(dyg_stream s) {
             ar.base = (char*)s->mem;
         }
     }
-    // hipCUB scratch: the largest of the sorts, selections and the scan
-    size_t t = 0, b = 0;
-    hipcub::CountingInputIterator<uint32_t> it(0);
-    DYG_HIP(s->ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, b, s->xk_in, s->xk_out, s->xi_in,
+    // hipCUB scratch of the key sorts
+    size_t t = 0;
+    DYG_HIP(s->ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, t, s->xk_in, s->xk_out, s->xi_in,
                                                        s->xi_out, (int)std::max<uint64_t>(cm, n),
                                                        0, 64, s->ctx->stream));
-    t = std::max(t, b);
-    DYG_HIP(s->ctx, hipcub::DeviceSelect::Flagged(nullptr, b, it, s->keep, s->sel, s->cnt, (int)n,
-                                                  s->ctx->stream));
-    t = std::max(t, b);
-    DYG_HIP(s->ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, b, s->pc, s->poff, (int)n,
-                                                     s->ctx->stream));
-    t = std::max(t, b);
     if (hipMalloc(&s->temp, t) != hipSuccess) {
         s->temp = nullptr;
         set_error(s->ctx, "stream: device allocation failed");
@@ -500,15 +561,6 @@ dyg_status sort_keys(dyg_stream s, unsigned long long* kin, unsigned long long* 
     size_t tb = s->temp_bytes;
     DYG_HIP(s->ctx, hipcub::DeviceRadixSort::SortPairs(s->temp, tb, kin, kout, iin, iout, (int)n,
                                                        0, key_bits(s->G->V), s->ctx->stream));
-    return DYG_OK;
-}
-
-dyg_status select_flagged(dyg_stream s, const uint8_t* flags, uint32_t* out, uint32_t* count,
-                          uint32_t n) {
-    size_t tb = s->temp_bytes;
-    hipcub::CountingInputIterator<uint32_t> it(0);
-    DYG_HIP(s->ctx, hipcub::DeviceSelect::Flagged(s->temp, tb, it, flags, out, count, (int)n,
-                                                  s->ctx->stream));
     return DYG_OK;
 }
 
@@ -590,19 +642,17 @@ dyg_status apply_insert(dyg_stream s, const uint32_t* du, const uint32_t* dv, co
         return apply_insert_small(s, du, dv, dw, n, dcls, dstats, timed, rep);
     hipStream_t st = s->ctx->stream;
     dyg_graph G = s->G, H = s->H;
-    LAUNCH_1D(k_batch_keys, n, du, dv, n, s->k_in, s->i_in);
+    LAUNCH_1D(k_batch_keys, n, du, dv, n, s->k_in, s->i_in, s->bcnt);
     dyg_status r = sort_keys(s, s->k_in, s->k_out, s->i_in, s->i_out, n);
     if (r != DYG_OK) return r;
     LAUNCH_1D(k_ins_groups, n, s->k_out, s->i_out, n, G->d_rows, G->d_adj, H->d_rows, H->d_adj,
-              dcls, s->flag_g, s->keep, s->mode);
+              dcls, s->flag_g, s->keep, s->mode, s->bcnt);
     r = device_edits_batch(G, du, dv, dw, s->flag_g, n);
     if (r != DYG_OK) return r;
     LAUNCH_1D(k_ins_weights, n, s->k_out, s->i_out, n, s->mode, dw, G->d_rows, G->d_adj,
               H->d_rows, H->d_adj);
-    r = select_flagged(s, s->keep, s->sel, s->cnt, n);
-    if (r != DYG_OK) return r;
-    LAUNCH_1D(k_gather, n, s->sel, s->cnt, du, dv, dw, s->base, false, s->cu, s->cv, s->cw,
-              s->ceid);
+    LAUNCH_1D(k_compact, n, s->keep, n, s->bcnt, du, dv, dw, s->base, false, s->cu, s->cv,
+              s->cw, s->ceid, s->sel, s->cnt);
     DYG_HIP(s->ctx, hipGetLastError());
     if (timed) DYG_HIP(s->ctx, hipEventRecord(s->ctx->ev0, st));
     WalkLaunch a = walk_args(s, H, s->sp.ins, n);
@@ -632,7 +682,7 @@ dyg_status recover_general(dyg_stream s, uint32_t n, const uint32_t* hc, int32_t
         return DYG_ERR_STATE;
     }
     LAUNCH_1D(k_cand_heur, n3, s->path3, n3, npath, s->xu, s->xv);
-    LAUNCH_1D(k_batch_keys, ncand, s->xu, s->xv, ncand, s->xk_in, s->xi_in);
+    LAUNCH_1D(k_batch_keys, ncand, s->xu, s->xv, ncand, s->xk_in, s->xi_in, (uint32_t*)nullptr);
     if (ncand) {
         dyg_status r = sort_keys(s, s->xk_in, s->xk_out, s->xi_in, s->xi_out, ncand);
         if (r != DYG_OK) return r;
@@ -713,19 +763,17 @@ dyg_status apply_delete(dyg_stream s, const uint32_t* du, const uint32_t* dv, ui
     hipStream_t st = s->ctx->stream;
     dyg_graph G = s->G, H = s->H;
     const uint32_t ms = s->sp.del.max_steps, hw = s->sp.heur_walkers;
-    LAUNCH_1D(k_batch_keys, n, du, dv, n, s->k_in, s->i_in);
+    LAUNCH_1D(k_batch_keys, n, du, dv, n, s->k_in, s->i_in, s->bcnt);
     dyg_status r = sort_keys(s, s->k_in, s->k_out, s->i_in, s->i_out, n);
     if (r != DYG_OK) return r;
     LAUNCH_1D(k_del_groups, n, s->k_out, s->i_out, n, G->d_rows, G->d_adj, H->d_rows, H->d_adj,
-              dcls, s->flag_g, s->flag_h, s->keep);
+              dcls, s->flag_g, s->flag_h, s->keep, s->bcnt);
     r = device_deletes_batch(G, du, dv, s->flag_g, n, nullptr);
     if (r != DYG_OK) return r;
     r = device_deletes_batch(H, du, dv, s->flag_h, n, nullptr);
     if (r != DYG_OK) return r;
-    r = select_flagged(s, s->keep, s->sel, s->cnt, n);
-    if (r != DYG_OK) return r;
-    LAUNCH_1D(k_gather, n, s->sel, s->cnt, du, dv, (const float*)nullptr, s->base, true, s->cu,
-              s->cv, (float*)nullptr, s->ceid);
+    LAUNCH_1D(k_compact, n, s->keep, n, s->bcnt, du, dv, (const float*)nullptr, s->base, true,
+              s->cu, s->cv, (float*)nullptr, s->ceid, s->sel, s->cnt);
     DYG_HIP(s->ctx, hipGetLastError());
     WalkLaunch a = walk_args(s, G, s->sp.del, n);
     a.w = nullptr;
@@ -734,15 +782,10 @@ dyg_status apply_delete(dyg_stream s, const uint32_t* du, const uint32_t* dv, ui
     if (timed) DYG_HIP(s->ctx, hipEventRecord(s->ctx->ev0, st));
     DYG_HIP(s->ctx, launch_walk_decremental(a, st));
     if (timed) DYG_HIP(s->ctx, hipEventRecord(s->ctx->ev1, st));
-    LAUNCH_1D(k_path_counts, n, s->cfound, s->cnt, n, s->pc, s->nopath);
-    size_t tb = s->temp_bytes;
-    DYG_HIP(s->ctx, hipcub::DeviceScan::ExclusiveSum(s->temp, tb, s->pc, s->poff, (int)n, st));
-    r = select_flagged(s, s->nopath, s->npsel, s->cnt + 1, n);
-    if (r != DYG_OK) return r;
-    uint32_t* hc = (uint32_t*)((char*)s->pin + kPinSizes);
-    hipLaunchKernelGGL(k_recovery_sizes, dim3(1), dim3(64), 0, st, s->poff, s->pc, n, s->cnt,
-                       (uint32_t*)(s->pin_dev + kPinSizes));
-    LAUNCH_1D(k_heur_endpoints, n, s->npsel, s->cnt, s->cu, s->cv, s->ceid, s->ep, s->ekey);
+    LAUNCH_1D(k_path_counts, n, s->cfound, s->cnt, n, s->pc, s->nopath, s->bsum);
+    LAUNCH_1D(k_recovery_lists, n, s->pc, s->nopath, n, s->bsum, s->cu, s->cv, s->ceid, s->poff,
+              s->ep, s->ekey, s->cnt, (uint32_t*)(s->pin_dev + kPinSizes));
+    const uint32_t* hc = (const uint32_t*)((char*)s->pin + kPinSizes);
     DYG_HIP(s->ctx, launch_heuristic(G->d_rows, G->d_adj, s->ep, s->ekey, 2 * n, hw,
                                      s->sp.del.seed, s->path3, st, s->cnt + 2));
     LAUNCH_1D(k_cand_paths, n, s->cfound, s->cnt, s->path, ms, s->poff, s->xu, s->xv);

@@ -246,8 +246,11 @@ __global__ __launch_bounds__((1 << LOGW) < 64 ? 64 : (1 << LOGW)) void k_walk(Wa
 // Work-queue form.  A workgroup of 256 lanes owns `E` consecutive edges and
 // their E*W (edge, walker) tasks, taken from an LDS counter; a lane whose
 // walker ends takes the next task at once, so a wave is never held by its
-// longest walker.  Refills are wave-aggregated (one LDS atomic per wave per
-// turn).  Per-edge winner keys and hop counts live in LDS (64-bit LDS max);
+// longest walker.  Each wave keeps a ring of up to 127 claimed tasks with
+// their first draw block, filled 64 at a time by the whole wave (one LDS
+// atomic and one Philox block per lane), so refills cost no divergent Philox
+// and the workgroup's pool can be large: many tasks per lane keep the tail
+// after the pool drains short.  Per-edge winner keys and hop counts live in LDS (64-bit LDS max);
 // pruning reads the edge's key.  Results equal the one-workgroup-per-edge
 // kernel's bit for bit: a walker's trajectory depends only on (edge, walker)
 // and the graph.
@@ -256,8 +259,8 @@ __global__ __launch_bounds__((1 << LOGW) < 64 ? 64 : (1 << LOGW)) void k_walk(Wa
 // every lane on one code path:
 //  * first and later hops share the draw, the load phase and the tests;
 //  * each edge's source row is staged in LDS (a refill reads no global memory);
-//  * draws come from an 8-entry ring of two Philox blocks.  Block 0 of every
-//    task is computed up front by the whole workgroup into LDS; later blocks
+//  * draws come from an 8-entry ring of two Philox blocks.  Block 0 of a
+//    task comes with it from the wave's task ring; later blocks
 //    are computed every fourth turn by all active lanes at once, into the
 //    half of the ring already consumed (a walker draws exactly once per
 //    turn, so the half not holding the next draw is always spent then).  The
@@ -265,15 +268,10 @@ __global__ __launch_bounds__((1 << LOGW) < 64 ? 64 : (1 << LOGW)) void k_walk(Wa
 //    of whenever any one lane runs dry.
 // ---------------------------------------------------------------------------
 constexpr int kPoolThreads = 256;
+constexpr int kPoolWaves = kPoolThreads / 64;
 constexpr float kPruneSlack = 1.0f + 0x1p-20f;
-constexpr int kPoolMaxEdges = 64;     // edges owned by one workgroup, at most
-constexpr int kPoolMaxTasks = 1024;   // and (edge, walker) tasks, at most
-
-__host__ __device__ constexpr uint32_t pool_edge_cap(int logw) {
-    return (uint32_t)((kPoolMaxTasks >> logw) < 1 ? 1
-                      : (kPoolMaxTasks >> logw) > kPoolMaxEdges ? kPoolMaxEdges
-                                                                : (kPoolMaxTasks >> logw));
-}
+constexpr int kPoolMaxEdges = 128;  // edges owned by one workgroup, at most
+constexpr uint32_t kRing = 128;     // a wave's claimed tasks (at most 127 pending)
 
 struct Lane {
     uint32_t edge;  // local edge index
@@ -320,8 +318,7 @@ struct Ring {
 template <int LOGW, bool INSERT, bool PRUNE>
 __global__ __launch_bounds__(kPoolThreads) void k_walk_pool(WalkLaunch a, uint32_t E) {
     constexpr uint32_t W = 1u << LOGW;
-    constexpr uint32_t kE = pool_edge_cap(LOGW);
-    constexpr uint32_t kT = kE * W;
+    constexpr uint32_t kE = kPoolMaxEdges;
     constexpr uint32_t kTag = INSERT ? kTagInsert : kTagDelete;
     __shared__ unsigned long long s_key[kE];
     __shared__ uint32_t s_hops[kE];
@@ -329,7 +326,8 @@ __global__ __launch_bounds__(kPoolThreads) void k_walk_pool(WalkLaunch a, uint32
     __shared__ float s_w[kE];
     __shared__ unsigned long long s_eid[kE];
     __shared__ uint2 s_row[kE];  // source rows: every walker of an edge starts there
-    __shared__ uint4 s_blk0[kT];  // draw block 0 of every task
+    __shared__ uint4 s_rblk[kPoolWaves][kRing];     // per wave: claimed tasks' block 0
+    __shared__ uint32_t s_rtask[kPoolWaves][kRing];  // and their task numbers
     __shared__ uint32_t s_next;
     __shared__ uint32_t s_cnt[3];
 
@@ -356,13 +354,12 @@ __global__ __launch_bounds__(kPoolThreads) void k_walk_pool(WalkLaunch a, uint32
     }
     __syncthreads();
     const uint32_t total = ne * W;
-    for (uint32_t t = tid; t < total; t += kPoolThreads) {
-        const uint64_t id = s_eid[t >> LOGW];
-        s_blk0[t] = philox_block(0u, t & (W - 1u), (uint32_t)id, stream_c3(kTag, id), k0, k1);
-    }
-    __syncthreads();
 
     const uint32_t lane = __lane_id();
+    uint4* const rblk = s_rblk[tid >> 6];
+    uint32_t* const rtask = s_rtask[tid >> 6];
+    uint32_t rhead = 0, rtail = 0;  // the wave's task ring (same in every lane)
+    bool pool_empty = false;        // the workgroup's counter has run out (same in every lane)
     bool active = false;
     bool drained = false;
     Lane L{};
@@ -371,18 +368,37 @@ __global__ __launch_bounds__(kPoolThreads) void k_walk_pool(WalkLaunch a, uint32
     uint32_t my_hops = 0;                    // current walker's draws
 
     for (uint32_t turn = 0;; ++turn) {
-        // ---- refill idle lanes (one LDS atomic per wave) ----
+        // ---- refill idle lanes from the wave's task ring ----
         const unsigned long long need = __ballot(!active && !drained);
         if (need) {
             const uint32_t cnt = __popcll(need);
-            const int leader = __ffsll((long long)need) - 1;
-            uint32_t base = 0;
-            if ((int)lane == leader) base = atomicAdd(&s_next, cnt);
-            base = __shfl(base, leader, 64);
+            if (rtail - rhead < cnt && !pool_empty) {
+                // claim 64 tasks (one LDS atomic), one block 0 per lane
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(&s_next, 64u);
+                base = __shfl(base, 0, 64);
+                const uint32_t task = base + lane;
+                if (task < total) {
+                    const uint64_t id = s_eid[task >> LOGW];
+                    const uint32_t q = (rtail + lane) & (kRing - 1u);
+                    rblk[q] = philox_block(0u, task & (W - 1u), (uint32_t)id, stream_c3(kTag, id),
+                                           k0, k1);
+                    rtask[q] = task;
+                }
+                const uint32_t got = base >= total ? 0u : min(64u, total - base);
+                rtail += got;
+                pool_empty = got < 64u;
+                // the ring entries this wave just wrote, before its lanes read them
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
+            const uint32_t avail = rtail - rhead;
             if (!active && !drained) {
                 const uint32_t rank = __popcll(need & ((1ull << lane) - 1ull));
-                const uint32_t task = base + rank;
-                if (task < total) {
+                if (rank < avail) {
+                    const uint32_t q = (rhead + rank) & (kRing - 1u);
+                    const uint32_t task = rtask[q];
                     L.edge = task >> LOGW;
                     L.walker = task & (W - 1u);
                     L.cur = s_src[L.edge];
@@ -392,13 +408,14 @@ __global__ __launch_bounds__(kPoolThreads) void k_walk_pool(WalkLaunch a, uint32
                     L.prev = 0;
                     L.R = 0.0f;
                     L.step = 1;
-                    rng.start(s_blk0[task], s_eid[L.edge], kTag);
+                    rng.start(rblk[q], s_eid[L.edge], kTag);
                     my_hops = 0;
                     active = true;
                 } else {
-                    drained = true;
+                    drained = true;  // the counter is out: nothing left to claim
                 }
             }
+            rhead += min(cnt, avail);
         }
         if (!__any(active)) break;
         // ---- every fourth turn: the next draw block of every active lane ----
@@ -570,9 +587,7 @@ uint32_t pool_slots() {
 // round of resident workgroups (so no workgroup starts late and adds its own
 // tail), and at least one task per lane.
 inline uint32_t pool_edges(uint32_t W, uint32_t requested, uint32_t n, uint32_t slots) {
-    uint32_t cap = (uint32_t)kPoolMaxTasks / W;
-    if (cap < 1) cap = 1;
-    if (cap > (uint32_t)kPoolMaxEdges) cap = kPoolMaxEdges;
+    const uint32_t cap = kPoolMaxEdges;
     uint32_t E = requested;
     if (!E) {
         E = (uint32_t)(((uint64_t)n + slots - 1) / slots);

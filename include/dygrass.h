@@ -129,7 +129,7 @@ typedef struct {
     uint32_t flags;     /* DYG_WALK_* */
     uint64_t seed;      /* Philox key of the draw streams */
     uint32_t block_edges; /* work-queue kernel: edges owned by one workgroup (0 = automatic,
-                             max 256); a launch-shape knob, results do not depend on it */
+                             max 128); a launch-shape knob, results do not depend on it */
     uint32_t reserved;
 } dyg_walk_params;
 
