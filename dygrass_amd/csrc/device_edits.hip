@@ -141,12 +141,12 @@ __global__ void k_del_rows(uint2* __restrict__ rows, uint4* __restrict__ adj,
     for (uint32_t e = p; e < p + k; ++e) {
         const uint32_t o = vals[e];
         const uint32_t nbr = (o & 1u) ? src[o >> 1] : dst[o >> 1];
-        uint32_t q = 0;
-        while (q < row.y && adj[row.x + q].x != nbr) ++q;
-        if (q == row.y) {  // not present: the caller's lists were inconsistent
+        const int s = slot_of(adj, row, nbr);
+        if (s < 0) {  // not present: the caller's lists were inconsistent
             atomicOr(err, 2u);
             continue;
         }
+        const uint32_t q = (uint32_t)s;
         const uint32_t last = row.y - 1u;
         if (q != last) adj[row.x + q] = adj[row.x + last];
         row.y = last;
@@ -174,11 +174,7 @@ __global__ __launch_bounds__(256) void k_refresh(
     const uint2 rr = rows[r];
     for (uint32_t q = lane; q < rr.y; q += 64u) {
         const uint32_t x = adj[rr.x + q].x;
-        const uint2 rx = rows[x];
-        for (uint32_t k = 0; k < rx.y; ++k) {
-            uint4* sl = &adj[rx.x + k];
-            if (sl->x == r) *reinterpret_cast<uint2*>(&sl->z) = rr;
-        }
+        refresh_pointers(adj, rows[x], r, rr);
     }
 }
 

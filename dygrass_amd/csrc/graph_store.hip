@@ -214,7 +214,7 @@ dyg_status graph_apply_inserts(dyg_graph g, const uint32_t* u, const uint32_t* v
 
     g->dev.cap_valid = false;  // relocations below change row capacities on the host
     std::vector<uint32_t> dirty, touched;
-    std::vector<uint32_t> slot_of(2 * (size_t)n);
+    std::vector<uint32_t> slot_at(2 * (size_t)n);
     dirty.reserve(4 * (size_t)n);
     for (size_t p = 0; p < apps.size();) {
         size_t q = p;
@@ -237,7 +237,7 @@ dyg_status graph_apply_inserts(dyg_graph g, const uint32_t* u, const uint32_t* v
             // the neighbour's descriptor is filled by the refresh below
             g->adj[slot] = make_uint4(apps[k].nbr, fbits(apps[k].w), 0, 0);
             dirty.push_back(slot);
-            slot_of[apps[k].seq] = (uint32_t)(d0 + (k - p));
+            slot_at[apps[k].seq] = (uint32_t)(d0 + (k - p));
         }
         g->deg[r] = (uint32_t)need;
         touched.push_back(r);
@@ -250,7 +250,7 @@ dyg_status graph_apply_inserts(dyg_graph g, const uint32_t* u, const uint32_t* v
         for (uint32_t i = 0; i < n; ++i) {
             const long a = u[i], b = v[i];
             const long key = key_of(a, b, g->mult);
-            const uint32_t sa = slot_of[2 * i], sb = slot_of[2 * i + 1];
+            const uint32_t sa = slot_at[2 * i], sb = slot_at[2 * i + 1];
             if (a <= b) g->key_map.insert({key, {sa, sb}});
             else g->key_map.insert({key, {sb, sa}});
         }

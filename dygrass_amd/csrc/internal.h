@@ -182,6 +182,44 @@ hipError_t launch_heuristic(const uint2* rows, const uint4* adj, const uint32_t*
 hipError_t launch_apply_delta(uint2* rows, uint4* adj, const uint32_t* delta, uint32_t n_write,
                               uint32_t n_row, hipStream_t s);
 
+// Slot of column b in a row r = (first slot, degree), or -1.  The columns
+// are read eight at a time as independent loads.
+__device__ __forceinline__ int slot_of(const uint4* __restrict__ adj, uint2 r, uint32_t b) {
+    const uint4* p = adj + r.x;
+    uint32_t q = 0;
+    for (; q + 8 <= r.y; q += 8) {
+        uint32_t c[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) c[j] = p[q + j].x;
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (c[j] == b) return (int)(q + j);
+    }
+    for (; q < r.y; ++q)
+        if (p[q].x == b) return (int)q;
+    return -1;
+}
+
+// In a row (first slot, degree) rx, every slot pointing to r gets r's
+// descriptor rr.  Only the slots' upper halves are written, so the columns
+// are read eight at a time as independent loads (a neighbour's row is one
+// short chain of loads, not one load per slot).
+__device__ __forceinline__ void refresh_pointers(uint4* __restrict__ adj, uint2 rx, uint32_t r,
+                                                 uint2 rr) {
+    uint4* p = adj + rx.x;
+    uint32_t k = 0;
+    for (; k + 8 <= rx.y; k += 8) {
+        uint32_t c[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) c[j] = p[k + j].x;
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (c[j] == r) *reinterpret_cast<uint2*>(&p[k + j].z) = rr;
+    }
+    for (; k < rx.y; ++k)
+        if (p[k].x == r) *reinterpret_cast<uint2*>(&p[k].z) = rr;
+}
+
 // One graph's device state for edits made inside a single workgroup
 // (stream_small.hip): same arena, capacities and counters as the edit pipeline.
 struct BlockGraph {

@@ -59,10 +59,7 @@ double now_ms() {
 
 __device__ __forceinline__ int row_find(const uint2* __restrict__ rows,
                                         const uint4* __restrict__ adj, uint32_t a, uint32_t b) {
-    const uint2 r = rows[a];
-    for (uint32_t q = 0; q < r.y; ++q)
-        if (adj[r.x + q].x == b) return (int)q;
-    return -1;
+    return slot_of(adj, rows[a], b);
 }
 
 // membership through the shorter of the two rows

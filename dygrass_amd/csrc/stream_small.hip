@@ -62,26 +62,9 @@ union TempU {
     uint32_t rk[2 * kThreads];         // counting sort of rows: rows in entry order
 };
 
-// Slot of b in a row (first slot, degree), or -1; four independent loads in
-// flight per round.
-__device__ __forceinline__ int find_slot(const uint4* adj, uint2 r, uint32_t b) {
-    const uint4* p = adj + r.x;
-    uint32_t q = 0;
-    for (; q + 4 <= r.y; q += 4) {
-        const uint32_t c0 = p[q].x, c1 = p[q + 1].x, c2 = p[q + 2].x, c3 = p[q + 3].x;
-        if (c0 == b) return (int)q;
-        if (c1 == b) return (int)q + 1;
-        if (c2 == b) return (int)q + 2;
-        if (c3 == b) return (int)q + 3;
-    }
-    for (; q < r.y; ++q)
-        if (p[q].x == b) return (int)q;
-    return -1;
-}
-
 __device__ __forceinline__ int find_in_row(const uint2* rows, const uint4* adj, uint32_t a,
                                            uint32_t b) {
-    return find_slot(adj, rows[a], b);
+    return slot_of(adj, rows[a], b);
 }
 
 __device__ __forceinline__ bool holds(const uint2* rows, const uint4* adj, uint32_t a, uint32_t b) {
@@ -262,7 +245,7 @@ __device__ void block_delete(const BlockGraph& g, TempU& tmp, uint32_t* srow, ui
         for (uint32_t z = p; z < end; ++z) {
             const uint32_t x = sent[z], e = x >> 1;
             const uint32_t nbr = (x & 1u) ? src[e] : dst[e];
-            const int s = find_slot(g.adj, row, nbr);
+            const int s = slot_of(g.adj, row, nbr);
             if (s < 0) {  // not present: the caller's lists were inconsistent
                 atomicOr((uint32_t*)(g.meta + 2), 2u);
                 continue;
@@ -586,11 +569,7 @@ __global__ __launch_bounds__(256) void k_refresh_rows(RefreshList l0, RefreshLis
     const uint2 rr = l.rows[r];
     for (uint32_t x = lane; x < rr.y; x += 64u) {
         const uint32_t nb = l.adj[rr.x + x].x;
-        const uint2 rn = l.rows[nb];
-        for (uint32_t y = 0; y < rn.y; ++y) {
-            uint4* sl = &l.adj[rn.x + y];
-            if (sl->x == r) *reinterpret_cast<uint2*>(&sl->z) = rr;
-        }
+        refresh_pointers(l.adj, l.rows[nb], r, rr);
     }
 }
 
