@@ -154,28 +154,46 @@ __global__ void k_del_rows(uint2* __restrict__ rows, uint4* __restrict__ adj,
     rows[r] = row;
 }
 
-// One thread per touched row r (its first append): rewrite r's descriptor in
-// every slot that points to r.  Each such slot is written only by r's thread
-// and only its upper half (.z,.w), so threads never race; the neighbour ids
-// and the row descriptors read here are final once k_write has run.
+// One group of LPR lanes per sorted entry; a touched row's first entry
+// rewrites r's descriptor in every slot that points to r, its lanes taking
+// the row's neighbours (so a lane's chain of dependent loads is one
+// neighbour's row, not all of them; LPR near the graph's mean degree keeps
+// the lanes busy).  Each such slot is written only by r's group and only its
+// upper half (.z,.w), so groups never race; the neighbour ids and the row
+// descriptors read here are final once k_write has run.
+template <uint32_t LPR>
 __global__ __launch_bounds__(256) void k_refresh(
     const uint2* __restrict__ rows, uint4* __restrict__ adj, const uint32_t* __restrict__ keys,
     const uint32_t* __restrict__ count, uint32_t* __restrict__ segstart,
     uint32_t* __restrict__ segcnt) {
-    // one wave per sorted entry; only a row's first entry does the work, its
-    // lanes taking the row's neighbours (so a lane's chain of dependent loads
-    // is one neighbour's row, not all of them)
-    const uint32_t p = blockIdx.x * (blockDim.x / 64u) + (threadIdx.x >> 6);
-    const uint32_t lane = __lane_id();
+    const uint32_t p = (blockIdx.x * 256u + threadIdx.x) / LPR;
+    const uint32_t lane = threadIdx.x & (LPR - 1u);
     if (p >= *count) return;
     const uint32_t r = keys[p];
     if (segstart[r] != p) return;
-    if (lane == 0) segcnt[r] = 0;  // ready for the next batch
     const uint2 rr = rows[r];
-    for (uint32_t q = lane; q < rr.y; q += 64u) {
+    for (uint32_t q = lane; q < rr.y; q += LPR) {
         const uint32_t x = adj[rr.x + q].x;
         refresh_pointers(adj, rows[x], r, rr);
     }
+    if (lane == 0) segcnt[r] = 0;  // ready for the next batch
+}
+
+void launch_refresh(dyg_graph g, const uint32_t* keys, const uint32_t* count, uint32_t entries,
+                    hipStream_t s) {
+    DevEdits& d = g->dev;
+    const uint32_t lpr = refresh_lanes(g);
+    const uint32_t blocks = (uint32_t)(((uint64_t)entries * lpr + 255) / 256);
+    if (blocks == 0) return;
+#define DYG_REFRESH(L)                                                                          \
+    case L:                                                                                     \
+        hipLaunchKernelGGL(k_refresh<L>, dim3(blocks), dim3(256), 0, s, g->d_rows, g->d_adj, keys, \
+                           count, d.segstart, d.segcnt);                                        \
+        break;
+    switch (lpr) {
+        DYG_REFRESH(4) DYG_REFRESH(8) DYG_REFRESH(16) DYG_REFRESH(32) default: DYG_REFRESH(64)
+    }
+#undef DYG_REFRESH
 }
 
 }  // namespace
@@ -303,8 +321,7 @@ dyg_status device_deletes_batch(dyg_graph g, const uint32_t* src, const uint32_t
     const uint32_t blocks = (2 * n + 255) / 256;
     hipLaunchKernelGGL(k_del_rows, dim3(blocks), dim3(256), 0, s, g->d_rows, g->d_adj, d.keys_out,
                        d.vals_out, count, d.segstart, d.segcnt, src, dst, err);
-    hipLaunchKernelGGL(k_refresh, dim3((2 * n + 3) / 4), dim3(256), 0, s, g->d_rows, g->d_adj,
-                       d.keys_out, count, d.segstart, d.segcnt);
+    launch_refresh(g, d.keys_out, count, 2 * n, s);
     DYG_HIP(ctx, hipGetLastError());
     return DYG_OK;
 }
@@ -327,8 +344,7 @@ dyg_status device_edits_batch(dyg_graph g, const uint32_t* src, const uint32_t* 
                        (unsigned long long)g->cap_slots, g->slack, err);
     hipLaunchKernelGGL(k_write, dim3(blocks), dim3(256), 0, s, g->d_adj, d.keys_out, d.vals_out,
                        count, d.segstart, d.segcnt, d.rowbase, src, dst, w);
-    hipLaunchKernelGGL(k_refresh, dim3((2 * n + 3) / 4), dim3(256), 0, s, g->d_rows, g->d_adj,
-                       d.keys_out, count, d.segstart, d.segcnt);
+    launch_refresh(g, d.keys_out, count, 2 * n, s);
     DYG_HIP(ctx, hipGetLastError());
     return DYG_OK;
 }

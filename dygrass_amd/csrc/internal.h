@@ -220,6 +220,15 @@ __device__ __forceinline__ void refresh_pointers(uint4* __restrict__ adj, uint2 
         if (p[k].x == r) *reinterpret_cast<uint2*>(&p[k].z) = rr;
 }
 
+// Lanes per touched row for the descriptor refresh: the power of two at or
+// above the graph's mean degree, within [4, 64].
+inline uint32_t refresh_lanes(const dyg_graph_s* g) {
+    const uint64_t mean = g->V ? (g->nslots + g->V - 1) / g->V : 1;
+    uint32_t l = 4;
+    while (l < 64 && l < mean) l <<= 1;
+    return l;
+}
+
 // One graph's device state for edits made inside a single workgroup
 // (stream_small.hip): same arena, capacities and counters as the edit pipeline.
 struct BlockGraph {
@@ -229,6 +238,7 @@ struct BlockGraph {
     unsigned long long* meta;  // DevEdits::meta of the graph
     unsigned long long cap_slots;
     uint32_t slack;
+    uint32_t lpr;  // lanes per row of its descriptor refresh (refresh_lanes)
 };
 // A batch's results, written by its last kernel straight into pinned host
 // memory (device-visible addresses): the outcome codes and 14 counter words,

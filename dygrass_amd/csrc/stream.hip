@@ -591,7 +591,7 @@ WalkLaunch walk_args(dyg_stream s, dyg_graph g, const dyg_walk_params& p, uint32
 
 BlockGraph block_graph(dyg_graph g) {
     return BlockGraph{g->d_rows, g->d_adj, g->dev.cap, g->dev.meta,
-                      (unsigned long long)g->cap_slots, g->slack};
+                      (unsigned long long)g->cap_slots, g->slack, refresh_lanes(g)};
 }
 
 // Small insertion batch: two one-workgroup kernels around the walk.

@@ -551,23 +551,25 @@ __global__ __launch_bounds__(kThreads) void k_small_del_post(SmallDelArgs a) {
     report_out(a.rep, a.cls, a.n, a.stats, t, kThreads);
 }
 
-// One wave per listed row: every slot pointing to the row gets its final
+// lpr lanes per listed row: every slot pointing to the row gets its final
 // descriptor (a lane per neighbour).  blockIdx.y selects one of two lists.
 struct RefreshList {
     const uint2* rows;
     uint4* adj;
     const uint32_t* list;
     const uint32_t* count;
+    uint32_t lpr;  // power of two, 4..64
 };
 
 __global__ __launch_bounds__(256) void k_refresh_rows(RefreshList l0, RefreshList l1) {
     const RefreshList l = blockIdx.y ? l1 : l0;
-    const uint32_t i = blockIdx.x * 4u + (threadIdx.x >> 6);
+    const uint32_t g = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t i = g / l.lpr;
     if (i >= *l.count) return;
-    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t lane = g & (l.lpr - 1u);
     const uint32_t r = l.list[i];
     const uint2 rr = l.rows[r];
-    for (uint32_t x = lane; x < rr.y; x += 64u) {
+    for (uint32_t x = lane; x < rr.y; x += l.lpr) {
         const uint32_t nb = l.adj[rr.x + x].x;
         refresh_pointers(l.adj, l.rows[nb], r, rr);
     }
@@ -582,34 +584,36 @@ __global__ __launch_bounds__(256) void k_report(Report r, const int32_t* __restr
 
 // Refreshes the rows of list a (and b) in one launch; max_rows bounds both.
 void refresh(const RefreshList& a, const RefreshList* b, uint32_t max_rows, hipStream_t s) {
-    hipLaunchKernelGGL(k_refresh_rows, dim3((max_rows + 3) / 4, b ? 2 : 1), dim3(256), 0, s, a,
-                       b ? *b : a);
+    const uint32_t lpr = b ? std::max(a.lpr, b->lpr) : a.lpr;
+    const uint32_t blocks = (uint32_t)(((uint64_t)max_rows * lpr + 255) / 256);
+    if (blocks == 0) return;
+    hipLaunchKernelGGL(k_refresh_rows, dim3(blocks, b ? 2 : 1), dim3(256), 0, s, a, b ? *b : a);
 }
 
 }  // namespace
 
 hipError_t launch_small_ins_pre(const SmallInsArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_small_ins_pre, dim3(1), dim3(kThreads), 0, s, a);
-    refresh({a.G.rows, a.G.adj, a.touched, a.cnt + 4}, nullptr, 2 * a.n, s);
+    refresh({a.G.rows, a.G.adj, a.touched, a.cnt + 4, a.G.lpr}, nullptr, 2 * a.n, s);
     return hipGetLastError();
 }
 
 hipError_t launch_small_ins_post(const SmallInsArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_small_ins_post, dim3(1), dim3(kThreads), 0, s, a);
-    refresh({a.H.rows, a.H.adj, a.touched, a.cnt + 4}, nullptr, 2 * a.n, s);
+    refresh({a.H.rows, a.H.adj, a.touched, a.cnt + 4, a.H.lpr}, nullptr, 2 * a.n, s);
     return hipGetLastError();
 }
 
 hipError_t launch_small_del_pre(const SmallDelArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_small_del_pre, dim3(1), dim3(kThreads), 0, s, a);
-    const RefreshList h{a.H.rows, a.H.adj, a.touched_h, a.cnt + 5};
-    refresh({a.G.rows, a.G.adj, a.touched_g, a.cnt + 4}, &h, 2 * a.n, s);
+    const RefreshList h{a.H.rows, a.H.adj, a.touched_h, a.cnt + 5, a.H.lpr};
+    refresh({a.G.rows, a.G.adj, a.touched_g, a.cnt + 4, a.G.lpr}, &h, 2 * a.n, s);
     return hipGetLastError();
 }
 
 hipError_t launch_small_del_post(const SmallDelArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_small_del_post, dim3(1), dim3(kThreads), 0, s, a);
-    refresh({a.H.rows, a.H.adj, a.touched_x, a.cnt + 6}, nullptr, 2 * kSmallCand, s);
+    refresh({a.H.rows, a.H.adj, a.touched_x, a.cnt + 6, a.H.lpr}, nullptr, 2 * kSmallCand, s);
     return hipGetLastError();
 }
 
