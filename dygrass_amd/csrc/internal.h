@@ -258,7 +258,7 @@ hipError_t launch_report(const Report& r, const int32_t* cls, uint32_t n,
 // An insertion batch of at most kSmallBatch edges in two one-workgroup
 // kernels around the walk (device pointers; u, v and w may point into
 // pinned host memory).
-constexpr uint32_t kSmallBatch = 1024;
+constexpr uint32_t kSmallBatch = 2048;
 struct SmallInsArgs {
     BlockGraph G, H;
     const uint32_t* u;
@@ -284,9 +284,10 @@ hipError_t launch_small_ins_pre(const SmallInsArgs& a, hipStream_t s);
 hipError_t launch_small_ins_post(const SmallInsArgs& a, hipStream_t s);
 
 // A deletion batch of at most kSmallBatch edges: one-workgroup kernels
-// before and after the walk on G; a recovery of more than kSmallCand
-// candidate edges is left to the general kernels (flag sizes[4]).
-constexpr uint32_t kSmallCand = 1024;
+// before and after the walk on G; a recovery of more candidate edges than
+// the post kernel holds (small_del_candidates, at most kSmallCand) is left
+// to the general kernels (flag sizes[4]).
+constexpr uint32_t kSmallCand = 2048;
 struct SmallDelArgs {
     BlockGraph G, H;
     const uint32_t* u;
@@ -318,6 +319,8 @@ struct SmallDelArgs {
     unsigned long long* stats;
     Report rep;
 };
+// Candidate edges the post kernel of an n-update deletion batch can hold.
+uint32_t small_del_candidates(uint32_t n);
 hipError_t launch_small_del_pre(const SmallDelArgs& a, hipStream_t s);
 hipError_t launch_small_del_post(const SmallDelArgs& a, hipStream_t s);
 

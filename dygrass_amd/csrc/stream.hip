@@ -740,7 +740,7 @@ dyg_status apply_delete_small(dyg_stream s, const uint32_t* du, const uint32_t* 
     DYG_HIP(s->ctx, launch_walk_decremental(w, st));
     if (timed) DYG_HIP(s->ctx, hipEventRecord(s->ctx->ev1, st));
     DYG_HIP(s->ctx, launch_small_del_post(a, st));
-    if ((uint64_t)n * (ms - 1) + 4ull * n * hw <= kSmallCand) return DYG_OK;
+    if ((uint64_t)n * (ms - 1) + 4ull * n * hw <= small_del_candidates(n)) return DYG_OK;
     dyg_status r = wait_spin(s->ctx, st, s->ev_sync);
     if (r != DYG_OK) return r;
     const uint32_t* hc = (const uint32_t*)((char*)s->pin + kPinSizes);

@@ -4,7 +4,8 @@
 // The general pipeline of a batch (stream.hip) is thirty to fifty launches;
 // for a batch of a few hundred edges every kernel runs for a few
 // microseconds and the launches themselves set the pace.  Here one
-// workgroup of 1024 lanes does a phase with the batch in LDS.
+// workgroup of 1024 lanes (two updates per lane) does a phase with the batch
+// in LDS (up to about 130 KB of the CU's 160 KB).
 //
 // Insertion batch:
 //   pre:  sort the (min,max) keys (stable: block radix sort, or a rank by
@@ -39,27 +40,38 @@ namespace dyg {
 namespace {
 
 constexpr int kThreads = 1024;
+static_assert(2 * kThreads == kSmallBatch && 2 * kThreads == kSmallCand,
+              "two items per lane at most");
 constexpr uint32_t kPadRow = 0xFFFFFFFFu;
 constexpr unsigned long long kPad64 = ~0ull;
 constexpr int32_t kOutWeight = 1, kOutFound = 2, kOutAdded = 3, kOutSkip = 4, kOutGOnly = 5,
                   kOutRecovered = 6, kOutHeuristic = 7, kOutInvalid = 8;
 
 using Scan = hipcub::BlockScan<uint32_t, kThreads>;
-using SortKeys64 = hipcub::BlockRadixSort<unsigned long long, kThreads, 1, uint32_t>;
-using SortRows = hipcub::BlockRadixSort<uint32_t, kThreads, 2, uint32_t>;
 
 // Up to this many items a sort ranks by counting (one wave, a few hundred
 // LDS reads); above it, a block radix sort (O(items) work per pass: counting
 // would be items^2 compares on one CU).
 constexpr uint32_t kCountMax = 64;
 
-// Phase-local scratch, one phase after another (a barrier between).
-union TempU {
-    SortKeys64::TempStorage k64;
-    SortRows::TempStorage rows;
-    Scan::TempStorage scan;
-    unsigned long long kin[kThreads];  // counting sort of keys: keys in input order
-    uint32_t rk[2 * kThreads];         // counting sort of rows: rows in entry order
+// A phase's shape: IPT batch items (edges) per lane, 2*IPT row entries per
+// lane.  One item per lane keeps the sorts and LDS small for the common small
+// batches; two per lane take batches of up to 2048.
+template <int IPT>
+struct Shape {
+    static constexpr int kIPT = IPT;
+    static constexpr int kEPT = 2 * IPT;
+    static constexpr uint32_t kItems = kThreads * IPT;
+    using SortKeys64 = hipcub::BlockRadixSort<unsigned long long, kThreads, IPT, uint32_t>;
+    using SortRows = hipcub::BlockRadixSort<uint32_t, kThreads, 2 * IPT, uint32_t>;
+    // Phase-local scratch, one phase after another (a barrier between).
+    union TempU {
+        typename SortKeys64::TempStorage k64;
+        typename SortRows::TempStorage rows;
+        typename Scan::TempStorage scan;
+        unsigned long long kin[kItems];  // counting sort of keys: keys in input order
+        uint32_t rk[2 * kItems];         // counting sort of rows: rows in entry order
+    };
 };
 
 __device__ __forceinline__ int find_in_row(const uint2* rows, const uint4* adj, uint32_t a,
@@ -81,119 +93,159 @@ __device__ __forceinline__ uint32_t row_pad(int row_bits) {
     return row_bits >= 32 ? 0xFFFFFFFFu : (1u << row_bits) - 1u;
 }
 
-// Stable sort of n keys (lane t < n holds kt, the others kPad64; bits
-// [0, bits) order them): key[rank] = kt, idx[rank] = t.  Lanes past n keep
+// Stable sort of n keys; lane t holds items kIPT*t + k (kPad64 past n; bits
+// [0, bits) order them): key[rank] = kt, idx[rank] = item.  Items past n keep
 // their place.
-__device__ void sort_keys64(TempU& tmp, unsigned long long* key, uint32_t* idx,
-                            unsigned long long kt, uint32_t t, uint32_t n, int bits) {
+template <int IPT>
+__device__ void sort_keys64(typename Shape<IPT>::TempU& tmp, unsigned long long* key,
+                            uint32_t* idx, const unsigned long long (&kt)[IPT], uint32_t t,
+                            uint32_t n, int bits) {
+    constexpr int kIPT = IPT;
     if (n <= kCountMax) {
-        tmp.kin[t] = kt;
+#pragma unroll
+        for (int k = 0; k < kIPT; ++k) tmp.kin[kIPT * t + k] = kt[k];
         __syncthreads();
-        uint32_t rank = t;
-        if (t < n) {
-            rank = 0;
-            for (uint32_t j = 0; j < n; ++j) {
-                const unsigned long long kj = tmp.kin[j];
-                rank += (kj < kt || (kj == kt && j < t)) ? 1u : 0u;
+#pragma unroll
+        for (int k = 0; k < kIPT; ++k) {
+            const uint32_t p = kIPT * t + k;
+            uint32_t rank = p;
+            if (p < n) {
+                rank = 0;
+                for (uint32_t j = 0; j < n; ++j) {
+                    const unsigned long long kj = tmp.kin[j];
+                    rank += (kj < kt[k] || (kj == kt[k] && j < p)) ? 1u : 0u;
+                }
             }
+            key[rank] = kt[k];
+            idx[rank] = p;
         }
-        key[rank] = kt;
-        idx[rank] = t;
     } else {
-        unsigned long long k[1] = {kt};
-        uint32_t v[1] = {t};
-        SortKeys64(tmp.k64).Sort(k, v, 0, bits);
-        key[t] = k[0];
-        idx[t] = v[0];
+        unsigned long long kk[kIPT];
+        uint32_t v[kIPT];
+#pragma unroll
+        for (int k = 0; k < kIPT; ++k) {
+            kk[k] = kt[k];
+            v[k] = kIPT * t + k;
+        }
+        typename Shape<IPT>::SortKeys64(tmp.k64).Sort(kk, v, 0, bits);
+#pragma unroll
+        for (int k = 0; k < kIPT; ++k) {
+            key[kIPT * t + k] = kk[k];
+            idx[kIPT * t + k] = v[k];
+        }
     }
     __syncthreads();
 }
 
-// Stable sort of the entries 2e (row r0 of lane e) and 2e+1 (r1) of the
-// first `span` lanes by row, pads (row_pad) last: srow/sent[p] = row and
-// entry at sorted position p < 2*span.
-__device__ void sort_rows(TempU& tmp, uint32_t* srow, uint32_t* sent, uint32_t r0, uint32_t r1,
-                          uint32_t t, uint32_t span, int bits) {
+// Stable sort by row of the row entries x = kEPT*t + q (row r[q]; entries
+// of the first `span` edges count, pads last): srow/sent[p] = row and entry
+// at sorted position p < 2*span.
+template <int IPT>
+__device__ void sort_rows(typename Shape<IPT>::TempU& tmp, uint32_t* srow, uint32_t* sent,
+                          const uint32_t (&r)[2 * IPT], uint32_t t, uint32_t span, int bits) {
+    constexpr int kEPT = 2 * IPT;
+    const uint32_t ne = 2 * span;
     if (span <= kCountMax) {
-        tmp.rk[2 * t] = r0;
-        tmp.rk[2 * t + 1] = r1;
+#pragma unroll
+        for (int q = 0; q < kEPT; ++q) tmp.rk[kEPT * t + q] = r[q];
         __syncthreads();
-        if (t < span) {
-            uint32_t a0 = 0, a1 = 0;
-            for (uint32_t y = 0; y < 2 * span; ++y) {
+#pragma unroll
+        for (int q = 0; q < kEPT; ++q) {
+            const uint32_t x = kEPT * t + q;
+            if (x >= ne) continue;
+            uint32_t a = 0;
+            for (uint32_t y = 0; y < ne; ++y) {
                 const uint32_t ry = tmp.rk[y];
-                a0 += (ry < r0 || (ry == r0 && y < 2 * t)) ? 1u : 0u;
-                a1 += (ry < r1 || (ry == r1 && y < 2 * t + 1)) ? 1u : 0u;
+                a += (ry < r[q] || (ry == r[q] && y < x)) ? 1u : 0u;
             }
-            srow[a0] = r0;
-            sent[a0] = 2 * t;
-            srow[a1] = r1;
-            sent[a1] = 2 * t + 1;
+            srow[a] = r[q];
+            sent[a] = x;
         }
     } else {
-        uint32_t k[2] = {r0, r1};
-        uint32_t v[2] = {2 * t, 2 * t + 1};
-        SortRows(tmp.rows).Sort(k, v, 0, bits);
-        srow[2 * t] = k[0];
-        srow[2 * t + 1] = k[1];
-        sent[2 * t] = v[0];
-        sent[2 * t + 1] = v[1];
+        uint32_t k[kEPT], v[kEPT];
+#pragma unroll
+        for (int q = 0; q < kEPT; ++q) {
+            k[q] = r[q];
+            v[q] = kEPT * t + q;
+        }
+        typename Shape<IPT>::SortRows(tmp.rows).Sort(k, v, 0, bits);
+#pragma unroll
+        for (int q = 0; q < kEPT; ++q) {
+            srow[kEPT * t + q] = k[q];
+            sent[kEPT * t + q] = v[q];
+        }
     }
     __syncthreads();
 }
 
 // Lists the rows r[q] flagged h[q] by this lane, in any order, and their count.
-__device__ __forceinline__ void list_rows(Scan::TempStorage& scan, const bool (&h)[2],
-                                          const uint32_t (&r)[2], uint32_t t,
+template <int kEPT>
+__device__ __forceinline__ void list_rows(Scan::TempStorage& scan, const bool (&h)[kEPT],
+                                          const uint32_t (&r)[kEPT], uint32_t t,
                                           uint32_t* __restrict__ touched,
                                           uint32_t* __restrict__ ntouched) {
+    uint32_t c = 0;
+#pragma unroll
+    for (int q = 0; q < kEPT; ++q) c += h[q] ? 1u : 0u;
     uint32_t pos = 0, total = 0;
-    Scan(scan).ExclusiveSum((h[0] ? 1u : 0u) + (h[1] ? 1u : 0u), pos, total);
-    if (h[0]) touched[pos++] = r[0];
-    if (h[1]) touched[pos] = r[1];
+    Scan(scan).ExclusiveSum(c, pos, total);
+#pragma unroll
+    for (int q = 0; q < kEPT; ++q)
+        if (h[q]) touched[pos++] = r[q];
     if (t == 0) *ntouched = total;
 }
 
-// Row groups of a sorted entry list: sorted position p (one of this lane's
-// two, 2t + j) starts a group when it holds a row and differs from p-1; the
-// group runs to `end`.
+// Row groups of a sorted entry list: sorted position p starts a group when
+// it holds a row and differs from p-1; the group runs to `end`.
 __device__ __forceinline__ bool group_head(const uint32_t* srow, uint32_t p, uint32_t lim,
                                            uint32_t pad, uint32_t& end) {
+    if (p >= lim) return false;
     const uint32_t r = srow[p];
-    if (p >= lim || r == pad || (p > 0 && srow[p - 1] == r)) return false;
+    if (r == pad || (p > 0 && srow[p - 1] == r)) return false;
     end = p + 1;
     while (end < lim && srow[end] == r) ++end;
     return true;
 }
 
-// Appends edge t (lane t < span: src[t]-dst[t] with weight w[t]) to both
-// rows when `take`.  Entry 2t is the append to src[t]'s row, 2t+1 to
-// dst[t]'s; each row's appends keep entry order (the general pipeline's
-// order), written by the lane holding the row's first sorted entry.
-// Returns after the rows are final, with the touched rows listed for
-// k_refresh_rows, which rewrites the neighbours' copies of their descriptors
-// across the whole GPU (in one workgroup that refresh would be deg^2
-// dependent loads per lane).
-__device__ void block_append(const BlockGraph& g, TempU& tmp, uint32_t* srow, uint32_t* sent,
-                             const uint32_t* src, const uint32_t* dst, const float* w,
-                             bool take, uint32_t t, uint32_t span, int bits,
-                             uint32_t* __restrict__ touched, uint32_t* __restrict__ ntouched) {
+// Appends edges e = kIPT*t + k (e < span: src[e]-dst[e] with weight w[e])
+// to both rows when take[k].  Entry 2e is the append to src[e]'s row, 2e+1
+// to dst[e]'s; each row's appends keep entry order (the general pipeline's
+// order), written by the lane holding the row's first sorted entry.  Returns
+// after the rows are final, with the touched rows listed for k_refresh_rows,
+// which rewrites the neighbours' copies of their descriptors across the
+// whole GPU (in one workgroup that refresh would be deg^2 dependent loads
+// per lane).
+template <int IPT>
+__device__ void block_append(const BlockGraph& g, typename Shape<IPT>::TempU& tmp,
+                             uint32_t* srow, uint32_t* sent, const uint32_t* src,
+                             const uint32_t* dst, const float* w, const bool (&take)[IPT],
+                             uint32_t t, uint32_t span, int bits, uint32_t* __restrict__ touched,
+                             uint32_t* __restrict__ ntouched) {
+    constexpr int kIPT = IPT, kEPT = 2 * IPT;
     const uint32_t pad = row_pad(bits);
-    sort_rows(tmp, srow, sent, take ? src[t] : pad, take ? dst[t] : pad, t, span, bits);
-    const int appended = __syncthreads_count(take ? 1 : 0);
-    bool h[2];
-    uint32_t r[2];
+    uint32_t re[kEPT];
+    uint32_t mine = 0;
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-        const uint32_t p = 2 * t + j;
+    for (int k = 0; k < kIPT; ++k) {
+        const uint32_t e = kIPT * t + k;
+        re[2 * k] = take[k] ? src[e] : pad;
+        re[2 * k + 1] = take[k] ? dst[e] : pad;
+        mine += take[k] ? 1u : 0u;
+    }
+    sort_rows<IPT>(tmp, srow, sent, re, t, span, bits);
+    bool h[kEPT];
+    uint32_t r[kEPT];
+#pragma unroll
+    for (int q = 0; q < kEPT; ++q) {
+        const uint32_t p = kEPT * t + q;
         uint32_t end = 0;
-        h[j] = group_head(srow, p, 2 * span, pad, end);
-        r[j] = h[j] ? srow[p] : pad;
-        if (!h[j]) continue;
+        h[q] = group_head(srow, p, 2 * span, pad, end);
+        r[q] = h[q] ? srow[p] : pad;
+        if (!h[q]) continue;
         const uint32_t cnt = end - p;
-        uint2 row = g.rows[r[j]];
+        uint2 row = g.rows[r[q]];
         const uint32_t d0 = row.y;
-        const uint32_t c = g.cap[r[j]];
+        const uint32_t c = g.cap[r[q]];
         if ((unsigned long long)d0 + cnt > c) {  // relocate to the arena tail, order kept
             unsigned long long ncap = 2ull * c;
             ncap = max(ncap, (unsigned long long)d0 + cnt + g.slack);
@@ -205,43 +257,54 @@ __device__ void block_append(const BlockGraph& g, TempU& tmp, uint32_t* srow, ui
             }
             for (uint32_t z = 0; z < d0; ++z) g.adj[ns + z] = g.adj[row.x + z];
             row.x = (uint32_t)ns;
-            g.cap[r[j]] = (uint32_t)ncap;
+            g.cap[r[q]] = (uint32_t)ncap;
         }
         for (uint32_t z = p; z < end; ++z) {
             const uint32_t x = sent[z], e = x >> 1;
             const uint32_t nbr = (x & 1u) ? src[e] : dst[e];
             g.adj[row.x + d0 + (z - p)] = make_uint4(nbr, __float_as_uint(w[e]), 0u, 0u);
         }
-        g.rows[r[j]] = make_uint2(row.x, d0 + cnt);
+        g.rows[r[q]] = make_uint2(row.x, d0 + cnt);
     }
-    if (t == 0 && appended) atomicAdd(&g.meta[1], (unsigned long long)appended);
+    if (mine) atomicAdd(&g.meta[1], (unsigned long long)mine);
     list_rows(tmp.scan, h, r, t, touched, ntouched);
     __syncthreads();
 }
 
-// Deletes edge t (lane t < span: src[t]-dst[t]) from both rows when `take`;
-// each row's removals in entry order (2t: src[t]'s row, 2t+1: dst[t]'s),
-// each by swap-with-last (the last live slot moves into the hole with its
-// copy of its neighbour's descriptor), by the lane holding the row's first
-// sorted entry.  A removal the row cannot make sets error bit 2.  Lists the
-// touched rows.
-__device__ void block_delete(const BlockGraph& g, TempU& tmp, uint32_t* srow, uint32_t* sent,
-                             const uint32_t* src, const uint32_t* dst, bool take, uint32_t t,
+// Deletes edges e = kIPT*t + k (e < span: src[e]-dst[e]) from both rows
+// when take[k]; each row's removals in entry order (2e: src[e]'s row, 2e+1:
+// dst[e]'s), each by swap-with-last (the last live slot moves into the hole
+// with its copy of its neighbour's descriptor), by the lane holding the
+// row's first sorted entry.  A removal the row cannot make sets error bit 2.
+// Lists the touched rows.
+template <int IPT>
+__device__ void block_delete(const BlockGraph& g, typename Shape<IPT>::TempU& tmp,
+                             uint32_t* srow, uint32_t* sent, const uint32_t* src,
+                             const uint32_t* dst, const bool (&take)[IPT], uint32_t t,
                              uint32_t span, int bits, uint32_t* __restrict__ touched,
                              uint32_t* __restrict__ ntouched) {
+    constexpr int kIPT = IPT, kEPT = 2 * IPT;
     const uint32_t pad = row_pad(bits);
-    sort_rows(tmp, srow, sent, take ? src[t] : pad, take ? dst[t] : pad, t, span, bits);
-    const int deleted = __syncthreads_count(take ? 1 : 0);
-    bool h[2];
-    uint32_t r[2];
+    uint32_t re[kEPT];
+    uint32_t mine = 0;
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-        const uint32_t p = 2 * t + j;
+    for (int k = 0; k < kIPT; ++k) {
+        const uint32_t e = kIPT * t + k;
+        re[2 * k] = take[k] ? src[e] : pad;
+        re[2 * k + 1] = take[k] ? dst[e] : pad;
+        mine += take[k] ? 1u : 0u;
+    }
+    sort_rows<IPT>(tmp, srow, sent, re, t, span, bits);
+    bool h[kEPT];
+    uint32_t r[kEPT];
+#pragma unroll
+    for (int q = 0; q < kEPT; ++q) {
+        const uint32_t p = kEPT * t + q;
         uint32_t end = 0;
-        h[j] = group_head(srow, p, 2 * span, pad, end);
-        r[j] = h[j] ? srow[p] : pad;
-        if (!h[j]) continue;
-        uint2 row = g.rows[r[j]];
+        h[q] = group_head(srow, p, 2 * span, pad, end);
+        r[q] = h[q] ? srow[p] : pad;
+        if (!h[q]) continue;
+        uint2 row = g.rows[r[q]];
         for (uint32_t z = p; z < end; ++z) {
             const uint32_t x = sent[z], e = x >> 1;
             const uint32_t nbr = (x & 1u) ? src[e] : dst[e];
@@ -254,9 +317,9 @@ __device__ void block_delete(const BlockGraph& g, TempU& tmp, uint32_t* srow, ui
             if ((uint32_t)s != last) g.adj[row.x + s] = g.adj[row.x + last];
             row.y = last;
         }
-        g.rows[r[j]] = row;
+        g.rows[r[q]] = row;
     }
-    if (t == 0 && deleted) atomicAdd(&g.meta[4], (unsigned long long)deleted);
+    if (mine) atomicAdd(&g.meta[4], (unsigned long long)mine);
     list_rows(tmp.scan, h, r, t, touched, ntouched);
     __syncthreads();
 }
@@ -290,201 +353,266 @@ __device__ __forceinline__ void report_out(const Report& r, const int32_t* __res
 }
 
 // ---------------------------------------------------------------- insertions
+template <int IPT>
 struct InsShared {
-    TempU tmp;
-    unsigned long long key[kThreads];  // sorted batch keys
-    uint32_t idx[kThreads];            // their batch positions
-    uint32_t srow[2 * kThreads];       // appends sorted by row: row, entry
-    uint32_t sent[2 * kThreads];
-    uint32_t src[kThreads], dst[kThreads];
-    float w[kThreads];
-    uint8_t keep[kThreads];            // by batch position
-    uint8_t mode[kThreads];            // by sorted position (heads)
+    static constexpr uint32_t kItems = Shape<IPT>::kItems;
+    typename Shape<IPT>::TempU tmp;
+    unsigned long long key[kItems];  // sorted batch keys
+    uint32_t idx[kItems];            // their batch positions
+    uint32_t srow[2 * kItems];       // appends sorted by row: row, entry
+    uint32_t sent[2 * kItems];
+    uint32_t src[kItems], dst[kItems];
+    float w[kItems];
+    uint8_t keep[kItems];            // by batch position
+    uint8_t mode[kItems];            // by sorted position (heads)
 };
 
+template <int IPT>
 __global__ __launch_bounds__(kThreads) void k_small_ins_pre(SmallInsArgs a) {
-    __shared__ InsShared sh;
+    constexpr int kIPT = IPT;
+    __shared__ InsShared<IPT> sh;
     const uint32_t t = threadIdx.x;
     const uint32_t n = a.n;
     // ---- sort the batch's (min,max) keys, stable ----
-    uint32_t ut = 0, vt = 0;
-    float wt = 0.0f;
-    unsigned long long kt = kPad64;
-    if (t < n) {
-        ut = a.u[t];
-        vt = a.v[t];
-        wt = a.w[t];
-        kt = edge_key(ut, vt);
+    uint32_t ut[kIPT], vt[kIPT];
+    float wt[kIPT];
+    unsigned long long kt[kIPT];
+#pragma unroll
+    for (int k = 0; k < kIPT; ++k) {
+        const uint32_t i = kIPT * t + k;
+        ut[k] = i < n ? a.u[i] : 0u;
+        vt[k] = i < n ? a.v[i] : 0u;
+        wt[k] = i < n ? a.w[i] : 0.0f;
+        kt[k] = i < n ? edge_key(ut[k], vt[k]) : kPad64;
+        sh.keep[i] = 0;
+        sh.mode[i] = 0;
     }
-    sh.keep[t] = 0;
-    sh.mode[t] = 0;
-    sort_keys64(sh.tmp, sh.key, sh.idx, kt, t, n, a.key_bits);
+    sort_keys64<IPT>(sh.tmp, sh.key, sh.idx, kt, t, n, a.key_bits);
     // ---- classify each distinct edge, occurrences in batch order ----
-    const unsigned long long k = sh.key[t];
-    if (t < n && k != kPad64 && (t == 0 || sh.key[t - 1] != k)) {
-        uint32_t q = t + 1;
-        while (q < n && sh.key[q] == k) ++q;
-        const uint32_t lo = (uint32_t)(k >> 32), hi = (uint32_t)k;
-        const bool in_g = holds(a.G.rows, a.G.adj, lo, hi);
-        for (uint32_t e = t; e < q; ++e) {
-            const bool first_new = !in_g && e == t;
-            a.cls[sh.idx[e]] = first_new ? 0 : kOutWeight;
-            sh.keep[sh.idx[e]] = first_new ? 1 : 0;
+#pragma unroll
+    for (int k = 0; k < kIPT; ++k) {
+        const uint32_t p = kIPT * t + k;
+        const unsigned long long kp = sh.key[p];
+        if (p < n && kp != kPad64 && (p == 0 || sh.key[p - 1] != kp)) {
+            uint32_t q = p + 1;
+            while (q < n && sh.key[q] == kp) ++q;
+            const uint32_t lo = (uint32_t)(kp >> 32), hi = (uint32_t)kp;
+            const bool in_g = holds(a.G.rows, a.G.adj, lo, hi);
+            for (uint32_t e = p; e < q; ++e) {
+                const bool first_new = !in_g && e == p;
+                a.cls[sh.idx[e]] = first_new ? 0 : kOutWeight;
+                sh.keep[sh.idx[e]] = first_new ? 1 : 0;
+            }
+            if (in_g) sh.mode[p] = 1 | (holds(a.H.rows, a.H.adj, lo, hi) ? 4 : 0);
+            else if (q - p > 1) sh.mode[p] = 2;
+        } else if (p < n && kp == kPad64) {
+            a.cls[sh.idx[p]] = kOutInvalid;
         }
-        if (in_g) sh.mode[t] = 1 | (holds(a.H.rows, a.H.adj, lo, hi) ? 4 : 0);
-        else if (q - t > 1) sh.mode[t] = 2;
-    } else if (t < n && k == kPad64) {
-        a.cls[sh.idx[t]] = kOutInvalid;
+        sh.src[kIPT * t + k] = ut[k];
+        sh.dst[kIPT * t + k] = vt[k];
+        sh.w[kIPT * t + k] = wt[k];
     }
-    sh.src[t] = ut;
-    sh.dst[t] = vt;
-    sh.w[t] = wt;
     __syncthreads();
     // ---- new edges into G, in batch order per row ----
-    const bool take = t < n && sh.keep[t];
-    block_append(a.G, sh.tmp, sh.srow, sh.sent, sh.src, sh.dst, sh.w, take, t, n, a.row_bits,
+    bool take[kIPT];
+#pragma unroll
+    for (int k = 0; k < kIPT; ++k) take[k] = kIPT * t + k < n && sh.keep[kIPT * t + k];
+    block_append<IPT>(a.G, sh.tmp, sh.srow, sh.sent, sh.src, sh.dst, sh.w, take, t, n, a.row_bits,
                  a.touched, a.cnt + 4);
     // ---- weight updates (after the appends: a repeated new edge adds to it) ----
-    const uint8_t md = sh.mode[t];
-    if (md) {
-        uint32_t q = t + 1;
-        while (q < n && sh.key[q] == k) ++q;
-        const uint32_t lo = (uint32_t)(k >> 32), hi = (uint32_t)k;
-        const uint32_t e0 = (md & 1) ? t : t + 1;
+#pragma unroll
+    for (int k = 0; k < kIPT; ++k) {
+        const uint32_t p = kIPT * t + k;
+        const uint8_t md = sh.mode[p];
+        if (!md) continue;
+        const unsigned long long kp = sh.key[p];
+        uint32_t q = p + 1;
+        while (q < n && sh.key[q] == kp) ++q;
+        const uint32_t lo = (uint32_t)(kp >> 32), hi = (uint32_t)kp;
+        const uint32_t e0 = (md & 1) ? p : p + 1;
         add_weights_seq(a.G, lo, hi, sh.idx, e0, q, a.w);
         if (md & 4) add_weights_seq(a.H, lo, hi, sh.idx, e0, q, a.w);
     }
     // ---- compact the kept edges (batch order) for the walk ----
-    uint32_t pos = 0, total = 0;
-    Scan(sh.tmp.scan).ExclusiveSum(take ? 1u : 0u, pos, total);
-    if (take) {
-        a.cu[pos] = ut;
-        a.cv[pos] = vt;
-        a.cw[pos] = wt;
-        a.ceid[pos] = a.base + t;
-        a.sel[pos] = t;
+    uint32_t in[kIPT], pos[kIPT], total = 0;
+#pragma unroll
+    for (int k = 0; k < kIPT; ++k) in[k] = take[k] ? 1u : 0u;
+    Scan(sh.tmp.scan).ExclusiveSum(in, pos, total);
+#pragma unroll
+    for (int k = 0; k < kIPT; ++k) {
+        if (!take[k]) continue;
+        const uint32_t i = kIPT * t + k;
+        a.cu[pos[k]] = ut[k];
+        a.cv[pos[k]] = vt[k];
+        a.cw[pos[k]] = wt[k];
+        a.ceid[pos[k]] = a.base + i;
+        a.sel[pos[k]] = i;
     }
     if (t == 0) a.cnt[0] = total;
 }
 
+template <int IPT>
 __global__ __launch_bounds__(kThreads) void k_small_ins_post(SmallInsArgs a) {
-    __shared__ InsShared sh;
+    constexpr int kIPT = IPT;
+    __shared__ InsShared<IPT> sh;
     const uint32_t t = threadIdx.x;
     const uint32_t m = a.cnt[0];
-    const bool live = t < m;
-    const bool add = live && a.found[t] == -1;
-    sh.src[t] = live ? a.cu[t] : 0u;
-    sh.dst[t] = live ? a.cv[t] : 0u;
-    sh.w[t] = live ? a.cw[t] : 0.0f;
+    bool live[kIPT], add[kIPT];
+#pragma unroll
+    for (int k = 0; k < kIPT; ++k) {
+        const uint32_t j = kIPT * t + k;
+        live[k] = j < m;
+        add[k] = live[k] && a.found[j] == -1;
+        sh.src[j] = live[k] ? a.cu[j] : 0u;
+        sh.dst[j] = live[k] ? a.cv[j] : 0u;
+        sh.w[j] = live[k] ? a.cw[j] : 0.0f;
+    }
     __syncthreads();
-    block_append(a.H, sh.tmp, sh.srow, sh.sent, sh.src, sh.dst, sh.w, add, t, m, a.row_bits,
+    block_append<IPT>(a.H, sh.tmp, sh.srow, sh.sent, sh.src, sh.dst, sh.w, add, t, m, a.row_bits,
                  a.touched, a.cnt + 4);
-    if (live) a.cls[a.sel[t]] = add ? kOutAdded : kOutFound;
+#pragma unroll
+    for (int k = 0; k < kIPT; ++k)
+        if (live[k]) a.cls[a.sel[kIPT * t + k]] = add[k] ? kOutAdded : kOutFound;
     __syncthreads();
     report_out(a.rep, a.cls, a.n, a.stats, t, kThreads);
 }
 
 // ----------------------------------------------------------------- deletions
+template <int IPT>
 struct DelPreShared {
-    TempU tmp;
-    unsigned long long key[kThreads];
-    uint32_t idx[kThreads];
-    uint32_t srow[2 * kThreads];
-    uint32_t sent[2 * kThreads];
-    uint32_t src[kThreads], dst[kThreads];
-    uint8_t fg[kThreads], fh[kThreads];  // by batch position: delete from G / from H
+    static constexpr uint32_t kItems = Shape<IPT>::kItems;
+    typename Shape<IPT>::TempU tmp;
+    unsigned long long key[kItems];
+    uint32_t idx[kItems];
+    uint32_t srow[2 * kItems];
+    uint32_t sent[2 * kItems];
+    uint32_t src[kItems], dst[kItems];
+    uint8_t fg[kItems], fh[kItems];  // by batch position: delete from G / from H
 };
 
+template <int IPT>
 __global__ __launch_bounds__(kThreads) void k_small_del_pre(SmallDelArgs a) {
-    __shared__ DelPreShared sh;
+    constexpr int kIPT = IPT;
+    __shared__ DelPreShared<IPT> sh;
     const uint32_t t = threadIdx.x;
     const uint32_t n = a.n;
-    uint32_t ut = 0, vt = 0;
-    unsigned long long kt = kPad64;
-    if (t < n) {
-        ut = a.u[t];
-        vt = a.v[t];
-        kt = edge_key(ut, vt);
+    uint32_t ut[kIPT], vt[kIPT];
+    unsigned long long kt[kIPT];
+#pragma unroll
+    for (int k = 0; k < kIPT; ++k) {
+        const uint32_t i = kIPT * t + k;
+        ut[k] = i < n ? a.u[i] : 0u;
+        vt[k] = i < n ? a.v[i] : 0u;
+        kt[k] = i < n ? edge_key(ut[k], vt[k]) : kPad64;
+        sh.fg[i] = 0;
+        sh.fh[i] = 0;
     }
-    sh.fg[t] = 0;
-    sh.fh[t] = 0;
-    sort_keys64(sh.tmp, sh.key, sh.idx, kt, t, n, a.key_bits);
+    sort_keys64<IPT>(sh.tmp, sh.key, sh.idx, kt, t, n, a.key_bits);
     // ---- classify: the first occurrence of an edge G holds is deleted ----
-    const unsigned long long k = sh.key[t];
-    if (t < n && (t == 0 || sh.key[t - 1] != k)) {
-        uint32_t q = t + 1;
-        while (q < n && sh.key[q] == k) ++q;
-        bool in_g = false, in_h = false;
-        if (k != kPad64) {
-            const uint32_t lo = (uint32_t)(k >> 32), hi = (uint32_t)k;
-            in_g = holds(a.G.rows, a.G.adj, lo, hi);
-            in_h = in_g && holds(a.H.rows, a.H.adj, lo, hi);
+#pragma unroll
+    for (int k = 0; k < kIPT; ++k) {
+        const uint32_t p = kIPT * t + k;
+        const unsigned long long kp = sh.key[p];
+        if (p < n && (p == 0 || sh.key[p - 1] != kp)) {
+            uint32_t q = p + 1;
+            while (q < n && sh.key[q] == kp) ++q;
+            bool in_g = false, in_h = false;
+            if (kp != kPad64) {
+                const uint32_t lo = (uint32_t)(kp >> 32), hi = (uint32_t)kp;
+                in_g = holds(a.G.rows, a.G.adj, lo, hi);
+                in_h = in_g && holds(a.H.rows, a.H.adj, lo, hi);
+            }
+            for (uint32_t e = p; e < q; ++e) {
+                const uint32_t i = sh.idx[e];
+                const bool first = e == p && in_g;
+                a.cls[i] = kp == kPad64 ? kOutInvalid : !first ? kOutSkip : in_h ? 0 : kOutGOnly;
+                sh.fg[i] = first ? 1 : 0;
+                sh.fh[i] = first && in_h ? 1 : 0;
+            }
         }
-        for (uint32_t e = t; e < q; ++e) {
-            const uint32_t i = sh.idx[e];
-            const bool first = e == t && in_g;
-            a.cls[i] = k == kPad64 ? kOutInvalid : !first ? kOutSkip : in_h ? 0 : kOutGOnly;
-            sh.fg[i] = first ? 1 : 0;
-            sh.fh[i] = first && in_h ? 1 : 0;
-        }
+        sh.src[kIPT * t + k] = ut[k];
+        sh.dst[kIPT * t + k] = vt[k];
     }
-    sh.src[t] = ut;
-    sh.dst[t] = vt;
     __syncthreads();
-    const bool dg = t < n && sh.fg[t], dh = t < n && sh.fh[t];
-    block_delete(a.G, sh.tmp, sh.srow, sh.sent, sh.src, sh.dst, dg, t, n, a.row_bits,
+    bool dg[kIPT], dh[kIPT];
+#pragma unroll
+    for (int k = 0; k < kIPT; ++k) {
+        const uint32_t i = kIPT * t + k;
+        dg[k] = i < n && sh.fg[i];
+        dh[k] = i < n && sh.fh[i];
+    }
+    block_delete<IPT>(a.G, sh.tmp, sh.srow, sh.sent, sh.src, sh.dst, dg, t, n, a.row_bits,
                  a.touched_g, a.cnt + 4);
-    block_delete(a.H, sh.tmp, sh.srow, sh.sent, sh.src, sh.dst, dh, t, n, a.row_bits,
+    block_delete<IPT>(a.H, sh.tmp, sh.srow, sh.sent, sh.src, sh.dst, dh, t, n, a.row_bits,
                  a.touched_h, a.cnt + 5);
     // ---- compact the edges H held (batch order), as (min, max), for the walk ----
-    uint32_t pos = 0, total = 0;
-    Scan(sh.tmp.scan).ExclusiveSum(dh ? 1u : 0u, pos, total);
-    if (dh) {
-        a.cu[pos] = min(ut, vt);
-        a.cv[pos] = max(ut, vt);
-        a.ceid[pos] = a.base + t;
-        a.sel[pos] = t;
+    uint32_t in[kIPT], pos[kIPT], total = 0;
+#pragma unroll
+    for (int k = 0; k < kIPT; ++k) in[k] = dh[k] ? 1u : 0u;
+    Scan(sh.tmp.scan).ExclusiveSum(in, pos, total);
+#pragma unroll
+    for (int k = 0; k < kIPT; ++k) {
+        if (!dh[k]) continue;
+        const uint32_t i = kIPT * t + k;
+        a.cu[pos[k]] = min(ut[k], vt[k]);
+        a.cv[pos[k]] = max(ut[k], vt[k]);
+        a.ceid[pos[k]] = a.base + i;
+        a.sel[pos[k]] = i;
     }
     if (t == 0) a.cnt[0] = total;
 }
 
+template <int IPT>
 struct DelPostShared {
-    TempU tmp;
-    uint32_t np[kThreads];                    // no-path list: kept-edge index
-    uint32_t cu[kSmallCand], cv[kSmallCand];  // candidates, in the general list's order
-    float cw[kSmallCand];
-    uint8_t add[kSmallCand];                  // by candidate: append to H
-    unsigned long long key[kSmallCand];       // candidate keys, sorted
-    uint32_t idx[kSmallCand];
-    uint32_t srow[2 * kSmallCand];
-    uint32_t sent[2 * kSmallCand];
+    static constexpr uint32_t kItems = Shape<IPT>::kItems;
+    typename Shape<IPT>::TempU tmp;
+    uint32_t np[kItems];                  // no-path list: kept-edge index
+    uint32_t cu[kItems], cv[kItems];      // candidates, in the general list's order
+    float cw[kItems];
+    uint8_t add[kItems];                  // by candidate: append to H
+    unsigned long long key[kItems];       // candidate keys, sorted
+    uint32_t idx[kItems];
+    uint32_t srow[2 * kItems];
+    uint32_t sent[2 * kItems];
 };
 
+template <int IPT>
 __global__ __launch_bounds__(kThreads) void k_small_del_post(SmallDelArgs a) {
-    __shared__ DelPostShared sh;
+    constexpr int kIPT = IPT;
+    __shared__ DelPostShared<IPT> sh;
     const uint32_t t = threadIdx.x;
     const uint32_t m = a.cnt[0];
-    const int32_t f = t < m ? a.found[t] : 0;
-    const uint32_t pc = f > 0 ? (uint32_t)f : 0u;
-    const bool np = f == -1;
-    uint32_t poff = 0, hops = 0, k = 0, nnp = 0;
+    int32_t f[kIPT];
+    uint32_t pc[kIPT], npf[kIPT], poff[kIPT], kr[kIPT], hops = 0, nnp = 0;
+#pragma unroll
+    for (int k = 0; k < kIPT; ++k) {
+        const uint32_t j = kIPT * t + k;
+        f[k] = j < m ? a.found[j] : 0;
+        pc[k] = f[k] > 0 ? (uint32_t)f[k] : 0u;
+        npf[k] = f[k] == -1 ? 1u : 0u;
+    }
     Scan(sh.tmp.scan).ExclusiveSum(pc, poff, hops);
     __syncthreads();
-    Scan(sh.tmp.scan).ExclusiveSum(np ? 1u : 0u, k, nnp);
+    Scan(sh.tmp.scan).ExclusiveSum(npf, kr, nnp);
     const uint32_t hw = a.hw;
     const uint64_t n3 = 2ull * nnp * hw;  // heuristic walks: two endpoints per no-path edge
     const uint64_t ncand = hops + 2 * n3;
-    if (ncand > kSmallCand) {
+    if (ncand > Shape<IPT>::kItems) {
         // the general recovery takes over: its lists, as its kernels leave them
-        if (t < m) {
-            a.pc[t] = pc;
-            a.poff[t] = poff;
-        }
-        if (np) {
-            a.ep[2 * k] = a.cu[t];
-            a.ep[2 * k + 1] = a.cv[t];
-            a.ekey[2 * k] = a.ceid[t] << 1;
-            a.ekey[2 * k + 1] = (a.ceid[t] << 1) | 1ull;
+#pragma unroll
+        for (int k = 0; k < kIPT; ++k) {
+            const uint32_t j = kIPT * t + k;
+            if (j < m) {
+                a.pc[j] = pc[k];
+                a.poff[j] = poff[k];
+            }
+            if (npf[k]) {
+                const uint32_t q = kr[k];
+                a.ep[2 * q] = a.cu[j];
+                a.ep[2 * q + 1] = a.cv[j];
+                a.ekey[2 * q] = a.ceid[j] << 1;
+                a.ekey[2 * q + 1] = (a.ceid[j] << 1) | 1ull;
+            }
         }
         if (t == 0) {
             a.cnt[1] = nnp;
@@ -499,14 +627,18 @@ __global__ __launch_bounds__(kThreads) void k_small_del_post(SmallDelArgs a) {
         }
         return;
     }
-    if (np) sh.np[k] = t;
-    sh.add[t] = 0;
     // ---- candidate list: path hops (kept-edge order), then heuristic hops ----
-    if (f > 0) {
-        const uint32_t* pj = a.path + (uint64_t)t * a.max_steps;
-        for (int q = 1; q <= f; ++q) {
-            sh.cu[poff + q - 1] = pj[q - 1];
-            sh.cv[poff + q - 1] = pj[q];
+#pragma unroll
+    for (int k = 0; k < kIPT; ++k) {
+        const uint32_t j = kIPT * t + k;
+        if (npf[k]) sh.np[kr[k]] = j;
+        sh.add[j] = 0;
+        if (f[k] > 0) {
+            const uint32_t* pj = a.path + (uint64_t)j * a.max_steps;
+            for (int q = 1; q <= f[k]; ++q) {
+                sh.cu[poff[k] + q - 1] = pj[q - 1];
+                sh.cv[poff[k] + q - 1] = pj[q];
+            }
         }
     }
     __syncthreads();
@@ -524,25 +656,37 @@ __global__ __launch_bounds__(kThreads) void k_small_del_post(SmallDelArgs a) {
     __syncthreads();
     // ---- first occurrence of each candidate that H lacks, with G's weight ----
     const uint32_t nc = (uint32_t)ncand;
-    const unsigned long long kt = t < nc ? edge_key(sh.cu[t], sh.cv[t]) : kPad64;
-    sort_keys64(sh.tmp, sh.key, sh.idx, kt, t, nc, a.key_bits);
-    const unsigned long long ks = sh.key[t];
-    if (t < nc && ks != kPad64 && (t == 0 || sh.key[t - 1] != ks)) {
+    unsigned long long kt[kIPT];
+#pragma unroll
+    for (int k = 0; k < kIPT; ++k) {
+        const uint32_t c = kIPT * t + k;
+        kt[k] = c < nc ? edge_key(sh.cu[c], sh.cv[c]) : kPad64;
+    }
+    sort_keys64<IPT>(sh.tmp, sh.key, sh.idx, kt, t, nc, a.key_bits);
+#pragma unroll
+    for (int k = 0; k < kIPT; ++k) {
+        const uint32_t p = kIPT * t + k;
+        const unsigned long long ks = sh.key[p];
+        if (p >= nc || ks == kPad64 || (p > 0 && sh.key[p - 1] == ks)) continue;
         const uint32_t lo = (uint32_t)(ks >> 32), hi = (uint32_t)ks;
-        if (!holds(a.H.rows, a.H.adj, lo, hi)) {
-            const int s = find_in_row(a.G.rows, a.G.adj, lo, hi);
-            if (s >= 0) {  // always: path and heuristic hops are edges of G
-                const uint32_t c = sh.idx[t];
-                sh.add[c] = 1;
-                sh.cw[c] = __uint_as_float(a.G.adj[a.G.rows[lo].x + s].y);
-            }
-        }
+        if (holds(a.H.rows, a.H.adj, lo, hi)) continue;
+        const int s = find_in_row(a.G.rows, a.G.adj, lo, hi);
+        if (s < 0) continue;  // unreachable: path and heuristic hops are edges of G
+        const uint32_t c = sh.idx[p];
+        sh.add[c] = 1;
+        sh.cw[c] = __uint_as_float(a.G.adj[a.G.rows[lo].x + s].y);
     }
     __syncthreads();
-    const bool take = t < nc && sh.add[t];
-    block_append(a.H, sh.tmp, sh.srow, sh.sent, sh.cu, sh.cv, sh.cw, take, t, nc, a.row_bits,
+    bool take[kIPT];
+#pragma unroll
+    for (int k = 0; k < kIPT; ++k) take[k] = kIPT * t + k < nc && sh.add[kIPT * t + k];
+    block_append<IPT>(a.H, sh.tmp, sh.srow, sh.sent, sh.cu, sh.cv, sh.cw, take, t, nc, a.row_bits,
                  a.touched_x, a.cnt + 6);
-    if (t < m) a.cls[a.sel[t]] = np ? kOutHeuristic : kOutRecovered;
+#pragma unroll
+    for (int k = 0; k < kIPT; ++k) {
+        const uint32_t j = kIPT * t + k;
+        if (j < m) a.cls[a.sel[j]] = npf[k] ? kOutHeuristic : kOutRecovered;
+    }
     if (t == 0) {
         a.sizes[4] = 0;
         a.stats[3] = ncand;  // reported as the batch's candidate count
@@ -593,27 +737,44 @@ void refresh(const RefreshList& a, const RefreshList* b, uint32_t max_rows, hipS
 }  // namespace
 
 hipError_t launch_small_ins_pre(const SmallInsArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_small_ins_pre, dim3(1), dim3(kThreads), 0, s, a);
+    if (a.n <= (uint32_t)kThreads)
+        hipLaunchKernelGGL(k_small_ins_pre<1>, dim3(1), dim3(kThreads), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_small_ins_pre<2>, dim3(1), dim3(kThreads), 0, s, a);
     refresh({a.G.rows, a.G.adj, a.touched, a.cnt + 4, a.G.lpr}, nullptr, 2 * a.n, s);
     return hipGetLastError();
 }
 
 hipError_t launch_small_ins_post(const SmallInsArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_small_ins_post, dim3(1), dim3(kThreads), 0, s, a);
+    if (a.n <= (uint32_t)kThreads)
+        hipLaunchKernelGGL(k_small_ins_post<1>, dim3(1), dim3(kThreads), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_small_ins_post<2>, dim3(1), dim3(kThreads), 0, s, a);
     refresh({a.H.rows, a.H.adj, a.touched, a.cnt + 4, a.H.lpr}, nullptr, 2 * a.n, s);
     return hipGetLastError();
 }
 
 hipError_t launch_small_del_pre(const SmallDelArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_small_del_pre, dim3(1), dim3(kThreads), 0, s, a);
+    if (a.n <= (uint32_t)kThreads)
+        hipLaunchKernelGGL(k_small_del_pre<1>, dim3(1), dim3(kThreads), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_small_del_pre<2>, dim3(1), dim3(kThreads), 0, s, a);
     const RefreshList h{a.H.rows, a.H.adj, a.touched_h, a.cnt + 5, a.H.lpr};
     refresh({a.G.rows, a.G.adj, a.touched_g, a.cnt + 4, a.G.lpr}, &h, 2 * a.n, s);
     return hipGetLastError();
 }
 
+uint32_t small_del_candidates(uint32_t n) {
+    return n <= (uint32_t)kThreads ? (uint32_t)kThreads : 2u * kThreads;
+}
+
 hipError_t launch_small_del_post(const SmallDelArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_small_del_post, dim3(1), dim3(kThreads), 0, s, a);
-    refresh({a.H.rows, a.H.adj, a.touched_x, a.cnt + 6, a.H.lpr}, nullptr, 2 * kSmallCand, s);
+    if (a.n <= (uint32_t)kThreads)
+        hipLaunchKernelGGL(k_small_del_post<1>, dim3(1), dim3(kThreads), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_small_del_post<2>, dim3(1), dim3(kThreads), 0, s, a);
+    refresh({a.H.rows, a.H.adj, a.touched_x, a.cnt + 6, a.H.lpr}, nullptr,
+            2 * small_del_candidates(a.n), s);
     return hipGetLastError();
 }
 

@@ -139,9 +139,10 @@ def test_deletion_stream_equals_decremental_updater(ctx, mesh_wl):
 @pytest.mark.parametrize("n_del", [40, 800])
 def test_small_deletion_batch_recovery_sizes(ctx, grid_wl, n_del):
     """A deletion batch of H edges within the one-workgroup size: a short
-    candidate list is recovered inside the post kernel, a long one (more
-    than 2048 candidates) is handed to the general recovery; both match the
-    oracle, and so does the insertion batch after them."""
+    candidate list is recovered inside the post kernel, a long one (2804
+    candidates, more than that kernel holds) is handed to the general
+    recovery; both match the oracle, and so does the insertion batch after
+    them."""
     from dygrass_amd import Stream, stream_params
     from oracle import stream as S
 
@@ -161,5 +162,35 @@ def test_small_deletion_batch_recovery_sizes(ctx, grid_wl, n_del):
         assert np.array_equal(got, want), (op, np.flatnonzero(got != want)[:10])
         if op == "del":
             assert (stats["candidates"] > 2048) == (n_del == 800), stats["candidates"]
+        _assert_same_graph(st.G, orc.G)
+        _assert_same_graph(st.H, orc.H)
+
+
+def test_batches_up_to_the_one_workgroup_limit(ctx, grid_wl):
+    """Batches near the one-workgroup size (2048 updates, two per lane) and
+    tiny ones (ranked by counting) match the oracle: insertion batches of
+    1900 and 50, deletion batches of 2048 (G edges, repeats, non-edges) and 30."""
+    from dygrass_amd import Stream, stream_params
+    from oracle import stream as S
+
+    wl = grid_wl
+    rng = np.random.default_rng(7)
+    sp = stream_params(w_ins=64, w_del=64, threshold=10.0, seed=9, max_batch=4096,
+                       max_inserts=8192)
+    st = Stream(ctx, wl.G.V, (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w), sp)
+    orc = S.Stream(wl.G.V, (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w), thr=10.0,
+                   w_ins=64, w_del=64, seed=9)
+    gi = rng.choice(len(wl.G), 2000, replace=False)
+    du = np.concatenate([wl.G.u[gi], rng.integers(0, wl.G.V, 40), wl.G.u[gi[:8]]])
+    dv = np.concatenate([wl.G.v[gi], rng.integers(0, wl.G.V, 40), wl.G.v[gi[:8]]])
+    hi = rng.choice(len(wl.H), 30, replace=False)
+    batches = [("ins", wl.ins.u[:1900], wl.ins.v[:1900], wl.ins.w[:1900]),
+               ("del", du[:2048], dv[:2048], None),
+               ("ins", wl.ins.u[1900:1950], wl.ins.v[1900:1950], wl.ins.w[1900:1950]),
+               ("del", wl.H.u[hi], wl.H.v[hi], None)]
+    for op, u, v, w in batches:
+        got, _ = st.apply(op, u, v, w)
+        want = orc.apply(op, u, v, w)
+        assert np.array_equal(got, want), (op, len(u), np.flatnonzero(got != want)[:10])
         _assert_same_graph(st.G, orc.G)
         _assert_same_graph(st.H, orc.H)
