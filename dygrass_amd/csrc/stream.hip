@@ -21,14 +21,16 @@
 // edge is a weight update of it; a second deletion is skipped).  Self-loops
 // are rejected per update.  G and H must be simple graphs with H inside G.
 //
-// Every step runs on the device: the batch's keys are sorted (stable, 64-bit
+// Every step runs on the device.  A batch of up to kSmallBatch updates runs
+// as one-workgroup kernels around the walk (stream_small.hip).  A larger one
+// takes the general pipeline: the batch's keys are sorted (stable, 64-bit
 // (min,max) keys) so that one thread per distinct edge sees its occurrences
 // in batch order; membership and positions come from scanning the smaller
 // row; row edits use the append and swap-with-last pipelines of
 // device_edits.hip; the walks use the work-queue kernels with the list
-// length in device memory.  An insertion batch never waits for the host
-// before its end; a deletion batch reads three counts once (after its walks)
-// to size the recovery step.
+// length in device memory.  A batch's last kernel writes its outcome codes
+// and counters into pinned host memory; the host waits once per batch (a
+// deletion batch also reads its recovery list sizes once, after its walks).
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
