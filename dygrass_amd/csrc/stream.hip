@@ -995,15 +995,31 @@ extern "C" dyg_status dyg_stream_apply(dyg_stream s, int op, const uint32_t* u, 
         set_error(ctx, "stream apply: batch larger than max_batch");
         return DYG_ERR_ARG;
     }
-    for (uint32_t i = 0; i < n; ++i)
-        if (u[i] >= s->G->V || v[i] >= s->G->V) {
-            set_error(ctx, "stream apply: vertex id out of range");
-            return DYG_ERR_ARG;
-        }
     const double t0 = now_ms();
     dyg_stream_stats rs{};
     rs.updates = n;
-    if (n && (s->sp.flags & DYG_STREAM_PLAIN)) {
+    const bool plain = (s->sp.flags & DYG_STREAM_PLAIN) != 0;
+    if (n) {
+        // range check, fused with the copy into pinned staging (device form)
+        uint32_t mx = 0;
+        if (plain) {
+            for (uint32_t i = 0; i < n; ++i) mx = std::max(mx, std::max(u[i], v[i]));
+        } else {
+            const PinLayout hp = pin_layout((char*)s->pin, n);
+            for (uint32_t i = 0; i < n; ++i) {
+                const uint32_t a = u[i], b = v[i];
+                hp.u[i] = a;
+                hp.v[i] = b;
+                mx = std::max(mx, std::max(a, b));
+            }
+        }
+        if (mx >= s->G->V) {
+            set_error(ctx, "stream apply: vertex id out of range");
+            return DYG_ERR_ARG;
+        }
+    }
+    uint64_t counts[16] = {};
+    if (n && plain) {
         s->h_cls.assign(n, 0);
         dyg_status r = op == DYG_STREAM_INSERT ? plain_insert(s, u, v, w, n, s->h_cls.data(), &rs)
                                                : plain_delete(s, u, v, n, s->h_cls.data(), &rs);
@@ -1016,8 +1032,6 @@ extern "C" dyg_status dyg_stream_apply(dyg_stream s, int op, const uint32_t* u, 
         hipStream_t str = ctx->stream;
         const PinLayout hp = pin_layout((char*)s->pin, n), dp = pin_layout(s->pin_dev, n);
         const bool ins = op == DYG_STREAM_INSERT;
-        std::memcpy(hp.u, u, n * 4ull);
-        std::memcpy(hp.v, v, n * 4ull);
         if (ins) std::memcpy(hp.w, w, n * 4ull);
         const uint32_t *du = s->in, *dv = s->in + n;
         const float* dw = (const float*)(s->in + 2ull * n);
@@ -1040,7 +1054,12 @@ extern "C" dyg_status dyg_stream_apply(dyg_stream s, int op, const uint32_t* u, 
         r = device_edits_absorb(s->G, hp.words + 4, nullptr);
         if (r == DYG_OK) r = device_edits_absorb(s->H, hp.words + 9, nullptr);
         if (r != DYG_OK) return r;
-        s->h_cls.assign(hp.cls, hp.cls + n);
+        // outcome codes to the caller and their counts, in one pass
+        for (uint32_t i = 0; i < n; ++i) {
+            const int32_t c = hp.cls[i];
+            if (outcome) outcome[i] = c;
+            counts[c & 15]++;
+        }
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess) rs.walk_ms = ms;
         rs.hops = hst[0];
@@ -1050,21 +1069,19 @@ extern "C" dyg_status dyg_stream_apply(dyg_stream s, int op, const uint32_t* u, 
         if (r == DYG_OK) r = device_edits_prepare(s->H, s->cmax, s->cmax);
         if (r != DYG_OK) return r;
     }
-    if (n) {
-        for (int32_t c : s->h_cls) {
-            switch (c) {
-                case kOutWeight: rs.weight_updates++; break;
-                case kOutFound: rs.found++; break;
-                case kOutAdded: rs.added++; break;
-                case kOutSkip: rs.skipped++; break;
-                case kOutGOnly: rs.g_only++; break;
-                case kOutRecovered: rs.recovered++; break;
-                case kOutHeuristic: rs.heuristic++; break;
-                default: rs.invalid++; break;
-            }
-        }
+    if (n && plain) {
+        for (int32_t c : s->h_cls) counts[c & 15]++;
         if (outcome) std::memcpy(outcome, s->h_cls.data(), n * 4ull);
     }
+    rs.weight_updates = counts[kOutWeight];
+    rs.found = counts[kOutFound];
+    rs.added = counts[kOutAdded];
+    rs.skipped = counts[kOutSkip];
+    rs.g_only = counts[kOutGOnly];
+    rs.recovered = counts[kOutRecovered];
+    rs.heuristic = counts[kOutHeuristic];
+    rs.invalid = n - (rs.weight_updates + rs.found + rs.added + rs.skipped + rs.g_only +
+                      rs.recovered + rs.heuristic);
     s->base += n;
     rs.total_ms = now_ms() - t0;
     if (st) *st = rs;
@@ -1337,12 +1354,12 @@ extern "C" dyg_status dyg_serve_submit(dyg_serve v, int op, const uint32_t* u, c
         set_error(ctx, "serve submit: request size must be in [1, max_request]");
         return DYG_ERR_ARG;
     }
-    const uint32_t V = v->s->G->V;
-    for (uint32_t i = 0; i < n; ++i)
-        if (u[i] >= V || w_v[i] >= V) {
-            set_error(ctx, "serve submit: vertex id out of range");
-            return DYG_ERR_ARG;
-        }
+    uint32_t mx = 0;
+    for (uint32_t i = 0; i < n; ++i) mx = std::max(mx, std::max(u[i], w_v[i]));
+    if (mx >= v->s->G->V) {
+        set_error(ctx, "serve submit: vertex id out of range");
+        return DYG_ERR_ARG;
+    }
     const uint64_t need = request_need(v, op, n);
     auto* r = new ServeReq;
     r->op = op;
