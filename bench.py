@@ -205,9 +205,11 @@ def run_stream(ctx, wl, plain: bool, seed: int, block_edges: int = 0) -> dict:
     from dygrass_amd import Stream, stream_params
 
     maxb = max(len(b[1]) for b in wl.batches)
+    # G keeps no neighbour descriptors: on this 12-NN graph refreshing them
+    # after insertions costs more than the longer deletion hops
     sp = stream_params(w_ins=wl.w_ins, w_del=wl.w_del, threshold=wl.threshold, seed=seed,
                        max_batch=maxb, max_inserts=wl.updates, plain=plain,
-                       block_edges=block_edges)
+                       block_edges=block_edges, dense_no_desc=not plain)
     st = Stream(ctx, wl.G.V, (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w), sp)
     ctx.sync()
     outs, lat, walk_ms, hops, agg = [], [], 0.0, 0, {}
@@ -252,7 +254,8 @@ def leg_batched(args, ctx) -> dict:
         "workload": f"{wl.name}: {len(wl.batches)} batches of log-uniform size 1..50000 "
                     f"({m} updates, 70% insertion batches), threshold {wl.threshold:g}, "
                     f"{wl.w_ins} insertion / {wl.w_del} deletion walkers, 100 steps "
-                    "(device-resident stream: work-queue walk kernels, device edits)",
+                    "(device-resident stream: work-queue walk kernels, device edits, dense graph "
+                    "without neighbour descriptors)",
         "V": int(wl.G.V), "E_G": int(len(wl.G)), "E_H": int(len(wl.H)), "updates": int(m),
         "batches": len(wl.batches), "walk_kernel_ms_per_step": head["walk_ms"],
         "walk_steps_per_s": head["hops"] / head["seconds"],
@@ -416,7 +419,7 @@ def leg_decremental(args, ctx) -> dict:
     log(f"decremental: {wl.name} V={G.V} E_G={len(G)} E_H={len(H)} deletions={M} "
         f"({time.time() - t0:.1f}s)")
     sp = stream_params(w_ins=512, w_del=512, threshold=0.0, seed=args.seed, max_batch=I,
-                       max_inserts=4 * M)
+                       max_inserts=4 * M, dense_no_desc=True)
     times, counts = [], None
     for i in range(args.warmup + args.steps):
         st = Stream(ctx, G.V, (G.u, G.v, G.w), (H.u, H.v, H.w), sp)
@@ -440,7 +443,8 @@ def leg_decremental(args, ctx) -> dict:
     total = float(np.sum(times))
     out = {"value": M * len(times) / total, "ms_per_step": total / len(times) * 1e3,
            "workload": f"{wl.name}: {M} deletions in 10 snapshot batches, 512 walkers, 100 "
-                       "steps (deletion-only stream, G and H on the device)",
+                       "steps (deletion-only stream, G and H on the device, G without "
+                       "neighbour descriptors)",
            "V": int(G.V), "E_G": int(len(G)), "E_H": int(len(H)), "deletions": M,
            "outcomes": counts, "H_after": len(h_dev)}
     if not args.no_plain:

@@ -75,6 +75,7 @@ class MtxInfo(C.Structure):
 DYG_STREAM_INSERT, DYG_STREAM_DELETE = 0, 1
 DYG_STREAM_PLAIN = 1
 DYG_STREAM_GENERAL_ONLY = 2
+DYG_STREAM_DENSE_NO_DESC = 4
 OUTCOMES = {1: "weight", 2: "found", 3: "added", 4: "skip", 5: "g_only", 6: "recovered",
             7: "heuristic", 8: "invalid"}
 
@@ -137,15 +138,17 @@ _SIGS = {
     "dyg_dual_free": (None, [vp]),
     "dyg_stream_create": (C.c_int32, [vp, C.c_uint32, u64p, u32p, f32p, u64p, u32p, f32p,
                                       C.POINTER(StreamParams), C.POINTER(vp)]),
-    "dyg_stream_apply": (C.c_int32, [vp, C.c_int, u32p, u32p, f32p, C.c_uint32, i32p,
+    # the per-batch calls take raw addresses (addr()): a typed ctypes pointer
+    # costs microseconds to build, which a batch of a few updates notices
+    "dyg_stream_apply": (C.c_int32, [vp, C.c_int, vp, vp, vp, C.c_uint32, vp,
                                      C.POINTER(StreamStats)]),
     "dyg_stream_graph": (vp, [vp, C.c_int]),
     "dyg_stream_position": (C.c_uint64, [vp]),
     "dyg_stream_free": (None, [vp]),
     "dyg_serve_open": (C.c_int32, [vp, C.c_uint32, C.c_uint32, C.c_uint64, C.POINTER(vp)]),
-    "dyg_serve_submit": (C.c_int32, [vp, C.c_int, u32p, u32p, f32p, C.c_uint32,
+    "dyg_serve_submit": (C.c_int32, [vp, C.c_int, vp, vp, vp, C.c_uint32,
                                      C.POINTER(C.c_uint64)]),
-    "dyg_serve_wait": (C.c_int32, [vp, C.c_uint64, i32p, C.POINTER(ServeResult)]),
+    "dyg_serve_wait": (C.c_int32, [vp, C.c_uint64, vp, C.POINTER(ServeResult)]),
     "dyg_serve_close": (None, [vp]),
     "dyg_mtx_read_graph": (C.c_int32, [C.c_char_p, C.c_int, C.c_int, C.POINTER(vp)]),
     "dyg_mtx_read_edges": (C.c_int32, [C.c_char_p, C.c_int, C.POINTER(vp)]),
@@ -223,6 +226,14 @@ def ptr(a: np.ndarray | None, ct):
         return None
     assert a.flags["C_CONTIGUOUS"], "array must be contiguous"
     return a.ctypes.data_as(ct)
+
+
+def addr(a: np.ndarray | None):
+    """Address of a contiguous numpy array, for void-pointer arguments (None -> NULL)."""
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "array must be contiguous"
+    return a.ctypes.data
 
 
 def check(status: int, ctx=None) -> None:

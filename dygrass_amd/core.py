@@ -380,17 +380,21 @@ class DualGraph:
 def stream_params(w_ins: int = 256, w_del: int = 512, threshold: float = 100.0, seed: int = 0,
                   max_batch: int = 50000, max_inserts: int = 0, heur_walkers: int = 16,
                   slack: int = 2, max_steps: int = 100, plain: bool = False,
-                  block_edges: int = 0, general_only: bool = False) -> L.StreamParams:
+                  block_edges: int = 0, general_only: bool = False,
+                  dense_no_desc: bool = False) -> L.StreamParams:
     """Stream parameters; plain=True selects the reference-shaped form (host
     edge maps and edit plans, one workgroup per edge, a wait per step);
-    general_only=True keeps small insertion batches on the general pipeline."""
+    general_only=True keeps small batches on the general pipeline;
+    dense_no_desc=True keeps no neighbour descriptors in G (no refresh after
+    its edits, deletion walks read the row table per hop)."""
     sp = L.StreamParams()
     sp.ins = params(w_ins, max_steps, threshold, seed, block_edges=block_edges)
     sp.del_ = params(w_del, max_steps, 0.0, seed, block_edges=block_edges)
     sp.heur_walkers, sp.max_batch, sp.max_inserts = heur_walkers, max_batch, max_inserts
     sp.slack = slack
     sp.flags = ((L.DYG_STREAM_PLAIN if plain else 0)
-                | (L.DYG_STREAM_GENERAL_ONLY if general_only else 0))
+                | (L.DYG_STREAM_GENERAL_ONLY if general_only else 0)
+                | (L.DYG_STREAM_DENSE_NO_DESC if dense_no_desc else 0))
     return sp
 
 
@@ -421,11 +425,11 @@ class Stream:
         u, v = _u32(u), _u32(v)
         n = len(u)
         ww = _f32(w) if w is not None else None
-        out = np.zeros(max(n, 1), np.int32)
+        out = np.empty(max(n, 1), np.int32)
         st = L.StreamStats()
         code = L.DYG_STREAM_INSERT if op == "ins" else L.DYG_STREAM_DELETE
-        check(self._lib.dyg_stream_apply(self.handle, code, ptr(u, L.u32p), ptr(v, L.u32p),
-                                         ptr(ww, L.f32p), n, ptr(out, L.i32p), C.byref(st)),
+        check(self._lib.dyg_stream_apply(self.handle, code, L.addr(u), L.addr(v), L.addr(ww), n,
+                                         L.addr(out), C.byref(st)),
               self.ctx.handle)
         return out[:n], {k: getattr(st, k) for k, _ in L.StreamStats._fields_}
 
@@ -461,15 +465,15 @@ class Server:
         ww = _f32(w) if w is not None else None
         t = C.c_uint64()
         code = L.DYG_STREAM_INSERT if op == "ins" else L.DYG_STREAM_DELETE
-        check(self._lib.dyg_serve_submit(self.handle, code, ptr(u, L.u32p), ptr(v, L.u32p),
-                                         ptr(ww, L.f32p), len(u), C.byref(t)),
+        check(self._lib.dyg_serve_submit(self.handle, code, L.addr(u), L.addr(v), L.addr(ww),
+                                         len(u), C.byref(t)),
               self.stream.ctx.handle)
         return t.value
 
     def wait(self, ticket: int, n: int) -> tuple[np.ndarray, dict]:
         out = np.zeros(max(n, 1), np.int32)
         res = L.ServeResult()
-        check(self._lib.dyg_serve_wait(self.handle, ticket, ptr(out, L.i32p), C.byref(res)),
+        check(self._lib.dyg_serve_wait(self.handle, ticket, L.addr(out), C.byref(res)),
               self.stream.ctx.handle)
         return out[:n], {k: getattr(res, k) for k, _ in L.ServeResult._fields_}
 

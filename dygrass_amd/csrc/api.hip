@@ -96,6 +96,7 @@ dyg_status walk_host(dyg_graph g, bool insert, const uint32_t* src, const uint32
     DYG_HIP(ctx, hipMemsetAsync(d_stats, 0, 3 * sizeof(unsigned long long), s));
     WalkLaunch a{};
     a.rows = g->d_rows;
+    a.nodesc = g->nodesc;
     a.adj = g->d_adj;
     a.src = d_src;
     a.dst = d_dst;
@@ -264,6 +265,7 @@ extern "C" dyg_status dyg_walk_incremental_dev(dyg_graph H, const uint32_t* src,
     if (cs != DYG_OK) return cs;
     WalkLaunch a{};
     a.rows = H->d_rows;
+    a.nodesc = H->nodesc;
     a.adj = H->d_adj;
     a.src = src;
     a.dst = dst;
@@ -296,6 +298,7 @@ extern "C" dyg_status dyg_walk_decremental_dev(dyg_graph G, const uint32_t* src,
     if (cs != DYG_OK) return cs;
     WalkLaunch a{};
     a.rows = G->d_rows;
+    a.nodesc = G->nodesc;
     a.adj = G->d_adj;
     a.src = src;
     a.dst = dst;
@@ -337,7 +340,8 @@ extern "C" dyg_status dyg_walk_heuristic(dyg_graph G, const uint32_t* endpoint,
     hipStream_t s = ctx->stream;
     DYG_HIP(ctx, hipMemcpyAsync(d_ep, endpoint, n * 4ull, hipMemcpyHostToDevice, s));
     DYG_HIP(ctx, hipMemcpyAsync(d_key, key, n * 8ull, hipMemcpyHostToDevice, s));
-    DYG_HIP(ctx, launch_heuristic(G->d_rows, G->d_adj, d_ep, d_key, n, walkers, seed, d_out, s));
+    DYG_HIP(ctx, launch_heuristic(G->d_rows, G->d_adj, d_ep, d_key, n, walkers, seed, d_out, s,
+                                  nullptr, G->nodesc));
     DYG_HIP(ctx, hipMemcpyAsync(path3, d_out, out * 4, hipMemcpyDeviceToHost, s));
     DYG_HIP(ctx, hipStreamSynchronize(s));
     return DYG_OK;

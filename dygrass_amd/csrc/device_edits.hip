@@ -114,6 +114,7 @@ __global__ void k_write(uint4* __restrict__ adj, const uint32_t* __restrict__ ke
     const uint32_t r = keys[p];
     const uint32_t base = rowbase[r];
     const uint32_t st = segstart[r];
+    if (st == p) segcnt[r] = 0;  // read by k_alloc only: ready for the next batch
     if (base != 0xFFFFFFFFu) {
         const uint32_t o = vals[p];
         const uint32_t k = o >> 1;
@@ -130,13 +131,14 @@ __global__ void k_del_rows(uint2* __restrict__ rows, uint4* __restrict__ adj,
                            const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
                            const uint32_t* __restrict__ count,
                            const uint32_t* __restrict__ segstart,
-                           const uint32_t* __restrict__ segcnt, const uint32_t* __restrict__ src,
+                           uint32_t* __restrict__ segcnt, const uint32_t* __restrict__ src,
                            const uint32_t* __restrict__ dst, uint32_t* __restrict__ err) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= *count) return;
     const uint32_t r = keys[p];
     if (segstart[r] != p) return;
     const uint32_t k = segcnt[r];
+    segcnt[r] = 0;  // ready for the next batch
     uint2 row = rows[r];
     for (uint32_t e = p; e < p + k; ++e) {
         const uint32_t o = vals[e];
@@ -160,7 +162,8 @@ __global__ void k_del_rows(uint2* __restrict__ rows, uint4* __restrict__ adj,
 // neighbour's row, not all of them; LPR near the graph's mean degree keeps
 // the lanes busy).  Each such slot is written only by r's group and only its
 // upper half (.z,.w), so groups never race; the neighbour ids and the row
-// descriptors read here are final once k_write has run.
+// descriptors read here are final once k_write has run.  (segcnt was
+// cleared by k_write / k_del_rows.)
 template <uint32_t LPR>
 __global__ __launch_bounds__(256) void k_refresh(
     const uint2* __restrict__ rows, uint4* __restrict__ adj, const uint32_t* __restrict__ keys,
@@ -176,11 +179,11 @@ __global__ __launch_bounds__(256) void k_refresh(
         const uint32_t x = adj[rr.x + q].x;
         refresh_pointers(adj, rows[x], r, rr);
     }
-    if (lane == 0) segcnt[r] = 0;  // ready for the next batch
 }
 
 void launch_refresh(dyg_graph g, const uint32_t* keys, const uint32_t* count, uint32_t entries,
                     hipStream_t s) {
+    if (g->nodesc) return;  // the graph keeps no descriptors
     DevEdits& d = g->dev;
     const uint32_t lpr = refresh_lanes(g);
     const uint32_t blocks = (uint32_t)(((uint64_t)entries * lpr + 255) / 256);

@@ -503,7 +503,8 @@ extern "C" dyg_status dyg_graph_download(dyg_graph g, uint64_t* row_ptr, uint32_
 
 extern "C" dyg_status dyg_graph_check(dyg_graph g, uint64_t* bad_slots) {
     // Every live slot's copy of its neighbour's row descriptor equals the
-    // neighbour's row (device state).
+    // neighbour's row (device state); a graph that keeps no descriptors only
+    // needs valid neighbour ids.
     if (!g) return DYG_ERR_ARG;
     dyg_ctx ctx = g->ctx;
     std::vector<uint2> rows(g->V);
@@ -517,7 +518,8 @@ extern "C" dyg_status dyg_graph_check(dyg_graph g, uint64_t* bad_slots) {
     for (uint32_t v = 0; v < g->V; ++v)
         for (uint32_t k = 0; k < rows[v].y; ++k) {
             const uint4 sl = adj[rows[v].x + k];
-            if (sl.x >= g->V || sl.z != rows[sl.x].x || sl.w != rows[sl.x].y) ++bad;
+            if (sl.x >= g->V || (!g->nodesc && (sl.z != rows[sl.x].x || sl.w != rows[sl.x].y)))
+                ++bad;
         }
     if (bad_slots) *bad_slots = bad;
     return DYG_OK;

@@ -59,6 +59,7 @@ dyg_status run_incremental(dyg_graph H, const RunBufs& b, const uint32_t* h_src,
     auto launch = [&](uint32_t b0, uint32_t n) -> hipError_t {
         WalkLaunch a{};
         a.rows = H->d_rows;
+        a.nodesc = H->nodesc;
         a.adj = H->d_adj;
         a.src = b.src + b0;
         a.dst = b.dst + b0;

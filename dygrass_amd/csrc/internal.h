@@ -93,6 +93,9 @@ struct dyg_graph_s {
     std::vector<uint32_t> start, deg, cap;
     std::vector<uint4> adj;
     uint64_t nslots = 0;  // sum of degrees
+    // slots do not keep their neighbour's descriptor current (the stream's
+    // dense graph: no refresh after its edits; walks on it read rows[] per hop)
+    bool nodesc = false;
     // reference edge map (random_walk_decremental/functions.cpp:243-258)
     bool keyed = false;
     long mult = 1;
@@ -171,12 +174,14 @@ struct WalkLaunch {
     uint32_t* path;          // deletions: [n*max_steps]
     unsigned long long* stats;  // optional [3]: hops, redirects, leaf stops
     const uint32_t* n_dev;   // optional: list length in device memory (<= n; grid sized by n)
+    bool nodesc;             // the graph's slots carry no neighbour descriptors
 };
 hipError_t launch_walk_incremental(const WalkLaunch& a, hipStream_t s);
 hipError_t launch_walk_decremental(const WalkLaunch& a, hipStream_t s);
 hipError_t launch_heuristic(const uint2* rows, const uint4* adj, const uint32_t* endpoint,
                             const uint64_t* key, uint32_t n, uint32_t walkers, uint64_t seed,
-                            uint32_t* path3, hipStream_t s, const uint32_t* n_dev = nullptr);
+                            uint32_t* path3, hipStream_t s, const uint32_t* n_dev = nullptr,
+                            bool nodesc = false);
 // Delta layout (uint32 words): writes (slot, col, wbits, nstart, ndeg), then
 // rows (v, start, deg).
 hipError_t launch_apply_delta(uint2* rows, uint4* adj, const uint32_t* delta, uint32_t n_write,
@@ -239,6 +244,7 @@ struct BlockGraph {
     unsigned long long cap_slots;
     uint32_t slack;
     uint32_t lpr;  // lanes per row of its descriptor refresh (refresh_lanes)
+    bool nodesc;   // no descriptors kept: no refresh (dyg_graph_s::nodesc)
 };
 // A batch's results, written by its last kernel straight into pinned host
 // memory (device-visible addresses): the outcome codes and 14 counter words,

@@ -46,10 +46,12 @@ __device__ __forceinline__ uint32_t stream_c3(uint32_t tag, uint64_t id) {
 // Heuristic two-hop walk from u (random_walk_decremental/main.cu:243-290):
 // first hop uniform over u's row, second hop over n1's row, non-backtracking
 // (a draw of u is redirected to that row's last slot).  Returns (u, n1, n2);
-// (u, u, u) for an isolated u.  Uses block 0 of the walker's stream.
+// (u, u, u) for an isolated u.  Uses block 0 of the walker's stream.  nodesc:
+// n1's row comes from rows[] (the graph's slots carry no descriptors).
 __device__ __forceinline__ uint3 heuristic_walk(const uint2* __restrict__ rows,
                                                 const uint4* __restrict__ adj, uint32_t u,
-                                                uint64_t key, uint32_t walker, uint64_t seed) {
+                                                uint64_t key, uint32_t walker, uint64_t seed,
+                                                bool nodesc) {
     const uint4 d = philox_block(0u, walker, (uint32_t)key, stream_c3(kTagHeuristic, key),
                                  (uint32_t)seed, (uint32_t)(seed >> 32));
     uint32_t n1 = u, n2 = u;
@@ -57,7 +59,7 @@ __device__ __forceinline__ uint3 heuristic_walk(const uint2* __restrict__ rows,
     if (ru.y != 0) {
         const uint4 s1 = adj[ru.x + d.x % ru.y];
         n1 = s1.x;
-        const uint2 r1 = make_uint2(s1.z, s1.w);  // r1.y >= 1: n1 is a neighbour of u
+        const uint2 r1 = nodesc ? rows[n1] : make_uint2(s1.z, s1.w);  // r1.y >= 1
         n2 = adj[r1.x + d.y % r1.y].x;
         if (n2 == u) n2 = adj[r1.x + r1.y - 1].x;
     }

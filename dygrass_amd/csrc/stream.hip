@@ -582,6 +582,7 @@ WalkLaunch walk_args(dyg_stream s, dyg_graph g, const dyg_walk_params& p, uint32
     a.found = s->cfound;
     a.cond = s->ccond;
     a.stats = s->stats;
+    a.nodesc = g->nodesc;
     return a;
 }
 
@@ -593,7 +594,7 @@ WalkLaunch walk_args(dyg_stream s, dyg_graph g, const dyg_walk_params& p, uint32
 
 BlockGraph block_graph(dyg_graph g) {
     return BlockGraph{g->d_rows, g->d_adj, g->dev.cap, g->dev.meta,
-                      (unsigned long long)g->cap_slots, g->slack, refresh_lanes(g)};
+                      (unsigned long long)g->cap_slots, g->slack, refresh_lanes(g), g->nodesc};
 }
 
 // Small insertion batch: two one-workgroup kernels around the walk.
@@ -748,7 +749,7 @@ dyg_status apply_delete_small(dyg_stream s, const uint32_t* du, const uint32_t* 
     const uint32_t* hc = (const uint32_t*)((char*)s->pin + kPinSizes);
     if (hc[4] == 0) return DYG_OK;  // the post kernel finished the batch
     DYG_HIP(s->ctx, launch_heuristic(s->G->d_rows, s->G->d_adj, s->ep, s->ekey, 2 * n, hw,
-                                     s->sp.del.seed, s->path3, st, s->cnt + 2));
+                                     s->sp.del.seed, s->path3, st, s->cnt + 2, s->G->nodesc));
     LAUNCH_1D(k_cand_paths, n, s->cfound, s->cnt, s->path, ms, s->poff, s->xu, s->xv);
     DYG_HIP(s->ctx, hipGetLastError());
     return recover_general(s, n, hc, dcls, dstats, out, rep);
@@ -786,7 +787,7 @@ dyg_status apply_delete(dyg_stream s, const uint32_t* du, const uint32_t* dv, ui
               s->ep, s->ekey, s->cnt, (uint32_t*)(s->pin_dev + kPinSizes));
     const uint32_t* hc = (const uint32_t*)((char*)s->pin + kPinSizes);
     DYG_HIP(s->ctx, launch_heuristic(G->d_rows, G->d_adj, s->ep, s->ekey, 2 * n, hw,
-                                     s->sp.del.seed, s->path3, st, s->cnt + 2));
+                                     s->sp.del.seed, s->path3, st, s->cnt + 2, G->nodesc));
     LAUNCH_1D(k_cand_paths, n, s->cfound, s->cnt, s->path, ms, s->poff, s->xu, s->xv);
     DYG_HIP(s->ctx, hipGetLastError());
     // the one wait of a deletion batch: recovery list sizes
@@ -969,6 +970,7 @@ extern "C" dyg_status dyg_stream_create(dyg_ctx ctx, uint32_t V, const uint64_t*
     r = dyg_graph_upload(ctx, V, g_row_ptr, g_col, g_w, sp->slack, plain, &s->G);
     if (r == DYG_OK) r = dyg_graph_upload(ctx, V, h_row_ptr, h_col, h_w, sp->slack, plain, &s->H);
     if (!plain) {
+        if (r == DYG_OK && (sp->flags & DYG_STREAM_DENSE_NO_DESC)) s->G->nodesc = true;
         if (r == DYG_OK) r = stream_alloc(s);
         if (r == DYG_OK) r = device_edits_begin(s->G, sp->max_batch, sp->max_inserts);
         if (r == DYG_OK) r = device_edits_begin(s->H, s->cmax, sp->max_inserts);

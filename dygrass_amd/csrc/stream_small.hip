@@ -647,7 +647,7 @@ __global__ __launch_bounds__(kThreads) void k_small_del_post(SmallDelArgs a) {
         const uint32_t e = sh.np[i >> 1];
         const uint32_t u = (i & 1u) ? a.cv[e] : a.cu[e];
         const uint64_t key = (a.ceid[e] << 1) | (uint64_t)(i & 1u);
-        const uint3 w3 = heuristic_walk(a.G.rows, a.G.adj, u, key, jw, a.seed);
+        const uint3 w3 = heuristic_walk(a.G.rows, a.G.adj, u, key, jw, a.seed, a.G.nodesc);
         sh.cu[hops + 2 * g] = w3.x;
         sh.cv[hops + 2 * g] = w3.y;
         sh.cu[hops + 2 * g + 1] = w3.y;
@@ -741,7 +741,8 @@ hipError_t launch_small_ins_pre(const SmallInsArgs& a, hipStream_t s) {
         hipLaunchKernelGGL(k_small_ins_pre<1>, dim3(1), dim3(kThreads), 0, s, a);
     else
         hipLaunchKernelGGL(k_small_ins_pre<2>, dim3(1), dim3(kThreads), 0, s, a);
-    refresh({a.G.rows, a.G.adj, a.touched, a.cnt + 4, a.G.lpr}, nullptr, 2 * a.n, s);
+    if (!a.G.nodesc)
+        refresh({a.G.rows, a.G.adj, a.touched, a.cnt + 4, a.G.lpr}, nullptr, 2 * a.n, s);
     return hipGetLastError();
 }
 
@@ -760,7 +761,10 @@ hipError_t launch_small_del_pre(const SmallDelArgs& a, hipStream_t s) {
     else
         hipLaunchKernelGGL(k_small_del_pre<2>, dim3(1), dim3(kThreads), 0, s, a);
     const RefreshList h{a.H.rows, a.H.adj, a.touched_h, a.cnt + 5, a.H.lpr};
-    refresh({a.G.rows, a.G.adj, a.touched_g, a.cnt + 4, a.G.lpr}, &h, 2 * a.n, s);
+    if (a.G.nodesc)
+        refresh(h, nullptr, 2 * a.n, s);
+    else
+        refresh({a.G.rows, a.G.adj, a.touched_g, a.cnt + 4, a.G.lpr}, &h, 2 * a.n, s);
     return hipGetLastError();
 }
 

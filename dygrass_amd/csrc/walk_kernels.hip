@@ -73,7 +73,7 @@ __device__ __forceinline__ uint4 ld(const uint4* __restrict__ p, uint32_t i) { r
 // PRUNE: stop once fl(1/R) is strictly below the best conductance already
 // found by a walker of the same edge; R never decreases, so such a walker can
 // no longer win and the edge's winner is unchanged.
-template <bool INSERT, bool RECORD, bool PRUNE>
+template <bool INSERT, bool RECORD, bool PRUNE, bool DESC = true>
 __device__ __forceinline__ void walk_lane(const uint2* __restrict__ rows,
                                           const uint4* __restrict__ adj, uint32_t s, uint32_t t,
                                           float we, float thr, int max_steps, Draws& rng,
@@ -99,7 +99,7 @@ __device__ __forceinline__ void walk_lane(const uint2* __restrict__ rows,
         return;
     }
     uint32_t prev = s, cur = sl.x;
-    row = make_uint2(sl.z, sl.w);
+    row = DESC ? make_uint2(sl.z, sl.w) : ld(rows, sl.x);
     int step = 2;
     while (step < max_steps) {
         if (row.y <= 1) {  // leaf (degree 0 cannot occur: deletions keep degree >= 1)
@@ -127,7 +127,7 @@ __device__ __forceinline__ void walk_lane(const uint2* __restrict__ rows,
         }
         prev = cur;
         cur = sl.x;
-        row = make_uint2(sl.z, sl.w);
+        row = DESC ? make_uint2(sl.z, sl.w) : ld(rows, sl.x);
         ++step;
     }
     o.R = R;
@@ -315,7 +315,9 @@ struct Ring {
     }
 };
 
-template <int LOGW, bool INSERT, bool PRUNE>
+// DESC false: the graph's slots carry no neighbour descriptors, so a hop
+// reads the next row from rows[] (a second dependent load).
+template <int LOGW, bool INSERT, bool PRUNE, bool DESC = true>
 __global__ __launch_bounds__(kPoolThreads) void k_walk_pool(WalkLaunch a, uint32_t E) {
     constexpr uint32_t W = 1u << LOGW;
     constexpr uint32_t kE = kPoolMaxEdges;
@@ -458,7 +460,7 @@ __global__ __launch_bounds__(kPoolThreads) void k_walk_pool(WalkLaunch a, uint32
                 }
                 L.prev = L.cur;
                 L.cur = sl.x;
-                L.row = make_uint2(sl.z, sl.w);
+                L.row = DESC ? make_uint2(sl.z, sl.w) : ld(a.rows, sl.x);
                 ++L.step;
                 if (L.step >= (int)a.max_steps) done = true;
             }
@@ -497,7 +499,7 @@ __global__ __launch_bounds__(kPoolThreads) void k_walk_pool(WalkLaunch a, uint32
         if (!INSERT && f > 0) {
             Draws rr(a.seed, kTagDelete, s_eid[tid], win);
             WalkOut ro;
-            walk_lane<false, true, false>(a.rows, a.adj, s_src[tid], s_dst[tid], 0.0f, 0.0f,
+            walk_lane<false, true, false, DESC>(a.rows, a.adj, s_src[tid], s_dst[tid], 0.0f, 0.0f,
                                           (int)a.max_steps, rr, nullptr,
                                           a.path + (uint64_t)e * a.max_steps, ro);
         }
@@ -515,12 +517,13 @@ __global__ __launch_bounds__(256) void k_heuristic(const uint2* __restrict__ row
                                                    const uint64_t* __restrict__ key, uint32_t n,
                                                    uint32_t walkers, uint64_t seed,
                                                    uint32_t* __restrict__ path3,
-                                                   const uint32_t* __restrict__ n_dev) {
+                                                   const uint32_t* __restrict__ n_dev,
+                                                   bool nodesc) {
     const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (n_dev && *n_dev < n) n = *n_dev;
     if (gid >= (uint64_t)n * walkers) return;
     const uint32_t i = (uint32_t)(gid / walkers), j = (uint32_t)(gid % walkers);
-    const uint3 t = heuristic_walk(rows, adj, endpoint[i], key[i], j, seed);
+    const uint3 t = heuristic_walk(rows, adj, endpoint[i], key[i], j, seed, nodesc);
     path3[gid * 3 + 0] = t.x;
     path3[gid * 3 + 1] = t.y;
     path3[gid * 3 + 2] = t.z;
@@ -564,7 +567,7 @@ hipError_t launch_walk_t(const WalkLaunch& a, hipStream_t s) {
 
 // Workgroups of the work-queue kernel that the device holds at once (CUs x
 // resident workgroups per CU), computed once per device.
-template <int LOGW, bool INSERT, bool PRUNE>
+template <int LOGW, bool INSERT, bool PRUNE, bool DESC>
 uint32_t pool_slots() {
     static std::atomic<uint32_t> cached[64];
     int dev = 0;
@@ -573,7 +576,7 @@ uint32_t pool_slots() {
     if (v) return v;
     int per_cu = 0, cus = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, reinterpret_cast<const void*>(&k_walk_pool<LOGW, INSERT, PRUNE>),
+            &per_cu, reinterpret_cast<const void*>(&k_walk_pool<LOGW, INSERT, PRUNE, DESC>),
             kPoolThreads, 0) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         per_cu <= 0 || cus <= 0)
@@ -599,13 +602,13 @@ inline uint32_t pool_edges(uint32_t W, uint32_t requested, uint32_t n, uint32_t 
     return E;
 }
 
-template <bool INSERT, bool PRUNE>
+template <bool INSERT, bool PRUNE, bool DESC>
 hipError_t launch_pool_t(const WalkLaunch& a, hipStream_t s) {
     if (a.n == 0) return hipSuccess;
     uint32_t slots = 2048;
     switch (a.walkers) {
 #define DYG_CASE(LW) \
-    case (1u << LW): slots = pool_slots<LW, INSERT, PRUNE>(); break;
+    case (1u << LW): slots = pool_slots<LW, INSERT, PRUNE, DESC>(); break;
         DYG_CASE(0) DYG_CASE(1) DYG_CASE(2) DYG_CASE(3) DYG_CASE(4) DYG_CASE(5)
         DYG_CASE(6) DYG_CASE(7) DYG_CASE(8) DYG_CASE(9) DYG_CASE(10)
 #undef DYG_CASE
@@ -617,7 +620,7 @@ hipError_t launch_pool_t(const WalkLaunch& a, hipStream_t s) {
     switch (a.walkers) {
 #define DYG_CASE(LW)                                                                        \
     case (1u << LW):                                                                        \
-        hipLaunchKernelGGL((k_walk_pool<LW, INSERT, PRUNE>), grid, dim3(kPoolThreads), 0, s, \
+        hipLaunchKernelGGL((k_walk_pool<LW, INSERT, PRUNE, DESC>), grid, dim3(kPoolThreads), 0, s, \
                            a, E);                                                           \
         break;
         DYG_CASE(0) DYG_CASE(1) DYG_CASE(2) DYG_CASE(3) DYG_CASE(4) DYG_CASE(5)
@@ -632,9 +635,18 @@ hipError_t launch_pool_t(const WalkLaunch& a, hipStream_t s) {
 template <bool INSERT>
 hipError_t launch_any(const WalkLaunch& a, hipStream_t s) {
     const bool prune = !(a.flags & DYG_WALK_NO_PRUNE);
+    if constexpr (INSERT) {
+        if (a.nodesc) return hipErrorNotSupported;  // insertion walks run on sparsifiers
+    } else {
+        // a graph without descriptors (the stream's dense graph) is walked by
+        // the work-queue kernel reading rows[] per hop; same results
+        if (a.nodesc)
+            return prune ? launch_pool_t<false, true, false>(a, s)
+                         : launch_pool_t<false, false, false>(a, s);
+    }
     if (a.flags & DYG_WALK_PER_EDGE)
         return prune ? launch_walk_t<INSERT, true>(a, s) : launch_walk_t<INSERT, false>(a, s);
-    return prune ? launch_pool_t<INSERT, true>(a, s) : launch_pool_t<INSERT, false>(a, s);
+    return prune ? launch_pool_t<INSERT, true, true>(a, s) : launch_pool_t<INSERT, false, true>(a, s);
 }
 
 }  // namespace
@@ -651,12 +663,13 @@ hipError_t launch_walk_decremental(const WalkLaunch& a, hipStream_t s) {
 
 hipError_t launch_heuristic(const uint2* rows, const uint4* adj, const uint32_t* endpoint,
                             const uint64_t* key, uint32_t n, uint32_t walkers, uint64_t seed,
-                            uint32_t* path3, hipStream_t s, const uint32_t* n_dev) {
+                            uint32_t* path3, hipStream_t s, const uint32_t* n_dev,
+                            bool nodesc) {
     const uint64_t total = (uint64_t)n * walkers;
     if (total == 0) return hipSuccess;
     const uint32_t blocks = (uint32_t)((total + 255) / 256);
     hipLaunchKernelGGL(k_heuristic, dim3(blocks), dim3(256), 0, s, rows, adj, endpoint, key, n,
-                       walkers, seed, path3, n_dev);
+                       walkers, seed, path3, n_dev, nodesc);
     return hipGetLastError();
 }
 

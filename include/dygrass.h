@@ -283,15 +283,21 @@ typedef struct {
     uint32_t max_batch;     /* largest batch apply() accepts */
     uint64_t max_inserts;   /* arena reservation hint: appends expected per graph */
     uint32_t slack;         /* spare slots given to a relocated row */
-    uint32_t flags;         /* DYG_STREAM_PLAIN, DYG_STREAM_GENERAL_ONLY or 0 */
+    uint32_t flags;         /* DYG_STREAM_* bits, or 0 */
 } dyg_stream_params;
 /* The reference's shape: host edge maps and host-planned edits, one
  * one-workgroup-per-edge walk launch per batch, waiting for every result
  * (same results as the default device-resident form). */
 #define DYG_STREAM_PLAIN 1u
 /* Device form only: always use the general per-batch pipeline (by default
- * insertion batches of at most 1024 edges take a two-kernel path). */
+ * batches of at most 2048 updates take the one-workgroup path). */
 #define DYG_STREAM_GENERAL_ONLY 2u
+/* Device form only: the dense graph G keeps no neighbour descriptors in its
+ * slots, so its edits need no descriptor refresh and deletion walks read the
+ * row table once per hop (a second dependent load).  Same results; faster
+ * when refreshing G after insertions costs more than the longer deletion
+ * hops (high-degree G, insertion-heavy streams). */
+#define DYG_STREAM_DENSE_NO_DESC 4u
 typedef struct {
     uint64_t updates, weight_updates, found, added, skipped, g_only, recovered, heuristic,
         invalid;

@@ -56,17 +56,20 @@ def _assert_same_graph(dev_graph, rows):
     assert dev_graph.check() == 0
 
 
-@pytest.mark.parametrize("seed,plain,general", [(0, False, False), (1, False, False),
-                                                (0, True, False), (2, False, True)])
-def test_stream_matches_oracle(ctx, grid_wl, seed, plain, general):
-    """Device-resident form (small insertion batches through the two-kernel
-    path, or everything through the general pipeline) and the plain form."""
+@pytest.mark.parametrize("seed,plain,general,nodesc",
+                         [(0, False, False, False), (1, False, False, False),
+                          (0, True, False, False), (2, False, True, False),
+                          (3, False, True, True), (4, False, False, True)])
+def test_stream_matches_oracle(ctx, grid_wl, seed, plain, general, nodesc):
+    """Device-resident form (batches through the one-workgroup path, or
+    everything through the general pipeline; G with or without neighbour
+    descriptors) and the plain form."""
     from dygrass_amd import Stream, stream_params
     from oracle import stream as S
 
     wl = grid_wl
     sp = stream_params(w_ins=64, w_del=64, threshold=10.0, seed=seed, max_batch=1100,
-                       max_inserts=4096, plain=plain, general_only=general)
+                       max_inserts=4096, plain=plain, general_only=general, dense_no_desc=nodesc)
     st = Stream(ctx, wl.G.V, (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w), sp)
     orc = S.Stream(wl.G.V, (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w), thr=10.0,
                    w_ins=64, w_del=64, seed=seed)
@@ -84,8 +87,8 @@ def test_stream_matches_oracle(ctx, grid_wl, seed, plain, general):
     assert seen >= {S.WEIGHT, S.FOUND, S.ADDED, S.SKIP, S.G_ONLY, S.RECOVERED, S.INVALID}
 
 
-@pytest.mark.parametrize("plain", [False, True])
-def test_stream_heuristic_recovery(ctx, plain):
+@pytest.mark.parametrize("plain,nodesc", [(False, False), (True, False), (False, True)])
+def test_stream_heuristic_recovery(ctx, plain, nodesc):
     """Deleting a bridge of G that H holds leaves no path: the heuristic runs."""
     from dygrass_amd import Stream, stream_params
     from oracle import stream as S
@@ -96,7 +99,7 @@ def test_stream_heuristic_recovery(ctx, plain):
     gw = np.array([1, 2, 3, 4, 5, 6, 7], np.float32)
     hu, hv, hw = gu[[0, 1, 3, 4, 6]], gv[[0, 1, 3, 4, 6]], gw[[0, 1, 3, 4, 6]]
     sp = stream_params(w_ins=8, w_del=8, threshold=10.0, seed=3, max_batch=16, max_inserts=64,
-                       plain=plain)
+                       plain=plain, dense_no_desc=nodesc)
     st = Stream(ctx, 6, (gu, gv, gw), (hu, hv, hw), sp)
     orc = S.Stream(6, (gu, gv, gw), (hu, hv, hw), thr=10.0, w_ins=8, w_del=8, seed=3)
     got, _ = st.apply("del", [2], [3])
@@ -166,7 +169,8 @@ def test_small_deletion_batch_recovery_sizes(ctx, grid_wl, n_del):
         _assert_same_graph(st.H, orc.H)
 
 
-def test_batches_up_to_the_one_workgroup_limit(ctx, grid_wl):
+@pytest.mark.parametrize("nodesc", [False, True])
+def test_batches_up_to_the_one_workgroup_limit(ctx, grid_wl, nodesc):
     """Batches near the one-workgroup size (2048 updates, two per lane) and
     tiny ones (ranked by counting) match the oracle: insertion batches of
     1900 and 50, deletion batches of 2048 (G edges, repeats, non-edges) and 30."""
@@ -176,7 +180,7 @@ def test_batches_up_to_the_one_workgroup_limit(ctx, grid_wl):
     wl = grid_wl
     rng = np.random.default_rng(7)
     sp = stream_params(w_ins=64, w_del=64, threshold=10.0, seed=9, max_batch=4096,
-                       max_inserts=8192)
+                       max_inserts=8192, dense_no_desc=nodesc)
     st = Stream(ctx, wl.G.V, (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w), sp)
     orc = S.Stream(wl.G.V, (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w), thr=10.0,
                    w_ins=64, w_del=64, seed=9)
