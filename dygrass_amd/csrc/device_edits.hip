@@ -96,7 +96,7 @@ __global__ void k_alloc(uint2* __restrict__ rows, uint4* __restrict__ adj,
             rowbase[r] = 0xFFFFFFFFu;
             return;
         }
-        for (uint32_t q = 0; q < d0; ++q) adj[ns + q] = adj[row.x + q];
+        copy_slots(adj + ns, adj + row.x, d0);
         row.x = (uint32_t)ns;
         cap[r] = (uint32_t)ncap;
     }

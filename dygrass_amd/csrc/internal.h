@@ -205,6 +205,21 @@ __device__ __forceinline__ int slot_of(const uint4* __restrict__ adj, uint2 r, u
     return -1;
 }
 
+// Moves a row's n slots to a fresh range (relocation to the arena tail; the
+// ranges do not overlap), eight loads in flight before their stores.
+__device__ __forceinline__ void copy_slots(uint4* __restrict__ dst, const uint4* __restrict__ src,
+                                           uint32_t n) {
+    for (uint32_t z = 0; z < n; z += 8) {
+        uint4 t[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (z + j < n) t[j] = src[z + j];
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (z + j < n) dst[z + j] = t[j];
+    }
+}
+
 // In a row (first slot, degree) rx, every slot pointing to r gets r's
 // descriptor rr.  Only the slots' upper halves are written, so the columns
 // are read eight at a time as independent loads (a neighbour's row is one

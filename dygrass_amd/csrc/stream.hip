@@ -84,6 +84,27 @@ __global__ void k_batch_keys(const uint32_t* __restrict__ u, const uint32_t* __r
     idx[i] = i;
 }
 
+// k_batch_keys reading the batch from pinned host memory (u, v, w: w null for
+// deletions) and staging it in device memory (su, sv, sw) for the kernels
+// after it: the batch crosses PCIe once, with no separate copy.
+__global__ void k_batch_keys_staged(const uint32_t* __restrict__ u, const uint32_t* __restrict__ v,
+                                    const float* __restrict__ w, uint32_t n,
+                                    unsigned long long* __restrict__ key,
+                                    uint32_t* __restrict__ idx, uint32_t* __restrict__ bcnt,
+                                    uint32_t* __restrict__ su, uint32_t* __restrict__ sv,
+                                    float* __restrict__ sw) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (threadIdx.x == 0) bcnt[blockIdx.x] = 0;
+    if (i >= n) return;
+    const uint32_t x = u[i], y = v[i];
+    su[i] = x;
+    sv[i] = y;
+    if (w) sw[i] = w[i];
+    const uint32_t a = min(x, y), b = max(x, y);
+    key[i] = a == b ? kPadKey : ((unsigned long long)a << 32) | b;
+    idx[i] = i;
+}
+
 // One thread per distinct key (first position of its run in the sorted order).
 // mode[p] at the head: 0 nothing more, 1 existing edge (weight updates for
 // all occurrences; bit 2 set when H holds it too), 2 new edge with further
@@ -637,12 +658,20 @@ dyg_status apply_insert_small(dyg_stream s, const uint32_t* du, const uint32_t* 
 // read-back of its recovery sizes) ending with the report `rep`.
 dyg_status apply_insert(dyg_stream s, const uint32_t* du, const uint32_t* dv, const float* dw,
                         uint32_t n, int32_t* dcls, unsigned long long* dstats, bool timed,
-                        dyg_stream_stats* out, const Report& rep) {
+                        dyg_stream_stats* out, const Report& rep, uint32_t* stage) {
     if (n <= std::min(kSmallBatch, s->sp.max_batch) && !(s->sp.flags & DYG_STREAM_GENERAL_ONLY))
         return apply_insert_small(s, du, dv, dw, n, dcls, dstats, timed, rep);
     hipStream_t st = s->ctx->stream;
     dyg_graph G = s->G, H = s->H;
-    LAUNCH_1D(k_batch_keys, n, du, dv, n, s->k_in, s->i_in, s->bcnt);
+    if (stage) {
+        LAUNCH_1D(k_batch_keys_staged, n, du, dv, dw, n, s->k_in, s->i_in, s->bcnt, stage,
+                  stage + n, (float*)(stage + 2ull * n));
+        du = stage;
+        dv = stage + n;
+        dw = (const float*)(stage + 2ull * n);
+    } else {
+        LAUNCH_1D(k_batch_keys, n, du, dv, n, s->k_in, s->i_in, s->bcnt);
+    }
     dyg_status r = sort_keys(s, s->k_in, s->k_out, s->i_in, s->i_out, n);
     if (r != DYG_OK) return r;
     LAUNCH_1D(k_ins_groups, n, s->k_out, s->i_out, n, G->d_rows, G->d_adj, H->d_rows, H->d_adj,
@@ -757,13 +786,20 @@ dyg_status apply_delete_small(dyg_stream s, const uint32_t* du, const uint32_t* 
 
 dyg_status apply_delete(dyg_stream s, const uint32_t* du, const uint32_t* dv, uint32_t n,
                         int32_t* dcls, unsigned long long* dstats, bool timed,
-                        dyg_stream_stats* out, const Report& rep) {
+                        dyg_stream_stats* out, const Report& rep, uint32_t* stage) {
     if (n <= std::min(kSmallBatch, s->sp.max_batch) && !(s->sp.flags & DYG_STREAM_GENERAL_ONLY))
         return apply_delete_small(s, du, dv, n, dcls, dstats, timed, out, rep);
     hipStream_t st = s->ctx->stream;
     dyg_graph G = s->G, H = s->H;
     const uint32_t ms = s->sp.del.max_steps, hw = s->sp.heur_walkers;
-    LAUNCH_1D(k_batch_keys, n, du, dv, n, s->k_in, s->i_in, s->bcnt);
+    if (stage) {
+        LAUNCH_1D(k_batch_keys_staged, n, du, dv, (const float*)nullptr, n, s->k_in, s->i_in,
+                  s->bcnt, stage, stage + n, (float*)nullptr);
+        du = stage;
+        dv = stage + n;
+    } else {
+        LAUNCH_1D(k_batch_keys, n, du, dv, n, s->k_in, s->i_in, s->bcnt);
+    }
     dyg_status r = sort_keys(s, s->k_in, s->k_out, s->i_in, s->i_out, n);
     if (r != DYG_OK) return r;
     LAUNCH_1D(k_del_groups, n, s->k_out, s->i_out, n, G->d_rows, G->d_adj, H->d_rows, H->d_adj,
@@ -1028,27 +1064,18 @@ extern "C" dyg_status dyg_stream_apply(dyg_stream s, int op, const uint32_t* u, 
         if (r != DYG_OK) return r;
         DYG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     } else if (n) {
-        // pinned staging: the inputs go in with one copy (a small insertion
-        // batch reads them in place), the last kernel writes the results
-        // back, and the batch has one wait
+        // pinned staging: the device reads the inputs from pinned memory (a
+        // one-workgroup batch in place, a larger one through its first
+        // kernel, which stages them in device memory), the last kernel writes
+        // the results back, and the batch has one wait
         hipStream_t str = ctx->stream;
         const PinLayout hp = pin_layout((char*)s->pin, n), dp = pin_layout(s->pin_dev, n);
         const bool ins = op == DYG_STREAM_INSERT;
         if (ins) std::memcpy(hp.w, w, n * 4ull);
-        const uint32_t *du = s->in, *dv = s->in + n;
-        const float* dw = (const float*)(s->in + 2ull * n);
-        if (ins && n <= std::min(kSmallBatch, s->sp.max_batch) &&
-            !(s->sp.flags & DYG_STREAM_GENERAL_ONLY)) {
-            du = dp.u;
-            dv = dp.v;
-            dw = dp.w;
-        } else {
-            DYG_HIP(ctx, hipMemcpyAsync(s->in, hp.u, n * (ins ? 12ull : 8ull),
-                                        hipMemcpyHostToDevice, str));
-        }
         const Report rep{dp.cls, dp.words, s->G->dev.meta, s->H->dev.meta};
-        dyg_status r = ins ? apply_insert(s, du, dv, dw, n, s->cls, s->stats, true, &rs, rep)
-                           : apply_delete(s, du, dv, n, s->cls, s->stats, true, &rs, rep);
+        dyg_status r = ins ? apply_insert(s, dp.u, dp.v, dp.w, n, s->cls, s->stats, true, &rs, rep,
+                                          s->in)
+                           : apply_delete(s, dp.u, dp.v, n, s->cls, s->stats, true, &rs, rep, s->in);
         if (r != DYG_OK) return r;
         r = wait_spin(ctx, str, s->ev_sync);
         if (r != DYG_OK) return r;
@@ -1237,8 +1264,8 @@ void serve_worker(dyg_serve_s* v) {
         const Report rep{dp.cls, dp.words, s->G->dev.meta, s->H->dev.meta};
         if (st == DYG_OK)
             st = r->op == DYG_STREAM_INSERT
-                     ? apply_insert(s, du, dv, dw, n, sl.dcls, sl.dstats, false, &rs, rep)
-                     : apply_delete(s, du, dv, n, sl.dcls, sl.dstats, false, &rs, rep);
+                     ? apply_insert(s, du, dv, dw, n, sl.dcls, sl.dstats, false, &rs, rep, nullptr)
+                     : apply_delete(s, du, dv, n, sl.dcls, sl.dstats, false, &rs, rep, nullptr);
         if (st == DYG_OK) {
             s->base += n;
             if (hipEventRecord(sl.e_done, cs) != hipSuccess) st = DYG_ERR_HIP;

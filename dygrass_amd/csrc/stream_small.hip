@@ -255,7 +255,7 @@ __device__ void block_append(const BlockGraph& g, typename Shape<IPT>::TempU& tm
                 atomicOr((uint32_t*)(g.meta + 2), 1u);
                 continue;
             }
-            for (uint32_t z = 0; z < d0; ++z) g.adj[ns + z] = g.adj[row.x + z];
+            copy_slots(g.adj + ns, g.adj + row.x, d0);
             row.x = (uint32_t)ns;
             g.cap[r[q]] = (uint32_t)ncap;
         }
