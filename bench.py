@@ -199,18 +199,47 @@ def leg_incremental(args, ctx) -> dict:
 
 
 # ---------------------------------------------------------------------------
-def run_stream(ctx, wl, plain: bool, seed: int, block_edges: int = 0) -> dict:
+def tuned(name: str, w_ins: int, w_del: int):
+    """The autotuner's setting for workload `name` (dygrass_amd/tuning/<name>.json,
+    made by `python -m dygrass_amd.autotune` on this GPU) and a summary for the
+    JSON line; without a committed tuning, the fixed walker counts given."""
+    from dygrass_amd import autotune
+
+    t = autotune.load(name)
+    if t is None:
+        s = autotune.Setting(w_ins=w_ins, w_del=w_del)
+        return s, {"source": None, "setting": s.asdict(), "note": "no committed tuning"}
+    s = autotune.setting_of(t)
+    return s, {"source": f"dygrass_amd/tuning/{name}.json", "setting": s.asdict(),
+               "anchor": t["anchor"]["setting"],
+               "growth_anchor": t["anchor"]["growth"], "growth": t["chosen"]["growth"],
+               "kappa_anchor": t["anchor"]["kappa"], "kappa": t["chosen"]["kappa"],
+               "tolerances": t["tolerances"], "calibration_updates": t["updates"],
+               "command": t["command"]}
+
+
+def sp_of(setting, threshold: float, seed: int, max_batch: int, max_inserts: int, plain: bool):
+    """Stream parameters of a tuned setting; the plain form takes its walker
+    counts and step budgets (same results), not its device-only shape knobs."""
+    from dygrass_amd import stream_params
+
+    s = setting
+    return stream_params(w_ins=s.w_ins, w_del=s.w_del, threshold=threshold, seed=seed,
+                         max_batch=max_batch, max_inserts=max_inserts, plain=plain,
+                         steps_ins=s.steps_ins, steps_del=s.steps_del,
+                         block_edges_ins=0 if plain else s.block_ins,
+                         block_edges_del=0 if plain else s.block_del,
+                         dense_no_desc=s.dense_no_desc and not plain)
+
+
+def run_stream(ctx, wl, plain: bool, seed: int, setting, csr=None) -> dict:
     """One pass of the whole batch list through a fresh stream; the stream is
     built (graphs uploaded, buffers reserved) outside the timed window."""
-    from dygrass_amd import Stream, stream_params
+    from dygrass_amd import Stream
 
     maxb = max(len(b[1]) for b in wl.batches)
-    # G keeps no neighbour descriptors: on this 12-NN graph refreshing them
-    # after insertions costs more than the longer deletion hops
-    sp = stream_params(w_ins=wl.w_ins, w_del=wl.w_del, threshold=wl.threshold, seed=seed,
-                       max_batch=maxb, max_inserts=wl.updates, plain=plain,
-                       block_edges=block_edges, dense_no_desc=not plain)
-    st = Stream(ctx, wl.G.V, (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w), sp)
+    sp = sp_of(setting, wl.threshold, seed, maxb, wl.updates, plain)
+    st = Stream(ctx, wl.G.V, (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w), sp, csr=csr)
     ctx.sync()
     outs, lat, walk_ms, hops, agg = [], [], 0.0, 0, {}
     ts = time.perf_counter()
@@ -233,18 +262,22 @@ def run_stream(ctx, wl, plain: bool, seed: int, block_edges: int = 0) -> dict:
 
 
 def leg_batched(args, ctx) -> dict:
-    from dygrass_amd import synth
+    from dygrass_amd import csr_from_edges, synth
 
     t0 = time.time()
     wl = synth.config_batched(args.scale, args.seed)
     m = wl.updates
     log(f"batched: {wl.name} V={wl.G.V} E_G={len(wl.G)} E_H={len(wl.H)} batches="
         f"{len(wl.batches)} updates={m} ({time.time() - t0:.1f}s)")
+    setting, tuning = tuned(wl.name, wl.w_ins, wl.w_del)
+    log(f"batched setting: {setting}")
+    csr = (csr_from_edges(wl.G.V, wl.G.u, wl.G.v, wl.G.w),
+           csr_from_edges(wl.G.V, wl.H.u, wl.H.v, wl.H.w))
     for _ in range(args.warmup):
-        run_stream(ctx, wl, False, args.seed, args.block_edges)
+        run_stream(ctx, wl, False, args.seed, setting, csr)
     runs = []
     for i in range(args.steps):
-        r = run_stream(ctx, wl, False, args.seed, args.block_edges)
+        r = run_stream(ctx, wl, False, args.seed, setting, csr)
         runs.append(r)
         log(f"batched step {i}: {r['seconds'] * 1e3:.1f} ms (walk kernels {r['walk_ms']:.1f} ms)")
     total = sum(r["seconds"] for r in runs)
@@ -253,18 +286,23 @@ def leg_batched(args, ctx) -> dict:
         "value": m * len(runs) / total, "ms_per_step": total / len(runs) * 1e3,
         "workload": f"{wl.name}: {len(wl.batches)} batches of log-uniform size 1..50000 "
                     f"({m} updates, 70% insertion batches), threshold {wl.threshold:g}, "
-                    f"{wl.w_ins} insertion / {wl.w_del} deletion walkers, 100 steps "
-                    "(device-resident stream: work-queue walk kernels, device edits, dense graph "
-                    "without neighbour descriptors)",
+                    f"{setting.w_ins} insertion / {setting.w_del} deletion walkers, "
+                    f"{setting.steps_ins}/{setting.steps_del} steps (autotuned; device-resident "
+                    "stream: work-queue walk kernels, device edits)",
         "V": int(wl.G.V), "E_G": int(len(wl.G)), "E_H": int(len(wl.H)), "updates": int(m),
         "batches": len(wl.batches), "walk_kernel_ms_per_step": head["walk_ms"],
         "walk_steps_per_s": head["hops"] / head["seconds"],
         "batch_latency_ms": {"p50": pct(head["latencies"], 50),
                              "p99": pct(head["latencies"], 99)},
         "outcomes": head["counts"],
+        "tuning": tuning,
+        "growth_H": int(len(head["H"][1]) // 2 - len(wl.H)),
     }
+    if tuning.get("source"):
+        # the stream is deterministic: this run's growth is the tuning run's
+        out["growth_matches_tuning"] = out["growth_H"] == tuning["growth"]
     if not args.no_plain:
-        pr = run_stream(ctx, wl, True, args.seed)
+        pr = run_stream(ctx, wl, True, args.seed, setting, csr)
         log(f"batched plain: {pr['seconds'] * 1e3:.1f} ms")
         out["plain"] = {"value": m / pr["seconds"], "ms_per_step": pr["seconds"] * 1e3,
                         "walk_kernel_ms_per_step": pr["walk_ms"],
@@ -279,11 +317,11 @@ def leg_batched(args, ctx) -> dict:
                            b.view(np.uint32) if b.dtype == np.float32 else b)
             for a, b in zip(pr["H"], head["H"]))
     if not args.no_cpu:
-        out["cpu"] = cpu_sample_batched(wl, args)
+        out["cpu"] = cpu_sample_batched(wl, args, setting)
     return out
 
 
-def cpu_sample_batched(wl, args) -> dict:
+def cpu_sample_batched(wl, args, setting) -> dict:
     """The oracle's insertion walks on H for the stream's first insertion
     batches (about 20k updates), OpenMP over each batch's edges."""
     from dygrass_amd import csr_from_edges
@@ -296,16 +334,17 @@ def cpu_sample_batched(wl, args) -> dict:
     for op, u, v, ww in wl.batches:
         if op == "ins" and n < 20000:
             ts = time.perf_counter()
-            og.walk_inc_batch(u, v, ww, wl.threshold, wl.w_ins, seed=args.seed,
+            og.walk_inc_batch(u, v, ww, wl.threshold, setting.w_ins, seed=args.seed,
                               edge_id=np.arange(base, base + len(u), dtype=np.uint64),
-                              threads=threads)
+                              threads=threads, max_steps=setting.steps_ins)
             secs += time.perf_counter() - ts
             n += len(u)
         base += len(u)
     log(f"batched cpu sample: {n / secs:.0f} walked updates/s on {threads} threads")
     return {"value": n / secs, "unit": "edge-updates/s", "cores": threads, "kind": "port",
             "sample": f"insertion walks of the stream's first insertion batches ({n} updates) "
-                      "on the original sparsifier, through the oracle, OpenMP over edges; "
+                      f"on the original sparsifier, {setting.w_ins} walkers, "
+                      f"{setting.steps_ins} steps, through the oracle, OpenMP over edges; "
                       "graph bookkeeping not included"}
 
 
@@ -323,9 +362,11 @@ def leg_served(args, ctx) -> dict:
     sizes = np.array([len(r[1]) for r in wl.requests])
     g_edges, h_edges = (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w)
 
+    setting, tuning = tuned(wl.name, wl.w_ins, wl.w_del)
+    log(f"served setting: {setting}")
+
     def sp(plain):
-        return stream_params(w_ins=wl.w_ins, w_del=wl.w_del, threshold=wl.threshold,
-                             seed=args.seed, max_batch=maxr, max_inserts=m, plain=plain)
+        return sp_of(setting, wl.threshold, args.seed, maxr, m, plain)
 
     def serve(arrivals):
         st = Stream(ctx, wl.G.V, g_edges, h_edges, sp(False))
@@ -360,9 +401,11 @@ def leg_served(args, ctx) -> dict:
         "value": m / span,
         "workload": f"{wl.name}: {n} requests, Poisson arrivals at {args.rate:g}/s, sizes "
                     f"log-uniform 1..10000 ({m} updates, 60% insertions), threshold "
-                    f"{wl.threshold:g}, {wl.w_ins}/{wl.w_del} walkers, {args.slots} slots",
+                    f"{wl.threshold:g}, {setting.w_ins}/{setting.w_del} walkers, "
+                    f"{setting.steps_ins}/{setting.steps_del} steps (autotuned), {args.slots} "
+                    "slots",
         "V": int(wl.G.V), "E_G": int(len(wl.G)), "E_H": int(len(wl.H)), "requests": n,
-        "updates": int(m), "span_s": span,
+        "updates": int(m), "span_s": span, "tuning": tuning,
         "latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99)},
         "latency_ms_le_1k": {"p50": pct(lat[small], 50), "p99": pct(lat[small], 99),
                              "requests": int(small.sum())},

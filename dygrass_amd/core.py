@@ -381,15 +381,21 @@ def stream_params(w_ins: int = 256, w_del: int = 512, threshold: float = 100.0, 
                   max_batch: int = 50000, max_inserts: int = 0, heur_walkers: int = 16,
                   slack: int = 2, max_steps: int = 100, plain: bool = False,
                   block_edges: int = 0, general_only: bool = False,
-                  dense_no_desc: bool = False) -> L.StreamParams:
+                  dense_no_desc: bool = False, steps_ins: int | None = None,
+                  steps_del: int | None = None, block_edges_ins: int | None = None,
+                  block_edges_del: int | None = None) -> L.StreamParams:
     """Stream parameters; plain=True selects the reference-shaped form (host
     edge maps and edit plans, one workgroup per edge, a wait per step);
     general_only=True keeps small batches on the general pipeline;
     dense_no_desc=True keeps no neighbour descriptors in G (no refresh after
-    its edits, deletion walks read the row table per hop)."""
+    its edits, deletion walks read the row table per hop).  steps_* and
+    block_edges_* set the insertion and deletion walks apart (default:
+    max_steps and block_edges for both)."""
     sp = L.StreamParams()
-    sp.ins = params(w_ins, max_steps, threshold, seed, block_edges=block_edges)
-    sp.del_ = params(w_del, max_steps, 0.0, seed, block_edges=block_edges)
+    sp.ins = params(w_ins, max_steps if steps_ins is None else steps_ins, threshold, seed,
+                    block_edges=block_edges if block_edges_ins is None else block_edges_ins)
+    sp.del_ = params(w_del, max_steps if steps_del is None else steps_del, 0.0, seed,
+                     block_edges=block_edges if block_edges_del is None else block_edges_del)
     sp.heur_walkers, sp.max_batch, sp.max_inserts = heur_walkers, max_batch, max_inserts
     sp.slack = slack
     sp.flags = ((L.DYG_STREAM_PLAIN if plain else 0)
@@ -402,12 +408,16 @@ class Stream:
     """Mixed insertion/deletion update stream on device-resident G and H
     (dyg_stream; dynamic_update/main.cu:92-124 batch loop semantics)."""
 
-    def __init__(self, ctx: Context, V: int, g_edges, h_edges, sp: L.StreamParams):
+    def __init__(self, ctx: Context, V: int, g_edges, h_edges, sp: L.StreamParams,
+                 csr: tuple | None = None):
+        """g_edges / h_edges: (u, v, w) edge lists; or csr = (G's, H's)
+        (row_ptr, col, w) from csr_from_edges, built once for many streams."""
         self.ctx = ctx
         self._lib = ctx._lib
         self.V = int(V)
-        grp, gcol, gw = csr_from_edges(V, *g_edges)
-        hrp, hcol, hw = csr_from_edges(V, *h_edges)
+        if csr is None:
+            csr = (csr_from_edges(V, *g_edges), csr_from_edges(V, *h_edges))
+        (grp, gcol, gw), (hrp, hcol, hw) = csr
         h = C.c_void_p()
         check(self._lib.dyg_stream_create(ctx.handle, self.V, ptr(grp, L.u64p), ptr(gcol, L.u32p),
                                           ptr(gw, L.f32p), ptr(hrp, L.u64p), ptr(hcol, L.u32p),

@@ -116,11 +116,14 @@ class Stream:
     """G, H and the walk parameters of one update stream."""
 
     def __init__(self, V, g_edges, h_edges, thr=100.0, w_ins=256, w_del=512, seed=0,
-                 max_steps=100, heur_walkers=16, threads=8):
+                 max_steps=100, heur_walkers=16, threads=8, steps_ins=None, steps_del=None):
         self.G = Rows(V, *g_edges)
         self.H = Rows(V, *h_edges)
         self.thr, self.w_ins, self.w_del = float(thr), w_ins, w_del
-        self.seed, self.max_steps, self.heur = seed, max_steps, heur_walkers
+        self.seed, self.heur = seed, heur_walkers
+        # walk bounds of the insertion and deletion walks (the reference: 100 each)
+        self.steps_ins = max_steps if steps_ins is None else steps_ins
+        self.steps_del = max_steps if steps_del is None else steps_del
         self.threads = threads
         self.base = 0  # stream position of the next update
 
@@ -147,7 +150,7 @@ class Stream:
             f, _, _, _ = hg.walk_inc_batch(np.asarray(u)[k], np.asarray(v)[k],
                                            np.asarray(w, np.float32)[k], self.thr, self.w_ins,
                                            seed=self.seed, edge_id=(self.base + k).astype(np.uint64),
-                                           threads=self.threads, max_steps=self.max_steps)
+                                           threads=self.threads, max_steps=self.steps_ins)
             for j, i in enumerate(keep):
                 if f[j] == -1:
                     a, b = min(int(u[i]), int(v[i])), max(int(u[i]), int(v[i]))
@@ -190,7 +193,7 @@ class Stream:
             f, _, _, _, path = gg.walk_dec_batch(src, dst, self.w_del, seed=self.seed,
                                                  edge_id=(self.base + k).astype(np.uint64),
                                                  threads=self.threads,
-                                                 max_steps=self.max_steps)
+                                                 max_steps=self.steps_del)
             for j, i in enumerate(keep):
                 if f[j] > 0:
                     for q in range(1, f[j] + 1):

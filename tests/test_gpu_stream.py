@@ -87,6 +87,34 @@ def test_stream_matches_oracle(ctx, grid_wl, seed, plain, general, nodesc):
     assert seen >= {S.WEIGHT, S.FOUND, S.ADDED, S.SKIP, S.G_ONLY, S.RECOVERED, S.INVALID}
 
 
+@pytest.mark.parametrize("general", [False, True])
+def test_stream_tuned_settings_match_oracle(ctx, grid_wl, general):
+    """The settings the autotuner moves, set apart per walk kind: walkers,
+    step budgets and workgroup shape of the insertion walks differ from the
+    deletion walks'; the step budgets are short enough that they cut walks
+    and paths (found_at near the bound, longer paths unrecovered)."""
+    from dygrass_amd import Stream, stream_params
+    from oracle import stream as S
+
+    wl = grid_wl
+    seed = 5
+    sp = stream_params(w_ins=128, w_del=32, threshold=10.0, seed=seed, max_batch=1100,
+                       max_inserts=4096, general_only=general, steps_ins=12, steps_del=7,
+                       block_edges_ins=3, block_edges_del=5, dense_no_desc=True)
+    st = Stream(ctx, wl.G.V, (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w), sp)
+    orc = S.Stream(wl.G.V, (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w), thr=10.0,
+                   w_ins=128, w_del=32, seed=seed, steps_ins=12, steps_del=7)
+    seen = set()
+    for op, u, v, w in _mixed_batches(wl, seed):
+        got, _ = st.apply(op, u, v, w)
+        want = orc.apply(op, u, v, w)
+        assert np.array_equal(got, want), (op, np.flatnonzero(got != want)[:10])
+        seen.update(np.unique(got).tolist())
+        _assert_same_graph(st.G, orc.G)
+        _assert_same_graph(st.H, orc.H)
+    assert seen >= {S.FOUND, S.ADDED, S.RECOVERED}
+
+
 @pytest.mark.parametrize("plain,nodesc", [(False, False), (True, False), (False, True)])
 def test_stream_heuristic_recovery(ctx, plain, nodesc):
     """Deleting a bridge of G that H holds leaves no path: the heuristic runs."""
