@@ -229,7 +229,8 @@ def sp_of(setting, threshold: float, seed: int, max_batch: int, max_inserts: int
                          steps_ins=s.steps_ins, steps_del=s.steps_del,
                          block_edges_ins=0 if plain else s.block_ins,
                          block_edges_del=0 if plain else s.block_del,
-                         dense_no_desc=s.dense_no_desc and not plain)
+                         dense_no_desc=s.dense_no_desc and not plain,
+                         graph=s.graph and not plain)
 
 
 def run_stream(ctx, wl, plain: bool, seed: int, setting, csr=None) -> dict:

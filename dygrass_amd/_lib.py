@@ -59,7 +59,7 @@ class StreamStats(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in ("updates", "weight_updates", "found", "added",
                                           "skipped", "g_only", "recovered", "heuristic",
                                           "invalid", "candidates", "hops")] + [
-        ("walk_ms", C.c_double), ("total_ms", C.c_double)]
+        ("walk_ms", C.c_double), ("total_ms", C.c_double), ("graph", C.c_uint64)]
 
 
 class ServeResult(C.Structure):
@@ -76,6 +76,7 @@ DYG_STREAM_INSERT, DYG_STREAM_DELETE = 0, 1
 DYG_STREAM_PLAIN = 1
 DYG_STREAM_GENERAL_ONLY = 2
 DYG_STREAM_DENSE_NO_DESC = 4
+DYG_STREAM_GRAPH = 8
 OUTCOMES = {1: "weight", 2: "found", 3: "added", 4: "skip", 5: "g_only", 6: "recovered",
             7: "heuristic", 8: "invalid"}
 

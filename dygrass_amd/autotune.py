@@ -8,13 +8,13 @@ command line (random_walk_incremental/main.cu:240-262, `<#walkers>` at most
 Those fixed values are the tuner's *anchor*: the quality the tuned setting
 must keep.
 
-Seven knobs, in two kinds:
+Eight knobs, in two kinds:
 
 * Shape knobs change no result, only the time, because every draw is keyed
   by (edge, walker, step) and never by where a walker runs: edges per
   workgroup of the work-queue walk kernel for insertion walks and for
-  deletion walks, and whether the dense graph G keeps neighbour descriptors
-  in its slots.  They are chosen by timing alone, and every shape's outcome
+  deletion walks, whether the dense graph G keeps neighbour descriptors in
+  its slots, and whether one-workgroup batch steps run as captured hipGraphs.  They are chosen by timing alone, and every shape's outcome
   codes and final sparsifier must equal the first shape's (checked; a
   mismatch is an error, not a tuning result).
 * Quality knobs change which paths the walks find: walkers per edge and
@@ -70,6 +70,7 @@ class Setting:
     block_ins: int = 0        # edges per workgroup, insertion walks (0 = automatic)
     block_del: int = 0        # edges per workgroup, deletion walks (0 = automatic)
     dense_no_desc: bool = False
+    graph: bool = False       # one-workgroup batch steps as captured hipGraphs
 
     def quality(self) -> tuple:
         return (self.w_ins, self.w_del, self.steps_ins, self.steps_del)
@@ -88,6 +89,7 @@ QUALITY_GRID = {
 }
 SHAPE_GRID = {
     "dense_no_desc": (False, True),
+    "graph": (False, True),
     "block_ins": (0, 1, 2, 4, 8, 16),
     "block_del": (0, 1, 2, 4, 8, 16),
 }
@@ -239,7 +241,7 @@ def stream_params_for(s: Setting, threshold: float, seed: int, max_batch: int, m
                          max_batch=max_batch, max_inserts=max_inserts,
                          steps_ins=s.steps_ins, steps_del=s.steps_del,
                          block_edges_ins=s.block_ins, block_edges_del=s.block_del,
-                         dense_no_desc=s.dense_no_desc)
+                         dense_no_desc=s.dense_no_desc, graph=s.graph)
 
 
 class StreamTrials:
@@ -361,8 +363,10 @@ def main(argv=None) -> int:
         # the served configuration fixes its walker counts; only the shape knobs
         # are tuned (kappa on its sparsifier, a tree plus random shortcuts, costs
         # minutes per evaluation through the sparse LU)
+        # (step graphs are not a knob here: the served loop launches directly)
         res = tune(tr.run, None, tr.edges_h0, anchor=Setting(w_ins=wl.w_ins, w_del=wl.w_del),
-                   quality_grid={}, repeats=a.repeats, log=log)
+                   quality_grid={}, repeats=a.repeats, log=log,
+                   shape_grid={k: v for k, v in SHAPE_GRID.items() if k != "graph"})
     res.update({"workload": name, "V": int(wl.G.V), "E_G": int(len(wl.G)),
                 "E_H": int(len(wl.H)), "updates": int(updates),
                 "batches": len(batches), "threshold": wl.threshold, "seed": a.seed,

@@ -383,14 +383,16 @@ def stream_params(w_ins: int = 256, w_del: int = 512, threshold: float = 100.0, 
                   block_edges: int = 0, general_only: bool = False,
                   dense_no_desc: bool = False, steps_ins: int | None = None,
                   steps_del: int | None = None, block_edges_ins: int | None = None,
-                  block_edges_del: int | None = None) -> L.StreamParams:
+                  block_edges_del: int | None = None, graph: bool = False) -> L.StreamParams:
     """Stream parameters; plain=True selects the reference-shaped form (host
     edge maps and edit plans, one workgroup per edge, a wait per step);
     general_only=True keeps small batches on the general pipeline;
     dense_no_desc=True keeps no neighbour descriptors in G (no refresh after
     its edits, deletion walks read the row table per hop).  steps_* and
     block_edges_* set the insertion and deletion walks apart (default:
-    max_steps and block_edges for both)."""
+    max_steps and block_edges for both); graph=True runs batches of at most
+    2048 updates as captured hipGraphs of their one-workgroup step (apply()
+    only; the served loop launches directly)."""
     sp = L.StreamParams()
     sp.ins = params(w_ins, max_steps if steps_ins is None else steps_ins, threshold, seed,
                     block_edges=block_edges if block_edges_ins is None else block_edges_ins)
@@ -400,7 +402,8 @@ def stream_params(w_ins: int = 256, w_del: int = 512, threshold: float = 100.0, 
     sp.slack = slack
     sp.flags = ((L.DYG_STREAM_PLAIN if plain else 0)
                 | (L.DYG_STREAM_GENERAL_ONLY if general_only else 0)
-                | (L.DYG_STREAM_DENSE_NO_DESC if dense_no_desc else 0))
+                | (L.DYG_STREAM_DENSE_NO_DESC if dense_no_desc else 0)
+                | (L.DYG_STREAM_GRAPH if graph else 0))
     return sp
 
 

@@ -276,6 +276,21 @@ struct Report {
 hipError_t launch_report(const Report& r, const int32_t* cls, uint32_t n,
                          unsigned long long* stats, hipStream_t s);
 
+// The per-batch values of a one-workgroup step run as a captured graph,
+// whose kernel arguments are fixed at capture: the host writes them into
+// pinned memory before each launch, and the pre and post kernels read them
+// at entry in place of the same fields of their arguments.
+struct StepDesc {
+    const uint32_t* u;
+    const uint32_t* v;
+    const float* w;               // insertions only
+    int32_t* rep_cls;             // Report::cls
+    unsigned long long* rep_words;  // Report::words
+    uint64_t base;
+    uint32_t n;
+    uint32_t pad;
+};
+
 // An insertion batch of at most kSmallBatch edges in two one-workgroup
 // kernels around the walk (device pointers; u, v and w may point into
 // pinned host memory).
@@ -300,6 +315,7 @@ struct SmallInsArgs {
     const int32_t* found;  // walk result per kept edge (post kernel)
     unsigned long long* stats;  // walk counters, reported and zeroed by the post kernel
     Report rep;
+    const StepDesc* desc;  // when set: u, v, w, n, base and rep's outputs come from here
 };
 hipError_t launch_small_ins_pre(const SmallInsArgs& a, hipStream_t s);
 hipError_t launch_small_ins_post(const SmallInsArgs& a, hipStream_t s);
@@ -339,11 +355,13 @@ struct SmallDelArgs {
     uint32_t* sizes;       // pinned: kept, no-path, endpoints, path hops, taken-over flag
     unsigned long long* stats;
     Report rep;
+    const StepDesc* desc;  // when set: u, v, n, base and rep's outputs come from here
 };
 // Candidate edges the post kernel of an n-update deletion batch can hold.
 uint32_t small_del_candidates(uint32_t n);
 hipError_t launch_small_del_pre(const SmallDelArgs& a, hipStream_t s);
 hipError_t launch_small_del_post(const SmallDelArgs& a, hipStream_t s);
+
 
 bool valid_walkers(uint32_t w);
 // Validates walkers (power of two, 1..1024) and max_steps (1..32767).

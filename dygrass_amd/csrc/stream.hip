@@ -435,8 +435,27 @@ struct dyg_stream_s {
     void* temp = nullptr;
     size_t temp_bytes = 0;
     std::vector<int32_t> h_cls;
+    // DYG_STREAM_GRAPH: the one-workgroup step of each operation and size
+    // class, captured on first use
+    struct StepGraph {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        BlockGraph g{}, h{};  // G's and H's layout baked into the captured nodes
+        bool failed = false;  // capture refused: the class launches directly
+    };
+    static constexpr int kStepClasses = 5;
+    StepGraph steps[2][kStepClasses];
+    StepDesc* desc = nullptr;      // pinned: the graphed batch's values
+    StepDesc* desc_dev = nullptr;  // the same, as the device reaches it
+    bool last_graph = false;       // the current batch ran as a step graph
     ~dyg_stream_s() {
+        for (auto& op : steps)
+            for (StepGraph& g : op) {
+                if (g.exec) (void)hipGraphExecDestroy(g.exec);
+                if (g.graph) (void)hipGraphDestroy(g.graph);
+            }
         if (ev_sync) (void)hipEventDestroy(ev_sync);
+        if (desc) (void)hipHostFree(desc);
         if (pin) (void)hipHostFree(pin);
         if (mem) (void)hipFree(mem);
         if (temp) (void)hipFree(temp);
@@ -618,11 +637,164 @@ BlockGraph block_graph(dyg_graph g) {
                       (unsigned long long)g->cap_slots, g->slack, refresh_lanes(g), g->nodesc};
 }
 
-// Small insertion batch: two one-workgroup kernels around the walk.
-dyg_status apply_insert_small(dyg_stream s, const uint32_t* du, const uint32_t* dv,
-                              const float* dw, uint32_t n, int32_t* dcls,
-                              unsigned long long* dstats, bool timed, const Report& rep) {
+// Size classes of the captured one-workgroup steps: a step graph's grids are
+// sized for its class maximum, the kernels read the batch's real counts from
+// device memory, and only the pre and post kernels' arguments change.
+constexpr uint32_t kStepClass[dyg_stream_s::kStepClasses] = {16, 64, 256, 1024, kSmallBatch};
+
+int step_class(uint32_t n) {
+    for (int c = 0; c < dyg_stream_s::kStepClasses; ++c)
+        if (n <= kStepClass[c]) return c;
+    return -1;
+}
+
+// The one-workgroup insertion step for a.n updates: pre kernel (and G's
+// descriptor refresh), walks over the kept list in H, post kernel (and H's
+// refresh).  `timed` brackets the walk with the context's events.
+dyg_status enqueue_ins_step(dyg_stream s, const SmallInsArgs& a, bool timed) {
     hipStream_t st = s->ctx->stream;
+    DYG_HIP(s->ctx, launch_small_ins_pre(a, st));
+    if (timed) DYG_HIP(s->ctx, hipEventRecord(s->ctx->ev0, st));
+    WalkLaunch w = walk_args(s, s->H, s->sp.ins, a.n);
+    w.stats = a.stats;
+    DYG_HIP(s->ctx, launch_walk_incremental(w, st));
+    if (timed) DYG_HIP(s->ctx, hipEventRecord(s->ctx->ev1, st));
+    DYG_HIP(s->ctx, launch_small_ins_post(a, st));
+    return DYG_OK;
+}
+
+// The deletion counterpart: pre kernel (and refreshes), walks over the kept
+// list in G with paths, post kernel (recovery and H's refresh).
+dyg_status enqueue_del_step(dyg_stream s, const SmallDelArgs& a, bool timed) {
+    hipStream_t st = s->ctx->stream;
+    DYG_HIP(s->ctx, launch_small_del_pre(a, st));
+    WalkLaunch w = walk_args(s, s->G, s->sp.del, a.n);
+    w.w = nullptr;
+    w.path = s->path;
+    w.stats = a.stats;
+    if (timed) DYG_HIP(s->ctx, hipEventRecord(s->ctx->ev0, st));
+    DYG_HIP(s->ctx, launch_walk_decremental(w, st));
+    if (timed) DYG_HIP(s->ctx, hipEventRecord(s->ctx->ev1, st));
+    DYG_HIP(s->ctx, launch_small_del_post(a, st));
+    return DYG_OK;
+}
+
+dyg_status enqueue_step(dyg_stream s, const SmallInsArgs& a, bool timed) {
+    return enqueue_ins_step(s, a, timed);
+}
+dyg_status enqueue_step(dyg_stream s, const SmallDelArgs& a, bool timed) {
+    return enqueue_del_step(s, a, timed);
+}
+
+// Captures the step of one operation and size class: grids for the class
+// maximum, the pre and post kernels reading the batch's values from the
+// stream's step descriptor, the walk timing events included.  False when the
+// runtime refuses any part of it: the class is then marked and launches
+// directly, with the same results.
+template <class Args>
+bool capture_step(dyg_stream s, dyg_stream_s::StepGraph& g, Args a, uint32_t cmax) {
+    hipStream_t st = s->ctx->stream;
+    g.failed = true;
+    if (!s->desc) {
+        void* h = nullptr;
+        void* d = nullptr;
+        if (hipHostMalloc(&h, sizeof(StepDesc), hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            return false;
+        }
+        if (hipHostGetDevicePointer(&d, h, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            (void)hipHostFree(h);
+            return false;
+        }
+        s->desc = (StepDesc*)h;
+        s->desc_dev = (StepDesc*)d;
+    }
+    a.n = cmax;  // sizes the grids; the kernels take the batch's n from the descriptor
+    a.desc = s->desc_dev;
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    const dyg_status r = enqueue_step(s, a, false);  // no event nodes in the graph
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &graph);
+    (void)hipGetLastError();
+    if (r != DYG_OK || e != hipSuccess || !graph ||
+        hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        g.exec = nullptr;
+        if (graph) (void)hipGraphDestroy(graph);
+        return false;
+    }
+    g.graph = graph;
+    g.g = a.G;
+    g.h = a.H;
+    g.failed = false;
+    return true;
+}
+
+bool same_layout(const BlockGraph& x, const BlockGraph& y) {
+    return x.rows == y.rows && x.adj == y.adj && x.cap == y.cap && x.meta == y.meta &&
+           x.cap_slots == y.cap_slots && x.slack == y.slack && x.lpr == y.lpr &&
+           x.nodesc == y.nodesc;
+}
+
+void fill_desc(StepDesc* d, const SmallInsArgs& a) {
+    d->u = a.u;
+    d->v = a.v;
+    d->w = a.w;
+    d->rep_cls = a.rep.cls;
+    d->rep_words = a.rep.words;
+    d->base = a.base;
+    d->n = a.n;
+}
+void fill_desc(StepDesc* d, const SmallDelArgs& a) {
+    d->u = a.u;
+    d->v = a.v;
+    d->w = nullptr;
+    d->rep_cls = a.rep.cls;
+    d->rep_words = a.rep.words;
+    d->base = a.base;
+    d->n = a.n;
+}
+
+// Runs a one-workgroup step through its captured graph: the batch's values
+// go into the step descriptor, then one graph launch.  Only for callers that
+// wait for each batch before the next (the descriptor must not change under
+// a launch still in flight).  Returns false (nothing enqueued) when the
+// class has no graph.
+template <class Args>
+bool run_step_graph(dyg_stream s, bool insert, const Args& a, dyg_status* out) {
+    const int c = step_class(a.n);
+    if (c < 0) return false;
+    dyg_stream_s::StepGraph& g = s->steps[insert ? 0 : 1][c];
+    if (g.exec && !(same_layout(g.g, a.G) && same_layout(g.h, a.H))) {
+        // the arena grew (or the edit buffers or refresh lanes changed) since
+        // the capture: the nodes hold the old addresses, so capture again
+        (void)hipGraphExecDestroy(g.exec);
+        (void)hipGraphDestroy(g.graph);
+        g.exec = nullptr;
+        g.graph = nullptr;
+    }
+    if (!g.exec && (g.failed || !capture_step(s, g, a, kStepClass[c]))) return false;
+    fill_desc(s->desc, a);
+    // the walk is a node inside the graph: the timing events bracket the step
+    hipError_t e = hipEventRecord(s->ctx->ev0, s->ctx->stream);
+    if (e == hipSuccess) e = hipGraphLaunch(g.exec, s->ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(s->ctx->ev1, s->ctx->stream);
+    if (e != hipSuccess) {
+        set_error(s->ctx, std::string("stream step graph: ") + hipGetErrorString(e));
+        *out = DYG_ERR_HIP;
+    } else {
+        *out = DYG_OK;
+    }
+    return true;
+}
+
+SmallInsArgs small_ins_args(dyg_stream s, const uint32_t* du, const uint32_t* dv,
+                            const float* dw, uint32_t n, int32_t* dcls,
+                            unsigned long long* dstats, const Report& rep) {
     SmallInsArgs a{};
     a.G = block_graph(s->G);
     a.H = block_graph(s->H);
@@ -644,23 +816,33 @@ dyg_status apply_insert_small(dyg_stream s, const uint32_t* du, const uint32_t* 
     a.touched = s->ep;  // 2 * max_batch entries, free during insertion batches
     a.stats = dstats;
     a.rep = rep;
-    DYG_HIP(s->ctx, launch_small_ins_pre(a, st));
-    if (timed) DYG_HIP(s->ctx, hipEventRecord(s->ctx->ev0, st));
-    WalkLaunch w = walk_args(s, s->H, s->sp.ins, n);
-    w.stats = dstats;
-    DYG_HIP(s->ctx, launch_walk_incremental(w, st));
-    if (timed) DYG_HIP(s->ctx, hipEventRecord(s->ctx->ev1, st));
-    DYG_HIP(s->ctx, launch_small_ins_post(a, st));
-    return DYG_OK;
+    a.desc = nullptr;
+    return a;
+}
+
+// Small insertion batch: two one-workgroup kernels around the walk (as a
+// captured graph when `graph`).
+dyg_status apply_insert_small(dyg_stream s, const uint32_t* du, const uint32_t* dv,
+                              const float* dw, uint32_t n, int32_t* dcls,
+                              unsigned long long* dstats, bool timed, const Report& rep,
+                              bool graph) {
+    const SmallInsArgs a = small_ins_args(s, du, dv, dw, n, dcls, dstats, rep);
+    dyg_status r;
+    if (graph && timed && run_step_graph(s, true, a, &r)) {
+        s->last_graph = true;
+        return r;
+    }
+    return enqueue_ins_step(s, a, timed);
 }
 
 // Both apply_* enqueue a whole batch (no wait except a deletion batch's one
 // read-back of its recovery sizes) ending with the report `rep`.
 dyg_status apply_insert(dyg_stream s, const uint32_t* du, const uint32_t* dv, const float* dw,
                         uint32_t n, int32_t* dcls, unsigned long long* dstats, bool timed,
-                        dyg_stream_stats* out, const Report& rep, uint32_t* stage) {
+                        dyg_stream_stats* out, const Report& rep, uint32_t* stage,
+                        bool graph = false) {
     if (n <= std::min(kSmallBatch, s->sp.max_batch) && !(s->sp.flags & DYG_STREAM_GENERAL_ONLY))
-        return apply_insert_small(s, du, dv, dw, n, dcls, dstats, timed, rep);
+        return apply_insert_small(s, du, dv, dw, n, dcls, dstats, timed, rep, graph);
     hipStream_t st = s->ctx->stream;
     dyg_graph G = s->G, H = s->H;
     if (stage) {
@@ -726,13 +908,8 @@ dyg_status recover_general(dyg_stream s, uint32_t n, const uint32_t* hc, int32_t
     return DYG_OK;
 }
 
-// Small deletion batch: one-workgroup kernels around the walk; waits only
-// when the recovery could outgrow the post kernel (then reads its flag).
-dyg_status apply_delete_small(dyg_stream s, const uint32_t* du, const uint32_t* dv, uint32_t n,
-                              int32_t* dcls, unsigned long long* dstats, bool timed,
-                              dyg_stream_stats* out, const Report& rep) {
-    hipStream_t st = s->ctx->stream;
-    const uint32_t ms = s->sp.del.max_steps, hw = s->sp.heur_walkers;
+SmallDelArgs small_del_args(dyg_stream s, const uint32_t* du, const uint32_t* dv, uint32_t n,
+                            int32_t* dcls, unsigned long long* dstats, const Report& rep) {
     SmallDelArgs a{};
     a.G = block_graph(s->G);
     a.H = block_graph(s->H);
@@ -753,8 +930,8 @@ dyg_status apply_delete_small(dyg_stream s, const uint32_t* du, const uint32_t* 
     a.ceid = s->ceid;
     a.found = s->cfound;
     a.path = s->path;
-    a.max_steps = ms;
-    a.hw = hw;
+    a.max_steps = s->sp.del.max_steps;
+    a.hw = s->sp.heur_walkers;
     a.seed = s->sp.del.seed;
     a.pc = s->pc;
     a.poff = s->poff;
@@ -763,17 +940,44 @@ dyg_status apply_delete_small(dyg_stream s, const uint32_t* du, const uint32_t* 
     a.sizes = (uint32_t*)(s->pin_dev + kPinSizes);
     a.stats = dstats;
     a.rep = rep;
-    DYG_HIP(s->ctx, launch_small_del_pre(a, st));
-    WalkLaunch w = walk_args(s, s->G, s->sp.del, n);
-    w.w = nullptr;
-    w.path = s->path;
-    w.stats = dstats;
-    if (timed) DYG_HIP(s->ctx, hipEventRecord(s->ctx->ev0, st));
-    DYG_HIP(s->ctx, launch_walk_decremental(w, st));
-    if (timed) DYG_HIP(s->ctx, hipEventRecord(s->ctx->ev1, st));
-    DYG_HIP(s->ctx, launch_small_del_post(a, st));
+    a.desc = nullptr;
+    return a;
+}
+
+// Captures every operation's one-workgroup step in every size class the
+// stream can use (DYG_STREAM_GRAPH), at creation: a replay then costs one
+// graph launch, and only a layout change (arena growth) captures again.
+void capture_all_steps(dyg_stream s) {
+    const Report rep{nullptr, nullptr, s->G->dev.meta, s->H->dev.meta};
+    const uint32_t top = std::min(kSmallBatch, s->sp.max_batch);
+    for (int c = 0; c < dyg_stream_s::kStepClasses; ++c) {
+        if (c > 0 && kStepClass[c - 1] >= top) break;  // no batch reaches this class
+        const uint32_t cm = kStepClass[c];
+        (void)capture_step(s, s->steps[0][c],
+                           small_ins_args(s, nullptr, nullptr, nullptr, cm, s->cls, s->stats, rep),
+                           cm);
+        (void)capture_step(s, s->steps[1][c],
+                           small_del_args(s, nullptr, nullptr, cm, s->cls, s->stats, rep), cm);
+    }
+}
+
+// Small deletion batch: one-workgroup kernels around the walk (as a captured
+// graph when `graph`); waits only when the recovery could outgrow the post
+// kernel (then reads its flag).
+dyg_status apply_delete_small(dyg_stream s, const uint32_t* du, const uint32_t* dv, uint32_t n,
+                              int32_t* dcls, unsigned long long* dstats, bool timed,
+                              dyg_stream_stats* out, const Report& rep, bool graph) {
+    hipStream_t st = s->ctx->stream;
+    const uint32_t ms = s->sp.del.max_steps, hw = s->sp.heur_walkers;
+    const SmallDelArgs a = small_del_args(s, du, dv, n, dcls, dstats, rep);
+    dyg_status r = DYG_OK;
+    if (graph && timed && run_step_graph(s, false, a, &r))
+        s->last_graph = true;
+    else
+        r = enqueue_del_step(s, a, timed);
+    if (r != DYG_OK) return r;
     if ((uint64_t)n * (ms - 1) + 4ull * n * hw <= small_del_candidates(n)) return DYG_OK;
-    dyg_status r = wait_spin(s->ctx, st, s->ev_sync);
+    r = wait_spin(s->ctx, st, s->ev_sync);
     if (r != DYG_OK) return r;
     const uint32_t* hc = (const uint32_t*)((char*)s->pin + kPinSizes);
     if (hc[4] == 0) return DYG_OK;  // the post kernel finished the batch
@@ -786,9 +990,10 @@ dyg_status apply_delete_small(dyg_stream s, const uint32_t* du, const uint32_t* 
 
 dyg_status apply_delete(dyg_stream s, const uint32_t* du, const uint32_t* dv, uint32_t n,
                         int32_t* dcls, unsigned long long* dstats, bool timed,
-                        dyg_stream_stats* out, const Report& rep, uint32_t* stage) {
+                        dyg_stream_stats* out, const Report& rep, uint32_t* stage,
+                        bool graph = false) {
     if (n <= std::min(kSmallBatch, s->sp.max_batch) && !(s->sp.flags & DYG_STREAM_GENERAL_ONLY))
-        return apply_delete_small(s, du, dv, n, dcls, dstats, timed, out, rep);
+        return apply_delete_small(s, du, dv, n, dcls, dstats, timed, out, rep, graph);
     hipStream_t st = s->ctx->stream;
     dyg_graph G = s->G, H = s->H;
     const uint32_t ms = s->sp.del.max_steps, hw = s->sp.heur_walkers;
@@ -1010,6 +1215,8 @@ extern "C" dyg_status dyg_stream_create(dyg_ctx ctx, uint32_t V, const uint64_t*
         if (r == DYG_OK) r = stream_alloc(s);
         if (r == DYG_OK) r = device_edits_begin(s->G, sp->max_batch, sp->max_inserts);
         if (r == DYG_OK) r = device_edits_begin(s->H, s->cmax, sp->max_inserts);
+        if (r == DYG_OK && (sp->flags & DYG_STREAM_GRAPH) && !(sp->flags & DYG_STREAM_GENERAL_ONLY))
+            capture_all_steps(s);
     }
     if (r != DYG_OK) {
         delete s;
@@ -1073,9 +1280,13 @@ extern "C" dyg_status dyg_stream_apply(dyg_stream s, int op, const uint32_t* u, 
         const bool ins = op == DYG_STREAM_INSERT;
         if (ins) std::memcpy(hp.w, w, n * 4ull);
         const Report rep{dp.cls, dp.words, s->G->dev.meta, s->H->dev.meta};
+        // step graphs are safe here: this call waits for its batch before returning
+        const bool graph = (s->sp.flags & DYG_STREAM_GRAPH) != 0;
+        s->last_graph = false;
         dyg_status r = ins ? apply_insert(s, dp.u, dp.v, dp.w, n, s->cls, s->stats, true, &rs, rep,
-                                          s->in)
-                           : apply_delete(s, dp.u, dp.v, n, s->cls, s->stats, true, &rs, rep, s->in);
+                                          s->in, graph)
+                           : apply_delete(s, dp.u, dp.v, n, s->cls, s->stats, true, &rs, rep, s->in,
+                                          graph);
         if (r != DYG_OK) return r;
         r = wait_spin(ctx, str, s->ev_sync);
         if (r != DYG_OK) return r;
@@ -1093,6 +1304,7 @@ extern "C" dyg_status dyg_stream_apply(dyg_stream s, int op, const uint32_t* u, 
         if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess) rs.walk_ms = ms;
         rs.hops = hst[0];
         if (hst[3]) rs.candidates = hst[3];  // counted on the device (small deletion batch)
+        rs.graph = s->last_graph ? 1 : 0;
         // keep a batch's worth of arena headroom for the next batch
         r = device_edits_prepare(s->G, s->sp.max_batch, s->sp.max_batch);
         if (r == DYG_OK) r = device_edits_prepare(s->H, s->cmax, s->cmax);

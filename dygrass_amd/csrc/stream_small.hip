@@ -367,8 +367,60 @@ struct InsShared {
     uint8_t mode[kItems];            // by sorted position (heads)
 };
 
+// A captured step graph's batch values (StepDesc), read once at entry.  The
+// host rewrites the pinned descriptor and inputs before every replay, and
+// pinned memory may be cached in an XCD's L2, where the previous batch's
+// lines can outlive a replay (a node inside a graph does not get the
+// system-scope acquire a directly launched kernel gets).  So the kernel takes
+// a system-scope acquire first (L1 and L2 invalidation) and reads the
+// descriptor with vector loads (volatile: not through the scalar cache).
+struct DescVals {
+    const uint32_t* u;
+    const uint32_t* v;
+    const float* w;
+    int32_t* cls;
+    unsigned long long* words;
+    uint64_t base;
+    uint32_t n;
+};
+__device__ inline DescVals read_desc(const StepDesc* p) {
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
+    const volatile StepDesc* d = p;
+    DescVals r;
+    r.u = d->u;
+    r.v = d->v;
+    r.w = d->w;
+    r.cls = d->rep_cls;
+    r.words = d->rep_words;
+    r.base = d->base;
+    r.n = d->n;
+    return r;
+}
+__device__ inline void take_desc(SmallInsArgs& a) {
+    if (!a.desc) return;
+    const DescVals d = read_desc(a.desc);
+    a.u = d.u;
+    a.v = d.v;
+    a.w = d.w;
+    a.n = d.n;
+    a.base = d.base;
+    a.rep.cls = d.cls;
+    a.rep.words = d.words;
+}
+__device__ inline void take_desc(SmallDelArgs& a) {
+    if (!a.desc) return;
+    const DescVals d = read_desc(a.desc);
+    a.u = d.u;
+    a.v = d.v;
+    a.n = d.n;
+    a.base = d.base;
+    a.rep.cls = d.cls;
+    a.rep.words = d.words;
+}
+
 template <int IPT>
 __global__ __launch_bounds__(kThreads) void k_small_ins_pre(SmallInsArgs a) {
+    take_desc(a);
     constexpr int kIPT = IPT;
     __shared__ InsShared<IPT> sh;
     const uint32_t t = threadIdx.x;
@@ -453,6 +505,7 @@ __global__ __launch_bounds__(kThreads) void k_small_ins_pre(SmallInsArgs a) {
 
 template <int IPT>
 __global__ __launch_bounds__(kThreads) void k_small_ins_post(SmallInsArgs a) {
+    take_desc(a);
     constexpr int kIPT = IPT;
     __shared__ InsShared<IPT> sh;
     const uint32_t t = threadIdx.x;
@@ -492,6 +545,7 @@ struct DelPreShared {
 
 template <int IPT>
 __global__ __launch_bounds__(kThreads) void k_small_del_pre(SmallDelArgs a) {
+    take_desc(a);
     constexpr int kIPT = IPT;
     __shared__ DelPreShared<IPT> sh;
     const uint32_t t = threadIdx.x;
@@ -578,6 +632,7 @@ struct DelPostShared {
 
 template <int IPT>
 __global__ __launch_bounds__(kThreads) void k_small_del_post(SmallDelArgs a) {
+    take_desc(a);
     constexpr int kIPT = IPT;
     __shared__ DelPostShared<IPT> sh;
     const uint32_t t = threadIdx.x;

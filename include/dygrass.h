@@ -298,6 +298,15 @@ typedef struct {
  * when refreshing G after insertions costs more than the longer deletion
  * hops (high-degree G, insertion-heavy streams). */
 #define DYG_STREAM_DENSE_NO_DESC 4u
+/* Device form, dyg_stream_apply only: a batch of at most 2048 updates runs as
+ * a hipGraph of its one-workgroup step (pre kernel, descriptor refreshes,
+ * walk with its timing events, post kernel, refresh), captured once per
+ * operation and size class (16, 64, 256, 1024, 2048 updates) with grids for
+ * the class maximum, and replayed with the batch's arguments set on its pre
+ * and post kernel nodes.  A class whose capture the runtime refuses launches
+ * directly.  Same results.  (No reference counterpart: the reference launches
+ * each kernel on the default stream.) */
+#define DYG_STREAM_GRAPH 8u
 typedef struct {
     uint64_t updates, weight_updates, found, added, skipped, g_only, recovered, heuristic,
         invalid;
@@ -305,6 +314,7 @@ typedef struct {
     uint64_t hops;        /* neighbour draws of the batch's walks */
     double walk_ms;       /* walk kernel (HIP events) */
     double total_ms;      /* whole apply() call, host clock */
+    uint64_t graph;       /* 1 when the batch ran as a captured step graph */
 } dyg_stream_stats;
 /* G and H in CSR (slot order as given), uploaded and owned by the stream. */
 dyg_status dyg_stream_create(dyg_ctx ctx, uint32_t V, const uint64_t* g_row_ptr,

@@ -56,20 +56,23 @@ def _assert_same_graph(dev_graph, rows):
     assert dev_graph.check() == 0
 
 
-@pytest.mark.parametrize("seed,plain,general,nodesc",
-                         [(0, False, False, False), (1, False, False, False),
-                          (0, True, False, False), (2, False, True, False),
-                          (3, False, True, True), (4, False, False, True)])
-def test_stream_matches_oracle(ctx, grid_wl, seed, plain, general, nodesc):
-    """Device-resident form (batches through the one-workgroup path, or
-    everything through the general pipeline; G with or without neighbour
-    descriptors) and the plain form."""
+@pytest.mark.parametrize("seed,plain,general,nodesc,graph",
+                         [(0, False, False, False, False), (1, False, False, False, False),
+                          (0, True, False, False, False), (2, False, True, False, False),
+                          (3, False, True, True, False), (4, False, False, True, False),
+                          (6, False, False, False, False), (6, False, False, False, True),
+                          (7, False, False, True, True)])
+def test_stream_matches_oracle(ctx, grid_wl, seed, plain, general, nodesc, graph):
+    """Device-resident form (batches through the one-workgroup path, directly
+    or as captured step graphs, or everything through the general pipeline;
+    G with or without neighbour descriptors) and the plain form."""
     from dygrass_amd import Stream, stream_params
     from oracle import stream as S
 
     wl = grid_wl
     sp = stream_params(w_ins=64, w_del=64, threshold=10.0, seed=seed, max_batch=1100,
-                       max_inserts=4096, plain=plain, general_only=general, dense_no_desc=nodesc)
+                       max_inserts=4096, plain=plain, general_only=general, dense_no_desc=nodesc,
+                       graph=graph)
     st = Stream(ctx, wl.G.V, (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w), sp)
     orc = S.Stream(wl.G.V, (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w), thr=10.0,
                    w_ins=64, w_del=64, seed=seed)
@@ -85,6 +88,54 @@ def test_stream_matches_oracle(ctx, grid_wl, seed, plain, general, nodesc):
     assert st.position == orc.base
     # the batches exercise every outcome
     assert seen >= {S.WEIGHT, S.FOUND, S.ADDED, S.SKIP, S.G_ONLY, S.RECOVERED, S.INVALID}
+
+
+@pytest.mark.parametrize("nodesc", [False, True])
+def test_step_graphs_replay_like_direct_launches(ctx, grid_wl, nodesc):
+    """Captured step graphs (DYG_STREAM_GRAPH) against direct launches on the
+    same batches: sizes in every size class and at its edges (1, 16, 17, 64,
+    65, 256, 257, 1024, 1025, 2048) and beyond the one-workgroup limit, each
+    class replayed several times with eager batches and waits in between;
+    every batch's outcome codes and walk statistics, and G and H after every
+    batch, must be equal, and the captured walk timing events must fire."""
+    from dygrass_amd import Stream, stream_params
+
+    wl = grid_wl
+    rng = np.random.default_rng(11)
+    G = wl.G
+    sizes = [1, 16, 17, 64, 65, 256, 257, 1024, 1025, 2048, 3000, 1, 5, 16, 300, 2048, 40, 1]
+    batches, k = [], 0
+    for i, n in enumerate(sizes):
+        if i % 3 == 2:  # deletions: G edges, sometimes H edges
+            idx = rng.choice(len(G), n, replace=False)
+            batches.append(("del", G.u[idx], G.v[idx], None))
+        else:
+            a = rng.integers(0, G.V, n)
+            b = rng.integers(0, G.V, n)
+            batches.append(("ins", a, b, rng.uniform(0.1, 10.0, n).astype(np.float32)))
+            k += n
+    streams = []
+    for graph in (False, True):
+        sp = stream_params(w_ins=64, w_del=32, threshold=10.0, seed=4, max_batch=3000,
+                           max_inserts=k + 16, dense_no_desc=nodesc, graph=graph,
+                           steps_del=40)
+        streams.append(Stream(ctx, wl.G.V, (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w),
+                              sp))
+    direct, graphed = streams
+    for op, u, v, w in batches:
+        o1, s1 = direct.apply(op, u, v, w)
+        o2, s2 = graphed.apply(op, u, v, w)
+        assert np.array_equal(o1, o2), (op, len(u))
+        for key in ("found", "added", "recovered", "heuristic", "hops", "candidates"):
+            assert s1[key] == s2[key], (op, len(u), key)
+        assert s2["walk_ms"] > 0.0 or s2["hops"] == 0
+        assert s1["graph"] == 0 and s2["graph"] == (1 if len(u) <= 2048 else 0), (op, len(u))
+        for g1, g2 in ((direct.G, graphed.G), (direct.H, graphed.H)):
+            a1, a2 = g1.download(), g2.download()
+            assert all(np.array_equal(x.view(np.uint32) if x.dtype == np.float32 else x,
+                                      y.view(np.uint32) if y.dtype == np.float32 else y)
+                       for x, y in zip(a1, a2))
+        assert graphed.H.check() == 0 and graphed.G.check() == 0
 
 
 @pytest.mark.parametrize("general", [False, True])
