@@ -25,6 +25,7 @@ vp = C.c_void_p
 DYG_WALK_NO_PRUNE = 1
 DYG_WALK_PER_EDGE = 2
 DYG_RUN_HOST_EDITS = 4
+DYG_WALK_DECISION_ONLY = 8
 
 STATUS = {
     0: "DYG_OK", -1: "DYG_ERR_ARG", -2: "DYG_ERR_HIP", -3: "DYG_ERR_NOMEM", -4: "DYG_ERR_IO",

@@ -42,13 +42,16 @@ class WalkResult:
 
 def params(walkers: int, max_steps: int = 100, threshold: float = 0.0, seed: int = 0,
            prune: bool = True, per_edge: bool = False, host_edits: bool = False,
-           block_edges: int = 0) -> L.WalkParams:
+           block_edges: int = 0, decision_only: bool = False) -> L.WalkParams:
     """Walk parameters.  per_edge: one workgroup per edge (the reference's
     launch shape) instead of the work-queue kernel; host_edits: plan each
     batch's appends on the host (dyg_incremental_update only); block_edges:
-    edges per workgroup of the work-queue kernel (0 = automatic)."""
+    edges per workgroup of the work-queue kernel (0 = automatic);
+    decision_only: insertion walks stop once any walker found the target
+    (found / not found exact, found_at and cond from some finder)."""
     flags = ((0 if prune else L.DYG_WALK_NO_PRUNE) | (L.DYG_WALK_PER_EDGE if per_edge else 0)
-             | (L.DYG_RUN_HOST_EDITS if host_edits else 0))
+             | (L.DYG_RUN_HOST_EDITS if host_edits else 0)
+             | (L.DYG_WALK_DECISION_ONLY if decision_only else 0))
     return L.WalkParams(walkers, max_steps, threshold, flags, seed, block_edges, 0)
 
 

@@ -657,6 +657,7 @@ dyg_status enqueue_ins_step(dyg_stream s, const SmallInsArgs& a, bool timed) {
     if (timed) DYG_HIP(s->ctx, hipEventRecord(s->ctx->ev0, st));
     WalkLaunch w = walk_args(s, s->H, s->sp.ins, a.n);
     w.stats = a.stats;
+    w.flags |= DYG_WALK_DECISION_ONLY;  // outcomes need found / not found only
     DYG_HIP(s->ctx, launch_walk_incremental(w, st));
     if (timed) DYG_HIP(s->ctx, hipEventRecord(s->ctx->ev1, st));
     DYG_HIP(s->ctx, launch_small_ins_post(a, st));
@@ -868,6 +869,7 @@ dyg_status apply_insert(dyg_stream s, const uint32_t* du, const uint32_t* dv, co
     if (timed) DYG_HIP(s->ctx, hipEventRecord(s->ctx->ev0, st));
     WalkLaunch a = walk_args(s, H, s->sp.ins, n);
     a.stats = dstats;
+    a.flags |= DYG_WALK_DECISION_ONLY;  // outcomes need found / not found only
     DYG_HIP(s->ctx, launch_walk_incremental(a, st));
     if (timed) DYG_HIP(s->ctx, hipEventRecord(s->ctx->ev1, st));
     r = device_edits_batch(H, s->cu, s->cv, s->cw, s->cfound, n, s->cnt);

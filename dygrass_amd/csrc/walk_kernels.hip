@@ -334,6 +334,7 @@ __global__ __launch_bounds__(kPoolThreads) void k_walk_pool(WalkLaunch a, uint32
     __shared__ uint32_t s_cnt[3];
 
     const uint32_t n = a.n_dev ? min(*a.n_dev, a.n) : a.n;  // list length (device or host)
+    const bool decide = INSERT && (a.flags & DYG_WALK_DECISION_ONLY) != 0u;
     const uint32_t e0 = blockIdx.x * E;
     if (e0 >= n) return;  // uniform per workgroup
     const uint32_t ne = min(E, n - e0);
@@ -452,10 +453,13 @@ __global__ __launch_bounds__(kPoolThreads) void k_walk_pool(WalkLaunch a, uint32
             } else {
                 if (PRUNE && !first) {
                     // conservative: prune only when fl(1/R) < c is certain;
-                    // rcp is within 1 ulp of 1/R, the margin covers 2^-20
+                    // rcp is within 1 ulp of 1/R, the margin covers 2^-20.
+                    // Decision only: any walker having found the target ends
+                    // the edge's walk.
                     const uint32_t bb = (uint32_t)(s_key[L.edge] >> 32);
                     if (bb != 0u &&
-                        __builtin_amdgcn_rcpf(L.R) * kPruneSlack < __uint_as_float(bb - 1u))
+                        (decide ||
+                         __builtin_amdgcn_rcpf(L.R) * kPruneSlack < __uint_as_float(bb - 1u)))
                         done = true;
                 }
                 L.prev = L.cur;

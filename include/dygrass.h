@@ -121,6 +121,13 @@ dyg_status dyg_graph_check(dyg_graph g, uint64_t* bad_slots);
  * batches follow each other on the stream with no host round trip.  Same
  * resulting graph either way (slot order per row). */
 #define DYG_RUN_HOST_EDITS 4u
+/* Insertion walks, work-queue kernel: once one walker of an edge has found
+ * its target, the edge's other walkers stop.  The decision (found or not,
+ * which is all random_walk_incremental/gpu_graph.cuh:302 reads) is exactly
+ * the reference's; found_at and cond then describe a walker that found the
+ * target, not necessarily the reference's winner.  The stream forms always
+ * use it (they report outcome codes only). */
+#define DYG_WALK_DECISION_ONLY 8u
 
 typedef struct {
     uint32_t walkers;   /* walkers per edge, power of two in [64,1024] (argv[3]; 512 in decremental) */

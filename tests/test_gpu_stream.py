@@ -126,7 +126,8 @@ def test_step_graphs_replay_like_direct_launches(ctx, grid_wl, nodesc):
         o1, s1 = direct.apply(op, u, v, w)
         o2, s2 = graphed.apply(op, u, v, w)
         assert np.array_equal(o1, o2), (op, len(u))
-        for key in ("found", "added", "recovered", "heuristic", "hops", "candidates"):
+        # (hop counts are work, not results: pruning depends on lane timing)
+        for key in ("found", "added", "recovered", "heuristic", "candidates"):
             assert s1[key] == s2[key], (op, len(u), key)
         assert s2["walk_ms"] > 0.0 or s2["hops"] == 0
         assert s1["graph"] == 0 and s2["graph"] == (1 if len(u) <= 2048 else 0), (op, len(u))

@@ -50,6 +50,31 @@ def test_incremental_walks_match_oracle_grid(ctx, grid_wl, prune, per_edge):
         assert got.stats["hops"] <= int(hops.sum())
 
 
+@pytest.mark.parametrize("W", [8, 64, 512])
+def test_decision_only_walks(ctx, grid_wl, W):
+    """DYG_WALK_DECISION_ONLY: found / not found equals the oracle's for every
+    edge; a found edge reports a walker that really found the target within
+    the bound, never with a larger conductance than the reference's winner
+    (the winner maximises it), and the walks do no more hops than pruning
+    alone."""
+    from dygrass_amd import DeviceGraph, params, walk_incremental
+
+    po = _oracle()
+    rp, col, w = csr(grid_wl.H)
+    H = DeviceGraph(ctx, grid_wl.H.V, rp, col, w)
+    s, t, we = _first_batch(grid_wl, 600)
+    got = walk_incremental(H, s, t, we, params(W, 100, 10.0, seed=3, decision_only=True))
+    full = walk_incremental(H, s, t, we, params(W, 100, 10.0, seed=3))
+    og = po.Graph(grid_wl.H.V, rp, col, w)
+    f, c, _, _ = og.walk_inc_batch(s, t, we, 10.0, W, seed=3, threads=8)
+    assert np.array_equal(got.found_at == -1, f == -1)
+    hit = f != -1
+    assert np.all((got.found_at[hit] >= 1) & (got.found_at[hit] <= 99))
+    assert np.all(got.cond[hit] > 0) and np.all(got.cond[hit] <= c[hit])
+    assert np.all(got.cond[~hit] == -1.0)
+    assert got.stats["hops"] <= full.stats["hops"]
+
+
 @pytest.mark.parametrize("per_edge", [False, True])
 @pytest.mark.parametrize("W", [1, 2, 8, 32, 64, 128, 256, 1024])
 def test_incremental_walker_counts(ctx, grid_wl, W, per_edge):
