@@ -449,6 +449,22 @@ class Stream:
               self.ctx.handle)
         return out[:n], {k: getattr(st, k) for k, _ in L.StreamStats._fields_}
 
+    def apply_dev(self, op: str, u, v, w=None, n: int | None = None) -> tuple[np.ndarray, dict]:
+        """One batch whose inputs are already in device memory: u, v (and w for
+        insertions) are DeviceArray objects or raw device addresses; n defaults
+        to len(u).  Same results as apply()."""
+        du = u.ptr if isinstance(u, DeviceArray) else int(u)
+        dv = v.ptr if isinstance(v, DeviceArray) else int(v)
+        dw = (w.ptr if isinstance(w, DeviceArray) else int(w)) if w is not None else None
+        if n is None:
+            n = u.n if isinstance(u, DeviceArray) else 0
+        out = np.empty(max(n, 1), np.int32)
+        st = L.StreamStats()
+        code = L.DYG_STREAM_INSERT if op == "ins" else L.DYG_STREAM_DELETE
+        check(self._lib.dyg_stream_apply_dev(self.handle, code, du, dv, dw, n, L.addr(out),
+                                             C.byref(st)), self.ctx.handle)
+        return out[:n], {k: getattr(st, k) for k, _ in L.StreamStats._fields_}
+
     def close(self) -> None:
         if self.handle:
             self._lib.dyg_stream_free(self.handle)

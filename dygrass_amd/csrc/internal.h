@@ -266,6 +266,10 @@ struct BlockGraph {
 // [0..3] the walk counters (zeroed once read, ready for the next batch),
 // [4..8] G's and [9..13] H's DevEdits::meta.
 constexpr int kReportWords = 14;
+// One more pinned word after them, cleared by the host before a batch: set
+// when a batch of device inputs held a vertex id out of range (the batch then
+// changed nothing).
+constexpr int kReportBad = 14;
 struct Report {
     int32_t* cls;                  // [n] outcome codes
     unsigned long long* words;     // [kReportWords]
@@ -316,6 +320,8 @@ struct SmallInsArgs {
     unsigned long long* stats;  // walk counters, reported and zeroed by the post kernel
     Report rep;
     const StepDesc* desc;  // when set: u, v, w, n, base and rep's outputs come from here
+    uint32_t vcheck;       // nonzero (device inputs): ids must be below it, checked by the
+                           // pre kernel before any change; a failing batch changes nothing
 };
 hipError_t launch_small_ins_pre(const SmallInsArgs& a, hipStream_t s);
 hipError_t launch_small_ins_post(const SmallInsArgs& a, hipStream_t s);
@@ -356,6 +362,7 @@ struct SmallDelArgs {
     unsigned long long* stats;
     Report rep;
     const StepDesc* desc;  // when set: u, v, n, base and rep's outputs come from here
+    uint32_t vcheck;       // as SmallInsArgs::vcheck
 };
 // Candidate edges the post kernel of an n-update deletion batch can hold.
 uint32_t small_del_candidates(uint32_t n);

@@ -73,6 +73,14 @@ __device__ __forceinline__ bool has_edge(const uint2* __restrict__ rows,
 
 // Also clears bcnt[block] (when given): the per-block counts a later
 // k_compact over the same n reads.
+// Range check of a batch of device inputs (larger than the one-workgroup
+// path, which checks inside its pre kernel): any id >= V sets *bad.
+__global__ void k_check_batch(const uint32_t* __restrict__ u, const uint32_t* __restrict__ v,
+                              uint32_t n, uint32_t V, unsigned long long* bad) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < n && (u[i] >= V || v[i] >= V)) *bad = 1ull;
+}
+
 __global__ void k_batch_keys(const uint32_t* __restrict__ u, const uint32_t* __restrict__ v,
                              uint32_t n, unsigned long long* __restrict__ key,
                              uint32_t* __restrict__ idx, uint32_t* __restrict__ bcnt) {
@@ -826,8 +834,9 @@ SmallInsArgs small_ins_args(dyg_stream s, const uint32_t* du, const uint32_t* dv
 dyg_status apply_insert_small(dyg_stream s, const uint32_t* du, const uint32_t* dv,
                               const float* dw, uint32_t n, int32_t* dcls,
                               unsigned long long* dstats, bool timed, const Report& rep,
-                              bool graph) {
-    const SmallInsArgs a = small_ins_args(s, du, dv, dw, n, dcls, dstats, rep);
+                              bool graph, uint32_t vcheck) {
+    SmallInsArgs a = small_ins_args(s, du, dv, dw, n, dcls, dstats, rep);
+    a.vcheck = vcheck;
     dyg_status r;
     if (graph && timed && run_step_graph(s, true, a, &r)) {
         s->last_graph = true;
@@ -841,9 +850,9 @@ dyg_status apply_insert_small(dyg_stream s, const uint32_t* du, const uint32_t* 
 dyg_status apply_insert(dyg_stream s, const uint32_t* du, const uint32_t* dv, const float* dw,
                         uint32_t n, int32_t* dcls, unsigned long long* dstats, bool timed,
                         dyg_stream_stats* out, const Report& rep, uint32_t* stage,
-                        bool graph = false) {
+                        bool graph = false, uint32_t vcheck = 0) {
     if (n <= std::min(kSmallBatch, s->sp.max_batch) && !(s->sp.flags & DYG_STREAM_GENERAL_ONLY))
-        return apply_insert_small(s, du, dv, dw, n, dcls, dstats, timed, rep, graph);
+        return apply_insert_small(s, du, dv, dw, n, dcls, dstats, timed, rep, graph, vcheck);
     hipStream_t st = s->ctx->stream;
     dyg_graph G = s->G, H = s->H;
     if (stage) {
@@ -968,10 +977,12 @@ void capture_all_steps(dyg_stream s) {
 // kernel (then reads its flag).
 dyg_status apply_delete_small(dyg_stream s, const uint32_t* du, const uint32_t* dv, uint32_t n,
                               int32_t* dcls, unsigned long long* dstats, bool timed,
-                              dyg_stream_stats* out, const Report& rep, bool graph) {
+                              dyg_stream_stats* out, const Report& rep, bool graph,
+                              uint32_t vcheck) {
     hipStream_t st = s->ctx->stream;
     const uint32_t ms = s->sp.del.max_steps, hw = s->sp.heur_walkers;
-    const SmallDelArgs a = small_del_args(s, du, dv, n, dcls, dstats, rep);
+    SmallDelArgs a = small_del_args(s, du, dv, n, dcls, dstats, rep);
+    a.vcheck = vcheck;
     dyg_status r = DYG_OK;
     if (graph && timed && run_step_graph(s, false, a, &r))
         s->last_graph = true;
@@ -982,6 +993,7 @@ dyg_status apply_delete_small(dyg_stream s, const uint32_t* du, const uint32_t* 
     r = wait_spin(s->ctx, st, s->ev_sync);
     if (r != DYG_OK) return r;
     const uint32_t* hc = (const uint32_t*)((char*)s->pin + kPinSizes);
+    if (((const unsigned long long*)s->pin)[kReportBad]) return DYG_OK;  // rejected: nothing to do
     if (hc[4] == 0) return DYG_OK;  // the post kernel finished the batch
     DYG_HIP(s->ctx, launch_heuristic(s->G->d_rows, s->G->d_adj, s->ep, s->ekey, 2 * n, hw,
                                      s->sp.del.seed, s->path3, st, s->cnt + 2, s->G->nodesc));
@@ -993,9 +1005,9 @@ dyg_status apply_delete_small(dyg_stream s, const uint32_t* du, const uint32_t* 
 dyg_status apply_delete(dyg_stream s, const uint32_t* du, const uint32_t* dv, uint32_t n,
                         int32_t* dcls, unsigned long long* dstats, bool timed,
                         dyg_stream_stats* out, const Report& rep, uint32_t* stage,
-                        bool graph = false) {
+                        bool graph = false, uint32_t vcheck = 0) {
     if (n <= std::min(kSmallBatch, s->sp.max_batch) && !(s->sp.flags & DYG_STREAM_GENERAL_ONLY))
-        return apply_delete_small(s, du, dv, n, dcls, dstats, timed, out, rep, graph);
+        return apply_delete_small(s, du, dv, n, dcls, dstats, timed, out, rep, graph, vcheck);
     hipStream_t st = s->ctx->stream;
     dyg_graph G = s->G, H = s->H;
     const uint32_t ms = s->sp.del.max_steps, hw = s->sp.heur_walkers;
@@ -1228,9 +1240,13 @@ extern "C" dyg_status dyg_stream_create(dyg_ctx ctx, uint32_t V, const uint64_t*
     return DYG_OK;
 }
 
-extern "C" dyg_status dyg_stream_apply(dyg_stream s, int op, const uint32_t* u, const uint32_t* v,
-                                       const float* w, uint32_t n, int32_t* outcome,
-                                       dyg_stream_stats* st) {
+namespace {
+
+// One batch through the stream; u, v, w are host arrays, or device pointers
+// when `dev` (then the device checks the ids before anything changes).
+dyg_status stream_apply(dyg_stream s, int op, const uint32_t* u, const uint32_t* v,
+                        const float* w, uint32_t n, int32_t* outcome, dyg_stream_stats* st,
+                        bool dev) {
     if (!s) return DYG_ERR_ARG;
     dyg_ctx ctx = s->ctx;
     if ((op != DYG_STREAM_INSERT && op != DYG_STREAM_DELETE) || (n && (!u || !v)) ||
@@ -1242,11 +1258,24 @@ extern "C" dyg_status dyg_stream_apply(dyg_stream s, int op, const uint32_t* u, 
         set_error(ctx, "stream apply: batch larger than max_batch");
         return DYG_ERR_ARG;
     }
+    const bool plain = (s->sp.flags & DYG_STREAM_PLAIN) != 0;
+    if (dev && plain) {
+        // the plain form plans on the host: bring the batch there and go on as usual
+        std::vector<uint32_t> hu(n), hv(n);
+        std::vector<float> hw(op == DYG_STREAM_INSERT ? n : 0);
+        if (n) {
+            DYG_HIP(ctx, hipMemcpy(hu.data(), u, n * 4ull, hipMemcpyDeviceToHost));
+            DYG_HIP(ctx, hipMemcpy(hv.data(), v, n * 4ull, hipMemcpyDeviceToHost));
+            if (op == DYG_STREAM_INSERT)
+                DYG_HIP(ctx, hipMemcpy(hw.data(), w, n * 4ull, hipMemcpyDeviceToHost));
+        }
+        return stream_apply(s, op, hu.data(), hv.data(), hw.empty() ? nullptr : hw.data(), n,
+                            outcome, st, false);
+    }
     const double t0 = now_ms();
     dyg_stream_stats rs{};
     rs.updates = n;
-    const bool plain = (s->sp.flags & DYG_STREAM_PLAIN) != 0;
-    if (n) {
+    if (n && !dev) {
         // range check, fused with the copy into pinned staging (device form)
         uint32_t mx = 0;
         if (plain) {
@@ -1273,25 +1302,50 @@ extern "C" dyg_status dyg_stream_apply(dyg_stream s, int op, const uint32_t* u, 
         if (r != DYG_OK) return r;
         DYG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     } else if (n) {
-        // pinned staging: the device reads the inputs from pinned memory (a
+        // host inputs: the device reads them from pinned staging (a
         // one-workgroup batch in place, a larger one through its first
-        // kernel, which stages them in device memory), the last kernel writes
-        // the results back, and the batch has one wait
+        // kernel, which stages them in device memory); device inputs are
+        // read where they are.  The last kernel writes the results back into
+        // pinned memory, and the batch has one wait.
         hipStream_t str = ctx->stream;
         const PinLayout hp = pin_layout((char*)s->pin, n), dp = pin_layout(s->pin_dev, n);
         const bool ins = op == DYG_STREAM_INSERT;
-        if (ins) std::memcpy(hp.w, w, n * 4ull);
+        const bool small =
+            n <= std::min(kSmallBatch, s->sp.max_batch) && !(s->sp.flags & DYG_STREAM_GENERAL_ONLY);
+        hp.words[kReportBad] = 0;
+        if (dev && !small) {
+            // a larger batch of device inputs: its ids are checked before it starts
+            hipLaunchKernelGGL(k_check_batch, dim3((n + 255) / 256), dim3(256), 0, str, u, v, n,
+                               s->G->V, dp.words + kReportBad);
+            dyg_status r = wait_spin(ctx, str, s->ev_sync);
+            if (r != DYG_OK) return r;
+            if (hp.words[kReportBad]) {
+                set_error(ctx, "stream apply: vertex id out of range");
+                return DYG_ERR_ARG;
+            }
+        }
+        if (ins && !dev) std::memcpy(hp.w, w, n * 4ull);
         const Report rep{dp.cls, dp.words, s->G->dev.meta, s->H->dev.meta};
-        // step graphs are safe here: this call waits for its batch before returning
-        const bool graph = (s->sp.flags & DYG_STREAM_GRAPH) != 0;
+        // step graphs are safe here: this call waits for its batch before
+        // returning (host inputs only: a graph's kernels take no id check)
+        const bool graph = (s->sp.flags & DYG_STREAM_GRAPH) != 0 && !dev;
+        const uint32_t vcheck = dev ? s->G->V : 0u;
+        const uint32_t* bu = dev ? u : dp.u;
+        const uint32_t* bv = dev ? v : dp.v;
+        const float* bw = dev ? w : dp.w;
+        uint32_t* stage = dev ? nullptr : s->in;
         s->last_graph = false;
-        dyg_status r = ins ? apply_insert(s, dp.u, dp.v, dp.w, n, s->cls, s->stats, true, &rs, rep,
-                                          s->in, graph)
-                           : apply_delete(s, dp.u, dp.v, n, s->cls, s->stats, true, &rs, rep, s->in,
-                                          graph);
+        dyg_status r = ins ? apply_insert(s, bu, bv, bw, n, s->cls, s->stats, true, &rs, rep,
+                                          stage, graph, vcheck)
+                           : apply_delete(s, bu, bv, n, s->cls, s->stats, true, &rs, rep, stage,
+                                          graph, vcheck);
         if (r != DYG_OK) return r;
         r = wait_spin(ctx, str, s->ev_sync);
         if (r != DYG_OK) return r;
+        if (hp.words[kReportBad]) {  // rejected by its pre kernel: nothing changed
+            set_error(ctx, "stream apply: vertex id out of range");
+            return DYG_ERR_ARG;
+        }
         const unsigned long long* hst = hp.words;
         r = device_edits_absorb(s->G, hp.words + 4, nullptr);
         if (r == DYG_OK) r = device_edits_absorb(s->H, hp.words + 9, nullptr);
@@ -1329,6 +1383,20 @@ extern "C" dyg_status dyg_stream_apply(dyg_stream s, int op, const uint32_t* u, 
     rs.total_ms = now_ms() - t0;
     if (st) *st = rs;
     return DYG_OK;
+}
+
+}  // namespace
+
+extern "C" dyg_status dyg_stream_apply(dyg_stream s, int op, const uint32_t* u, const uint32_t* v,
+                                       const float* w, uint32_t n, int32_t* outcome,
+                                       dyg_stream_stats* st) {
+    return stream_apply(s, op, u, v, w, n, outcome, st, false);
+}
+
+extern "C" dyg_status dyg_stream_apply_dev(dyg_stream s, int op, const uint32_t* u,
+                                           const uint32_t* v, const float* w, uint32_t n,
+                                           int32_t* outcome, dyg_stream_stats* st) {
+    return stream_apply(s, op, u, v, w, n, outcome, st, true);
 }
 
 extern "C" dyg_graph dyg_stream_graph(dyg_stream s, int which) {

@@ -418,6 +418,28 @@ __device__ inline void take_desc(SmallDelArgs& a) {
     a.rep.words = d.words;
 }
 
+// Device inputs (vcheck != 0): a batch holding an id >= vcheck is rejected
+// before it changes anything: nothing kept for the walk, no touched rows for
+// the refreshes, and the report's flag word set.  Uniform across the block.
+template <int IPT, class Args>
+__device__ bool reject_batch(const uint32_t (&ut)[IPT], const uint32_t (&vt)[IPT], uint32_t n,
+                             const Args& a, uint32_t t) {
+    bool bad = false;
+#pragma unroll
+    for (int k = 0; k < IPT; ++k) {
+        const uint32_t i = IPT * t + k;
+        if (i < n && (ut[k] >= a.vcheck || vt[k] >= a.vcheck)) bad = true;
+    }
+    if (!__syncthreads_or(bad)) return false;
+    if (t == 0) {
+        a.cnt[0] = 0;  // kept edges (the walk's list length)
+        a.cnt[4] = 0;  // rows listed for the refreshes
+        a.cnt[5] = 0;
+        a.rep.words[kReportBad] = 1ull;
+    }
+    return true;
+}
+
 template <int IPT>
 __global__ __launch_bounds__(kThreads) void k_small_ins_pre(SmallInsArgs a) {
     take_desc(a);
@@ -439,6 +461,7 @@ __global__ __launch_bounds__(kThreads) void k_small_ins_pre(SmallInsArgs a) {
         sh.keep[i] = 0;
         sh.mode[i] = 0;
     }
+    if (a.vcheck && reject_batch<IPT>(ut, vt, n, a, t)) return;
     sort_keys64<IPT>(sh.tmp, sh.key, sh.idx, kt, t, n, a.key_bits);
     // ---- classify each distinct edge, occurrences in batch order ----
 #pragma unroll
@@ -561,6 +584,7 @@ __global__ __launch_bounds__(kThreads) void k_small_del_pre(SmallDelArgs a) {
         sh.fg[i] = 0;
         sh.fh[i] = 0;
     }
+    if (a.vcheck && reject_batch<IPT>(ut, vt, n, a, t)) return;
     sort_keys64<IPT>(sh.tmp, sh.key, sh.idx, kt, t, n, a.key_bits);
     // ---- classify: the first occurrence of an edge G holds is deleted ----
 #pragma unroll

@@ -332,6 +332,16 @@ dyg_status dyg_stream_create(dyg_ctx ctx, uint32_t V, const uint64_t* g_row_ptr,
  * (optional) receives DYG_OUT_* of update i.  Returns when the batch is done. */
 dyg_status dyg_stream_apply(dyg_stream s, int op, const uint32_t* u, const uint32_t* v,
                             const float* w, uint32_t n, int32_t* outcome, dyg_stream_stats* st);
+/* Same, with the batch already in device memory: u, v, w are device pointers
+ * (w ignored for deletions); outcome stays a host array.  The device checks
+ * the ids (a batch of at most 2048 updates inside its first kernel, a larger
+ * one before it starts); a batch holding an id >= V fails with DYG_ERR_ARG
+ * and changes nothing.  No reference counterpart: the reference reads its
+ * batches from files into host memory (random_walk_incremental/functions.cpp:
+ * 261-370) and copies them per batch (gpu_graph.cuh:395-413). */
+dyg_status dyg_stream_apply_dev(dyg_stream s, int op, const uint32_t* u, const uint32_t* v,
+                                const float* w, uint32_t n, int32_t* outcome,
+                                dyg_stream_stats* st);
 /* 0: G, 1: H (owned by the stream; use with dyg_graph_info/download/check). */
 dyg_graph dyg_stream_graph(dyg_stream s, int which);
 /* Updates applied so far (the edge id of the next update). */

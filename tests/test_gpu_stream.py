@@ -278,3 +278,88 @@ def test_batches_up_to_the_one_workgroup_limit(ctx, grid_wl, nodesc):
         assert np.array_equal(got, want), (op, len(u), np.flatnonzero(got != want)[:10])
         _assert_same_graph(st.G, orc.G)
         _assert_same_graph(st.H, orc.H)
+
+
+def _dev(ctx, a, dtype):
+    from dygrass_amd import DeviceArray
+
+    return DeviceArray.from_numpy(ctx, np.ascontiguousarray(np.asarray(a, dtype)))
+
+
+@pytest.mark.parametrize("nodesc", [False, True])
+def test_apply_dev_matches_host_inputs(ctx, grid_wl, nodesc):
+    """dyg_stream_apply_dev (batch already in device memory) against
+    dyg_stream_apply (host arrays) on the same mixed batches, one-workgroup
+    and general sizes: outcome codes, G and H equal after every batch."""
+    from dygrass_amd import Stream, stream_params
+
+    wl = grid_wl
+    batches = _mixed_batches(wl, 8, n_ins=300)
+    rng = np.random.default_rng(8)
+    big = rng.choice(len(wl.G), 2600, replace=False)
+    batches.append(("del", wl.G.u[big], wl.G.v[big], None))
+    a = rng.integers(0, wl.G.V, 2500)
+    b = rng.integers(0, wl.G.V, 2500)
+    batches.append(("ins", a, b, rng.uniform(0.1, 10, 2500).astype(np.float32)))
+    mk = lambda: Stream(ctx, wl.G.V, (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w),
+                        stream_params(w_ins=64, w_del=64, threshold=10.0, seed=8, max_batch=3000,
+                                      max_inserts=8192, dense_no_desc=nodesc))
+    host, dev = mk(), mk()
+    for op, u, v, w in batches:
+        o1, _ = host.apply(op, u, v, w)
+        du, dv = _dev(ctx, u, np.uint32), _dev(ctx, v, np.uint32)
+        dw = _dev(ctx, w, np.float32) if w is not None else None
+        o2, _ = dev.apply_dev(op, du, dv, dw)
+        assert np.array_equal(o1, o2), (op, len(u))
+        _assert_same_graph(dev.G, _rows_of(host.G))
+        _assert_same_graph(dev.H, _rows_of(host.H))
+    assert dev.position == host.position
+
+
+class _rows_of:
+    """A device graph's download in the form _assert_same_graph expects."""
+
+    def __init__(self, g):
+        self._csr = g.download()
+
+    def csr(self):
+        return self._csr
+
+
+@pytest.mark.parametrize("n", [10, 2500])
+@pytest.mark.parametrize("op", ["ins", "del"])
+def test_apply_dev_rejects_out_of_range_ids(ctx, grid_wl, n, op):
+    """A device batch holding an id >= V fails with DYG_ERR_ARG and changes
+    nothing (G, H, position); the next valid batch then gives what a stream
+    that never saw the bad batch gives."""
+    from dygrass_amd import DygError, Stream, stream_params
+
+    wl = grid_wl
+    rng = np.random.default_rng(n)
+    mk = lambda: Stream(ctx, wl.G.V, (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w),
+                        stream_params(w_ins=32, w_del=32, threshold=10.0, seed=2, max_batch=3000,
+                                      max_inserts=8192))
+    st, ref = mk(), mk()
+    if op == "ins":
+        u = rng.integers(0, wl.G.V, n)
+        v = rng.integers(0, wl.G.V, n)
+        w = rng.uniform(0.1, 10, n).astype(np.float32)
+    else:
+        idx = rng.choice(len(wl.G), n, replace=False)
+        u, v, w = wl.G.u[idx], wl.G.v[idx], None
+    bad_u = np.array(u, np.uint32)
+    bad_u[n // 2] = wl.G.V  # one id out of range
+    before = (st.G.download(), st.H.download())
+    with pytest.raises(DygError, match="out of range"):
+        st.apply_dev(op, _dev(ctx, bad_u, np.uint32), _dev(ctx, v, np.uint32),
+                     _dev(ctx, w, np.float32) if w is not None else None)
+    for g, b in zip((st.G, st.H), before):
+        a = g.download()
+        assert all(np.array_equal(x, y) for x, y in zip(a, b))
+    assert st.position == 0
+    o1, _ = st.apply_dev(op, _dev(ctx, u, np.uint32), _dev(ctx, v, np.uint32),
+                         _dev(ctx, w, np.float32) if w is not None else None)
+    o2, _ = ref.apply(op, u, v, w)
+    assert np.array_equal(o1, o2)
+    _assert_same_graph(st.H, _rows_of(ref.H))
+    _assert_same_graph(st.G, _rows_of(ref.G))
