@@ -101,3 +101,37 @@ def test_shape_only_tuning_needs_no_kappa():
     assert Setting(**res["chosen"]["setting"]) == Setting(w_ins=256, w_del=512, block_ins=4,
                                                           block_del=2, dense_no_desc=True)
     assert res["anchor"]["kappa"] is None and res["kappa_checks"] == []
+
+
+def test_committed_tunings_hold_their_bounds():
+    """Every committed tuning file: the chosen setting is a Setting, its growth
+    and kappa satisfy the recorded tolerances against the anchor (when kappa
+    was measured), every kappa check of an accepted quality step is within
+    bounds, and every shape trial (results must not move) shows the chosen
+    setting's growth exactly."""
+    import glob
+    import json
+    import os
+
+    from dygrass_amd.autotune import SHAPE_GRID, load, setting_of
+
+    root = os.path.join(os.path.dirname(__file__), "..", "dygrass_amd", "tuning")
+    files = sorted(glob.glob(os.path.join(root, "*.json")))
+    assert files
+    for path in files:
+        t = json.load(open(path))
+        name = os.path.basename(path)[:-5]
+        assert load(name) == t and t["workload"] == name
+        chosen = setting_of(t)
+        a, c, tol = t["anchor"], t["chosen"], t["tolerances"]
+        if a["kappa"] is not None:
+            assert c["growth"] <= a["growth"] * (1 + tol["growth"]), name
+            assert abs(c["kappa"] - a["kappa"]) <= tol["kappa"] * abs(a["kappa"]), name
+            chosen_q = [chosen.w_ins, chosen.w_del, chosen.steps_ins, chosen.steps_del]
+            assert any(k["quality"] == chosen_q for k in t["kappa_checks"]), name
+        else:  # shape-only tuning: the anchor's walker counts and budgets
+            assert setting_of(t, "anchor").quality() == chosen.quality(), name
+        shapes = [r for r in t["trials"] if r["knob"] in SHAPE_GRID]
+        assert shapes, name
+        assert all(r["growth"] == c["growth"] for r in shapes), name
+        assert c["seconds"] <= a["seconds"] * 1.05, name
