@@ -1,5 +1,6 @@
-"""Full-size checks on the GPU (1M-node random geometric graph, 200k
-insertions in 10 snapshot batches, threshold 100, 256 walkers).
+"""Full-size checks on the GPU: the 1M-node random geometric graph with 200k
+insertions in 10 snapshot batches (threshold 100, 256 walkers), the 2M-node
+mesh with 100k deletions, and the 4M-point 12-NN batched stream.
 
 Every one of the 200k decisions must equal the oracle's, and the final
 sparsifier must equal the oracle's slot for slot.  Size-independent
@@ -75,3 +76,83 @@ def test_full_size_decremental_matches_oracle(ctx):
     assert sum(b["queued"] for b in log) == st.filtered
     assert sum(b["no_path"] for b in log) == st.no_path
     assert len(log) == 10 and st.filtered > 20000
+
+
+def _keys(V, u, v):
+    u = np.asarray(u, np.int64)
+    v = np.asarray(v, np.int64)
+    return np.minimum(u, v) * V + np.maximum(u, v)
+
+
+def _edges(g, V):
+    """Each undirected edge of a device graph once: sorted keys, weight bits."""
+    rp, col, w = g.download()
+    rows = np.repeat(np.arange(V, dtype=np.int64), np.diff(rp).astype(np.int64))
+    once = col.astype(np.int64) > rows
+    k = rows[once] * V + col[once].astype(np.int64)
+    order = np.argsort(k)
+    return k[order], w[once].view(np.uint32)[order]
+
+
+def test_full_size_batched_stream_properties(ctx):
+    """The batched-stream workload at full size (4M-point 12-NN graph, 64
+    batches, 379k updates) with the committed autotuned setting: outcome codes
+    and H equal the plain form's slot for slot (the reference-shaped
+    implementation: host edge maps and plans, one workgroup per edge); G ends
+    as exactly G0 - deletions + insertions; H lies inside G with G's weight
+    bits; every no-path insertion is in H; deleted edges are in neither."""
+    from dygrass_amd import Stream, autotune, csr_from_edges, stream_params, synth
+    from oracle import stream as S
+
+    wl = synth.config_batched(1.0, 0)
+    V = wl.G.V
+    t = autotune.load(wl.name)
+    s = autotune.setting_of(t) if t else autotune.Setting(w_ins=256, w_del=512)
+    csr = (csr_from_edges(V, wl.G.u, wl.G.v, wl.G.w), csr_from_edges(V, wl.H.u, wl.H.v, wl.H.w))
+    maxb = max(len(b[1]) for b in wl.batches)
+
+    def run(plain):
+        sp = stream_params(w_ins=s.w_ins, w_del=s.w_del, threshold=wl.threshold, seed=0,
+                           max_batch=maxb, max_inserts=wl.updates, plain=plain,
+                           steps_ins=s.steps_ins, steps_del=s.steps_del,
+                           block_edges_ins=0 if plain else s.block_ins,
+                           block_edges_del=0 if plain else s.block_del,
+                           dense_no_desc=s.dense_no_desc and not plain)
+        st = Stream(ctx, V, None, None, sp, csr=csr)
+        outs = [st.apply(op, u, v, w)[0] for op, u, v, w in wl.batches]
+        return st, outs
+
+    dev, outs = run(False)
+    plain, pouts = run(True)
+    for a, b in zip(outs, pouts):
+        assert np.array_equal(a, b)
+    for x, y in zip(dev.H.download(), plain.H.download()):
+        assert np.array_equal(x.view(np.uint32) if x.dtype == np.float32 else x,
+                              y.view(np.uint32) if y.dtype == np.float32 else y)
+    plain.close()
+    assert dev.H.check() == 0
+
+    ins = [(b, o) for b, o in zip(wl.batches, outs) if b[0] == "ins"]
+    dels = [(b, o) for b, o in zip(wl.batches, outs) if b[0] == "del"]
+    ik = np.concatenate([_keys(V, b[1], b[2]) for b, _ in ins])
+    iw = np.concatenate([np.asarray(b[3], np.float32) for b, _ in ins]).view(np.uint32)
+    io = np.concatenate([o for _, o in ins])
+    dk = np.concatenate([_keys(V, b[1], b[2]) for b, _ in dels])
+    do = np.concatenate([o for _, o in dels])
+    assert np.all(np.isin(io, [S.FOUND, S.ADDED])) and np.all(np.isin(do, [
+        S.G_ONLY, S.RECOVERED, S.HEURISTIC]))  # new edges only, deletions of G edges only
+    g0k = _keys(V, wl.G.u, wl.G.v)
+    g0w = np.asarray(wl.G.w, np.float32).view(np.uint32)
+    keep = ~np.isin(g0k, dk)
+    want_k = np.concatenate([g0k[keep], ik])
+    want_w = np.concatenate([g0w[keep], iw])
+    order = np.argsort(want_k)
+    gk, gw = _edges(dev.G, V)
+    assert np.array_equal(gk, want_k[order]) and np.array_equal(gw, want_w[order])
+    hk, hw = _edges(dev.H, V)
+    pos = np.searchsorted(gk, hk)
+    assert np.all(pos < len(gk)) and np.array_equal(gk[pos], hk)  # H inside G
+    assert np.array_equal(gw[pos], hw)                           # with G's weights
+    assert np.all(np.isin(ik[io == S.ADDED], hk))
+    assert not np.any(np.isin(dk, hk))
+    dev.close()
