@@ -27,8 +27,9 @@ Eight knobs, in two kinds:
   screened on growth and kappa is computed once per knob, at the lowest value
   that passed the screen, stepping back up the knob's values while it misses.
 
-The served configuration fixes its walker counts and step budgets, so its
-tuning moves the shape knobs only.
+The served workload's sparsifier holds random shortcuts, so kappa there costs
+minutes per evaluation through the sparse LU: its quality knobs answer to the
+growth screen alone, and its tuning records kappa as unchecked.
 
 Search: coordinate descent from the anchor.  Each quality knob walks down
 its candidate list (fewer walkers, shorter budgets) while candidates pass
@@ -131,7 +132,8 @@ def tune(run, kappa, edges_h0: int, *, anchor: Setting = ANCHOR, quality_grid=No
     """Coordinate-descent search (module docstring).  `edges_h0` is the
     sparsifier's edge count before the stream.  With an empty quality_grid
     only the shape knobs are tuned, at the anchor's walker counts and step
-    budgets, and `kappa` is not called (it may be None)."""
+    budgets; with kappa None the quality knobs answer to the growth screen
+    alone (the result records kappa as None)."""
     qg = QUALITY_GRID if quality_grid is None else quality_grid
     sg = SHAPE_GRID if shape_grid is None else shape_grid
     say = log or (lambda m: None)
@@ -154,13 +156,17 @@ def tune(run, kappa, edges_h0: int, *, anchor: Setting = ANCHOR, quality_grid=No
     say(f"anchor {anchor.quality()}: {a.seconds * 1e3:.1f} ms (walks {a.walk_ms:.2f} ms), "
         f"growth {g_anchor}")
 
-    # kappa of the anchor (skipped for a shape-only tuning: no quality knob moves)
-    k_anchor = float(kappa(anchor)) if qg else None
-    kappa_checks = [(anchor.quality(), k_anchor)] if qg else []
-    if qg:
+    # kappa of the anchor (skipped for a shape-only tuning, where no quality
+    # knob moves, and when no kappa is given: the growth screen alone decides)
+    use_kappa = bool(qg) and kappa is not None
+    k_anchor = float(kappa(anchor)) if use_kappa else None
+    kappa_checks = [(anchor.quality(), k_anchor)] if use_kappa else []
+    if use_kappa:
         say(f"kappa anchor {k_anchor:.6g}")
 
     def kappa_ok(s: Setting) -> bool:
+        if not use_kappa:
+            return True
         k = float(kappa(s))
         kappa_checks.append((s.quality(), k))
         ok = abs(k - k_anchor) <= tol_kappa * abs(k_anchor)
@@ -197,7 +203,7 @@ def tune(run, kappa, edges_h0: int, *, anchor: Setting = ANCHOR, quality_grid=No
             cand = dataclasses.replace(cur, **{knob: val})
             if kappa_ok(cand):
                 cur, cur_t = cand, t
-                k_cur = kappa_checks[-1][1]
+                k_cur = kappa_checks[-1][1] if use_kappa else None
                 break
             passed.pop()
 
@@ -360,13 +366,12 @@ def main(argv=None) -> int:
         res = tune(tr.run, tr.kappa, tr.edges_h0, repeats=a.repeats, tol_growth=a.tol_growth,
                    tol_kappa=a.tol_kappa, log=log)
     else:
-        # the served configuration fixes its walker counts; only the shape knobs
-        # are tuned (kappa on its sparsifier, a tree plus random shortcuts, costs
-        # minutes per evaluation through the sparse LU)
-        # (step graphs are not a knob here: the served loop launches directly)
-        res = tune(tr.run, None, tr.edges_h0, anchor=Setting(w_ins=wl.w_ins, w_del=wl.w_del),
-                   quality_grid={}, repeats=a.repeats, log=log,
-                   shape_grid={k: v for k, v in SHAPE_GRID.items() if k != "graph"})
+        # kappa on the served sparsifier (a tree plus random shortcuts) costs
+        # minutes per evaluation through the sparse LU, so the quality knobs
+        # answer to the growth screen alone; step graphs are not a knob here
+        # (the served loop launches directly)
+        res = tune(tr.run, None, tr.edges_h0, repeats=a.repeats, tol_growth=a.tol_growth,
+                   log=log, shape_grid={k: v for k, v in SHAPE_GRID.items() if k != "graph"})
     res.update({"workload": name, "V": int(wl.G.V), "E_G": int(len(wl.G)),
                 "E_H": int(len(wl.H)), "updates": int(updates),
                 "batches": len(batches), "threshold": wl.threshold, "seed": a.seed,

@@ -129,9 +129,19 @@ def test_committed_tunings_hold_their_bounds():
             assert abs(c["kappa"] - a["kappa"]) <= tol["kappa"] * abs(a["kappa"]), name
             chosen_q = [chosen.w_ins, chosen.w_del, chosen.steps_ins, chosen.steps_del]
             assert any(k["quality"] == chosen_q for k in t["kappa_checks"]), name
-        else:  # shape-only tuning: the anchor's walker counts and budgets
-            assert setting_of(t, "anchor").quality() == chosen.quality(), name
+        else:  # growth screen only (or a shape-only tuning)
+            assert c["kappa"] is None and t["kappa_checks"] == [], name
+            assert c["growth"] <= a["growth"] * (1 + tol["growth"]), name
         shapes = [r for r in t["trials"] if r["knob"] in SHAPE_GRID]
         assert shapes, name
         assert all(r["growth"] == c["growth"] for r in shapes), name
         assert c["seconds"] <= a["seconds"] * 1.05, name
+
+
+def test_growth_only_tuning():
+    """kappa None with quality knobs: the growth screen alone decides; by hand,
+    w_ins 128 and w_del 128 pass (64 fails each), every budget passes."""
+    res = tune(model_run, None, H0, repeats=1, shape_grid={})
+    assert Setting(**res["chosen"]["setting"]) == Setting(w_ins=128, w_del=128, steps_ins=32,
+                                                          steps_del=32)
+    assert res["chosen"]["kappa"] is None and res["kappa_checks"] == []
