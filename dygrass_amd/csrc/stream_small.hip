@@ -370,10 +370,10 @@ struct InsShared {
 // A captured step graph's batch values (StepDesc), read once at entry.  The
 // host rewrites the pinned descriptor and inputs before every replay, and
 // pinned memory may be cached in an XCD's L2, where the previous batch's
-// lines can outlive a replay (a node inside a graph does not get the
-// system-scope acquire a directly launched kernel gets).  So the kernel takes
-// a system-scope acquire first (L1 and L2 invalidation) and reads the
-// descriptor with vector loads (volatile: not through the scalar cache).
+// lines could outlive a replay: the runtime's fences between the nodes of a
+// graph need only reach device scope.  So the kernel takes a system-scope
+// acquire first (L1 and L2 invalidation) and reads the descriptor with vector
+// loads (volatile: not through the scalar cache).
 struct DescVals {
     const uint32_t* u;
     const uint32_t* v;

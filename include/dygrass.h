@@ -321,7 +321,8 @@ typedef struct {
     uint64_t hops;        /* neighbour draws of the batch's walks */
     double walk_ms;       /* walk kernel (HIP events) */
     double total_ms;      /* whole apply() call, host clock */
-    uint64_t graph;       /* 1 when the batch ran as a captured step graph */
+    uint64_t graph;       /* 1 when the batch ran as a captured step graph (its walk_ms
+                             then times the whole step: the walk is a node inside it) */
 } dyg_stream_stats;
 /* G and H in CSR (slot order as given), uploaded and owned by the stream. */
 dyg_status dyg_stream_create(dyg_ctx ctx, uint32_t V, const uint64_t* g_row_ptr,
