@@ -63,6 +63,22 @@ def test_path_graph_step_bound():
     assert g.walk_inc(0, 100, 1.0, 1e30, 4)[3] == 4 * 99  # 99 draws, then the bound
 
 
+@pytest.mark.parametrize("L", [2, 7, 32, 48])
+def test_path_graph_step_budget(L):
+    """A step budget L (the tuner's 32 and 48 among them) bounds the hop index
+    at L - 1, for insertion and deletion walks alike: on a path the route is
+    unique, so a target d hops away is found at hop d exactly when d <= L - 1,
+    after d draws per walker; otherwise every walker stops after L - 1 draws."""
+    (g, _), _ = path_graph(120)
+    for d in (L - 1, L):
+        f, _, _, hops = g.walk_inc(0, d, 1.0, 1e30, 8, max_steps=L)
+        fd, _, _, _, _ = g.walk_dec(0, d, 8, max_steps=L)
+        if d <= L - 1:
+            assert f == d and fd == d and hops == 8 * d
+        else:
+            assert f == -1 and fd == -1 and hops == 8 * (L - 1)
+
+
 @pytest.mark.parametrize("thr", [3.0, 10.0, 25.0])
 def test_distortion_cut_on_path(thr):
     (g, _), w = path_graph(60, seed=4)
