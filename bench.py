@@ -262,7 +262,7 @@ def run_stream(ctx, wl, plain: bool, seed: int, setting, csr=None, dev_batches=N
     sp = sp_of(setting, wl.threshold, seed, maxb, wl.updates, plain)
     st = Stream(ctx, wl.G.V, (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w), sp, csr=csr)
     ctx.sync()
-    outs, lat, walk_ms, hops, agg = [], [], 0.0, 0, {}
+    outs, lat, walk_ms, hops, agg, walks = [], [], 0.0, 0, {}, []
     src = dev_batches if dev_batches is not None else wl.batches
     ts = time.perf_counter()
     for op, u, v, w in src:
@@ -272,6 +272,8 @@ def run_stream(ctx, wl, plain: bool, seed: int, setting, csr=None, dev_batches=N
         outs.append(o)
         walk_ms += stt["walk_ms"]
         hops += stt["hops"]
+        walks.append({"op": op, "kept": int(stt["found"] + stt["added"]), "hops": int(stt["hops"]),
+                      "redirects": int(stt["redirects"]), "walk_ms": float(stt["walk_ms"])})
         for k in ("weight_updates", "found", "added", "skipped", "g_only", "recovered",
                   "heuristic", "invalid"):
             agg[k] = agg.get(k, 0) + stt[k]
@@ -280,7 +282,32 @@ def run_stream(ctx, wl, plain: bool, seed: int, setting, csr=None, dev_batches=N
     H = st.H.download()  # final sparsifier (untimed)
     st.close()
     return dict(seconds=dt, outcomes=np.concatenate(outs), latencies=np.array(lat) * 1e3,
-                walk_ms=walk_ms, hops=hops, counts=agg, H=H)
+                walk_ms=walk_ms, hops=hops, counts=agg, H=H, walks=walks)
+
+
+def stream_roofline(runs, walkers: int, workload: str) -> dict:
+    """Roofline of the batched stream's dominant kernel, the insertion walk,
+    over every insertion batch's walk launch in the timed passes: algorithmic
+    bytes (8 B per walker for the source row, 16 B per neighbour draw and per
+    redirect, 20 B per edge of I/O; hop and redirect counts from the kernel's
+    own counters) over the launches' HIP-event durations."""
+    nbytes, ms, launches = 0.0, 0.0, 0
+    for r in runs:
+        for b in r["walks"]:
+            if b["op"] != "ins" or b["kept"] == 0:
+                continue
+            nbytes += (8.0 * walkers * b["kept"] + 16.0 * (b["hops"] + b["redirects"])
+                       + 20.0 * b["kept"])
+            ms += b["walk_ms"]
+            launches += 1
+    achieved = nbytes / (ms / 1e3) / 1e9 if ms > 0 else 0.0
+    kernel = f"k_walk_pool<{int(np.log2(walkers))},true,true,true>"
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kernel, workload),
+            "kernel": f"k_walk_pool<W={walkers}, insert, prune, decision only>",
+            "workload": workload, "launches": launches,
+            "bytes_per_launch": nbytes / max(launches, 1),
+            "avg_launch_ms": ms / max(launches, 1)}
 
 
 def leg_batched(args, ctx) -> dict:
@@ -320,6 +347,7 @@ def leg_batched(args, ctx) -> dict:
         "batch_latency_ms": {"p50": pct(head["latencies"], 50),
                              "p99": pct(head["latencies"], 99)},
         "outcomes": head["counts"],
+        "roofline": stream_roofline(runs, setting.w_ins, wl.name),
         "tuning": tuning,
         "growth_H": int(len(head["H"][1]) // 2 - len(wl.H)),
         "inputs": "every batch in HBM before the timed window (dyg_stream_apply_dev)",
@@ -620,7 +648,8 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": head.pop("workload"), "form": head_name},
-        "roofline": inc.pop("roofline") if inc else None,
+        "roofline": (head.pop("roofline", None)
+                     or (inc.get("roofline") if inc is not None else None)),
         "cpu_baseline": cpu,
     }
     for name, leg in legs.items():

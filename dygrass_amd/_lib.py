@@ -60,7 +60,8 @@ class StreamStats(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in ("updates", "weight_updates", "found", "added",
                                           "skipped", "g_only", "recovered", "heuristic",
                                           "invalid", "candidates", "hops")] + [
-        ("walk_ms", C.c_double), ("total_ms", C.c_double), ("graph", C.c_uint64)]
+        ("walk_ms", C.c_double), ("total_ms", C.c_double), ("graph", C.c_uint64),
+        ("redirects", C.c_uint64)]
 
 
 class ServeResult(C.Structure):

@@ -1359,6 +1359,7 @@ dyg_status stream_apply(dyg_stream s, int op, const uint32_t* u, const uint32_t*
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess) rs.walk_ms = ms;
         rs.hops = hst[0];
+        rs.redirects = hst[1];
         if (hst[3]) rs.candidates = hst[3];  // counted on the device (small deletion batch)
         rs.graph = s->last_graph ? 1 : 0;
         // keep a batch's worth of arena headroom for the next batch

@@ -323,6 +323,8 @@ typedef struct {
     double total_ms;      /* whole apply() call, host clock */
     uint64_t graph;       /* 1 when the batch ran as a captured step graph (its walk_ms
                              then times the whole step: the walk is a node inside it) */
+    uint64_t redirects;   /* non-backtracking redirects to a row's last slot (each one
+                             more slot read) in the batch's walks */
 } dyg_stream_stats;
 /* G and H in CSR (slot order as given), uploaded and owned by the stream. */
 dyg_status dyg_stream_create(dyg_ctx ctx, uint32_t V, const uint64_t* g_row_ptr,
