@@ -90,6 +90,32 @@ def test_stream_matches_oracle(ctx, grid_wl, seed, plain, general, nodesc, graph
     assert seen >= {S.WEIGHT, S.FOUND, S.ADDED, S.SKIP, S.G_ONLY, S.RECOVERED, S.INVALID}
 
 
+@pytest.mark.parametrize("general", [False, True])
+def test_stream_many_walkers_match_oracle(ctx, grid_wl, general):
+    """512 insertion and 1024 deletion walkers per edge (two and four walkers
+    per lane of a workgroup) on small batches: outcomes, G and H equal the
+    oracle's after every batch."""
+    from dygrass_amd import Stream, stream_params
+    from oracle import stream as S
+
+    wl = grid_wl
+    seed = 9
+    sp = stream_params(w_ins=512, w_del=1024, threshold=10.0, seed=seed, max_batch=1100,
+                       max_inserts=4096, general_only=general, steps_ins=60, steps_del=40)
+    st = Stream(ctx, wl.G.V, (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w), sp)
+    orc = S.Stream(wl.G.V, (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w), thr=10.0,
+                   w_ins=512, w_del=1024, seed=seed, steps_ins=60, steps_del=40)
+    seen = set()
+    for op, u, v, w in _mixed_batches(wl, seed, n_ins=200, n_del=120):
+        got, _ = st.apply(op, u, v, w)
+        want = orc.apply(op, u, v, w)
+        assert np.array_equal(got, want), (op, np.flatnonzero(got != want)[:10])
+        seen.update(np.unique(got).tolist())
+        _assert_same_graph(st.G, orc.G)
+        _assert_same_graph(st.H, orc.H)
+    assert seen >= {S.FOUND, S.ADDED, S.RECOVERED}
+
+
 @pytest.mark.parametrize("nodesc", [False, True])
 def test_step_graphs_replay_like_direct_launches(ctx, grid_wl, nodesc):
     """Captured step graphs (DYG_STREAM_GRAPH) against direct launches on the
