@@ -380,8 +380,9 @@ def leg_batched(args, ctx) -> dict:
 
 
 def cpu_sample_batched(wl, args, setting) -> dict:
-    """The oracle's insertion walks on H for the stream's first insertion
-    batches (about 20k updates), OpenMP over each batch's edges."""
+    """The oracle's insertion walks on the original H for the stream's
+    insertion batches (all of them, or as many as 30 s of CPU time allow),
+    OpenMP over each batch's edges."""
     from dygrass_amd import csr_from_edges
     from oracle import pyoracle as po
 
@@ -390,7 +391,7 @@ def cpu_sample_batched(wl, args, setting) -> dict:
     og = po.Graph(wl.G.V, rp, col, w)
     n, base, secs = 0, 0, 0.0
     for op, u, v, ww in wl.batches:
-        if op == "ins" and n < 20000:
+        if op == "ins" and secs < 30.0:
             ts = time.perf_counter()
             og.walk_inc_batch(u, v, ww, wl.threshold, setting.w_ins, seed=args.seed,
                               edge_id=np.arange(base, base + len(u), dtype=np.uint64),
@@ -400,7 +401,8 @@ def cpu_sample_batched(wl, args, setting) -> dict:
         base += len(u)
     log(f"batched cpu sample: {n / secs:.0f} walked updates/s on {threads} threads")
     return {"value": n / secs, "unit": "edge-updates/s", "cores": threads, "kind": "port",
-            "sample": f"insertion walks of the stream's first insertion batches ({n} updates) "
+            "sample": f"insertion walks of the stream's insertion batches ({n} updates, "
+                      f"{secs:.1f} s of CPU time) "
                       f"on the original sparsifier, {setting.w_ins} walkers, "
                       f"{setting.steps_ins} steps, through the oracle, OpenMP over edges; "
                       "graph bookkeeping not included"}
