@@ -85,7 +85,10 @@ int main(int argc, char* argv[]) {
     p.walkers = (uint32_t)n_walkers;
     p.max_steps = 100;
     p.threshold = distortion;
-    p.flags = 0;
+    // only found / not found is read below (as the reference's host reads only
+    // flag == -1, random_walk_incremental/gpu_graph.cuh:302): an edge's walkers
+    // may stop once one of them found the target
+    p.flags = DYG_WALK_DECISION_ONLY;
     p.seed = env_u64("DYGRASS_SEED", 0);
 
     std::vector<int32_t> found(M ? M : 1);
