@@ -147,6 +147,8 @@ _SIGS = {
                                      C.POINTER(StreamStats)]),
     "dyg_stream_apply_dev": (C.c_int32, [vp, C.c_int, vp, vp, vp, C.c_uint32, vp,
                                          C.POINTER(StreamStats)]),
+    "dyg_stream_apply_many_dev": (C.c_int32, [vp, C.c_uint32, vp, vp, vp, vp, vp, vp,
+                                              C.POINTER(StreamStats)]),
     "dyg_stream_graph": (vp, [vp, C.c_int]),
     "dyg_stream_position": (C.c_uint64, [vp]),
     "dyg_stream_free": (None, [vp]),

@@ -465,6 +465,25 @@ class Stream:
                                              C.byref(st)), self.ctx.handle)
         return out[:n], {k: getattr(st, k) for k, _ in L.StreamStats._fields_}
 
+    def apply_many_dev(self, ops, offsets, u, v, w=None) -> tuple[np.ndarray, list]:
+        """Batches b = 0.. in one call: ops[b] ("ins" / "del") over updates
+        offsets[b]:offsets[b+1] of the device arrays u, v, w (DeviceArray or
+        raw addresses; w entries of deletion batches are ignored).  Returns every
+        update's outcome code and the per-batch stats."""
+        nb = len(ops)
+        code = np.array([L.DYG_STREAM_INSERT if o == "ins" else L.DYG_STREAM_DELETE
+                         for o in ops], np.int32)
+        off = np.ascontiguousarray(offsets, np.uint64)
+        assert len(off) == nb + 1
+        addr = lambda a: (a.ptr if isinstance(a, DeviceArray) else int(a)) if a is not None else None
+        out = np.empty(max(int(off[-1]), 1), np.int32)
+        sts = (L.StreamStats * max(nb, 1))()
+        check(self._lib.dyg_stream_apply_many_dev(self.handle, nb, L.addr(code), L.addr(off),
+                                                  addr(u), addr(v), addr(w), L.addr(out), sts),
+              self.ctx.handle)
+        stats = [{k: getattr(sts[b], k) for k, _ in L.StreamStats._fields_} for b in range(nb)]
+        return out[:int(off[-1])], stats
+
     def close(self) -> None:
         if self.handle:
             self._lib.dyg_stream_free(self.handle)

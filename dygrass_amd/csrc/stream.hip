@@ -1400,6 +1400,30 @@ extern "C" dyg_status dyg_stream_apply_dev(dyg_stream s, int op, const uint32_t*
     return stream_apply(s, op, u, v, w, n, outcome, st, true);
 }
 
+extern "C" dyg_status dyg_stream_apply_many_dev(dyg_stream s, uint32_t nb, const int* op,
+                                                const uint64_t* off, const uint32_t* u,
+                                                const uint32_t* v, const float* w,
+                                                int32_t* outcome, dyg_stream_stats* st) {
+    if (!s) return DYG_ERR_ARG;
+    if (nb && (!op || !off)) {
+        set_error(s->ctx, "stream apply many: missing batch list");
+        return DYG_ERR_ARG;
+    }
+    for (uint32_t b = 0; b < nb; ++b)  // the whole list, before anything changes
+        if (off[b + 1] < off[b] || off[b + 1] - off[b] > 0xFFFFFFFFull) {
+            set_error(s->ctx, "stream apply many: bad batch offsets");
+            return DYG_ERR_ARG;
+        }
+    for (uint32_t b = 0; b < nb; ++b) {
+        const uint64_t o = off[b];
+        const dyg_status r =
+            stream_apply(s, op[b], u + o, v + o, w ? w + o : nullptr, (uint32_t)(off[b + 1] - o),
+                         outcome ? outcome + o : nullptr, st ? st + b : nullptr, true);
+        if (r != DYG_OK) return r;
+    }
+    return DYG_OK;
+}
+
 extern "C" dyg_graph dyg_stream_graph(dyg_stream s, int which) {
     if (!s) return nullptr;
     return which == 0 ? s->G : s->H;

@@ -345,6 +345,18 @@ dyg_status dyg_stream_apply(dyg_stream s, int op, const uint32_t* u, const uint3
 dyg_status dyg_stream_apply_dev(dyg_stream s, int op, const uint32_t* u, const uint32_t* v,
                                 const float* w, uint32_t n, int32_t* outcome,
                                 dyg_stream_stats* st);
+/* A list of nb batches in one call, all in device memory: batch b is op[b]
+ * over updates [off[b], off[b+1]) of u, v and w (w entries of deletion
+ * batches are ignored; w may be NULL when every batch deletes); exactly nb
+ * calls of dyg_stream_apply_dev in order, without a return to the caller in
+ * between.  outcome (optional, host) receives every update's code at its
+ * position, st (optional) nb per-batch stats.  Offsets are checked for the
+ * whole list first (DYG_ERR_ARG, nothing applied); otherwise the call stops at
+ * the first failing batch and returns its status (the batches before it stay
+ * applied). */
+dyg_status dyg_stream_apply_many_dev(dyg_stream s, uint32_t nb, const int* op,
+                                     const uint64_t* off, const uint32_t* u, const uint32_t* v,
+                                     const float* w, int32_t* outcome, dyg_stream_stats* st);
 /* 0: G, 1: H (owned by the stream; use with dyg_graph_info/download/check). */
 dyg_graph dyg_stream_graph(dyg_stream s, int which);
 /* Updates applied so far (the edge id of the next update). */

@@ -389,3 +389,48 @@ def test_apply_dev_rejects_out_of_range_ids(ctx, grid_wl, n, op):
     assert np.array_equal(o1, o2)
     _assert_same_graph(st.H, _rows_of(ref.H))
     _assert_same_graph(st.G, _rows_of(ref.G))
+
+
+def test_apply_many_dev_matches_apply_dev(ctx, grid_wl):
+    """dyg_stream_apply_many_dev (a batch list in one call) against per-batch
+    dyg_stream_apply_dev: outcome codes, per-batch stats, G and H equal; bad
+    offsets are rejected before anything changes."""
+    from dygrass_amd import DygError, Stream, stream_params
+
+    wl = grid_wl
+    batches = _mixed_batches(wl, 12, n_ins=300)
+    rng = np.random.default_rng(12)
+    big = rng.choice(len(wl.G), 2600, replace=False)
+    batches.append(("del", wl.G.u[big], wl.G.v[big], None))
+    batches.append(("ins", wl.ins.u[1200:1210], wl.ins.v[1200:1210], wl.ins.w[1200:1210]))
+    mk = lambda: Stream(ctx, wl.G.V, (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w),
+                        stream_params(w_ins=64, w_del=64, threshold=10.0, seed=12,
+                                      max_batch=3000, max_inserts=8192))
+    one, many = mk(), mk()
+    want, wst = [], []
+    for op, u, v, w in batches:
+        o, st = one.apply_dev(op, _dev(ctx, u, np.uint32), _dev(ctx, v, np.uint32),
+                              _dev(ctx, w, np.float32) if w is not None else None)
+        want.append(o)
+        wst.append(st)
+    off = np.concatenate([[0], np.cumsum([len(b[1]) for b in batches])]).astype(np.uint64)
+    U = np.concatenate([np.asarray(b[1], np.uint32) for b in batches])
+    Vv = np.concatenate([np.asarray(b[2], np.uint32) for b in batches])
+    Wt = np.concatenate([np.asarray(b[3], np.float32) if b[3] is not None
+                         else np.zeros(len(b[1]), np.float32) for b in batches])
+    du, dv, dw = _dev(ctx, U, np.uint32), _dev(ctx, Vv, np.uint32), _dev(ctx, Wt, np.float32)
+    ops = [b[0] for b in batches]
+    bad = off.copy()
+    bad[2] = bad[1] - 1  # a batch ending before it starts
+    with pytest.raises(DygError, match="offsets"):
+        many.apply_many_dev(ops[:2], bad[:3], du, dv, dw)
+    assert many.position == 0
+    got, gst = many.apply_many_dev(ops, off, du, dv, dw)
+    assert np.array_equal(got, np.concatenate(want))
+    for a, b in zip(gst, wst):
+        for k in ("updates", "found", "added", "skipped", "g_only", "recovered", "heuristic",
+                  "weight_updates", "invalid"):
+            assert a[k] == b[k], k
+    _assert_same_graph(many.G, _rows_of(one.G))
+    _assert_same_graph(many.H, _rows_of(one.H))
+    assert many.position == one.position
