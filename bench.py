@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark of the random-walk sparsifier updaters on one MI355X.
 
-Three workloads, all seeded synthetic graphs, every result checked against a
+Five workloads, all seeded synthetic graphs, every result checked against a
 second implementation in the same run:
 
 batched (the headline `value`): a 4M-point 12-NN random geometric graph
@@ -25,9 +25,15 @@ incremental: a 1M-point 8-NN random geometric graph, 200k insertions as 10
 decremental: a 2M-node triangulated mesh, 100k deletions as 10 snapshot
   batches (the plain-decremental configuration); deletion-only stream vs
   the dual graph.
+grid: the 128x128 grid with 2,000 insertions in 10 snapshot batches,
+  threshold 10, 512 walkers (the plain incremental configuration that also
+  runs through the CPU walk path), GPU beside the oracle's whole run.
+
+CPU baselines (the oracle, OpenMP, on the cores this process may use) run
+after every GPU leg.
 
   python bench.py [--steps K] [--warmup W]
-                  [--workload all|batched|served|incremental|decremental]
+                  [--workload all|batched|served|incremental|decremental|grid]
 """
 from __future__ import annotations
 
@@ -56,15 +62,14 @@ def pct(a, q) -> float:
     return float(np.percentile(np.asarray(a), q)) if len(a) else 0.0
 
 
-def algorithmic_bytes(stats: dict, updates: int, walkers: int) -> float:
-    """Bytes the walks must read.  Per walker the source's row record (8 B:
-    first slot + degree); per neighbour draw one 16 B slot (neighbour 4 +
-    weight 4 + the neighbour's first slot 4 + degree 4, i.e. the row-start,
-    degree, column and weight words of a hop); +16 B for each
-    non-backtracking redirect to the row's last slot.  A leaf is recognised
-    from the degree already read.  Per edge 12 B of input and 8 B of output."""
-    return (8.0 * updates * walkers + 16.0 * stats["hops"] + 16.0 * stats["redirects"]
-            + 20.0 * updates)
+def algorithmic_bytes(hops: int, redirects: int, edges: int, per_edge: float = 20.0) -> float:
+    """Bytes a walk launch must move: 16 B per neighbour draw, the first
+    included (row start 4 + degree 4 of the row drawn from, column 4 + weight 4
+    of the drawn slot); 8 B per non-backtracking redirect (column + weight of
+    the row's last slot); per edge 12 B in and 8 B out for insertions (16 B
+    for deletions, whose winner path of 4 B per vertex is not counted).  Hop
+    and redirect counts come from the kernel's own counters."""
+    return 16.0 * hops + 8.0 * redirects + per_edge * edges
 
 
 def measured_traffic(kernel: str, workload: str):
@@ -80,14 +85,20 @@ def measured_traffic(kernel: str, workload: str):
 
 
 # ---------------------------------------------------------------------------
-def leg_incremental(args, ctx) -> dict:
+def leg_incremental(args, ctx, grid: bool = False) -> dict:
+    """Snapshot-batched insertions through dyg_incremental_update_dev (device
+    edits) against the plain form and the oracle's whole run on the host's
+    cores: the 1M-point 8-NN graph (200k insertions, threshold 100, 256
+    walkers) or, with grid, the 128x128 grid (2,000 insertions, threshold 10,
+    512 walkers)."""
     from dygrass_amd import (DeviceArray, DeviceGraph, csr_from_edges, incremental_update,
                              incremental_update_dev, params, synth)
 
     t0 = time.time()
-    wl = synth.config_rgg_insert(args.scale, args.seed)
+    wl = (synth.config_grid_insert if grid else synth.config_rgg_insert)(args.scale, args.seed)
+    leg = "grid" if grid else "incremental"
     m = len(wl.ins)
-    log(f"incremental: {wl.name} V={wl.G.V} E_H={len(wl.H)} updates={m} "
+    log(f"{leg}: {wl.name} V={wl.G.V} E_H={len(wl.H)} updates={m} "
         f"({time.time() - t0:.1f}s)")
     rp, col, w = csr_from_edges(wl.H.V, wl.H.u, wl.H.v, wl.H.w)
     I = (m + 9) // 10
@@ -128,7 +139,7 @@ def leg_incremental(args, ctx) -> dict:
             times.append(dt)
             walk_ms.append(st["walk_ms"])
             upd_ms.append(st["update_ms"])
-            log(f"incremental {'plain' if plain else 'device'} step {i}: {dt * 1e3:.2f} ms "
+            log(f"{leg} {'plain' if plain else 'device'} step {i}: {dt * 1e3:.2f} ms "
                 f"(walk kernels {st['walk_ms']:.2f} ms, graph edits {st['update_ms']:.2f} ms)")
         total = float(np.sum(times))
         return dict(found=found, stats=st, total=total, steps=steps, value=m * steps / total,
@@ -138,9 +149,10 @@ def leg_incremental(args, ctx) -> dict:
     head = run_mode(False, args.steps, args.warmup)
     stats = head["stats"]
     launches = int(stats["batches"])
-    bytes_run = algorithmic_bytes(stats, m, wl.walkers)
+    bytes_run = algorithmic_bytes(stats["hops"], stats["redirects"], m)
     achieved = bytes_run / (head["walk_ms"] / 1e3) / 1e9
-    traffic = measured_traffic("k_walk_pool<8,true,true>", wl.name)
+    traffic = measured_traffic(f"k_walk_pool<{int(np.log2(wl.walkers))},true,true,true>",
+                               wl.name)
     out = {
         "value": head["value"], "ms_per_step": head["ms_per_step"],
         "workload": f"{wl.name}: {launches} snapshot batches of {I} insertions, threshold "
@@ -185,19 +197,28 @@ def leg_incremental(args, ctx) -> dict:
                                       "Lanczos on the host; plain and device H identical, so "
                                       "kappa is equal by construction",
                         "seconds": time.perf_counter() - ts}
-        log(f"incremental kappa: {k_before:.1f} -> {k_after:.1f}")
+        log(f"{leg} kappa: {k_before:.1f} -> {k_after:.1f}")
     if not args.no_cpu:
-        from oracle import pyoracle as po
+        found = head["found"]
 
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
-        og = po.Graph(wl.H.V, rp, col, w)
-        ts = time.perf_counter()
-        cfound, _ = og.incremental(wl.ins.u, wl.ins.v, wl.ins.w, wl.threshold, wl.walkers,
-                                   seed=args.seed, threads=threads)
-        cdt = time.perf_counter() - ts
-        out["cpu"] = {"value": m / cdt, "cores": threads,
-                      "decisions_identical": bool(np.array_equal(head["found"], cfound))}
-        log(f"incremental cpu: {m / cdt:.0f} edge-updates/s on {threads} threads")
+        def cpu_job():
+            from oracle import pyoracle as po
+
+            threads = cpu_threads(args)
+            og = po.Graph(wl.H.V, rp, col, w)
+            ts = time.perf_counter()
+            cfound, _ = og.incremental(wl.ins.u, wl.ins.v, wl.ins.w, wl.threshold, wl.walkers,
+                                       seed=args.seed, threads=threads)
+            cdt = time.perf_counter() - ts
+            log(f"{leg} cpu: {m / cdt:.0f} edge-updates/s on {threads} threads")
+            return {"value": m / cdt, "unit": "edge-updates/s", "cores": threads,
+                    "nproc": os.cpu_count(), "kind": "port",
+                    "sample": "the whole workload (every snapshot batch's walks and the "
+                              "sparsifier appends) through the oracle, OpenMP over each "
+                              "batch's edges",
+                    "decisions_identical": bool(np.array_equal(found, cfound))}
+
+        out["_cpu_job"] = cpu_job
     return out
 
 
@@ -312,27 +333,39 @@ def run_stream(ctx, wl, plain: bool, seed: int, setting, csr=None, dev_list=None
 
 def stream_roofline(runs, walkers: int, workload: str) -> dict:
     """Roofline of the batched stream's dominant kernel, the insertion walk,
-    over every insertion batch's walk launch in the timed passes: algorithmic
-    bytes (8 B per walker for the source row, 16 B per neighbour draw and per
-    redirect, 20 B per edge of I/O; hop and redirect counts from the kernel's
-    own counters) over the launches' HIP-event durations."""
+    over every insertion batch's walk launch in the timed passes:
+    algorithmic bytes (algorithmic_bytes: 16 B per neighbour draw, 8 B per
+    redirect, 20 B per edge; counts from the kernel's own counters) over the
+    launches' HIP-event durations.  `per_launch` lists the last pass's
+    launches as [edges, hops, redirects, microseconds], so the figure can be
+    recomputed from them (and from a kernel trace's durations)."""
     nbytes, ms, launches = 0.0, 0.0, 0
+    hops = redirects = edges = 0
     for r in runs:
         for b in r["walks"]:
             if b["op"] != "ins" or b["kept"] == 0:
                 continue
-            nbytes += (8.0 * walkers * b["kept"] + 16.0 * (b["hops"] + b["redirects"])
-                       + 20.0 * b["kept"])
+            nbytes += algorithmic_bytes(b["hops"], b["redirects"], b["kept"])
+            hops += b["hops"]
+            redirects += b["redirects"]
+            edges += b["kept"]
             ms += b["walk_ms"]
             launches += 1
     achieved = nbytes / (ms / 1e3) / 1e9 if ms > 0 else 0.0
     kernel = f"k_walk_pool<{int(np.log2(walkers))},true,true,true>"
+    last = [[b["kept"], b["hops"], b["redirects"], round(b["walk_ms"] * 1e3, 2)]
+            for b in runs[-1]["walks"] if b["op"] == "ins" and b["kept"] > 0]
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kernel, workload),
             "kernel": f"k_walk_pool<W={walkers}, insert, prune, decision only>",
             "workload": workload, "launches": launches,
             "bytes_per_launch": nbytes / max(launches, 1),
-            "avg_launch_ms": ms / max(launches, 1)}
+            "avg_launch_ms": ms / max(launches, 1),
+            "hops_per_launch": hops / max(launches, 1),
+            "redirects_per_launch": redirects / max(launches, 1),
+            "edges_per_launch": edges / max(launches, 1),
+            "bytes_basis": "16 B per neighbour draw + 8 B per redirect + 20 B per edge",
+            "per_launch": last}
 
 
 def leg_batched(args, ctx) -> dict:
@@ -401,37 +434,56 @@ def leg_batched(args, ctx) -> dict:
                            b.view(np.uint32) if b.dtype == np.float32 else b)
             for a, b in zip(pr["H"], head["H"]))
     if not args.no_cpu:
-        out["cpu"] = cpu_sample_batched(wl, args, setting)
+        gpu_out = head["outcomes"]
+        out["_cpu_job"] = lambda: cpu_stream_batched(wl, args, setting, csr, gpu_out)
     return out
 
 
-def cpu_sample_batched(wl, args, setting) -> dict:
-    """The oracle's insertion walks on the original H for the stream's
-    insertion batches (all of them, or as many as 30 s of CPU time allow),
-    OpenMP over each batch's edges."""
-    from dygrass_amd import csr_from_edges
+def cpu_threads(args) -> int:
+    """Host threads for the CPU baselines: --cpu-threads, else the cores this
+    process may use (OMP_NUM_THREADS when the environment sets it: a GPU box
+    allots its cores per GPU that way), else nproc."""
+    if args.cpu_threads:
+        return args.cpu_threads
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return min(n, int(env)) if env.isdigit() and int(env) > 0 else n
+
+
+def cpu_stream_batched(wl, args, setting, csr, gpu_outcomes) -> dict:
+    """The whole batched stream through the C++ stream oracle on the host's
+    cores: every batch's classification, G and H edits, walks (OpenMP over the
+    batch's edges; insertion walks decide only, as on the GPU) and recovery,
+    from the original graphs, batch after batch while the first 30 s last.
+    Its outcome codes are compared with the GPU pass's."""
     from oracle import pyoracle as po
 
-    threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
-    rp, col, w = csr_from_edges(wl.G.V, wl.H.u, wl.H.v, wl.H.w)
-    og = po.Graph(wl.G.V, rp, col, w)
-    n, base, secs = 0, 0, 0.0
-    for op, u, v, ww in wl.batches:
-        if op == "ins" and secs < 30.0:
-            ts = time.perf_counter()
-            og.walk_inc_batch(u, v, ww, wl.threshold, setting.w_ins, seed=args.seed,
-                              edge_id=np.arange(base, base + len(u), dtype=np.uint64),
-                              threads=threads, max_steps=setting.steps_ins)
-            secs += time.perf_counter() - ts
-            n += len(u)
-        base += len(u)
-    log(f"batched cpu sample: {n / secs:.0f} walked updates/s on {threads} threads")
-    return {"value": n / secs, "unit": "edge-updates/s", "cores": threads, "kind": "port",
-            "sample": f"insertion walks of the stream's insertion batches ({n} updates, "
-                      f"{secs:.1f} s of CPU time) "
-                      f"on the original sparsifier, {setting.w_ins} walkers, "
-                      f"{setting.steps_ins} steps, through the oracle, OpenMP over edges; "
-                      "graph bookkeeping not included"}
+    threads = cpu_threads(args)
+    s = setting
+    orc = po.StreamCpp(wl.G.V, thr=wl.threshold, w_ins=s.w_ins, w_del=s.w_del, seed=args.seed,
+                       steps_ins=s.steps_ins, steps_del=s.steps_del, threads=threads, csr=csr)
+    outs, nb = [], 0
+    for op, u, v, w in wl.batches:
+        outs.append(orc.apply(op, u, v, w))
+        nb += 1
+        if orc.stats()["total_seconds"] > 30.0:
+            break
+    st = orc.stats()
+    orc.close()
+    got = np.concatenate(outs)
+    n = len(got)
+    same = bool(np.array_equal(got, gpu_outcomes[:n]))
+    log(f"batched cpu: {n / st['total_seconds']:.0f} edge-updates/s on {threads} threads "
+        f"({nb} of {len(wl.batches)} batches, outcomes {'equal' if same else 'DIFFER'})")
+    return {"value": n / st["total_seconds"], "unit": "edge-updates/s", "cores": threads,
+            "nproc": os.cpu_count(), "kind": "port",
+            "sample": f"the stream's first {nb} of {len(wl.batches)} batches ({n} updates: "
+                      "classification, G and H edits, walks and recovery) through the C++ "
+                      f"stream oracle, OpenMP over each batch's walks, OMP_PROC_BIND="
+                      f"{os.environ.get('OMP_PROC_BIND', 'unset')}; "
+                      f"{st['walk_seconds']:.1f} of {st['total_seconds']:.1f} s in walks",
+            "outcomes_identical": same, "batches": nb,
+            "walk_seconds": st["walk_seconds"], "total_seconds": st["total_seconds"]}
 
 
 # ---------------------------------------------------------------------------
@@ -500,8 +552,19 @@ def leg_served(args, ctx) -> dict:
         f"p99 {pct(lat, 99):.2f} ms (<=1k edges: p99 {pct(lat[small], 99):.2f} ms)")
     cspan, _ = serve(np.zeros(n))  # every request waiting at t=0: capacity
     out["capacity"] = {"value": m / cspan, "span_s": cspan,
+                       "requests_per_s": n / cspan,
                        "note": "same trace, all requests arriving at once"}
-    log(f"served capacity: {m / cspan:.0f} updates/s")
+    log(f"served capacity: {m / cspan:.0f} updates/s ({n / cspan:.0f} requests/s)")
+    curve = []
+    for rate in [float(x) for x in args.rates.split(",") if x.strip()]:
+        # the same trace with its arrival times compressed to `rate`
+        rspan, rlat = serve(wl.arrivals * (args.rate / rate))
+        curve.append({"rate": rate, "updates_per_s": m / rspan,
+                      "p50_ms": pct(rlat, 50), "p99_ms": pct(rlat, 99),
+                      "p50_ms_le_1k": pct(rlat[small], 50), "p99_ms_le_1k": pct(rlat[small], 99)})
+        log(f"served at {rate:g}/s: p50 {pct(rlat, 50):.2f} ms, p99 {pct(rlat, 99):.2f} ms "
+            f"(<=1k edges: p99 {pct(rlat[small], 99):.2f} ms)")
+    out["latency_curve"] = curve
     if not args.no_plain and args.plain_requests > 0:
         k = min(n, args.plain_requests)
         pst = Stream(ctx, wl.G.V, g_edges, h_edges, sp(True))
@@ -549,22 +612,25 @@ def leg_decremental(args, ctx) -> dict:
         f"({time.time() - t0:.1f}s)")
     sp = stream_params(w_ins=512, w_del=512, threshold=0.0, seed=args.seed, max_batch=I,
                        max_inserts=4 * M, dense_no_desc=True)
-    times, counts = [], None
+    times, counts, walks = [], None, []
     dev_batches = upload_batches(ctx, [("del", D.u[b0:b0 + I], D.v[b0:b0 + I], None)
                                        for b0 in range(0, M, I)])  # resident in HBM
     for i in range(args.warmup + args.steps):
         st = Stream(ctx, G.V, (G.u, G.v, G.w), (H.u, H.v, H.w), sp)
         ctx.sync()
         ts = time.perf_counter()
-        agg = {}
+        agg, walk = {}, []
         for _, du, dv, _ in dev_batches:
             _, stt = st.apply_dev("del", du, dv)
             for k in ("skipped", "g_only", "recovered", "heuristic"):
                 agg[k] = agg.get(k, 0) + stt[k]
+            walk.append((stt["recovered"] + stt["heuristic"], stt["hops"], stt["redirects"],
+                         stt["walk_ms"]))
         ctx.sync()
         dt = time.perf_counter() - ts
         if i >= args.warmup:
             times.append(dt)
+            walks.extend(walk)
             log(f"decremental device step {i - args.warmup}: {dt * 1e3:.1f} ms")
         rp, col, _ = st.H.download()
         st.close()
@@ -578,6 +644,19 @@ def leg_decremental(args, ctx) -> dict:
                        "neighbour descriptors, batches resident in HBM)",
            "V": int(G.V), "E_G": int(len(G)), "E_H": int(len(H)), "deletions": M,
            "outcomes": counts, "H_after": len(h_dev)}
+    nbytes = sum(algorithmic_bytes(h, r, k, per_edge=16.0) for k, h, r, _ in walks)
+    wms = sum(x[3] for x in walks)
+    achieved = nbytes / (wms / 1e3) / 1e9 if wms > 0 else 0.0
+    out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": achieved / HBM_PEAK_GBS,
+                       "traffic": measured_traffic("k_walk_pool<9,false,true,false>", wl.name),
+                       "kernel": "k_walk_pool<W=512, delete, prune, row table per hop>",
+                       "workload": wl.name, "launches": len(walks),
+                       "bytes_per_launch": nbytes / max(len(walks), 1),
+                       "avg_launch_ms": wms / max(len(walks), 1),
+                       "hops_per_launch": sum(x[1] for x in walks) / max(len(walks), 1),
+                       "bytes_basis": "16 B per neighbour draw + 8 B per redirect + 16 B per "
+                                      "edge (winner path not counted)"}
     if not args.no_plain:
         grp, gcol, gw = csr_from_edges(G.V, G.u, G.v, G.w)
         dual = DualGraph(ctx, G.V, grp, gcol, gw, H.u, H.v)
@@ -609,16 +688,19 @@ def main() -> int:
     ap.add_argument("--scale", type=float, default=1.0, help="workload scale (tests only)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--workload", choices=["all", "batched", "served", "incremental",
-                                           "decremental"], default="all")
+                                           "decremental", "grid"], default="all")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparisons")
     ap.add_argument("--no-kappa", action="store_true", help="skip the FP64 kappa check")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores (max 16)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="0 = the cores this process may use (OMP_NUM_THREADS when set)")
     ap.add_argument("--block-edges", type=int, default=0,
                     help="edges per workgroup of the work-queue walk kernel (0 = automatic)")
     ap.add_argument("--requests", type=int, default=1500, help="served: requests in the trace")
     ap.add_argument("--rate", type=float, default=1000.0, help="served: arrivals per second")
     ap.add_argument("--slots", type=int, default=8, help="served: requests admitted at once")
+    ap.add_argument("--rates", default="2000,3000,4000,5000",
+                    help="served: further arrival rates (per second) for the latency curve")
     ap.add_argument("--plain-requests", type=int, default=40,
                     help="served: requests of the trace the plain form serves")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -632,6 +714,10 @@ def main() -> int:
               "this build does not implement", file=sys.stderr)
         return 2
 
+    # CPU baselines bind their OpenMP threads to neighbouring cores; they run
+    # after every GPU leg, so the binding (which the main thread takes too)
+    # never reaches the served loop's threads
+    os.environ.setdefault("OMP_PROC_BIND", "close")
     from dygrass_amd import Context, DeviceArray
 
     ctx = Context(0)
@@ -646,22 +732,23 @@ def main() -> int:
         log(f"calibration copy: {args.copy_calibration_mb} MiB device to device")
         a.close()
         b.close()
-    want = (["batched", "served", "incremental", "decremental"] if args.workload == "all"
-            else [args.workload])
+    want = (["batched", "served", "incremental", "decremental", "grid"]
+            if args.workload == "all" else [args.workload])
     legs = {}
     for name in want:
         legs[name] = {"batched": leg_batched, "served": leg_served,
                       "incremental": leg_incremental,
-                      "decremental": leg_decremental}[name](args, ctx)
+                      "decremental": leg_decremental,
+                      "grid": lambda a, c: leg_incremental(a, c, grid=True)}[name](args, ctx)
+    for name, leg in legs.items():  # CPU baselines, after all GPU work
+        job = leg.pop("_cpu_job", None)
+        if job is not None:
+            leg["cpu"] = job()
+            leg["speedup_vs_cpu"] = leg["value"] / leg["cpu"]["value"]
     head_name = want[0]
     head = legs[head_name]
     inc = legs.get("incremental")
     cpu = head.pop("cpu", None)
-    if head_name == "incremental" and cpu is not None:
-        cpu = {"value": cpu["value"], "unit": "edge-updates/s", "cores": cpu["cores"],
-               "kind": "port", "sample": "the whole workload through the oracle, OpenMP over "
-                                         "each batch's edges",
-               "decisions_identical": cpu["decisions_identical"]}
     out = {
         "metric": METRIC,
         "value": head["value"],

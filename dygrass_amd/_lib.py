@@ -157,6 +157,7 @@ _SIGS = {
                                      C.POINTER(C.c_uint64)]),
     "dyg_serve_wait": (C.c_int32, [vp, C.c_uint64, vp, C.POINTER(ServeResult)]),
     "dyg_serve_close": (None, [vp]),
+    "dyg_serve_fail_next_copy": (C.c_int32, [vp]),
     "dyg_mtx_read_graph": (C.c_int32, [C.c_char_p, C.c_int, C.c_int, C.POINTER(vp)]),
     "dyg_mtx_read_edges": (C.c_int32, [C.c_char_p, C.c_int, C.POINTER(vp)]),
     "dyg_mtx_read_auto": (C.c_int32, [C.c_char_p, C.POINTER(MtxInfo), C.POINTER(vp)]),

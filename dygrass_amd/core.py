@@ -528,6 +528,11 @@ class Server:
               self.stream.ctx.handle)
         return out[:n], {k: getattr(res, k) for k, _ in L.ServeResult._fields_}
 
+    def fail_next_copy(self) -> None:
+        """Testing hook (dyg_serve_fail_next_copy): the next admitted request
+        that copies its input reports the copy as failed."""
+        check(self._lib.dyg_serve_fail_next_copy(self.handle), self.stream.ctx.handle)
+
     def close(self) -> None:
         if self.handle:
             self._lib.dyg_serve_close(self.handle)

@@ -11,11 +11,15 @@
 namespace dyg {
 
 namespace {
-thread_local std::string g_tls_error;
+thread_local std::string g_tls_error;   // last failure of this thread
+thread_local std::string g_tls_report;  // what dyg_last_error returned to this thread
 }
 
 void set_error(dyg_ctx ctx, const std::string& msg) {
-    if (ctx) ctx->last_error = msg;
+    if (ctx) {
+        std::lock_guard<std::mutex> lk(ctx->err_mu);
+        ctx->last_error = msg;
+    }
     g_tls_error = msg;
 }
 
@@ -181,8 +185,15 @@ extern "C" void dyg_ctx_destroy(dyg_ctx ctx) {
 }
 
 extern "C" const char* dyg_last_error(dyg_ctx ctx) {
-    if (ctx) return ctx->last_error.c_str();
-    return g_tls_error.c_str();
+    // a copy owned by the calling thread: a server's worker may set the
+    // context's message at any time
+    if (ctx) {
+        std::lock_guard<std::mutex> lk(ctx->err_mu);
+        g_tls_report = ctx->last_error;
+    } else {
+        g_tls_report = g_tls_error;
+    }
+    return g_tls_report.c_str();
 }
 
 extern "C" void* dyg_ctx_stream(dyg_ctx ctx) { return ctx ? (void*)ctx->stream : nullptr; }

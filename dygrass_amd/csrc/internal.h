@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <utility>
@@ -42,6 +43,9 @@ struct dyg_ctx_s {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // written by the caller's thread and, while a server is open, by its
+    // worker threads: always under err_mu
+    std::mutex err_mu;
     std::string last_error;
     dyg::Pinned stage;         // H2D / D2H staging
     dyg::DevBuf dscratch;      // per-call device staging of inputs/outputs

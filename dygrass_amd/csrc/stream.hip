@@ -44,6 +44,7 @@
 #include <unordered_set>
 #include <vector>
 
+#include "admission.h"
 #include "internal.h"
 
 namespace dyg {
@@ -993,7 +994,9 @@ dyg_status apply_delete_small(dyg_stream s, const uint32_t* du, const uint32_t* 
     r = wait_spin(s->ctx, st, s->ev_sync);
     if (r != DYG_OK) return r;
     const uint32_t* hc = (const uint32_t*)((char*)s->pin + kPinSizes);
-    if (((const unsigned long long*)s->pin)[kReportBad]) return DYG_OK;  // rejected: nothing to do
+    // rejected by the pre kernel's id check (device inputs only; the word is
+    // the stream's own, cleared before this batch): nothing to do
+    if (vcheck && ((const unsigned long long*)s->pin)[kReportBad]) return DYG_OK;
     if (hc[4] == 0) return DYG_OK;  // the post kernel finished the batch
     DYG_HIP(s->ctx, launch_heuristic(s->G->d_rows, s->G->d_adj, s->ep, s->ekey, 2 * n, hw,
                                      s->sp.del.seed, s->path3, st, s->cnt + 2, s->G->nodesc));
@@ -1456,7 +1459,6 @@ struct ServeSlot {
     void* host = nullptr;     // pinned, laid out by pin_layout; device address hdev
     char* hdev = nullptr;
     hipEvent_t e_in = nullptr, e_done = nullptr;
-    bool busy = false;
 };
 
 struct ServeReq {
@@ -1480,10 +1482,11 @@ struct dyg_serve_s {
     hipStream_t copy = nullptr;  // inputs in
     uint32_t slot_cap = 0;
     std::vector<ServeSlot> slots;
-    uint64_t budget = 0, resident = 0, admitted = 0;
+    dyg::AdmissionGate gate;  // slots and the memory budget (under mu)
     uint64_t used_est_g = 0, used_est_h = 0;  // arena bump pointers, upper estimates
     std::mutex mu;
-    std::condition_variable cv_work, cv_inflight, cv_done, cv_admit;
+    std::condition_variable cv_work, cv_inflight, cv_done;
+    bool fail_next_copy = false;  // testing hook (dyg_serve_fail_next_copy)
     std::deque<ServeReq*> queue, inflight;
     std::unordered_map<uint64_t, ServeReq*> reqs;
     std::thread worker, completer;
@@ -1526,7 +1529,8 @@ dyg_status ensure_headroom(dyg_serve_s* v, const ServeReq& r) {
     size_t mfree = 0, mtotal = 0;  // arena growth is resident state from now on
     if (hipMemGetInfo(&mfree, &mtotal) == hipSuccess) {
         std::lock_guard<std::mutex> lk(v->mu);
-        v->resident = std::max<uint64_t>(v->resident, mtotal - mfree - v->admitted);
+        const uint64_t used = mtotal - mfree, adm = v->gate.admitted();
+        v->gate.raise_resident(used > adm ? used - adm : 0);
     }
     if (v->used_est_g > s->G->cap_slots || v->used_est_h > s->H->cap_slots) {
         set_error(s->ctx, "serve: arena headroom could not be reserved");
@@ -1554,7 +1558,13 @@ void serve_worker(dyg_serve_s* v) {
         }
         ServeSlot& sl = v->slots[r->slot];
         r->t_start = now_ms();
-        dyg_status st = v->failed;
+        // a request whose submit failed (its input never reached the device)
+        // is never applied, and neither is anything after a failure
+        dyg_status st = r->status;
+        if (st == DYG_OK) {
+            std::lock_guard<std::mutex> lk(v->mu);
+            st = v->failed;
+        }
         if (st == DYG_OK) st = ensure_headroom(v, *r);
         if (st == DYG_OK && r->copied && hipStreamWaitEvent(cs, sl.e_in, 0) != hipSuccess)
             st = DYG_ERR_HIP;
@@ -1579,12 +1589,16 @@ void serve_worker(dyg_serve_s* v) {
         }
         std::lock_guard<std::mutex> lk(v->mu);
         if (st != DYG_OK) {
-            r->status = st;
-            r->error = dyg_last_error(s->ctx);
+            if (r->status == DYG_OK) {
+                r->status = st;
+                r->error = v->failed != DYG_OK ? "serve: stopped after an error: " + v->fail_msg
+                                               : std::string(dyg_last_error(s->ctx));
+            }
             if (v->failed == DYG_OK) {
                 v->failed = st;
                 v->fail_msg = r->error;
             }
+            v->gate.fail(v->failed);
         }
         v->inflight.push_back(r);
         v->cv_inflight.notify_one();
@@ -1615,11 +1629,9 @@ void serve_completer(dyg_serve_s* v) {
         }
         r->t_done = now_ms();
         std::lock_guard<std::mutex> lk(v->mu);
-        sl.busy = false;
-        v->admitted -= r->need;
+        v->gate.release(r->slot, r->need);
         r->done = true;
         v->cv_done.notify_all();
-        v->cv_admit.notify_all();
     }
 }
 
@@ -1664,15 +1676,19 @@ extern "C" dyg_status dyg_serve_open(dyg_stream s, uint32_t slots, uint32_t max_
         dyg_serve_close(v);
         return DYG_ERR_NOMEM;
     }
-    v->budget = budget_bytes ? std::min<uint64_t>(budget_bytes, mtotal) : mtotal;
-    v->resident = mtotal - mfree;  // graphs, stream buffers and slots already in place
-    if (v->resident >= v->budget) {
+    const uint64_t budget = budget_bytes ? std::min<uint64_t>(budget_bytes, mtotal) : mtotal;
+    // graphs, stream buffers and slots already in place
+    v->gate.open(slots, budget, mtotal - mfree);
+    if (v->gate.resident() >= budget) {
         set_error(ctx, "serve: resident state already exceeds the memory budget");
         dyg_serve_close(v);
         return DYG_ERR_ADMISSION;
     }
     v->used_est_g = s->G->used_slots;
     v->used_est_h = s->H->used_slots;
+    // served requests report into their slots; the stream's own rejection
+    // word must not carry over from an earlier rejected dyg_stream_apply_dev
+    ((unsigned long long*)s->pin)[kReportBad] = 0;
     v->worker = std::thread(serve_worker, v);
     v->completer = std::thread(serve_completer, v);
     *out = v;
@@ -1705,31 +1721,18 @@ extern "C" dyg_status dyg_serve_submit(dyg_serve v, int op, const uint32_t* u, c
     uint32_t slot = 0;
     {
         std::unique_lock<std::mutex> lk(v->mu);
-        if (v->failed != DYG_OK) {
-            set_error(ctx, "serve: stopped after an error: " + v->fail_msg);
-            delete r;
-            return v->failed;
-        }
-        if (v->resident + need > v->budget) {
-            set_error(ctx, "serve submit: request cannot fit the memory budget");
-            delete r;
-            return DYG_ERR_ADMISSION;
-        }
         // admission: a free slot and room in the budget; tickets (and the
         // order of application) follow the order of admission
-        v->cv_admit.wait(lk, [&] {
-            if (v->resident + v->admitted + need > v->budget) return false;
-            for (const ServeSlot& sl : v->slots)
-                if (!sl.busy) return true;
-            return false;
-        });
-        for (slot = 0; v->slots[slot].busy; ++slot) {
+        const dyg_status adm = v->gate.admit(lk, need, &slot);
+        if (adm != DYG_OK) {
+            set_error(ctx, adm == DYG_ERR_ADMISSION
+                               ? std::string("serve submit: request cannot fit the memory budget")
+                           : adm == DYG_ERR_STATE ? std::string("serve submit: server is closing")
+                                                  : "serve: stopped after an error: " + v->fail_msg);
+            delete r;
+            return adm;
         }
-        v->slots[slot].busy = true;
-        v->admitted += need;
         r->slot = slot;
-        r->ticket = v->next_ticket++;
-        v->reqs[r->ticket] = r;
         // the input goes to the device now, ahead of the requests before it
         // (a small insertion request is read in place from pinned memory)
         ServeSlot& sl = v->slots[slot];
@@ -1739,17 +1742,43 @@ extern "C" dyg_status dyg_serve_submit(dyg_serve v, int op, const uint32_t* u, c
         std::memcpy(hp.v, w_v, n * 4ull);
         if (ins) std::memcpy(hp.w, w, n * 4ull);
         r->copied = !(ins && n <= kSmallBatch && !(v->s->sp.flags & DYG_STREAM_GENERAL_ONLY));
-        if (r->copied &&
-            !(hipMemcpyAsync(sl.din, hp.u, n * (ins ? 12ull : 8ull), hipMemcpyHostToDevice,
-                             v->copy) == hipSuccess &&
-              hipEventRecord(sl.e_in, v->copy) == hipSuccess)) {
-            r->status = DYG_ERR_HIP;
-            r->error = "serve submit: input copy failed";
+        bool copy_ok = true;
+        if (r->copied) {
+            if (v->fail_next_copy) {  // testing hook: report a failure, issue nothing
+                v->fail_next_copy = false;
+                copy_ok = false;
+            } else {
+                copy_ok = hipMemcpyAsync(sl.din, hp.u, n * (ins ? 12ull : 8ull),
+                                         hipMemcpyHostToDevice, v->copy) == hipSuccess &&
+                          hipEventRecord(sl.e_in, v->copy) == hipSuccess;
+            }
         }
+        if (!copy_ok) {
+            // the request never reaches the worker; the device's state is no
+            // longer known, so the server stops: every later request fails
+            v->gate.release(slot, need);
+            if (v->failed == DYG_OK) {
+                v->failed = DYG_ERR_HIP;
+                v->fail_msg = "serve submit: input copy failed";
+            }
+            v->gate.fail(v->failed);
+            set_error(ctx, "serve submit: input copy failed");
+            delete r;
+            return DYG_ERR_HIP;
+        }
+        r->ticket = v->next_ticket++;
+        v->reqs[r->ticket] = r;
         v->queue.push_back(r);
         *ticket = r->ticket;
     }
     v->cv_work.notify_one();
+    return DYG_OK;
+}
+
+extern "C" dyg_status dyg_serve_fail_next_copy(dyg_serve v) {
+    if (!v) return DYG_ERR_ARG;
+    std::lock_guard<std::mutex> lk(v->mu);
+    v->fail_next_copy = true;
     return DYG_OK;
 }
 
@@ -1786,6 +1815,7 @@ extern "C" void dyg_serve_close(dyg_serve v) {
     {
         std::lock_guard<std::mutex> lk(v->mu);
         v->stop = true;
+        v->gate.close();
     }
     v->cv_work.notify_all();
     v->cv_inflight.notify_all();

@@ -382,10 +382,18 @@ typedef struct {
 dyg_status dyg_serve_open(dyg_stream s, uint32_t slots, uint32_t max_request,
                           uint64_t budget_bytes, dyg_serve* out);
 /* Blocks until the request is admitted (a slot and memory free); returns
- * DYG_ERR_ADMISSION at once when the request could never fit the budget.
- * The arrays are copied before the call returns. */
+ * DYG_ERR_ADMISSION when the request cannot fit the budget (at once, or
+ * after the server's resident state grew while it waited), DYG_ERR_STATE
+ * when the server is closing, and the server's failure status once a
+ * request has failed (the server then stops: no later request is applied).
+ * The arrays are copied before the call returns.  A failed input copy fails
+ * the call with DYG_ERR_HIP, applies nothing and stops the server. */
 dyg_status dyg_serve_submit(dyg_serve v, int op, const uint32_t* u, const uint32_t* v_,
                             const float* w, uint32_t n, uint64_t* ticket);
+/* Testing hook: the next admitted request that copies its input reports the
+ * copy as failed without issuing it (the failure path of dyg_serve_submit,
+ * forced from the host).  No reference counterpart. */
+dyg_status dyg_serve_fail_next_copy(dyg_serve v);
 /* Waits for one request; outcome[i] (optional) receives DYG_OUT_* codes. */
 dyg_status dyg_serve_wait(dyg_serve v, uint64_t ticket, int32_t* outcome,
                           dyg_serve_result* res);

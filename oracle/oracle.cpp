@@ -1195,3 +1195,228 @@ int orc_decremental_run(const char* dir, uint64_t seed, int threads, const char*
 }
 
 }  // extern "C"
+
+/* ==================================================================== */
+/* Mixed insertion/deletion stream (C++ restatement of oracle/stream.py, */
+/* the sequential stream oracle; same semantics, same order of edits)     */
+/* ==================================================================== */
+namespace {
+
+/* Slot of b in row a, or -1 (rows are short: a scan). */
+inline int row_pos(const orc_graph* g, uint32_t a, uint32_t b) {
+    const std::vector<Slot>& r = g->rows[a];
+    for (size_t i = 0; i < r.size(); ++i)
+        if (r[i].col == b) return (int)i;
+    return -1;
+}
+
+inline void row_append(orc_graph* g, uint32_t a, uint32_t b, float w) {
+    g->rows[a].push_back(Slot{b, w});
+    g->rows[b].push_back(Slot{a, w});
+}
+
+inline void row_add_weight(orc_graph* g, uint32_t a, uint32_t b, float dw) {
+    Slot& x = g->rows[a][row_pos(g, a, b)];
+    x.w = x.w + dw;
+    Slot& y = g->rows[b][row_pos(g, b, a)];
+    y.w = y.w + dw;
+}
+
+/* swap-with-last in row a, then in row b */
+inline void row_delete(orc_graph* g, uint32_t a, uint32_t b) {
+    for (int side = 0; side < 2; ++side) {
+        const uint32_t x = side ? b : a, y = side ? a : b;
+        std::vector<Slot>& r = g->rows[x];
+        r[row_pos(g, x, y)] = r.back();
+        r.pop_back();
+    }
+}
+
+/* Insertion walks deciding only: an edge is found once any walker reaches
+ * its target (the stream reports found / not found; the walks themselves
+ * are random_walk_incremental/main.cu:79-163). */
+bool inc_decide(const orc_graph* H, uint32_t s, uint32_t t, float w_e, float thr, uint32_t W,
+                uint32_t max_steps, uint64_t seed, uint64_t eid, uint64_t* hops) {
+    for (uint32_t j = 0; j < W; ++j) {
+        Stream rng(seed, ORC_TAG_INSERT, eid, j);
+        float R;
+        if (walk_inc_one(H, s, t, w_e, thr, max_steps, rng, R, *hops, nullptr) != -1) return true;
+    }
+    return false;
+}
+
+}  // namespace
+
+struct orc_stream {
+    orc_graph G, H;
+    float thr;
+    uint32_t w_ins, w_del, steps_ins, steps_del, heur;
+    uint64_t seed;
+    int threads;
+    uint64_t base = 0;
+    orc_stream_stats st{};
+};
+
+namespace {
+
+double since(double t0) { return now_s() - t0; }
+
+void stream_recover(orc_stream* s, uint32_t x, uint32_t y) {
+    if (x == y) return;
+    const uint32_t a = std::min(x, y), b = std::max(x, y);
+    if (row_pos(&s->H, a, b) < 0) row_append(&s->H, a, b, s->G.rows[a][row_pos(&s->G, a, b)].w);
+}
+
+void stream_insert(orc_stream* s, const uint32_t* u, const uint32_t* v, const float* w,
+                   uint32_t n, int32_t* out) {
+    std::vector<uint32_t> keep;
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t a = std::min(u[i], v[i]), b = std::max(u[i], v[i]);
+        if (a == b) {
+            out[i] = 8;  // invalid
+        } else if (row_pos(&s->G, a, b) >= 0) {
+            row_add_weight(&s->G, a, b, w[i]);
+            if (row_pos(&s->H, a, b) >= 0) row_add_weight(&s->H, a, b, w[i]);
+            out[i] = 1;  // weight update
+        } else {
+            row_append(&s->G, a, b, w[i]);
+            keep.push_back(i);
+        }
+    }
+    const int64_t nk = (int64_t)keep.size();
+    std::vector<uint8_t> found(nk);
+    std::vector<uint64_t> hops(nk, 0);
+    const double tw = now_s();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(s->threads > 0 ? s->threads : 1)
+    for (int64_t j = 0; j < nk; ++j) {
+        const uint32_t i = keep[j];
+        found[j] = inc_decide(&s->H, u[i], v[i], w[i], s->thr, s->w_ins, s->steps_ins, s->seed,
+                              s->base + i, &hops[j]);
+    }
+    s->st.walk_seconds += since(tw);
+    for (int64_t j = 0; j < nk; ++j) {
+        const uint32_t i = keep[j];
+        s->st.hops += hops[j];
+        if (!found[j]) {
+            row_append(&s->H, std::min(u[i], v[i]), std::max(u[i], v[i]), w[i]);
+            out[i] = 3;  // added
+        } else {
+            out[i] = 2;  // found
+        }
+    }
+}
+
+void stream_delete(orc_stream* s, const uint32_t* u, const uint32_t* v, uint32_t n,
+                   int32_t* out) {
+    std::vector<uint32_t> keep;
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t a = std::min(u[i], v[i]), b = std::max(u[i], v[i]);
+        if (a == b) {
+            out[i] = 8;
+        } else if (row_pos(&s->G, a, b) < 0) {
+            out[i] = 4;  // skip
+        } else {
+            row_delete(&s->G, a, b);
+            if (row_pos(&s->H, a, b) >= 0) {
+                row_delete(&s->H, a, b);
+                keep.push_back(i);
+            } else {
+                out[i] = 5;  // G only
+            }
+        }
+    }
+    const int64_t nk = (int64_t)keep.size();
+    const uint32_t ms = s->steps_del;
+    std::vector<uint32_t> src(nk), dst(nk), path((size_t)nk * ms);
+    std::vector<uint64_t> eid(nk), hops(nk);
+    std::vector<int32_t> f(nk);
+    std::vector<float> c(nk);
+    for (int64_t j = 0; j < nk; ++j) {
+        const uint32_t i = keep[j];
+        src[j] = std::min(u[i], v[i]);
+        dst[j] = std::max(u[i], v[i]);
+        eid[j] = s->base + i;
+    }
+    const double tw = now_s();
+    orc_walk_dec_batch(&s->G, (uint32_t)nk, src.data(), dst.data(), eid.data(), s->w_del, ms,
+                       s->seed, s->threads, f.data(), c.data(), nullptr, hops.data(),
+                       path.data());
+    s->st.walk_seconds += since(tw);
+    for (int64_t j = 0; j < nk; ++j) {
+        s->st.hops += hops[j];
+        if (f[j] > 0) {
+            for (int q = 1; q <= f[j]; ++q)
+                stream_recover(s, path[(size_t)j * ms + q - 1], path[(size_t)j * ms + q]);
+            out[keep[j]] = 6;  // recovered
+        }
+    }
+    std::vector<uint32_t> p3((size_t)s->heur * 3);
+    for (int64_t j = 0; j < nk; ++j) {
+        if (f[j] > 0) continue;
+        const uint64_t e = s->base + keep[j];
+        for (int side = 0; side < 2; ++side) {
+            heuristic_one(&s->G, side ? dst[j] : src[j], s->heur, s->seed, (e << 1) | side,
+                          p3.data());
+            for (uint32_t q = 0; q < s->heur; ++q) {
+                stream_recover(s, p3[q * 3], p3[q * 3 + 1]);
+                stream_recover(s, p3[q * 3 + 1], p3[q * 3 + 2]);
+            }
+        }
+        out[keep[j]] = 7;  // heuristic
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+orc_stream* orc_stream_new(uint32_t V, const uint64_t* g_rp, const uint32_t* g_col,
+                           const float* g_w, const uint64_t* h_rp, const uint32_t* h_col,
+                           const float* h_w, float thr, uint32_t w_ins, uint32_t w_del,
+                           uint32_t steps_ins, uint32_t steps_del, uint32_t heur, uint64_t seed,
+                           int threads) {
+    auto* s = new orc_stream;
+    for (int k = 0; k < 2; ++k) {
+        const uint64_t* rp = k ? h_rp : g_rp;
+        const uint32_t* col = k ? h_col : g_col;
+        const float* w = k ? h_w : g_w;
+        orc_graph& g = k ? s->H : s->G;
+        g.rows.resize(V);
+        for (uint32_t x = 0; x < V; ++x)
+            for (uint64_t p = rp[x]; p < rp[x + 1]; ++p) g.rows[x].push_back(Slot{col[p], w[p]});
+    }
+    s->thr = thr;
+    s->w_ins = w_ins;
+    s->w_del = w_del;
+    s->steps_ins = steps_ins;
+    s->steps_del = steps_del;
+    s->heur = heur;
+    s->seed = seed;
+    s->threads = threads;
+    return s;
+}
+
+int orc_stream_apply(orc_stream* s, int op, const uint32_t* u, const uint32_t* v, const float* w,
+                     uint32_t n, int32_t* out) {
+    const uint32_t V = (uint32_t)s->G.rows.size();
+    for (uint32_t i = 0; i < n; ++i)
+        if (u[i] >= V || v[i] >= V) return -1;
+    const double t0 = now_s();
+    if (op == 0)
+        stream_insert(s, u, v, w, n, out);
+    else
+        stream_delete(s, u, v, n, out);
+    s->base += n;
+    s->st.updates += n;
+    s->st.batches += 1;
+    s->st.total_seconds += since(t0);
+    return 0;
+}
+
+const orc_graph* orc_stream_graph(const orc_stream* s, int which) { return which ? &s->H : &s->G; }
+
+void orc_stream_get_stats(const orc_stream* s, orc_stream_stats* st) { *st = s->st; }
+
+void orc_stream_free(orc_stream* s) { delete s; }
+
+}  // extern "C"

@@ -140,6 +140,32 @@ int orc_decremental_lines(uint32_t V, const uint32_t* g_u, const uint32_t* g_v, 
 int orc_decremental_run(const char* dir, uint64_t seed, int threads, const char* log_path,
                         orc_run_stats* st);
 
+/* ---- mixed insertion/deletion stream ----
+ * The C++ restatement of oracle/stream.py (same semantics and order of edits:
+ * dynamic_update/gpu_graph.cuh:361-439 preprocessing, :451-465 weight update,
+ * :620-700 sparsifier update, with the plain updaters' walks and heuristic).
+ * Insertion walks decide only (an edge is found once any walker reaches the
+ * target); deletion walks run every walker and recover the winner's path.
+ * Walks are OpenMP-parallel over a batch's edges; the graph edits are
+ * sequential, in batch order.  Outcome codes as oracle/stream.py. */
+typedef struct orc_stream orc_stream;
+typedef struct {
+    uint64_t updates, batches, hops;
+    double walk_seconds;  /* wall time in the walks */
+    double total_seconds; /* wall time of orc_stream_apply calls */
+} orc_stream_stats;
+orc_stream* orc_stream_new(uint32_t V, const uint64_t* g_rp, const uint32_t* g_col,
+                           const float* g_w, const uint64_t* h_rp, const uint32_t* h_col,
+                           const float* h_w, float thr, uint32_t w_ins, uint32_t w_del,
+                           uint32_t steps_ins, uint32_t steps_del, uint32_t heur, uint64_t seed,
+                           int threads);
+/* op 0 insertion, 1 deletion; out[n] receives outcome codes.  -1 on a bad id. */
+int orc_stream_apply(orc_stream* s, int op, const uint32_t* u, const uint32_t* v, const float* w,
+                     uint32_t n, int32_t* out);
+const orc_graph* orc_stream_graph(const orc_stream* s, int which); /* 0 G, 1 H */
+void orc_stream_get_stats(const orc_stream* s, orc_stream_stats* st);
+void orc_stream_free(orc_stream* s);
+
 /* ---- parse restatements, exposed for pinning against the reference's own parser ---- */
 /* random_walk_incremental/functions.cpp:58-239 (0-based, undirected).  Returns V;
  * fills begin[V+1], adj/weight[2L] when the pointers are non-NULL.  *lines gets L. */

@@ -32,7 +32,19 @@ class RunStats(C.Structure):
                 ("walk_seconds", C.c_double), ("total_seconds", C.c_double)]
 
 
+class StreamStats(C.Structure):
+    _fields_ = [("updates", C.c_uint64), ("batches", C.c_uint64), ("hops", C.c_uint64),
+                ("walk_seconds", C.c_double), ("total_seconds", C.c_double)]
+
+
 _SIGS = {
+    "orc_stream_new": (vp, [C.c_uint32, u64p, u32p, f32p, u64p, u32p, f32p, C.c_float,
+                            C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                            C.c_uint64, C.c_int]),
+    "orc_stream_apply": (C.c_int, [vp, C.c_int, u32p, u32p, f32p, C.c_uint32, i32p]),
+    "orc_stream_graph": (vp, [vp, C.c_int]),
+    "orc_stream_get_stats": (None, [vp, C.POINTER(StreamStats)]),
+    "orc_stream_free": (None, [vp]),
     "orc_philox4x32_10": (None, [u32p, u32p, u32p]),
     "orc_draw": (C.c_uint32, [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32]),
     "orc_tree_reduce": (C.c_uint32, [f32p, C.c_uint32]),
@@ -322,3 +334,82 @@ def read_targets(path: str, weighted: bool = True, divisor: int = 10):
     w = np.zeros(max(m, 1), np.float32)
     lib().orc_read_targets(path.encode(), int(weighted), divisor, _p(nodes, u32p), _p(w, f32p))
     return nodes[: 2 * m], w[:m]
+
+
+def _csr_of(V, u, v, w):
+    """Row lists in slot order from an edge list: edge (a,b,w) appends b to
+    a's row, then a to b's row (the order every graph here is built in)."""
+    u = np.ascontiguousarray(u, np.uint32)
+    v = np.ascontiguousarray(v, np.uint32)
+    w = np.ascontiguousarray(w, np.float32)
+    m = len(u)
+    src = np.empty(2 * m, np.uint32)
+    dst = np.empty(2 * m, np.uint32)
+    ww = np.empty(2 * m, np.float32)
+    src[0::2], src[1::2] = u, v
+    dst[0::2], dst[1::2] = v, u
+    ww[0::2], ww[1::2] = w, w
+    order = np.argsort(src, kind="stable")
+    rp = np.zeros(V + 1, np.uint64)
+    np.cumsum(np.bincount(src, minlength=V), out=rp[1:])
+    return rp, np.ascontiguousarray(dst[order]), np.ascontiguousarray(ww[order])
+
+
+class StreamCpp:
+    """The stream oracle in C++ (orc_stream_*): same semantics and outcome
+    codes as oracle/stream.py, fast enough for full-size workloads and used
+    as the CPU baseline of the stream forms (walks OpenMP-parallel over each
+    batch's edges, graph edits sequential).  Insertion walks decide only.
+    g_edges / h_edges: (u, v, w) edge lists, or csr=((rp, col, w), (rp, col, w))."""
+
+    def __init__(self, V, g_edges=None, h_edges=None, thr=100.0, w_ins=256, w_del=512, seed=0,
+                 steps_ins=100, steps_del=100, heur_walkers=16, threads=1, csr=None):
+        self.V = int(V)
+        if csr is None:
+            csr = (_csr_of(V, *g_edges), _csr_of(V, *h_edges))
+        (grp, gcol, gw), (hrp, hcol, hw) = csr
+        a = [np.ascontiguousarray(x, t) for x, t in ((grp, np.uint64), (gcol, np.uint32),
+                                                      (gw, np.float32), (hrp, np.uint64),
+                                                      (hcol, np.uint32), (hw, np.float32))]
+        self.h = lib().orc_stream_new(self.V, _p(a[0], u64p), _p(a[1], u32p), _p(a[2], f32p),
+                                      _p(a[3], u64p), _p(a[4], u32p), _p(a[5], f32p), thr, w_ins,
+                                      w_del, steps_ins, steps_del, heur_walkers, seed, threads)
+
+    def apply(self, op: str, u, v, w=None) -> np.ndarray:
+        u = np.ascontiguousarray(u, np.uint32)
+        v = np.ascontiguousarray(v, np.uint32)
+        n = len(u)
+        ww = np.ascontiguousarray(w, np.float32) if op == "ins" else None
+        out = np.zeros(max(n, 1), np.int32)
+        rc = lib().orc_stream_apply(self.h, 0 if op == "ins" else 1, _p(u, u32p), _p(v, u32p),
+                                    _p(ww, f32p), n, _p(out, i32p))
+        if rc != 0:
+            raise ValueError("oracle stream: vertex id out of range")
+        return out[:n]
+
+    def csr(self, which: int):
+        """(row_ptr, col, w) of G (0) or H (1) in slot order."""
+        g = lib().orc_stream_graph(self.h, which)
+        V = lib().orc_graph_nv(g)
+        n = lib().orc_graph_nslots(g)
+        rp = np.zeros(V + 1, np.uint64)
+        col = np.zeros(max(n, 1), np.uint32)
+        w = np.zeros(max(n, 1), np.float32)
+        lib().orc_graph_export(g, _p(rp, u64p), _p(col, u32p), _p(w, f32p))
+        return rp, col[:n], w[:n]
+
+    def stats(self) -> dict:
+        st = StreamStats()
+        lib().orc_stream_get_stats(self.h, C.byref(st))
+        return {k: getattr(st, k) for k, _ in StreamStats._fields_}
+
+    def close(self) -> None:
+        if self.h:
+            lib().orc_stream_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -156,3 +156,94 @@ def test_full_size_batched_stream_properties(ctx):
     assert np.all(np.isin(ik[io == S.ADDED], hk))
     assert not np.any(np.isin(dk, hk))
     dev.close()
+
+
+@pytest.mark.timeout(900)
+def test_full_size_served_properties(ctx):
+    """The served workload at full size (8M-vertex 3D grid plus 16M random
+    shortcuts, ~10M-edge sparsifier, 1500 requests of log-uniform size
+    1..10000, 60/40 insert/delete) with the committed autotuned setting, every
+    request submitted at once: each request's outcome codes equal those of
+    dyg_stream_apply applied one request at a time in admission order, G and
+    H equal slot for slot; G ends as G0 - deletions + insertions (repeats as
+    fp32 weight sums in order); H lies inside G with G's weight bits; every
+    no-path insertion is in H; deleted edges are in neither."""
+    from dygrass_amd import Server, Stream, autotune, csr_from_edges, stream_params, synth
+    from oracle import stream as S
+
+    from conftest import serve_all
+
+    wl = synth.config_served(1.0, 0)
+    V = wl.G.V
+    t = autotune.load(wl.name)
+    s = autotune.setting_of(t) if t else autotune.Setting(w_ins=64, w_del=512)
+    maxr = max(len(r[1]) for r in wl.requests)
+    csr = (csr_from_edges(V, wl.G.u, wl.G.v, wl.G.w), csr_from_edges(V, wl.H.u, wl.H.v, wl.H.w))
+    sp = stream_params(w_ins=s.w_ins, w_del=s.w_del, threshold=wl.threshold, seed=0,
+                       max_batch=maxr, max_inserts=wl.updates, steps_ins=s.steps_ins,
+                       steps_del=s.steps_del, block_edges_ins=s.block_ins,
+                       block_edges_del=s.block_del, dense_no_desc=s.dense_no_desc)
+
+    served = Stream(ctx, V, None, None, sp, csr=csr)
+    srv = Server(served, slots=8, max_request=maxr)
+    got, _ = serve_all(srv, wl.requests, timeout=600)
+    srv.close()
+    alone = Stream(ctx, V, None, None, sp, csr=csr)
+    for (op, u, v, w), g in zip(wl.requests, got):
+        want, _ = alone.apply(op, u, v, w)
+        assert np.array_equal(g, want)
+    for a, b in ((served.G, alone.G), (served.H, alone.H)):
+        for x, y in zip(a.download(), b.download()):
+            assert np.array_equal(x.view(np.uint32) if x.dtype == np.float32 else x,
+                                  y.view(np.uint32) if y.dtype == np.float32 else y)
+    alone.close()
+    assert served.H.check() == 0
+
+    ins = [(r, o) for r, o in zip(wl.requests, got) if r[0] == "ins"]
+    dels = [(r, o) for r, o in zip(wl.requests, got) if r[0] == "del"]
+    ik = np.concatenate([_keys(V, r[1], r[2]) for r, _ in ins])
+    iw = np.concatenate([np.asarray(r[3], np.float32) for r, _ in ins])
+    io = np.concatenate([o for _, o in ins])
+    dk = np.concatenate([_keys(V, r[1], r[2]) for r, _ in dels])
+    do = np.concatenate([o for _, o in dels])
+    g0k = _keys(V, wl.G.u, wl.G.v)
+    assert len(np.unique(dk)) == len(dk) and np.all(np.isin(dk, g0k))  # distinct G0 edges
+    assert not np.any(np.isin(ik, dk))
+    assert np.all(np.isin(do, [S.G_ONLY, S.RECOVERED, S.HEURISTIC]))
+    # an insertion updates a weight exactly when its edge is in G0 or came earlier
+    _, first = np.unique(ik, return_index=True)
+    repeat = np.ones(len(ik), bool)
+    repeat[first] = False
+    in_g0 = np.isin(ik, g0k)
+    assert np.array_equal(io == S.WEIGHT, repeat | in_g0)
+    assert np.all(np.isin(io[~(repeat | in_g0)], [S.FOUND, S.ADDED]))
+    # expected G: G0 without the deletions, the insertions' weights summed in order
+    g0w = np.asarray(wl.G.w, np.float32)
+    special = set(ik[repeat | in_g0].tolist())  # keys whose weight is a sum
+    g0_of = dict(zip(g0k[np.isin(g0k, list(special))].tolist(),
+                     g0w[np.isin(g0k, list(special))].tolist()))
+    weights = {}
+    for k, w in zip(ik.tolist(), iw.tolist()):
+        if k not in special:
+            continue
+        if k not in weights:
+            weights[k] = np.float32(g0_of[k]) + np.float32(w) if k in g0_of else np.float32(w)
+        else:
+            weights[k] = np.float32(weights[k] + np.float32(w))
+    firsts = ~(repeat | in_g0)
+    keep = ~np.isin(g0k, dk)
+    want_k = np.concatenate([g0k[keep], ik[firsts]])
+    want_w = np.concatenate([g0w[keep], iw[firsts]])
+    order = np.argsort(want_k)
+    want_k, want_w = want_k[order], want_w[order]
+    for k, w in weights.items():
+        want_w[np.searchsorted(want_k, k)] = w
+    gk, gw = _edges(served.G, V)
+    assert np.array_equal(gk, want_k) and np.array_equal(gw, want_w.view(np.uint32))
+    hk, hw = _edges(served.H, V)
+    pos = np.searchsorted(gk, hk)
+    assert np.all(pos < len(gk)) and np.array_equal(gk[pos], hk)  # H inside G
+    assert np.array_equal(gw[pos], hw)                           # with G's weights
+    assert np.all(np.isin(ik[io == S.ADDED], hk))
+    assert not np.any(np.isin(dk, hk)) and not np.any(np.isin(dk, gk))
+    served.close()

@@ -13,7 +13,7 @@ import threading
 import numpy as np
 import pytest
 
-from conftest import f32bits
+from conftest import f32bits, serve_all
 
 pytestmark = pytest.mark.gpu
 
@@ -110,3 +110,109 @@ def test_serve_admission(ctx, grid_wl):
     with pytest.raises(DygError) as e:
         Server(st, slots=1, max_request=64, budget_bytes=1 << 20)
     assert "ADMISSION" in str(e.value)
+
+
+def test_serve_failed_copy_applies_nothing(ctx, grid_wl):
+    """A request whose input copy fails (forced from the host through the
+    testing hook: no copy is issued) is refused by submit, is never applied,
+    and stops the server; the request before it keeps its result."""
+    from dygrass_amd import DygError, Server, Stream, stream_params
+
+    wl = grid_wl
+    sp = stream_params(w_ins=64, w_del=64, threshold=10.0, seed=3, max_batch=512,
+                       max_inserts=4096)
+    g_edges, h_edges = (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w)
+    a = ("ins", wl.ins.u[:50], wl.ins.v[:50], wl.ins.w[:50])  # read in place: no copy
+    idx = np.arange(0, len(wl.G), len(wl.G) // 120)[:120]
+    b = ("del", wl.G.u[idx], wl.G.v[idx], None)               # copied to the device
+
+    alone = Stream(ctx, wl.G.V, g_edges, h_edges, sp)
+    want, _ = alone.apply(*a)
+
+    st = Stream(ctx, wl.G.V, g_edges, h_edges, sp)
+    srv = Server(st, slots=2, max_request=512)
+    ta = srv.submit(*a)
+    srv.fail_next_copy()
+    with pytest.raises(DygError) as e:
+        srv.submit(*b)
+    assert "copy failed" in str(e.value)
+    got, _ = srv.wait(ta, 50)
+    assert np.array_equal(got, want)
+    with pytest.raises(DygError) as e:  # the server has stopped
+        srv.submit(*a)
+    assert "stopped after an error" in str(e.value)
+    srv.close()
+    _same(alone.G, st.G)
+    _same(alone.H, st.H)
+    assert st.position == alone.position == 50
+    st.close()
+    alone.close()
+
+
+def test_serve_after_rejected_device_batch(ctx, grid_wl):
+    """A device batch rejected for an out-of-range id leaves the stream's
+    rejection word set; a served deletion request that waits for its
+    recovery list sizes (more candidates than the one-workgroup post kernel
+    holds) must still recover its paths, as dyg_stream_apply does."""
+    from dygrass_amd import DeviceArray, DygError, Server, Stream, stream_params
+
+    wl = grid_wl
+    sp = stream_params(w_ins=64, w_del=128, threshold=10.0, seed=4, max_batch=512,
+                       max_inserts=4096)  # 100 steps, 16 heuristic walkers per endpoint
+    g_edges, h_edges = (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w)
+    hu, hv = wl.H.u[::40][:150], wl.H.v[::40][:150]  # sparsifier edges: walked
+    req = ("del", hu, hv, None)
+
+    alone = Stream(ctx, wl.G.V, g_edges, h_edges, sp)
+    want, wst = alone.apply(*req)
+    assert wst["recovered"] + wst["heuristic"] > 0
+
+    st = Stream(ctx, wl.G.V, g_edges, h_edges, sp)
+    bad_u = DeviceArray.from_numpy(ctx, np.array([0, wl.G.V + 5], np.uint32))
+    bad_v = DeviceArray.from_numpy(ctx, np.array([1, 2], np.uint32))
+    with pytest.raises(DygError) as e:
+        st.apply_dev("del", bad_u, bad_v)
+    assert "out of range" in str(e.value)
+    srv = Server(st, slots=2, max_request=512)
+    got = serve_all(srv, [req])[0][0]
+    srv.close()
+    assert np.array_equal(got, want)
+    _same(alone.G, st.G)
+    _same(alone.H, st.H)
+    for a in (bad_u, bad_v):
+        a.close()
+    st.close()
+    alone.close()
+
+
+def test_serve_grid3d_family_matches_oracle(ctx):
+    """The served workload's family (3D grid plus random shortcuts, Poisson
+    trace of mixed requests) at a reduced size: every request's outcome codes
+    through the server equal the stream oracle's, and so do G and H."""
+    from dygrass_amd import Server, Stream, stream_params, synth
+    from oracle import stream as S
+
+    wl = synth.config_served(0.002, 0, count=80)
+    maxr = max(len(r[1]) for r in wl.requests)
+    g_edges, h_edges = (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w)
+    kw = dict(w_ins=64, w_del=512, threshold=wl.threshold, seed=0, steps_ins=32, steps_del=32)
+    sp = stream_params(**{k: kw[k] for k in ("w_ins", "w_del", "threshold", "seed",
+                                             "steps_ins", "steps_del")},
+                       max_batch=maxr, max_inserts=wl.updates)
+    st = Stream(ctx, wl.G.V, g_edges, h_edges, sp)
+    srv = Server(st, slots=8, max_request=maxr)
+    got, _ = serve_all(srv, wl.requests)
+    srv.close()
+    orc = S.Stream(wl.G.V, g_edges, h_edges, thr=wl.threshold, w_ins=64, w_del=512, seed=0,
+                   steps_ins=32, steps_del=32)
+    kinds = set()
+    for (op, u, v, w), a in zip(wl.requests, got):
+        want = orc.apply(op, u, v, w)
+        assert np.array_equal(a, want)
+        kinds.update(np.unique(want).tolist())
+    assert {S.FOUND, S.ADDED, S.G_ONLY, S.RECOVERED} <= kinds
+    for dev, (orp, ocol, ow) in ((st.G, orc.G.csr()), (st.H, orc.H.csr())):
+        rp, col, w = dev.download()
+        assert np.array_equal(rp, orp) and np.array_equal(col, ocol)
+        assert np.array_equal(f32bits(w), f32bits(ow))
+    st.close()

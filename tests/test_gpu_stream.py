@@ -8,43 +8,9 @@ swap-with-last deletions and the recovery order together.
 import numpy as np
 import pytest
 
-from conftest import f32bits
+from conftest import f32bits, mixed_batches
 
 pytestmark = pytest.mark.gpu
-
-
-def _mixed_batches(wl, seed, n_ins=300, n_del=200):
-    """Insertions (new edges, existing G edges, in-batch repeats, a self-loop)
-    and deletions (G edges, H edges, non-edges, repeats), alternating."""
-    rng = np.random.default_rng(seed)
-    G = wl.G
-    out = []
-    k = 0
-    for b in range(4):
-        if b % 2 == 0:
-            u = list(wl.ins.u[k:k + n_ins])
-            v = list(wl.ins.v[k:k + n_ins])
-            w = list(wl.ins.w[k:k + n_ins])
-            k += n_ins
-            ex = rng.choice(len(G), 25, replace=False)  # existing edges: weight updates
-            u += list(G.v[ex]); v += list(G.u[ex]); w += list(rng.uniform(0.1, 2, 25))
-            rep = rng.choice(n_ins, 6, replace=False)  # repeats inside the batch
-            u += [u[i] for i in rep]; v += [v[i] for i in rep]; w += list(rng.uniform(0.1, 2, 6))
-            u.append(5); v.append(5); w.append(1.0)  # self-loop
-            perm = rng.permutation(len(u))
-            out.append(("ins", np.array(u)[perm], np.array(v)[perm],
-                        np.array(w, np.float32)[perm]))
-        else:
-            idx = rng.choice(len(G), n_del, replace=False)
-            hidx = rng.choice(len(wl.H), 30, replace=False)
-            u = list(G.u[idx]) + list(wl.H.u[hidx]) + list(rng.integers(0, G.V, 10))
-            v = list(G.v[idx]) + list(wl.H.v[hidx]) + list(rng.integers(0, G.V, 10))
-            u += u[:5]; v += v[:5]  # repeats
-            # a previously inserted edge, deleted again
-            u.append(int(wl.ins.u[0])); v.append(int(wl.ins.v[0]))
-            perm = rng.permutation(len(u))
-            out.append(("del", np.array(u)[perm], np.array(v)[perm], None))
-    return out
 
 
 def _assert_same_graph(dev_graph, rows):
@@ -77,7 +43,7 @@ def test_stream_matches_oracle(ctx, grid_wl, seed, plain, general, nodesc, graph
     orc = S.Stream(wl.G.V, (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w), thr=10.0,
                    w_ins=64, w_del=64, seed=seed)
     seen = set()
-    for op, u, v, w in _mixed_batches(wl, seed, n_ins=300 if seed != 1 else 1000):
+    for op, u, v, w in mixed_batches(wl, seed, n_ins=300 if seed != 1 else 1000):
         got, stats = st.apply(op, u, v, w)
         want = orc.apply(op, u, v, w)
         assert np.array_equal(got, want), (op, np.flatnonzero(got != want)[:10])
@@ -106,7 +72,7 @@ def test_stream_many_walkers_match_oracle(ctx, grid_wl, general):
     orc = S.Stream(wl.G.V, (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w), thr=10.0,
                    w_ins=512, w_del=1024, seed=seed, steps_ins=60, steps_del=40)
     seen = set()
-    for op, u, v, w in _mixed_batches(wl, seed, n_ins=200, n_del=120):
+    for op, u, v, w in mixed_batches(wl, seed, n_ins=200, n_del=120):
         got, _ = st.apply(op, u, v, w)
         want = orc.apply(op, u, v, w)
         assert np.array_equal(got, want), (op, np.flatnonzero(got != want)[:10])
@@ -183,7 +149,7 @@ def test_stream_tuned_settings_match_oracle(ctx, grid_wl, general):
     orc = S.Stream(wl.G.V, (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w), thr=10.0,
                    w_ins=128, w_del=32, seed=seed, steps_ins=12, steps_del=7)
     seen = set()
-    for op, u, v, w in _mixed_batches(wl, seed):
+    for op, u, v, w in mixed_batches(wl, seed):
         got, _ = st.apply(op, u, v, w)
         want = orc.apply(op, u, v, w)
         assert np.array_equal(got, want), (op, np.flatnonzero(got != want)[:10])
@@ -320,7 +286,7 @@ def test_apply_dev_matches_host_inputs(ctx, grid_wl, nodesc):
     from dygrass_amd import Stream, stream_params
 
     wl = grid_wl
-    batches = _mixed_batches(wl, 8, n_ins=300)
+    batches = mixed_batches(wl, 8, n_ins=300)
     rng = np.random.default_rng(8)
     big = rng.choice(len(wl.G), 2600, replace=False)
     batches.append(("del", wl.G.u[big], wl.G.v[big], None))
@@ -398,7 +364,7 @@ def test_apply_many_dev_matches_apply_dev(ctx, grid_wl):
     from dygrass_amd import DygError, Stream, stream_params
 
     wl = grid_wl
-    batches = _mixed_batches(wl, 12, n_ins=300)
+    batches = mixed_batches(wl, 12, n_ins=300)
     rng = np.random.default_rng(12)
     big = rng.choice(len(wl.G), 2600, replace=False)
     batches.append(("del", wl.G.u[big], wl.G.v[big], None))

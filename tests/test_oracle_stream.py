@@ -120,3 +120,57 @@ def test_bridge_deletion_runs_the_heuristic():
     # each recovered edge carries G's weight
     for a, b in h:
         assert st.H.weight(a, b) == st.G.weight(a, b)
+
+
+def _same_rows(a, b):
+    rp, col, w = a
+    orp, ocol, ow = b
+    assert np.array_equal(rp, orp) and np.array_equal(col, ocol)
+    assert np.array_equal(f32bits(w), f32bits(ow))
+
+
+def test_cpp_stream_equals_python_stream(grid_wl):
+    """The C++ stream oracle (orc_stream_*, the full-size checker and CPU
+    baseline) against the sequential Python one it restates: mixed batches
+    with weight updates, in-batch repeats, self-loops, skipped and G-only
+    deletions and path recovery; outcome codes and both graphs slot for slot
+    after every batch, single- and multi-threaded."""
+    from conftest import mixed_batches
+
+    wl = grid_wl
+    g, h = (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w)
+    for threads, seed in ((1, 3), (8, 5)):
+        py = S.Stream(wl.G.V, g, h, thr=10.0, w_ins=32, w_del=64, seed=seed, steps_ins=48,
+                      steps_del=32)
+        cc = po.StreamCpp(wl.G.V, g, h, thr=10.0, w_ins=32, w_del=64, seed=seed, steps_ins=48,
+                          steps_del=32, threads=threads)
+        kinds = set()
+        for op, u, v, w in mixed_batches(wl, seed, n_ins=200, n_del=150):
+            a = py.apply(op, u, v, w)
+            assert np.array_equal(cc.apply(op, u, v, w), a)
+            kinds.update(a.tolist())
+            _same_rows(cc.csr(0), py.G.csr())
+            _same_rows(cc.csr(1), py.H.csr())
+        assert {S.WEIGHT, S.FOUND, S.ADDED, S.SKIP, S.G_ONLY, S.RECOVERED, S.INVALID} <= kinds
+        st = cc.stats()
+        assert st["batches"] == 4 and st["hops"] > 0
+        cc.close()
+
+
+def test_cpp_stream_heuristic_equals_python_stream():
+    """Bridge deletions (no path, two-hop recovery) in both stream oracles."""
+    # two triangles joined by the bridge 2-3; H = the two paths plus the bridge
+    gu = [0, 1, 0, 3, 4, 3, 2]
+    gv = [1, 2, 2, 4, 5, 5, 3]
+    gw = np.array([1, 2, 3, 1, 2, 3, 0.5], np.float32)
+    hidx = [0, 1, 3, 4, 6]
+    py = _stream(gu, gv, gw, hidx, W=16, seed=2)
+    gu, gv = np.array(gu), np.array(gv)
+    h = np.array(hidx)
+    cc = po.StreamCpp(6, (gu, gv, gw), (gu[h], gv[h], gw[h]), thr=10.0, w_ins=16, w_del=16,
+                      seed=2, threads=2)
+    a = py.apply("del", [3], [2])
+    assert list(a) == [S.HEURISTIC]
+    assert np.array_equal(cc.apply("del", [3], [2]), a)
+    _same_rows(cc.csr(1), py.H.csr())
+    _same_rows(cc.csr(0), py.G.csr())
