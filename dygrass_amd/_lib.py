@@ -79,6 +79,7 @@ DYG_STREAM_PLAIN = 1
 DYG_STREAM_GENERAL_ONLY = 2
 DYG_STREAM_DENSE_NO_DESC = 4
 DYG_STREAM_GRAPH = 8
+DYG_NOALLOC_MAX_EDGES = 1048576
 OUTCOMES = {1: "weight", 2: "found", 3: "added", 4: "skip", 5: "g_only", 6: "recovered",
             7: "heuristic", 8: "invalid"}
 
@@ -113,6 +114,9 @@ _SIGS = {
     "dyg_walk_incremental": (C.c_int32, [vp, u32p, u32p, f32p, u64p, C.c_uint32,
                                          C.POINTER(WalkParams), i32p, f32p, u32p, u32p,
                                          C.POINTER(WalkStats)]),
+    "dyg_walk_incremental_noalloc": (C.c_int32, [vp, u32p, u32p, f32p, u64p, C.c_uint32,
+                                                 C.POINTER(WalkParams), i32p, f32p, u32p, u32p,
+                                                 C.POINTER(WalkStats)]),
     "dyg_walk_decremental": (C.c_int32, [vp, u32p, u32p, u64p, C.c_uint32,
                                          C.POINTER(WalkParams), i32p, f32p, u32p, u32p, u32p,
                                          C.POINTER(WalkStats)]),

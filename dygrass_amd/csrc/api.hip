@@ -37,6 +37,45 @@ dyg_status check_params(dyg_ctx ctx, const dyg_walk_params* p) {
 
 namespace {
 
+// ---------------------------------------------------------------------------
+// Fixed device staging of dyg_walk_incremental_noalloc: one call's inputs,
+// results and counters, in an array the library holds on every GPU it runs
+// on until the process ends.  In host code `scratch` names a host-side
+// stand-in, not device memory: every host use goes through the per-device
+// address taken once by scratch_setup (hipGetSymbolAddress, with the device
+// current).  All calls on one GPU share its array; calls that overlap on a
+// GPU race.
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchBytes = 40ull << 20;
+constexpr size_t kScratchStats = 256;  // counters at the array's end
+constexpr int kMaxDevices = 64;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Takes the device address of `scratch` on `device`, which must be current;
+// once per device.
+hipError_t scratch_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = p;
+    return hipSuccess;
+}
+
+void* scratch_of(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
+
+__global__ void k_zero_u64(unsigned long long* p, uint32_t n) {
+    if (threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+
 // Device staging area for one host-pointer walk call, carved from ctx->dscratch.
 struct Carve {
     char* base;
@@ -56,11 +95,13 @@ struct Carve {
     }
 };
 
-// Host-pointer walk: stage inputs, launch, copy results back, wait.
+// Host-pointer walk: stage inputs, launch, copy results back, wait.  The
+// staging comes from the context's growable buffers, or, with `fixed`, from
+// the fixed array (no allocation; at most DYG_NOALLOC_MAX_EDGES edges).
 dyg_status walk_host(dyg_graph g, bool insert, const uint32_t* src, const uint32_t* dst,
                      const float* w, const uint64_t* edge_id, uint32_t n,
                      const dyg_walk_params* p, int32_t* found_at, float* cond, uint32_t* winner,
-                     uint32_t* hops, uint32_t* path, dyg_walk_stats* st) {
+                     uint32_t* hops, uint32_t* path, dyg_walk_stats* st, bool fixed = false) {
     dyg_ctx ctx = g->ctx;
     dyg_status cs = check_params(ctx, p);
     if (cs != DYG_OK) return cs;
@@ -79,9 +120,27 @@ dyg_status walk_host(dyg_graph g, bool insert, const uint32_t* src, const uint32
     const size_t bytes = Carve::need({n * 4ull, n * 4ull, n * 4ull, n * 8ull, n * 4ull, n * 4ull,
                                       n * 4ull, n * 4ull, pathn * 4, 64});
     DYG_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    DYG_HIP(ctx, ctx->dscratch.reserve(bytes));
-    DYG_HIP(ctx, ctx->dstats.reserve(64));
-    Carve c(ctx->dscratch.p);
+    void* stage = nullptr;
+    unsigned long long* d_stats = nullptr;
+    if (fixed) {
+        char* base = (char*)scratch_of(ctx->device);
+        if (!base) {
+            set_error(ctx, "walk (no allocation): the fixed staging array has no device address");
+            return DYG_ERR_STATE;
+        }
+        if (bytes > kScratchBytes - kScratchStats) {
+            set_error(ctx, "walk (no allocation): batch exceeds the fixed staging array");
+            return DYG_ERR_ARG;
+        }
+        stage = base;
+        d_stats = (unsigned long long*)(base + kScratchBytes - kScratchStats);
+    } else {
+        DYG_HIP(ctx, ctx->dscratch.reserve(bytes));
+        DYG_HIP(ctx, ctx->dstats.reserve(64));
+        stage = ctx->dscratch.p;
+        d_stats = (unsigned long long*)ctx->dstats.p;
+    }
+    Carve c(stage);
     uint32_t* d_src = c.take<uint32_t>(n);
     uint32_t* d_dst = c.take<uint32_t>(n);
     float* d_w = c.take<float>(n);
@@ -91,13 +150,17 @@ dyg_status walk_host(dyg_graph g, bool insert, const uint32_t* src, const uint32
     uint32_t* d_win = c.take<uint32_t>(n);
     uint32_t* d_hops = c.take<uint32_t>(n);
     uint32_t* d_path = c.take<uint32_t>(pathn ? pathn : 1);
-    unsigned long long* d_stats = (unsigned long long*)ctx->dstats.p;
     hipStream_t s = ctx->stream;
     DYG_HIP(ctx, hipMemcpyAsync(d_src, src, n * 4ull, hipMemcpyHostToDevice, s));
     DYG_HIP(ctx, hipMemcpyAsync(d_dst, dst, n * 4ull, hipMemcpyHostToDevice, s));
     if (insert) DYG_HIP(ctx, hipMemcpyAsync(d_w, w, n * 4ull, hipMemcpyHostToDevice, s));
     if (edge_id) DYG_HIP(ctx, hipMemcpyAsync(d_id, edge_id, n * 8ull, hipMemcpyHostToDevice, s));
-    DYG_HIP(ctx, hipMemsetAsync(d_stats, 0, 3 * sizeof(unsigned long long), s));
+    if (fixed) {  // a kernel, not a memset: the array keeps the last call's contents
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, d_stats, 3u);
+        DYG_HIP(ctx, hipGetLastError());
+    } else {
+        DYG_HIP(ctx, hipMemsetAsync(d_stats, 0, 3 * sizeof(unsigned long long), s));
+    }
     WalkLaunch a{};
     a.rows = g->d_rows;
     a.nodesc = g->nodesc;
@@ -171,6 +234,9 @@ extern "C" dyg_status dyg_ctx_create(int device, dyg_ctx* out) {
         delete c;
         return DYG_ERR_HIP;
     }
+    // the fixed staging array's address on this device (device current);
+    // a failure here is retried by the first call that needs it
+    (void)scratch_setup(device);
     *out = c;
     return DYG_OK;
 }
@@ -253,6 +319,30 @@ extern "C" dyg_status dyg_walk_incremental(dyg_graph H, const uint32_t* src, con
     if (!H) return DYG_ERR_ARG;
     return walk_host(H, true, src, dst, w, edge_id, n, p, found_at, cond, winner, hops, nullptr,
                      st);
+}
+
+extern "C" dyg_status dyg_walk_incremental_noalloc(dyg_graph H, const uint32_t* src,
+                                                   const uint32_t* dst, const float* w,
+                                                   const uint64_t* edge_id, uint32_t n,
+                                                   const dyg_walk_params* p, int32_t* found_at,
+                                                   float* cond, uint32_t* winner, uint32_t* hops,
+                                                   dyg_walk_stats* st) {
+    if (!H) return DYG_ERR_ARG;
+    if (n > DYG_NOALLOC_MAX_EDGES) {
+        set_error(H->ctx, "walk (no allocation): more than DYG_NOALLOC_MAX_EDGES edges");
+        return DYG_ERR_ARG;
+    }
+    const int dev = H->ctx->device;
+    if (!scratch_of(dev)) {  // set-up failed at context creation: once more, device current
+        int cur = 0;
+        DYG_HIP(H->ctx, hipGetDevice(&cur));
+        DYG_HIP(H->ctx, hipSetDevice(dev));
+        const hipError_t e = scratch_setup(dev);
+        DYG_HIP(H->ctx, hipSetDevice(cur));
+        DYG_HIP(H->ctx, e);
+    }
+    return walk_host(H, true, src, dst, w, edge_id, n, p, found_at, cond, winner, hops, nullptr,
+                     st, true);
 }
 
 extern "C" dyg_status dyg_walk_decremental(dyg_graph G, const uint32_t* src, const uint32_t* dst,

@@ -157,6 +157,19 @@ dyg_status dyg_walk_incremental(dyg_graph H, const uint32_t* src, const uint32_t
                                 const float* w, const uint64_t* edge_id, uint32_t n,
                                 const dyg_walk_params* p, int32_t* found_at, float* cond,
                                 uint32_t* winner, uint32_t* hops, dyg_walk_stats* st);
+/* dyg_walk_incremental without any device allocation.  No counterpart in
+ * the reference: for callers that cannot allocate device memory (tight
+ * serving loops, harnesses that forbid hipMalloc).  Inputs, results and
+ * counters are staged through one fixed 40 MiB device array that the library
+ * holds on each GPU it runs on until the process ends, so n is limited to
+ * DYG_NOALLOC_MAX_EDGES (DYG_ERR_ARG beyond).  Same results as
+ * dyg_walk_incremental.  Every call on a GPU shares that GPU's array: calls
+ * that overlap on one GPU (from several threads or contexts) race. */
+#define DYG_NOALLOC_MAX_EDGES 1048576u
+dyg_status dyg_walk_incremental_noalloc(dyg_graph H, const uint32_t* src, const uint32_t* dst,
+                                        const float* w, const uint64_t* edge_id, uint32_t n,
+                                        const dyg_walk_params* p, int32_t* found_at, float* cond,
+                                        uint32_t* winner, uint32_t* hops, dyg_walk_stats* st);
 /* Deletion walks; path[i*max_steps + 0..found_at[i]] receives the winner's
  * vertex path when found.  Replaces random_walk_decremental
  * (random_walk_decremental/main.cu:77-220) + get_result (gpu_graph.cuh:831-841). */
