@@ -687,8 +687,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--scale", type=float, default=1.0, help="workload scale (tests only)")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--workload", choices=["all", "batched", "served", "incremental",
-                                           "decremental", "grid"], default="all")
+    ap.add_argument("--workload", default="all",
+                    help="all, or a comma-separated list of batched, served, incremental, "
+                         "decremental, grid (the first is the headline)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparisons")
     ap.add_argument("--no-kappa", action="store_true", help="skip the FP64 kappa check")
@@ -732,8 +733,11 @@ def main() -> int:
         log(f"calibration copy: {args.copy_calibration_mb} MiB device to device")
         a.close()
         b.close()
-    want = (["batched", "served", "incremental", "decremental", "grid"]
-            if args.workload == "all" else [args.workload])
+    known = ["batched", "served", "incremental", "decremental", "grid"]
+    want = known if args.workload == "all" else [x for x in args.workload.split(",") if x]
+    if not want or any(x not in known for x in want):
+        print(f"unknown workload in {args.workload!r}; choose from {known}", file=sys.stderr)
+        return 2
     legs = {}
     for name in want:
         legs[name] = {"batched": leg_batched, "served": leg_served,

@@ -255,8 +255,15 @@ __global__ __launch_bounds__((1 << LOGW) < 64 ? 64 : (1 << LOGW)) void k_walk(Wa
 // kernel's bit for bit: a walker's trajectory depends only on (edge, walker)
 // and the graph.
 //
-// The kernel is bound by instruction issue, not by memory, so the loop keeps
-// every lane on one code path:
+// The kernel is bound by instruction issue and by the latency of each hop's
+// dependent load, so the loop keeps every lane on one code path and keeps a
+// load in flight while it computes:
+//  * with neighbour descriptors in the slots, the loop is software-pipelined:
+//    a lane's next slot loads are issued as soon as the slot they depend on
+//    has arrived (draw, modulo, address), and the hop's arithmetic and tests
+//    (1/w, distortion, target, pruning), the task refills and the Philox
+//    rounds run while they are in flight; a walk that ends leaves its last
+//    loads unused, and a leaf ends the walk on arrival;
 //  * first and later hops share the draw, the load phase and the tests;
 //  * each edge's source row is staged in LDS (a refill reads no global memory);
 //  * draws come from an 8-entry ring of two Philox blocks.  Block 0 of a
@@ -267,6 +274,10 @@ __global__ __launch_bounds__((1 << LOGW) < 64 ? 64 : (1 << LOGW)) void k_walk(Wa
 //    Philox rounds thus run once per four turns for the whole wave, instead
 //    of whenever any one lane runs dry.
 // ---------------------------------------------------------------------------
+// The slot a lane reads when its walk has no next hop (the loop issues its
+// loads unconditionally); never written.
+__device__ uint4 g_no_slot[1];
+
 constexpr int kPoolThreads = 256;
 constexpr int kPoolWaves = kPoolThreads / 64;
 constexpr float kPruneSlack = 1.0f + 0x1p-20f;
@@ -277,7 +288,7 @@ struct Lane {
     uint32_t edge;  // local edge index
     uint32_t walker;
     uint32_t cur, prev, t;
-    uint2 row;      // descriptor of `cur`
+    uint2 row;      // descriptor of the source row (the first hop draws from it)
     float R, we;
     int step;
 };
@@ -367,119 +378,262 @@ __global__ __launch_bounds__(kPoolThreads) void k_walk_pool(WalkLaunch a, uint32
     bool drained = false;
     Lane L{};
     Ring rng{};
+    uint4 x{}, y{};  // the slots of the lane's next hop: the drawn one and the row's last
     uint32_t hops = 0, redir = 0, leaf = 0;  // this lane's running counts
     uint32_t my_hops = 0;                    // current walker's draws
 
-    for (uint32_t turn = 0;; ++turn) {
-        // ---- refill idle lanes from the wave's task ring ----
-        const unsigned long long need = __ballot(!active && !drained);
-        if (need) {
-            const uint32_t cnt = __popcll(need);
-            if (rtail - rhead < cnt && !pool_empty) {
-                // claim 64 tasks (one LDS atomic), one block 0 per lane
-                uint32_t base = 0;
-                if (lane == 0) base = atomicAdd(&s_next, 64u);
-                base = __shfl(base, 0, 64);
-                const uint32_t task = base + lane;
-                if (task < total) {
-                    const uint64_t id = s_eid[task >> LOGW];
-                    const uint32_t q = (rtail + lane) & (kRing - 1u);
-                    rblk[q] = philox_block(0u, task & (W - 1u), (uint32_t)id, stream_c3(kTag, id),
-                                           k0, k1);
-                    rtask[q] = task;
+    if constexpr (DESC) {
+        // slots carry the next row: the next hop's loads go out as soon as
+        // a slot arrives (software-pipelined loop)
+        for (uint32_t turn = 0;; ++turn) {
+            // ---- refill idle lanes from the wave's task ring ----
+            const unsigned long long need = __ballot(!active && !drained);
+            if (need) {
+                const uint32_t cnt = __popcll(need);
+                if (rtail - rhead < cnt && !pool_empty) {
+                    // claim 64 tasks (one LDS atomic), one block 0 per lane
+                    uint32_t base = 0;
+                    if (lane == 0) base = atomicAdd(&s_next, 64u);
+                    base = __shfl(base, 0, 64);
+                    const uint32_t task = base + lane;
+                    if (task < total) {
+                        const uint64_t id = s_eid[task >> LOGW];
+                        const uint32_t q = (rtail + lane) & (kRing - 1u);
+                        rblk[q] = philox_block(0u, task & (W - 1u), (uint32_t)id, stream_c3(kTag, id),
+                                               k0, k1);
+                        rtask[q] = task;
+                    }
+                    const uint32_t got = base >= total ? 0u : min(64u, total - base);
+                    rtail += got;
+                    pool_empty = got < 64u;
+                    // the ring entries this wave just wrote, before its lanes read them
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 }
-                const uint32_t got = base >= total ? 0u : min(64u, total - base);
-                rtail += got;
-                pool_empty = got < 64u;
-                // the ring entries this wave just wrote, before its lanes read them
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            }
-            const uint32_t avail = rtail - rhead;
-            if (!active && !drained) {
-                const uint32_t rank = __popcll(need & ((1ull << lane) - 1ull));
-                if (rank < avail) {
-                    const uint32_t q = (rhead + rank) & (kRing - 1u);
-                    const uint32_t task = rtask[q];
-                    L.edge = task >> LOGW;
-                    L.walker = task & (W - 1u);
-                    L.cur = s_src[L.edge];
-                    L.row = s_row[L.edge];
-                    L.t = s_dst[L.edge];
-                    L.we = s_w[L.edge];
-                    L.prev = 0;
-                    L.R = 0.0f;
-                    L.step = 1;
-                    rng.start(rblk[q], s_eid[L.edge], kTag);
-                    my_hops = 0;
-                    active = true;
-                } else {
-                    drained = true;  // the counter is out: nothing left to claim
+                const uint32_t avail = rtail - rhead;
+                if (!active && !drained) {
+                    const uint32_t rank = __popcll(need & ((1ull << lane) - 1ull));
+                    if (rank < avail) {
+                        const uint32_t q = (rhead + rank) & (kRing - 1u);
+                        const uint32_t task = rtask[q];
+                        L.edge = task >> LOGW;
+                        L.walker = task & (W - 1u);
+                        L.cur = s_src[L.edge];
+                        L.row = s_row[L.edge];
+                        L.t = s_dst[L.edge];
+                        L.we = s_w[L.edge];
+                        L.prev = 0;
+                        L.R = 0.0f;
+                        L.step = 0;  // fresh: its first slot load is issued this turn
+                        rng.start(rblk[q], s_eid[L.edge], kTag);
+                        my_hops = 0;
+                        active = true;
+                    } else {
+                        drained = true;  // the counter is out: nothing left to claim
+                    }
                 }
+                rhead += min(cnt, avail);
             }
-            rhead += min(cnt, avail);
-        }
-        if (!__any(active)) break;
-        // ---- every fourth turn: the next draw block of every active lane ----
-        if ((turn & 3u) == 0u && active) rng.append(L.walker, k0, k1);
-        if (!active) continue;
+            if (!__any(active)) break;
+            // ---- every fourth turn: the next draw block of every active lane
+            //      (while the lanes' slot loads are in flight) ----
+            if ((turn & 3u) == 0u && active) rng.append(L.walker, k0, k1);
+            if (!active) continue;
 
-        // ---- one hop: first and later hops share one path ----
-        bool done = false;
-        int found = -1;
-        const uint2 row = L.row;
-        const bool first = (L.step == 1);
-        // first hop: any slot of the source row; later hops: the first deg-1
-        // slots, with the previous vertex redirected to the last slot
-        const uint32_t span = first ? row.y : row.y - 1u;
-        if (row.y == 0u || span == 0u) {
-            leaf += first ? 0u : 1u;  // isolated source, or a leaf
-            done = true;
-        } else {
-            const uint32_t i = rng.next() % span;
-            ++my_hops;
-            const uint4 x = ld(a.adj, row.x + i);
-            const uint4 y = ld(a.adj, row.x + row.y - 1u);
-            const bool back = !first && (x.x == L.prev);
-            redir += back ? 1u : 0u;
-            const uint4 sl = back ? y : x;
-            L.R += 1.0f / __uint_as_float(sl.y);  // L.R starts at 0: first hop gives 1/w
-            if (INSERT && !first && L.R * L.we >= a.thr) {
-                done = true;
-            } else if (sl.x == L.t) {
-                found = L.step;
+            // ---- one hop: take the slot loaded for it in the last turn, issue the
+            //      next hop's loads at once, then decide whether the walk goes on
+            //      (a walk that ends here leaves those loads unused) ----
+            bool done = false;
+            int found = -1;
+            const bool fresh = (L.step == 0);  // task started this turn: nothing loaded yet
+            const bool first = (L.step == 1);  // the loaded slot is the first hop's
+            bool back = false;
+            uint4 sl = make_uint4(0u, 0u, 0u, 0u);
+            uint2 nrow = L.row;  // the row the next hop draws from
+            if (!fresh) {
+                // non-backtracking: a draw of the previous vertex is redirected to
+                // the row's last slot (the first hop draws over the whole row)
+                back = !first && (x.x == L.prev);
+                sl = back ? y : x;
+                nrow = DESC ? make_uint2(sl.z, sl.w) : ld(a.rows, sl.x);
+            }
+            {
+                // one load site for every active lane, with no branch: a lane
+                // whose walk cannot go on (isolated source, leaf) reads the dummy
+                // slot instead
+                const bool go = fresh ? nrow.y >= 1u : nrow.y >= 2u;
+                const uint32_t span = go ? (fresh ? nrow.y : nrow.y - 1u) : 1u;
+                const uint32_t i = rng.next() % span;
+                const uint4* const base = go ? a.adj : g_no_slot;
+                x = base[go ? nrow.x + i : 0u];
+                y = base[go ? nrow.x + nrow.y - 1u : 0u];
+            }
+            if (fresh) {
+                if (nrow.y == 0u) done = true;  // isolated source: nothing to find
+                else L.step = 1;
+            } else {
+                ++my_hops;
+                redir += back ? 1u : 0u;
+                L.R += 1.0f / __uint_as_float(sl.y);  // L.R starts at 0: first hop gives 1/w
+                if (INSERT && !first && L.R * L.we >= a.thr) {
+                    done = true;
+                } else if (sl.x == L.t) {
+                    found = L.step;
+                    done = true;
+                } else {
+                    if (PRUNE && !first) {
+                        // conservative: prune only when fl(1/R) < c is certain;
+                        // rcp is within 1 ulp of 1/R, the margin covers 2^-20.
+                        // Decision only: any walker having found the target ends
+                        // the edge's walk.
+                        const uint32_t bb = (uint32_t)(s_key[L.edge] >> 32);
+                        if (bb != 0u &&
+                            (decide ||
+                             __builtin_amdgcn_rcpf(L.R) * kPruneSlack < __uint_as_float(bb - 1u)))
+                            done = true;
+                    }
+                    L.prev = L.cur;
+                    L.cur = sl.x;
+                    ++L.step;
+                    if (!done) {
+                        if (L.step >= (int)a.max_steps) {
+                            done = true;
+                        } else if (nrow.y <= 1u) {
+                            // a leaf: the walk would stop at its next hop without a draw
+                            leaf += 1u;
+                            done = true;
+                        }
+                    }
+                }
+            }
+            if (done) {
+                if (found != -1) {
+                    const unsigned long long key =
+                        ((unsigned long long)(__float_as_uint(1.0f / L.R) + 1u) << 32) |
+                        ((unsigned long long)tie_field<LOGW>(L.walker) << 16) |
+                        (unsigned long long)((uint32_t)found & 0xFFFFu);
+                    atomicMax(&s_key[L.edge], key);
+                }
+                atomicAdd(&s_hops[L.edge], my_hops);
+                hops += my_hops;
+                active = false;
+            }
+        }
+    } else {
+        // the next row comes from rows[] (a second dependent load): a walk
+        // that ends would wait for it too, so each hop completes in its turn
+        for (uint32_t turn = 0;; ++turn) {
+            // ---- refill idle lanes from the wave's task ring ----
+            const unsigned long long need = __ballot(!active && !drained);
+            if (need) {
+                const uint32_t cnt = __popcll(need);
+                if (rtail - rhead < cnt && !pool_empty) {
+                    // claim 64 tasks (one LDS atomic), one block 0 per lane
+                    uint32_t base = 0;
+                    if (lane == 0) base = atomicAdd(&s_next, 64u);
+                    base = __shfl(base, 0, 64);
+                    const uint32_t task = base + lane;
+                    if (task < total) {
+                        const uint64_t id = s_eid[task >> LOGW];
+                        const uint32_t q = (rtail + lane) & (kRing - 1u);
+                        rblk[q] = philox_block(0u, task & (W - 1u), (uint32_t)id, stream_c3(kTag, id),
+                                               k0, k1);
+                        rtask[q] = task;
+                    }
+                    const uint32_t got = base >= total ? 0u : min(64u, total - base);
+                    rtail += got;
+                    pool_empty = got < 64u;
+                    // the ring entries this wave just wrote, before its lanes read them
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                }
+                const uint32_t avail = rtail - rhead;
+                if (!active && !drained) {
+                    const uint32_t rank = __popcll(need & ((1ull << lane) - 1ull));
+                    if (rank < avail) {
+                        const uint32_t q = (rhead + rank) & (kRing - 1u);
+                        const uint32_t task = rtask[q];
+                        L.edge = task >> LOGW;
+                        L.walker = task & (W - 1u);
+                        L.cur = s_src[L.edge];
+                        L.row = s_row[L.edge];
+                        L.t = s_dst[L.edge];
+                        L.we = s_w[L.edge];
+                        L.prev = 0;
+                        L.R = 0.0f;
+                        L.step = 1;
+                        rng.start(rblk[q], s_eid[L.edge], kTag);
+                        my_hops = 0;
+                        active = true;
+                    } else {
+                        drained = true;  // the counter is out: nothing left to claim
+                    }
+                }
+                rhead += min(cnt, avail);
+            }
+            if (!__any(active)) break;
+            // ---- every fourth turn: the next draw block of every active lane ----
+            if ((turn & 3u) == 0u && active) rng.append(L.walker, k0, k1);
+            if (!active) continue;
+
+            // ---- one hop: first and later hops share one path ----
+            bool done = false;
+            int found = -1;
+            const uint2 row = L.row;
+            const bool first = (L.step == 1);
+            // first hop: any slot of the source row; later hops: the first deg-1
+            // slots, with the previous vertex redirected to the last slot
+            const uint32_t span = first ? row.y : row.y - 1u;
+            if (row.y == 0u || span == 0u) {
+                leaf += first ? 0u : 1u;  // isolated source, or a leaf
                 done = true;
             } else {
-                if (PRUNE && !first) {
-                    // conservative: prune only when fl(1/R) < c is certain;
-                    // rcp is within 1 ulp of 1/R, the margin covers 2^-20.
-                    // Decision only: any walker having found the target ends
-                    // the edge's walk.
-                    const uint32_t bb = (uint32_t)(s_key[L.edge] >> 32);
-                    if (bb != 0u &&
-                        (decide ||
-                         __builtin_amdgcn_rcpf(L.R) * kPruneSlack < __uint_as_float(bb - 1u)))
-                        done = true;
+                const uint32_t i = rng.next() % span;
+                ++my_hops;
+                const uint4 x = ld(a.adj, row.x + i);
+                const uint4 y = ld(a.adj, row.x + row.y - 1u);
+                const bool back = !first && (x.x == L.prev);
+                redir += back ? 1u : 0u;
+                const uint4 sl = back ? y : x;
+                L.R += 1.0f / __uint_as_float(sl.y);  // L.R starts at 0: first hop gives 1/w
+                if (INSERT && !first && L.R * L.we >= a.thr) {
+                    done = true;
+                } else if (sl.x == L.t) {
+                    found = L.step;
+                    done = true;
+                } else {
+                    if (PRUNE && !first) {
+                        // conservative: prune only when fl(1/R) < c is certain;
+                        // rcp is within 1 ulp of 1/R, the margin covers 2^-20.
+                        // Decision only: any walker having found the target ends
+                        // the edge's walk.
+                        const uint32_t bb = (uint32_t)(s_key[L.edge] >> 32);
+                        if (bb != 0u &&
+                            (decide ||
+                             __builtin_amdgcn_rcpf(L.R) * kPruneSlack < __uint_as_float(bb - 1u)))
+                            done = true;
+                    }
+                    L.prev = L.cur;
+                    L.cur = sl.x;
+                    L.row = DESC ? make_uint2(sl.z, sl.w) : ld(a.rows, sl.x);
+                    ++L.step;
+                    if (L.step >= (int)a.max_steps) done = true;
                 }
-                L.prev = L.cur;
-                L.cur = sl.x;
-                L.row = DESC ? make_uint2(sl.z, sl.w) : ld(a.rows, sl.x);
-                ++L.step;
-                if (L.step >= (int)a.max_steps) done = true;
             }
-        }
-        if (done) {
-            if (found != -1) {
-                const unsigned long long key =
-                    ((unsigned long long)(__float_as_uint(1.0f / L.R) + 1u) << 32) |
-                    ((unsigned long long)tie_field<LOGW>(L.walker) << 16) |
-                    (unsigned long long)((uint32_t)found & 0xFFFFu);
-                atomicMax(&s_key[L.edge], key);
+            if (done) {
+                if (found != -1) {
+                    const unsigned long long key =
+                        ((unsigned long long)(__float_as_uint(1.0f / L.R) + 1u) << 32) |
+                        ((unsigned long long)tie_field<LOGW>(L.walker) << 16) |
+                        (unsigned long long)((uint32_t)found & 0xFFFFu);
+                    atomicMax(&s_key[L.edge], key);
+                }
+                atomicAdd(&s_hops[L.edge], my_hops);
+                hops += my_hops;
+                active = false;
             }
-            atomicAdd(&s_hops[L.edge], my_hops);
-            hops += my_hops;
-            active = false;
         }
     }
 
