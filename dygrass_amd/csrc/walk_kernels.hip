@@ -282,7 +282,7 @@ constexpr int kPoolThreads = 256;
 constexpr int kPoolWaves = kPoolThreads / 64;
 constexpr float kPruneSlack = 1.0f + 0x1p-20f;
 constexpr int kPoolMaxEdges = 128;  // edges owned by one workgroup, at most
-constexpr uint32_t kRing = 128;     // a wave's claimed tasks (at most 127 pending)
+constexpr uint32_t kRing = 64;      // a wave's claimed tasks (a claim fills it up)
 
 struct Lane {
     uint32_t edge;  // local edge index
@@ -291,6 +291,42 @@ struct Lane {
     uint2 row;      // descriptor of the source row (the first hop draws from it)
     float R, we;
     int step;
+};
+
+// Draw ring in LDS (the pipelined loop): a lane's column of an 8-row array
+// holds its draws, draw d at row d & 7, so block b fills rows 4(b & 1) ..
+// 4(b & 1) + 3.  A draw is one LDS read (issued ahead of the slot wait),
+// instead of a chain of register selects.
+struct LdsRing {
+    uint32_t* col;  // the lane's column; rows kPoolThreads words apart
+    uint32_t q;     // index of the next draw (mod 8)
+    uint32_t blk;   // next block to compute
+    uint32_t c2, c3;
+
+    __device__ __forceinline__ void start(uint4 b0, uint64_t id, uint32_t tag) {
+        col[0] = b0.x;
+        col[kPoolThreads] = b0.y;
+        col[2 * kPoolThreads] = b0.z;
+        col[3 * kPoolThreads] = b0.w;
+        q = 0;
+        blk = 1;
+        c2 = (uint32_t)id;
+        c3 = stream_c3(tag, id);
+    }
+    __device__ __forceinline__ void append(uint32_t walker, uint32_t k0, uint32_t k1) {
+        const uint4 x = philox_block(blk, walker, c2, c3, k0, k1);
+        uint32_t* const p = col + (blk & 1u) * (4u * kPoolThreads);
+        p[0] = x.x;
+        p[kPoolThreads] = x.y;
+        p[2 * kPoolThreads] = x.z;
+        p[3 * kPoolThreads] = x.w;
+        ++blk;
+    }
+    __device__ __forceinline__ uint32_t next() {
+        const uint32_t v = col[q * kPoolThreads];
+        q = (q + 1u) & 7u;
+        return v;
+    }
 };
 
 // Draw ring: r[d & 7] holds draw d of the walker; block b sits in half b & 1.
@@ -343,6 +379,7 @@ __global__ __launch_bounds__(kPoolThreads) void k_walk_pool(WalkLaunch a, uint32
     __shared__ uint32_t s_rtask[kPoolWaves][kRing];  // and their task numbers
     __shared__ uint32_t s_next;
     __shared__ uint32_t s_cnt[3];
+    __shared__ uint32_t s_draw[DESC ? 8 : 1][kPoolThreads];  // LdsRing columns
 
     const uint32_t n = a.n_dev ? min(*a.n_dev, a.n) : a.n;  // list length (device or host)
     const bool decide = INSERT && (a.flags & DYG_WALK_DECISION_ONLY) != 0u;
@@ -385,27 +422,31 @@ __global__ __launch_bounds__(kPoolThreads) void k_walk_pool(WalkLaunch a, uint32
     if constexpr (DESC) {
         // slots carry the next row: the next hop's loads go out as soon as
         // a slot arrives (software-pipelined loop)
+        LdsRing drw{};
+        drw.col = &s_draw[0][tid];
         for (uint32_t turn = 0;; ++turn) {
             // ---- refill idle lanes from the wave's task ring ----
             const unsigned long long need = __ballot(!active && !drained);
             if (need) {
                 const uint32_t cnt = __popcll(need);
                 if (rtail - rhead < cnt && !pool_empty) {
-                    // claim 64 tasks (one LDS atomic), one block 0 per lane
+                    // fill the ring (one LDS atomic), one block 0 per lane; it
+                    // holds fewer than cnt <= 64 tasks, so there is room for one
+                    const uint32_t room = kRing - (rtail - rhead);
                     uint32_t base = 0;
-                    if (lane == 0) base = atomicAdd(&s_next, 64u);
+                    if (lane == 0) base = atomicAdd(&s_next, room);
                     base = __shfl(base, 0, 64);
                     const uint32_t task = base + lane;
-                    if (task < total) {
+                    if (lane < room && task < total) {
                         const uint64_t id = s_eid[task >> LOGW];
                         const uint32_t q = (rtail + lane) & (kRing - 1u);
                         rblk[q] = philox_block(0u, task & (W - 1u), (uint32_t)id, stream_c3(kTag, id),
                                                k0, k1);
                         rtask[q] = task;
                     }
-                    const uint32_t got = base >= total ? 0u : min(64u, total - base);
+                    const uint32_t got = base >= total ? 0u : min(room, total - base);
                     rtail += got;
-                    pool_empty = got < 64u;
+                    pool_empty = got < room;
                     // the ring entries this wave just wrote, before its lanes read them
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                     __builtin_amdgcn_wave_barrier();
@@ -426,7 +467,7 @@ __global__ __launch_bounds__(kPoolThreads) void k_walk_pool(WalkLaunch a, uint32
                         L.prev = 0;
                         L.R = 0.0f;
                         L.step = 0;  // fresh: its first slot load is issued this turn
-                        rng.start(rblk[q], s_eid[L.edge], kTag);
+                        drw.start(rblk[q], s_eid[L.edge], kTag);
                         my_hops = 0;
                         active = true;
                     } else {
@@ -438,7 +479,7 @@ __global__ __launch_bounds__(kPoolThreads) void k_walk_pool(WalkLaunch a, uint32
             if (!__any(active)) break;
             // ---- every fourth turn: the next draw block of every active lane
             //      (while the lanes' slot loads are in flight) ----
-            if ((turn & 3u) == 0u && active) rng.append(L.walker, k0, k1);
+            if ((turn & 3u) == 0u && active) drw.append(L.walker, k0, k1);
             if (!active) continue;
 
             // ---- one hop: take the slot loaded for it in the last turn, issue the
@@ -451,6 +492,7 @@ __global__ __launch_bounds__(kPoolThreads) void k_walk_pool(WalkLaunch a, uint32
             bool back = false;
             uint4 sl = make_uint4(0u, 0u, 0u, 0u);
             uint2 nrow = L.row;  // the row the next hop draws from
+            const uint32_t r = drw.next();  // read before the wait for the slots
             if (!fresh) {
                 // non-backtracking: a draw of the previous vertex is redirected to
                 // the row's last slot (the first hop draws over the whole row)
@@ -464,7 +506,7 @@ __global__ __launch_bounds__(kPoolThreads) void k_walk_pool(WalkLaunch a, uint32
                 // slot instead
                 const bool go = fresh ? nrow.y >= 1u : nrow.y >= 2u;
                 const uint32_t span = go ? (fresh ? nrow.y : nrow.y - 1u) : 1u;
-                const uint32_t i = rng.next() % span;
+                const uint32_t i = r % span;
                 const uint4* const base = go ? a.adj : g_no_slot;
                 x = base[go ? nrow.x + i : 0u];
                 y = base[go ? nrow.x + nrow.y - 1u : 0u];
@@ -529,21 +571,23 @@ __global__ __launch_bounds__(kPoolThreads) void k_walk_pool(WalkLaunch a, uint32
             if (need) {
                 const uint32_t cnt = __popcll(need);
                 if (rtail - rhead < cnt && !pool_empty) {
-                    // claim 64 tasks (one LDS atomic), one block 0 per lane
+                    // fill the ring (one LDS atomic), one block 0 per lane; it
+                    // holds fewer than cnt <= 64 tasks, so there is room for one
+                    const uint32_t room = kRing - (rtail - rhead);
                     uint32_t base = 0;
-                    if (lane == 0) base = atomicAdd(&s_next, 64u);
+                    if (lane == 0) base = atomicAdd(&s_next, room);
                     base = __shfl(base, 0, 64);
                     const uint32_t task = base + lane;
-                    if (task < total) {
+                    if (lane < room && task < total) {
                         const uint64_t id = s_eid[task >> LOGW];
                         const uint32_t q = (rtail + lane) & (kRing - 1u);
                         rblk[q] = philox_block(0u, task & (W - 1u), (uint32_t)id, stream_c3(kTag, id),
                                                k0, k1);
                         rtask[q] = task;
                     }
-                    const uint32_t got = base >= total ? 0u : min(64u, total - base);
+                    const uint32_t got = base >= total ? 0u : min(room, total - base);
                     rtail += got;
-                    pool_empty = got < 64u;
+                    pool_empty = got < room;
                     // the ring entries this wave just wrote, before its lanes read them
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                     __builtin_amdgcn_wave_barrier();
