@@ -48,6 +48,7 @@ struct dyg_ctx_s {
     std::mutex err_mu;
     std::string last_error;
     dyg::Pinned stage;         // H2D / D2H staging
+    dyg::Pinned report;        // per-batch reports of a list of stream batches in flight
     dyg::DevBuf dscratch;      // per-call device staging of inputs/outputs
     dyg::DevBuf dstats;        // device counters (hops, redirects, leaf stops)
     std::vector<hipEvent_t> evpool;  // timing events for multi-launch runs

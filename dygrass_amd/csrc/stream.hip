@@ -457,6 +457,9 @@ struct dyg_stream_s {
     StepDesc* desc = nullptr;      // pinned: the graphed batch's values
     StepDesc* desc_dev = nullptr;  // the same, as the device reaches it
     bool last_graph = false;       // the current batch ran as a step graph
+    // events bracketing the current batch's walk (the context's pair, or a
+    // batch's own pair while a list of batches is in flight)
+    hipEvent_t wev0 = nullptr, wev1 = nullptr;
     ~dyg_stream_s() {
         for (auto& op : steps)
             for (StepGraph& g : op) {
@@ -663,12 +666,12 @@ int step_class(uint32_t n) {
 dyg_status enqueue_ins_step(dyg_stream s, const SmallInsArgs& a, bool timed) {
     hipStream_t st = s->ctx->stream;
     DYG_HIP(s->ctx, launch_small_ins_pre(a, st));
-    if (timed) DYG_HIP(s->ctx, hipEventRecord(s->ctx->ev0, st));
+    if (timed) DYG_HIP(s->ctx, hipEventRecord(s->wev0, st));
     WalkLaunch w = walk_args(s, s->H, s->sp.ins, a.n);
     w.stats = a.stats;
     w.flags |= DYG_WALK_DECISION_ONLY;  // outcomes need found / not found only
     DYG_HIP(s->ctx, launch_walk_incremental(w, st));
-    if (timed) DYG_HIP(s->ctx, hipEventRecord(s->ctx->ev1, st));
+    if (timed) DYG_HIP(s->ctx, hipEventRecord(s->wev1, st));
     DYG_HIP(s->ctx, launch_small_ins_post(a, st));
     return DYG_OK;
 }
@@ -682,9 +685,9 @@ dyg_status enqueue_del_step(dyg_stream s, const SmallDelArgs& a, bool timed) {
     w.w = nullptr;
     w.path = s->path;
     w.stats = a.stats;
-    if (timed) DYG_HIP(s->ctx, hipEventRecord(s->ctx->ev0, st));
+    if (timed) DYG_HIP(s->ctx, hipEventRecord(s->wev0, st));
     DYG_HIP(s->ctx, launch_walk_decremental(w, st));
-    if (timed) DYG_HIP(s->ctx, hipEventRecord(s->ctx->ev1, st));
+    if (timed) DYG_HIP(s->ctx, hipEventRecord(s->wev1, st));
     DYG_HIP(s->ctx, launch_small_del_post(a, st));
     return DYG_OK;
 }
@@ -790,9 +793,9 @@ bool run_step_graph(dyg_stream s, bool insert, const Args& a, dyg_status* out) {
     if (!g.exec && (g.failed || !capture_step(s, g, a, kStepClass[c]))) return false;
     fill_desc(s->desc, a);
     // the walk is a node inside the graph: the timing events bracket the step
-    hipError_t e = hipEventRecord(s->ctx->ev0, s->ctx->stream);
+    hipError_t e = hipEventRecord(s->wev0, s->ctx->stream);
     if (e == hipSuccess) e = hipGraphLaunch(g.exec, s->ctx->stream);
-    if (e == hipSuccess) e = hipEventRecord(s->ctx->ev1, s->ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(s->wev1, s->ctx->stream);
     if (e != hipSuccess) {
         set_error(s->ctx, std::string("stream step graph: ") + hipGetErrorString(e));
         *out = DYG_ERR_HIP;
@@ -876,12 +879,12 @@ dyg_status apply_insert(dyg_stream s, const uint32_t* du, const uint32_t* dv, co
     LAUNCH_1D(k_compact, n, s->keep, n, s->bcnt, du, dv, dw, s->base, false, s->cu, s->cv,
               s->cw, s->ceid, s->sel, s->cnt);
     DYG_HIP(s->ctx, hipGetLastError());
-    if (timed) DYG_HIP(s->ctx, hipEventRecord(s->ctx->ev0, st));
+    if (timed) DYG_HIP(s->ctx, hipEventRecord(s->wev0, st));
     WalkLaunch a = walk_args(s, H, s->sp.ins, n);
     a.stats = dstats;
     a.flags |= DYG_WALK_DECISION_ONLY;  // outcomes need found / not found only
     DYG_HIP(s->ctx, launch_walk_incremental(a, st));
-    if (timed) DYG_HIP(s->ctx, hipEventRecord(s->ctx->ev1, st));
+    if (timed) DYG_HIP(s->ctx, hipEventRecord(s->wev1, st));
     r = device_edits_batch(H, s->cu, s->cv, s->cw, s->cfound, n, s->cnt);
     if (r != DYG_OK) return r;
     LAUNCH_1D(k_outcome, n, s->sel, s->cnt, s->cfound, dcls, kOutFound, kOutAdded);
@@ -1037,9 +1040,9 @@ dyg_status apply_delete(dyg_stream s, const uint32_t* du, const uint32_t* dv, ui
     a.w = nullptr;
     a.path = s->path;
     a.stats = dstats;
-    if (timed) DYG_HIP(s->ctx, hipEventRecord(s->ctx->ev0, st));
+    if (timed) DYG_HIP(s->ctx, hipEventRecord(s->wev0, st));
     DYG_HIP(s->ctx, launch_walk_decremental(a, st));
-    if (timed) DYG_HIP(s->ctx, hipEventRecord(s->ctx->ev1, st));
+    if (timed) DYG_HIP(s->ctx, hipEventRecord(s->wev1, st));
     LAUNCH_1D(k_path_counts, n, s->cfound, s->cnt, n, s->pc, s->nopath, s->bsum);
     LAUNCH_1D(k_recovery_lists, n, s->pc, s->nopath, n, s->bsum, s->cu, s->cv, s->ceid, s->poff,
               s->ep, s->ekey, s->cnt, (uint32_t*)(s->pin_dev + kPinSizes));
@@ -1223,6 +1226,8 @@ extern "C" dyg_status dyg_stream_create(dyg_ctx ctx, uint32_t V, const uint64_t*
     }
     auto* s = new dyg_stream_s;
     s->ctx = ctx;
+    s->wev0 = ctx->ev0;
+    s->wev1 = ctx->ev1;
     s->sp = *sp;
     const bool plain = (sp->flags & DYG_STREAM_PLAIN) != 0;
     r = dyg_graph_upload(ctx, V, g_row_ptr, g_col, g_w, sp->slack, plain, &s->G);
@@ -1360,7 +1365,7 @@ dyg_status stream_apply(dyg_stream s, int op, const uint32_t* u, const uint32_t*
             counts[c & 15]++;
         }
         float ms = 0.f;
-        if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess) rs.walk_ms = ms;
+        if (hipEventElapsedTime(&ms, s->wev0, s->wev1) == hipSuccess) rs.walk_ms = ms;
         rs.hops = hst[0];
         rs.redirects = hst[1];
         if (hst[3]) rs.candidates = hst[3];  // counted on the device (small deletion batch)
@@ -1387,6 +1392,174 @@ dyg_status stream_apply(dyg_stream s, int op, const uint32_t* u, const uint32_t*
     rs.total_ms = now_ms() - t0;
     if (st) *st = rs;
     return DYG_OK;
+}
+
+// A list of batches already in device memory, enqueued back to back: the
+// ids of the whole list are checked once before anything changes; each
+// batch reports into its own region of the context's pinned report buffer
+// and times its walk with its own events; the host waits only where a
+// deletion batch needs its recovery list sizes (and when the arenas might
+// run out of room, which a running worst-case estimate tells), then once at
+// the end.  Same kernels, same order, so the same results as one
+// stream_apply per batch.
+constexpr size_t kListWords = 16;  // report words per batch (kReportWords used)
+
+dyg_status stream_apply_list(dyg_stream s, uint32_t nb, const int* op, const uint64_t* off,
+                             const uint32_t* u, const uint32_t* v, const float* w,
+                             int32_t* outcome, dyg_stream_stats* st) {
+    dyg_ctx ctx = s->ctx;
+    hipStream_t str = ctx->stream;
+    const uint64_t o0 = off[0], total = off[nb] - off[0];
+    for (uint32_t b = 0; b < nb; ++b) {
+        const uint64_t n = off[b + 1] - off[b];
+        if ((op[b] != DYG_STREAM_INSERT && op[b] != DYG_STREAM_DELETE) ||
+            (n && (!u || !v)) || (n && op[b] == DYG_STREAM_INSERT && !w)) {
+            set_error(ctx, "stream apply many: bad operation or missing array");
+            return DYG_ERR_ARG;
+        }
+        if (n > s->sp.max_batch) {
+            set_error(ctx, "stream apply many: batch larger than max_batch");
+            return DYG_ERR_ARG;
+        }
+    }
+    // every id of the list, before anything changes
+    const PinLayout hp = pin_layout((char*)s->pin, 0), dp = pin_layout(s->pin_dev, 0);
+    hp.words[kReportBad] = 0;
+    for (uint64_t c = 0; c < total; c += (1u << 30)) {
+        const uint32_t k = (uint32_t)std::min<uint64_t>(total - c, 1u << 30);
+        hipLaunchKernelGGL(k_check_batch, dim3((k + 255) / 256), dim3(256), 0, str, u + o0 + c,
+                           v + o0 + c, k, s->G->V, dp.words + kReportBad);
+    }
+    DYG_HIP(ctx, hipGetLastError());
+    dyg_status r = wait_spin(ctx, str, s->ev_sync);
+    if (r != DYG_OK) return r;
+    if (hp.words[kReportBad]) {
+        set_error(ctx, "stream apply many: vertex id out of range");
+        return DYG_ERR_ARG;
+    }
+    // report regions: kListWords words per batch, then every outcome code
+    const size_t words_bytes = (size_t)nb * kListWords * 8;
+    DYG_HIP(ctx, ctx->report.reserve(words_bytes + total * 4 + 256));
+    void* rd = nullptr;
+    DYG_HIP(ctx, hipHostGetDevicePointer(&rd, ctx->report.p, 0));
+    auto* rw_h = (unsigned long long*)ctx->report.p;
+    auto* rw_d = (unsigned long long*)rd;
+    auto* rc_h = (int32_t*)((char*)ctx->report.p + words_bytes);
+    auto* rc_d = (int32_t*)((char*)rd + words_bytes);
+    ctx->reserve_events(3 * (size_t)nb + 1);
+    if (ctx->evpool.size() < 3 * (size_t)nb + 1) {
+        set_error(ctx, "stream apply many: event creation failed");
+        return DYG_ERR_HIP;
+    }
+    hipEvent_t* ev = ctx->evpool.data();  // per batch: start, walk start, walk end; then end
+    std::vector<dyg_stream_stats> rs(nb);
+    const uint32_t ms = s->sp.del.max_steps, hw = s->sp.heur_walkers;
+    uint64_t est_g = s->G->used_slots, est_h = s->H->used_slots;  // arena bump pointers, upper
+    int last = -1;  // last batch enqueued
+    // brings the host's view of both arenas up to the last enqueued batch
+    // (after a wait)
+    auto absorb_last = [&]() -> dyg_status {
+        if (last < 0) return DYG_OK;
+        const unsigned long long* wd = rw_h + (size_t)last * kListWords;
+        dyg_status q = device_edits_absorb(s->G, wd + 4, nullptr);
+        if (q == DYG_OK) q = device_edits_absorb(s->H, wd + 9, nullptr);
+        return q;
+    };
+    dyg_status fail = DYG_OK;
+    uint32_t done_b = 0;
+    for (uint32_t b = 0; b < nb && fail == DYG_OK; ++b) {
+        const uint64_t o = off[b];
+        const uint32_t n = (uint32_t)(off[b + 1] - o);
+        rs[b].updates = n;
+        if (n == 0) {
+            done_b = b + 1;
+            continue;
+        }
+        // the batch's worst-case arena growth (as the served loop estimates it)
+        const bool del = op[b] == DYG_STREAM_DELETE;
+        const uint64_t add = 4ull * (2ull * n) * (s->sp.slack + 4) + 8ull * n;
+        const uint64_t cand = (uint64_t)n * (ms - 1) + 4ull * n * hw;
+        const uint64_t add_h = del ? add + 8ull * cand : add;
+        if (est_g + add > s->G->cap_slots || est_h + add_h > s->H->cap_slots) {
+            fail = wait_spin(ctx, str, s->ev_sync);
+            if (fail == DYG_OK) fail = absorb_last();
+            if (fail == DYG_OK)
+                fail = device_edits_prepare(s->G, s->sp.max_batch, 2ull * n + s->sp.max_batch);
+            if (fail == DYG_OK) fail = device_edits_prepare(s->H, s->cmax, 2ull * n + cand);
+            if (fail != DYG_OK) break;
+            est_g = s->G->used_slots;
+            est_h = s->H->used_slots;
+            if (est_g + add > s->G->cap_slots || est_h + add_h > s->H->cap_slots) {
+                set_error(ctx, "stream apply many: arena headroom could not be reserved");
+                fail = DYG_ERR_NOMEM;
+                break;
+            }
+        }
+        est_g += add;
+        est_h += add_h;
+        const Report rep{rc_d + (o - o0), rw_d + (size_t)b * kListWords, s->G->dev.meta,
+                         s->H->dev.meta};
+        s->wev0 = ev[3 * b + 1];
+        s->wev1 = ev[3 * b + 2];
+        fail = hipEventRecord(ev[3 * b], str) == hipSuccess ? DYG_OK : DYG_ERR_HIP;
+        if (fail == DYG_OK)
+            fail = del ? apply_delete(s, u + o, v + o, n, s->cls, s->stats, true, &rs[b], rep,
+                                      nullptr)
+                       : apply_insert(s, u + o, v + o, w + o, n, s->cls, s->stats, true, &rs[b],
+                                      rep, nullptr);
+        if (fail != DYG_OK) break;
+        last = (int)b;
+        done_b = b + 1;
+        s->base += n;
+    }
+    s->wev0 = ctx->ev0;
+    s->wev1 = ctx->ev1;
+    // the list's one wait, then the reports of every batch enqueued
+    DYG_HIP(ctx, hipEventRecord(ev[3 * (size_t)nb], str));
+    r = wait_spin(ctx, str, s->ev_sync);
+    if (r == DYG_OK) r = absorb_last();
+    if (fail == DYG_OK) fail = r;
+    for (uint32_t b = 0; b < done_b; ++b) {
+        const uint64_t o = off[b] - o0;
+        const uint32_t n = (uint32_t)rs[b].updates;
+        if (n == 0) {
+            if (st) st[b] = rs[b];
+            continue;
+        }
+        const unsigned long long* wd = rw_h + (size_t)b * kListWords;
+        uint64_t counts[16] = {};
+        for (uint32_t i = 0; i < n; ++i) {
+            const int32_t c = rc_h[o + i];
+            if (outcome) outcome[o + i] = c;
+            counts[c & 15]++;
+        }
+        dyg_stream_stats& x = rs[b];
+        x.weight_updates = counts[kOutWeight];
+        x.found = counts[kOutFound];
+        x.added = counts[kOutAdded];
+        x.skipped = counts[kOutSkip];
+        x.g_only = counts[kOutGOnly];
+        x.recovered = counts[kOutRecovered];
+        x.heuristic = counts[kOutHeuristic];
+        x.invalid = n - (x.weight_updates + x.found + x.added + x.skipped + x.g_only +
+                         x.recovered + x.heuristic);
+        x.hops = wd[0];
+        x.redirects = wd[1];
+        if (wd[3]) x.candidates = wd[3];
+        float t = 0.f;
+        if (hipEventElapsedTime(&t, ev[3 * b + 1], ev[3 * b + 2]) == hipSuccess) x.walk_ms = t;
+        // the batch's span on the device timeline: its start to the next one's
+        uint32_t nx = b + 1;
+        while (nx < done_b && rs[nx].updates == 0) ++nx;
+        const hipEvent_t end = nx < done_b ? ev[3 * nx] : ev[3 * (size_t)nb];
+        if (hipEventElapsedTime(&t, ev[3 * b], end) == hipSuccess) x.total_ms = t;
+        if (st) st[b] = x;
+    }
+    if (fail != DYG_OK) return fail;
+    // a batch's worth of arena headroom for what comes next
+    r = device_edits_prepare(s->G, s->sp.max_batch, s->sp.max_batch);
+    if (r == DYG_OK) r = device_edits_prepare(s->H, s->cmax, s->cmax);
+    return r;
 }
 
 }  // namespace
@@ -1417,6 +1590,8 @@ extern "C" dyg_status dyg_stream_apply_many_dev(dyg_stream s, uint32_t nb, const
             set_error(s->ctx, "stream apply many: bad batch offsets");
             return DYG_ERR_ARG;
         }
+    if (nb && !(s->sp.flags & DYG_STREAM_PLAIN))
+        return stream_apply_list(s, nb, op, off, u, v, w, outcome, st);
     for (uint32_t b = 0; b < nb; ++b) {
         const uint64_t o = off[b];
         const dyg_status r =

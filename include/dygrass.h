@@ -360,13 +360,16 @@ dyg_status dyg_stream_apply_dev(dyg_stream s, int op, const uint32_t* u, const u
                                 dyg_stream_stats* st);
 /* A list of nb batches in one call, all in device memory: batch b is op[b]
  * over updates [off[b], off[b+1]) of u, v and w (w entries of deletion
- * batches are ignored; w may be NULL when every batch deletes); exactly nb
- * calls of dyg_stream_apply_dev in order, without a return to the caller in
- * between.  outcome (optional, host) receives every update's code at its
- * position, st (optional) nb per-batch stats.  Offsets are checked for the
- * whole list first (DYG_ERR_ARG, nothing applied); otherwise the call stops at
- * the first failing batch and returns its status (the batches before it stay
- * applied). */
+ * batches are ignored; w may be NULL when every batch deletes); the results
+ * of nb calls of dyg_stream_apply_dev in order.  The batches are enqueued
+ * back to back: the host waits only where a deletion batch's recovery list
+ * sizes must reach it, and once at the end; each batch reports into its own
+ * region and times its walk with its own events.  outcome (optional, host)
+ * receives every update's code at its position, st (optional) nb per-batch
+ * stats (total_ms: the batch's span on the device timeline).  Offsets, and
+ * then every vertex id of the list (on the device), are checked first
+ * (DYG_ERR_ARG, nothing applied); a later failure stops the call with the
+ * batches before it applied. */
 dyg_status dyg_stream_apply_many_dev(dyg_stream s, uint32_t nb, const int* op,
                                      const uint64_t* off, const uint32_t* u, const uint32_t* v,
                                      const float* w, int32_t* outcome, dyg_stream_stats* st);

@@ -400,3 +400,50 @@ def test_apply_many_dev_matches_apply_dev(ctx, grid_wl):
     _assert_same_graph(many.G, _rows_of(one.G))
     _assert_same_graph(many.H, _rows_of(one.H))
     assert many.position == one.position
+
+
+def test_apply_many_dev_checks_ids_first_and_runs_lists_in_turn(ctx, grid_wl):
+    """A list holding an out-of-range id anywhere is refused before anything
+    changes; an empty batch inside a list is a no-op; two lists in a row give
+    what one list of all their batches gives (the reports, events and arena
+    headroom carry over between calls)."""
+    from dygrass_amd import DygError, Stream, stream_params
+
+    wl = grid_wl
+    batches = mixed_batches(wl, 21, n_ins=250, n_del=150)
+    batches.insert(2, ("ins", wl.ins.u[:0], wl.ins.v[:0], wl.ins.w[:0]))
+    mk = lambda: Stream(ctx, wl.G.V, (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w),
+                        stream_params(w_ins=64, w_del=64, threshold=10.0, seed=21,
+                                      max_batch=3000, max_inserts=8192))
+
+    def dev_list(bs):
+        off = np.concatenate([[0], np.cumsum([len(b[1]) for b in bs])]).astype(np.uint64)
+        U = np.concatenate([np.asarray(b[1], np.uint32) for b in bs])
+        Vv = np.concatenate([np.asarray(b[2], np.uint32) for b in bs])
+        Wt = np.concatenate([np.asarray(b[3], np.float32) if b[3] is not None
+                             else np.zeros(len(b[1]), np.float32) for b in bs])
+        return ([b[0] for b in bs], off, _dev(ctx, U, np.uint32), _dev(ctx, Vv, np.uint32),
+                _dev(ctx, Wt, np.float32))
+
+    whole, split = mk(), mk()
+    ops, off, du, dv, dw = dev_list(batches)
+    want, _ = whole.apply_many_dev(ops, off, du, dv, dw)
+
+    bad = [list(b) for b in batches[:3]]
+    bad[2] = ("del", np.array([0, wl.G.V + 3], np.uint32), np.array([1, 2], np.uint32), None)
+    with pytest.raises(DygError, match="out of range"):
+        split.apply_many_dev(*dev_list(bad))
+    assert split.position == 0
+    fresh = mk()
+    _assert_same_graph(split.G, _rows_of(fresh.G))
+    _assert_same_graph(split.H, _rows_of(fresh.H))
+
+    got = []
+    for part in (batches[:3], batches[3:]):
+        o, sts = split.apply_many_dev(*dev_list(part))
+        got.append(o)
+        assert [s_["updates"] for s_ in sts] == [len(b[1]) for b in part]
+    assert np.array_equal(np.concatenate(got), want)
+    _assert_same_graph(split.G, _rows_of(whole.G))
+    _assert_same_graph(split.H, _rows_of(whole.H))
+    assert split.position == whole.position == int(off[-1])
