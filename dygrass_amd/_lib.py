@@ -13,6 +13,7 @@ import numpy as np
 _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, "lib", "libdygrass.so")
 SYNTH_PATH = os.path.join(_PKG, "lib", "libdygrass_synth.so")
+KAPPA_PATH = os.path.join(_PKG, "lib", "libdygrass_kappa.so")
 BIN_DIR = os.path.join(_PKG, "bin")
 
 u32p = C.POINTER(C.c_uint32)
@@ -159,6 +160,8 @@ _SIGS = {
     "dyg_serve_open": (C.c_int32, [vp, C.c_uint32, C.c_uint32, C.c_uint64, C.POINTER(vp)]),
     "dyg_serve_submit": (C.c_int32, [vp, C.c_int, vp, vp, vp, C.c_uint32,
                                      C.POINTER(C.c_uint64)]),
+    "dyg_serve_submit_dev": (C.c_int32, [vp, C.c_int, vp, vp, vp, C.c_uint32,
+                                         C.POINTER(C.c_uint64)]),
     "dyg_serve_wait": (C.c_int32, [vp, C.c_uint64, vp, C.POINTER(ServeResult)]),
     "dyg_serve_close": (None, [vp]),
     "dyg_serve_fail_next_copy": (C.c_int32, [vp]),
@@ -230,6 +233,21 @@ def synth_lib() -> C.CDLL:
     if _synth is None:
         _synth = _bind(SYNTH_PATH, _SYNTH_SIGS)
     return _synth
+
+
+_KAPPA_SIGS = {
+    "dyg_kappa_spmv": (None, [C.c_int64, vp, vp, vp, vp, vp, C.c_int]),
+    "dyg_kappa_pcg": (C.c_int, [C.c_int64, vp, vp, vp, vp, vp, C.c_double, C.c_int, C.c_int]),
+}
+_kappa = None
+
+
+def kappa_lib() -> C.CDLL:
+    """The host FP64 kernels of the kappa check (libdygrass_kappa.so)."""
+    global _kappa
+    if _kappa is None:
+        _kappa = _bind(KAPPA_PATH, _KAPPA_SIGS)
+    return _kappa
 
 
 def ptr(a: np.ndarray | None, ct):

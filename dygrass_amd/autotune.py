@@ -27,9 +27,12 @@ Eight knobs, in two kinds:
   screened on growth and kappa is computed once per knob, at the lowest value
   that passed the screen, stepping back up the knob's values while it misses.
 
-The served workload's sparsifier holds random shortcuts, so kappa there costs
-minutes per evaluation through the sparse LU: its quality knobs answer to the
-growth screen alone, and its tuning records kappa as unchecked.
+The served workload's sparsifier holds random shortcuts, which fill a sparse
+LU factorisation in: its kappa applies L_H^-1 by conjugate gradients on the
+host's cores instead (kappa method "pcg", about 100 s at 8M vertices).
+`--check-kappa` records kappa for the anchor and the chosen setting of an
+existing tuning (the chosen setting must lie within tol_kappa of the
+anchor), for a tuning made while kappa was out of reach there.
 
 Search: coordinate descent from the anchor.  Each quality knob walks down
 its candidate list (fewer walkers, shorter budgets) while candidates pass
@@ -254,9 +257,11 @@ class StreamTrials:
     """Runs the calibration stream (G, H, batches) on the device, one fresh
     Stream per pass (the CSR arrays are built once)."""
 
-    def __init__(self, ctx, V: int, g_edges, h_edges, batches, threshold: float, seed: int = 0):
+    def __init__(self, ctx, V: int, g_edges, h_edges, batches, threshold: float, seed: int = 0,
+                 kappa_method: str = "lu"):
         from .core import csr_from_edges
 
+        self.kappa_method = kappa_method
         self.ctx, self.V, self.batches = ctx, int(V), batches
         self.threshold, self.seed = float(threshold), seed
         self.g_edges = tuple(np.asarray(x) for x in g_edges)
@@ -297,13 +302,14 @@ class StreamTrials:
         finally:
             st.close()
 
-    def kappa(self, s: Setting) -> float:
+    def kappa(self, s: Setting, progress=None) -> float:
         from .kappa import kappa
 
         self.run(s, keep=True)  # the final graphs of this setting (deterministic pass)
         G, H = self.final
         self.final = None
-        return kappa(self.V, G, H, tol=1e-4)  # far below the 2% bound it is compared at
+        # tolerance far below the 2% bound it is compared at
+        return kappa(self.V, G, H, tol=1e-4, method=self.kappa_method, progress=progress)
 
 
 def _edges_once(csr):
@@ -330,6 +336,52 @@ def setting_of(tuning: dict, which: str = "chosen") -> Setting:
     return Setting(**tuning[which]["setting"])
 
 
+def check_kappa(tr: StreamTrials, tuning: dict, method: str, log,
+                which_settings=("anchor", "chosen")) -> dict:
+    """kappa of the anchor's and the chosen setting's final graphs on the
+    tuning's calibration stream, written into a copy of the tuning; each
+    setting's pass must reproduce the growth the tuning recorded.
+    `which_settings` limits the work to some of them (the others keep the
+    kappa the tuning file holds, so a check can be split over runs)."""
+    if tuning is None:
+        raise TuneError("check-kappa: tuning file not found")
+    if tr.updates != tuning["updates"]:
+        raise TuneError(f"check-kappa: calibration stream of {tr.updates} updates, the tuning "
+                        f"was made on {tuning['updates']}")
+    out = dict(tuning)
+    checks = []
+    for which in ("anchor", "chosen"):
+        s = setting_of(tuning, which)
+        if which not in which_settings:
+            checks.append({"quality": list(s.quality()), "kappa": tuning[which]["kappa"]})
+            continue
+        t0 = time.time()
+
+        def progress(p, which=which, t0=t0):
+            if p["solves"] % 5 == 0:
+                log(f"kappa {which}: {p['solves']} solves, {p['cg_iterations']} CG iterations, "
+                    f"{time.time() - t0:.0f} s")
+
+        k = tr.kappa(s, progress=progress)
+        t = tr.run(s)
+        growth = t.edges_h - tr.edges_h0
+        if growth != tuning[which]["growth"]:
+            raise TuneError(f"check-kappa: {which} grew H by {growth}, the tuning recorded "
+                            f"{tuning[which]['growth']}")
+        out[which] = dict(tuning[which], kappa=k)
+        checks.append({"quality": list(s.quality()), "kappa": k})
+        log(f"kappa {which} {s.quality()}: {k:.6g} ({time.time() - t0:.0f} s, {method})")
+    out["kappa_checks"] = checks
+    out["kappa_method"] = method
+    ka, kc = out["anchor"]["kappa"], out["chosen"]["kappa"]
+    tol = tuning["tolerances"]["kappa"]
+    if ka is not None and kc is not None:
+        out["kappa_within_tolerance"] = abs(kc - ka) <= tol * abs(ka)
+        log(f"chosen vs anchor kappa: {kc:.6g} vs {ka:.6g} "
+            f"({'within' if out['kappa_within_tolerance'] else 'OUTSIDE'} {tol:.0%})")
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description="tune the stream form's walk settings")
     ap.add_argument("--workload", choices=["batched", "served"], default="batched")
@@ -341,7 +393,15 @@ def main(argv=None) -> int:
     ap.add_argument("--tol-growth", type=float, default=0.02)
     ap.add_argument("--tol-kappa", type=float, default=0.02)
     ap.add_argument("--out", required=True)
+    ap.add_argument("--check-kappa", metavar="TUNING",
+                    help="instead of a search: kappa of the anchor and the chosen setting of "
+                         "this tuning file, on its calibration stream (written to --out)")
+    ap.add_argument("--kappa-of", choices=["anchor", "chosen", "both"], default="both",
+                    help="check-kappa: which settings to compute (others keep the file's)")
+    ap.add_argument("--kappa-method", choices=["lu", "pcg"], default=None,
+                    help="how kappa applies L_H^-1 (default: lu for batched, pcg for served)")
     a = ap.parse_args(argv)
+    method = a.kappa_method or ("lu" if a.workload == "batched" else "pcg")
 
     from . import synth
     from .core import Context
@@ -361,17 +421,28 @@ def main(argv=None) -> int:
         f"updates={updates} ({time.time() - t0:.1f}s)")
     ctx = Context(0)
     tr = StreamTrials(ctx, wl.G.V, (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w),
-                      batches, wl.threshold, a.seed)
+                      batches, wl.threshold, a.seed, kappa_method=method)
+    if a.check_kappa:
+        which = ("anchor", "chosen") if a.kappa_of == "both" else (a.kappa_of,)
+        res = check_kappa(tr, load(name, a.check_kappa), method, log, which)
+        res["kappa_command"] = ("python -m dygrass_amd.autotune "
+                                + " ".join(argv or sys.argv[1:]))
+        with open(a.out, "w") as f:
+            json.dump(res, f, indent=1)
+        ctx.close()
+        return 0
     if a.workload == "batched":
         res = tune(tr.run, tr.kappa, tr.edges_h0, repeats=a.repeats, tol_growth=a.tol_growth,
                    tol_kappa=a.tol_kappa, log=log)
     else:
-        # kappa on the served sparsifier (a tree plus random shortcuts) costs
-        # minutes per evaluation through the sparse LU, so the quality knobs
-        # answer to the growth screen alone; step graphs are not a knob here
-        # (the served loop launches directly)
-        res = tune(tr.run, None, tr.edges_h0, repeats=a.repeats, tol_growth=a.tol_growth,
-                   log=log, shape_grid={k: v for k, v in SHAPE_GRID.items() if k != "graph"})
+        # kappa on the served sparsifier (a tree plus random shortcuts) runs
+        # through CG solves on the host (about 100 s per evaluation at 8M
+        # vertices); step graphs are not a knob here (the served loop
+        # launches directly)
+        res = tune(tr.run, tr.kappa, tr.edges_h0, repeats=a.repeats, tol_growth=a.tol_growth,
+                   tol_kappa=a.tol_kappa, log=log,
+                   shape_grid={k: v for k, v in SHAPE_GRID.items() if k != "graph"})
+        res["kappa_method"] = method
     res.update({"workload": name, "V": int(wl.G.V), "E_G": int(len(wl.G)),
                 "E_H": int(len(wl.H)), "updates": int(updates),
                 "batches": len(batches), "threshold": wl.threshold, "seed": a.seed,

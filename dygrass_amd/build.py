@@ -8,6 +8,8 @@ package so they travel with the source tree:
     dygrass_amd/lib/libdygrass.so
     dygrass_amd/bin/incremental_update
     dygrass_amd/bin/decremental_update
+    dygrass_amd/lib/libdygrass_synth.so   (workload generators, host only)
+    dygrass_amd/lib/libdygrass_kappa.so   (kappa check kernels, host only)
 
 Run as ``python -m dygrass_amd.build`` (or through __graft_entry__.build()).
 """
@@ -89,6 +91,11 @@ def build(verbose: bool = True) -> None:
     _run([CXX, *HOST_FLAGS, "-O3", "-fopenmp", "-shared", os.path.join(SRC, "synth.cpp"),
           "-o", synth + ".tmp"])
     os.replace(synth + ".tmp", synth)
+    # FP64 sparse kernels of the kappa check (host only, OpenMP)
+    kap = os.path.join(LIB, "libdygrass_kappa.so")
+    _run([CXX, *HOST_FLAGS, "-O3", "-fopenmp", "-shared", os.path.join(SRC, "kappa.cpp"),
+          "-o", kap + ".tmp"])
+    os.replace(kap + ".tmp", kap)
     if verbose:
         print(f"built {so}, {synth} and {', '.join(CLIS)}")
 

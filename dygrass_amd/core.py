@@ -521,6 +521,21 @@ class Server:
               self.stream.ctx.handle)
         return t.value
 
+    def submit_dev(self, op: str, u, v, w=None, n: int | None = None) -> int:
+        """A request already in device memory (DeviceArray objects or raw device
+        addresses; kept unchanged until its wait returns): ids are checked on
+        the device before admission.  Returns the ticket."""
+        du = u.ptr if isinstance(u, DeviceArray) else int(u)
+        dv = v.ptr if isinstance(v, DeviceArray) else int(v)
+        dw = (w.ptr if isinstance(w, DeviceArray) else int(w)) if w is not None else None
+        if n is None:
+            n = u.n if isinstance(u, DeviceArray) else 0
+        t = C.c_uint64()
+        code = L.DYG_STREAM_INSERT if op == "ins" else L.DYG_STREAM_DELETE
+        check(self._lib.dyg_serve_submit_dev(self.handle, code, du, dv, dw, n, C.byref(t)),
+              self.stream.ctx.handle)
+        return t.value
+
     def wait(self, ticket: int, n: int) -> tuple[np.ndarray, dict]:
         out = np.zeros(max(n, 1), np.int32)
         res = L.ServeResult()

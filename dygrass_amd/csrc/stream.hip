@@ -1647,6 +1647,10 @@ struct ServeReq {
     std::vector<int32_t> outcome;
     uint64_t hops = 0;
     bool copied = false;  // input copied to the slot's device buffer (else read in place)
+    // device inputs (dyg_serve_submit_dev): the caller's arrays, read in place
+    const uint32_t* du = nullptr;
+    const uint32_t* dv = nullptr;
+    const float* dw = nullptr;
     bool done = false;
 };
 
@@ -1662,6 +1666,12 @@ struct dyg_serve_s {
     std::mutex mu;
     std::condition_variable cv_work, cv_inflight, cv_done;
     bool fail_next_copy = false;  // testing hook (dyg_serve_fail_next_copy)
+    // id check of device inputs (dyg_serve_submit_dev), one at a time on the
+    // copy stream: a pinned flag word and the event that orders it
+    std::mutex chk_mu;
+    unsigned long long* chk = nullptr;
+    unsigned long long* chk_dev = nullptr;
+    hipEvent_t e_chk = nullptr;
     std::deque<ServeReq*> queue, inflight;
     std::unordered_map<uint64_t, ServeReq*> reqs;
     std::thread worker, completer;
@@ -1752,6 +1762,10 @@ void serve_worker(dyg_serve_s* v) {
             du = sl.din;
             dv = sl.din + n;
             dw = (const float*)(sl.din + 2ull * n);
+        } else if (r->du) {  // device inputs, ids checked at submit
+            du = r->du;
+            dv = r->dv;
+            dw = r->dw;
         }
         const Report rep{dp.cls, dp.words, s->G->dev.meta, s->H->dev.meta};
         if (st == DYG_OK)
@@ -1864,14 +1878,28 @@ extern "C" dyg_status dyg_serve_open(dyg_stream s, uint32_t slots, uint32_t max_
     // served requests report into their slots; the stream's own rejection
     // word must not carry over from an earlier rejected dyg_stream_apply_dev
     ((unsigned long long*)s->pin)[kReportBad] = 0;
+    void* cd = nullptr;
+    if (hipHostMalloc((void**)&v->chk, 256, hipHostMallocDefault) != hipSuccess ||
+        hipHostGetDevicePointer(&cd, v->chk, 0) != hipSuccess ||
+        hipEventCreateWithFlags(&v->e_chk, hipEventDisableTiming) != hipSuccess) {
+        set_error(ctx, "serve: pinned allocation failed");
+        dyg_serve_close(v);
+        return DYG_ERR_NOMEM;
+    }
+    v->chk_dev = (unsigned long long*)cd;
     v->worker = std::thread(serve_worker, v);
     v->completer = std::thread(serve_completer, v);
     *out = v;
     return DYG_OK;
 }
 
-extern "C" dyg_status dyg_serve_submit(dyg_serve v, int op, const uint32_t* u, const uint32_t* w_v,
-                                       const float* w, uint32_t n, uint64_t* ticket) {
+namespace {
+
+// Both submits: host arrays (dev false) are range-checked on the host and
+// copied into the slot; device arrays are range-checked by a kernel on the
+// copy stream before admission and read in place by the request.
+dyg_status serve_submit(dyg_serve v, int op, const uint32_t* u, const uint32_t* w_v,
+                        const float* w, uint32_t n, uint64_t* ticket, bool dev) {
     if (!v || !ticket || (op != DYG_STREAM_INSERT && op != DYG_STREAM_DELETE) ||
         (n && (!u || !w_v || (op == DYG_STREAM_INSERT && !w))))
         return DYG_ERR_ARG;
@@ -1881,11 +1909,27 @@ extern "C" dyg_status dyg_serve_submit(dyg_serve v, int op, const uint32_t* u, c
         set_error(ctx, "serve submit: request size must be in [1, max_request]");
         return DYG_ERR_ARG;
     }
-    uint32_t mx = 0;
-    for (uint32_t i = 0; i < n; ++i) mx = std::max(mx, std::max(u[i], w_v[i]));
-    if (mx >= v->s->G->V) {
-        set_error(ctx, "serve submit: vertex id out of range");
-        return DYG_ERR_ARG;
+    if (dev) {
+        std::lock_guard<std::mutex> lk(v->chk_mu);
+        v->chk[0] = 0;
+        hipLaunchKernelGGL(k_check_batch, dim3((n + 255) / 256), dim3(256), 0, v->copy, u, w_v,
+                           n, v->s->G->V, v->chk_dev);
+        if (hipGetLastError() != hipSuccess || hipEventRecord(v->e_chk, v->copy) != hipSuccess ||
+            spin_event(v->e_chk) != hipSuccess) {
+            set_error(ctx, "serve submit: id check failed on the device");
+            return DYG_ERR_HIP;
+        }
+        if (v->chk[0]) {
+            set_error(ctx, "serve submit: vertex id out of range");
+            return DYG_ERR_ARG;
+        }
+    } else {
+        uint32_t mx = 0;
+        for (uint32_t i = 0; i < n; ++i) mx = std::max(mx, std::max(u[i], w_v[i]));
+        if (mx >= v->s->G->V) {
+            set_error(ctx, "serve submit: vertex id out of range");
+            return DYG_ERR_ARG;
+        }
     }
     const uint64_t need = request_need(v, op, n);
     auto* r = new ServeReq;
@@ -1913,11 +1957,18 @@ extern "C" dyg_status dyg_serve_submit(dyg_serve v, int op, const uint32_t* u, c
         ServeSlot& sl = v->slots[slot];
         const PinLayout hp = pin_layout((char*)sl.host, n);
         const bool ins = op == DYG_STREAM_INSERT;
-        std::memcpy(hp.u, u, n * 4ull);
-        std::memcpy(hp.v, w_v, n * 4ull);
-        if (ins) std::memcpy(hp.w, w, n * 4ull);
-        r->copied = !(ins && n <= kSmallBatch && !(v->s->sp.flags & DYG_STREAM_GENERAL_ONLY));
         bool copy_ok = true;
+        if (dev) {
+            r->du = u;
+            r->dv = w_v;
+            r->dw = ins ? w : nullptr;
+            r->copied = false;
+        } else {
+            std::memcpy(hp.u, u, n * 4ull);
+            std::memcpy(hp.v, w_v, n * 4ull);
+            if (ins) std::memcpy(hp.w, w, n * 4ull);
+            r->copied = !(ins && n <= kSmallBatch && !(v->s->sp.flags & DYG_STREAM_GENERAL_ONLY));
+        }
         if (r->copied) {
             if (v->fail_next_copy) {  // testing hook: report a failure, issue nothing
                 v->fail_next_copy = false;
@@ -1948,6 +1999,19 @@ extern "C" dyg_status dyg_serve_submit(dyg_serve v, int op, const uint32_t* u, c
     }
     v->cv_work.notify_one();
     return DYG_OK;
+}
+
+}  // namespace
+
+extern "C" dyg_status dyg_serve_submit(dyg_serve v, int op, const uint32_t* u, const uint32_t* w_v,
+                                       const float* w, uint32_t n, uint64_t* ticket) {
+    return serve_submit(v, op, u, w_v, w, n, ticket, false);
+}
+
+extern "C" dyg_status dyg_serve_submit_dev(dyg_serve v, int op, const uint32_t* u,
+                                           const uint32_t* w_v, const float* w, uint32_t n,
+                                           uint64_t* ticket) {
+    return serve_submit(v, op, u, w_v, w, n, ticket, true);
 }
 
 extern "C" dyg_status dyg_serve_fail_next_copy(dyg_serve v) {
@@ -2013,6 +2077,8 @@ extern "C" void dyg_serve_close(dyg_serve v) {
     for (auto& kv : v->reqs) delete kv.second;
     for (ServeReq* r : v->queue) delete r;
     for (ServeReq* r : v->inflight) delete r;
+    if (v->chk) (void)hipHostFree(v->chk);
+    if (v->e_chk) (void)hipEventDestroy(v->e_chk);
     if (v->copy) (void)hipStreamDestroy(v->copy);
     delete v;
 }

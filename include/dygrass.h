@@ -406,6 +406,16 @@ dyg_status dyg_serve_open(dyg_stream s, uint32_t slots, uint32_t max_request,
  * the call with DYG_ERR_HIP, applies nothing and stops the server. */
 dyg_status dyg_serve_submit(dyg_serve v, int op, const uint32_t* u, const uint32_t* v_,
                             const float* w, uint32_t n, uint64_t* ticket);
+/* dyg_serve_submit with the request already in device memory: u, v_ and w
+ * are device pointers (w ignored for deletions), ready when the call is made
+ * and left unchanged until dyg_serve_wait returns for the ticket (the
+ * request reads them in place; nothing is copied).  Its ids are checked on
+ * the device (on the server's copy stream, beside the requests in flight)
+ * before admission: an id >= V fails the call with DYG_ERR_ARG and the server
+ * goes on.  No reference counterpart (the reference reads batches from files
+ * into host memory, random_walk_incremental/functions.cpp:261-370). */
+dyg_status dyg_serve_submit_dev(dyg_serve v, int op, const uint32_t* u, const uint32_t* v_,
+                                const float* w, uint32_t n, uint64_t* ticket);
 /* Testing hook: the next admitted request that copies its input reports the
  * copy as failed without issuing it (the failure path of dyg_serve_submit,
  * forced from the host).  No reference counterpart. */

@@ -216,3 +216,54 @@ def test_serve_grid3d_family_matches_oracle(ctx):
         assert np.array_equal(rp, orp) and np.array_equal(col, ocol)
         assert np.array_equal(f32bits(w), f32bits(ow))
     st.close()
+
+
+def test_serve_device_inputs_equal_host_inputs(ctx, grid_wl):
+    """dyg_serve_submit_dev: the same request sequence submitted from device
+    arrays (read in place, small and general-path sizes, insertions and
+    deletions) gives the outcome codes and graphs of host submits; a request
+    with an out-of-range id is refused at submit and the server goes on."""
+    from dygrass_amd import DeviceArray, DygError, Server, Stream, stream_params
+
+    wl = grid_wl
+    reqs = _requests(wl, 9, count=18)
+    rng = np.random.default_rng(9)
+    big = rng.choice(len(wl.G), 2500, replace=False)  # above the one-workgroup size
+    reqs.insert(5, ("del", wl.G.u[big], wl.G.v[big], None))
+    reqs.insert(9, ("ins", wl.ins.u[1500:1980], wl.ins.v[1500:1980], wl.ins.w[1500:1980]))
+    sp = stream_params(w_ins=64, w_del=64, threshold=10.0, seed=6, max_batch=3000,
+                       max_inserts=16384)
+    g_edges, h_edges = (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w)
+
+    host = Stream(ctx, wl.G.V, g_edges, h_edges, sp)
+    srv = Server(host, slots=4, max_request=3000)
+    want, _ = serve_all(srv, reqs)
+    srv.close()
+
+    dev = Stream(ctx, wl.G.V, g_edges, h_edges, sp)
+    srv = Server(dev, slots=4, max_request=3000)
+    arrays = [(DeviceArray.from_numpy(ctx, np.asarray(u, np.uint32)),
+               DeviceArray.from_numpy(ctx, np.asarray(v, np.uint32)),
+               DeviceArray.from_numpy(ctx, np.asarray(w, np.float32)) if w is not None else None)
+              for _, u, v, w in reqs]
+    bu = DeviceArray.from_numpy(ctx, np.array([1, 2, wl.G.V], np.uint32))
+    bv = DeviceArray.from_numpy(ctx, np.array([3, 4, 5], np.uint32))
+    tickets = []
+    for i, ((op, u, v, w), (du, dv, dw)) in enumerate(zip(reqs, arrays)):
+        if i == 7:
+            with pytest.raises(DygError, match="out of range"):
+                srv.submit_dev("del", bu, bv)
+        tickets.append(srv.submit_dev(op, du, dv, dw))
+    got = [srv.wait(t, len(r[1]))[0] for t, r in zip(tickets, reqs)]
+    srv.close()
+    for a, b in zip(got, want):
+        assert np.array_equal(a, b)
+    _same(host.G, dev.G)
+    _same(host.H, dev.H)
+    assert dev.position == host.position
+    for t in arrays:
+        for a in t:
+            if a is not None:
+                a.close()
+    host.close()
+    dev.close()

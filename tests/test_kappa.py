@@ -99,3 +99,45 @@ def test_kappa_cli_reads_both_files(tmp_path, capsys):
     assert main([str(tmp_path / "g.mtx"), str(tmp_path / "h.mtx")]) == 0
     got = float(capsys.readouterr().out.split()[-1])
     assert got == pytest.approx(n, rel=1e-3)  # 1 + resistance of the 29-edge path
+
+
+def test_pcg_kappa_equals_dense_and_lu():
+    """The CG path (L_H^-1 by diagonal-preconditioned conjugate gradients on
+    the host's cores) against the dense generalized eigenproblem on a small
+    3D grid with random shortcuts (the served workload's family) forced
+    through Lanczos, and against the sparse-LU path; also independent of the
+    thread count (block-ordered reductions)."""
+    from dygrass_amd import synth
+
+    wl = synth.config_served(12 ** 3 / 8e6, 0, count=4)  # 12^3 vertices
+    G = (wl.G.u, wl.G.v, wl.G.w)
+    H = (wl.H.u, wl.H.v, wl.H.w)
+    V = wl.G.V
+    assert V == 12 ** 3 and len(wl.H) < len(wl.G)
+    dense = float(la.eigh(_dense_lap(V, *G), _dense_lap(V, *H), eigvals_only=True)[-1])
+    info = {}
+    k1 = kappa(V, G, H, tol=1e-12, method="pcg", threads=1, info=info, dense_limit=0)
+    k4 = kappa(V, G, H, tol=1e-12, method="pcg", threads=4, dense_limit=0)
+    klu = kappa(V, G, H, tol=1e-12, method="lu", dense_limit=0)
+    assert info["solves"] > 0 and info["cg_iterations"] > 0
+    assert abs(k1 - dense) <= 1e-9 * dense
+    assert abs(klu - dense) <= 1e-9 * dense
+    assert k1 == k4
+    assert dense > 1.0
+
+
+def test_pcg_kappa_cycle_closed_form():
+    """kappa of a weighted cycle over its spanning path through the CG path
+    (the closed form 1 + w_c * R_path, as for the sparse-LU path)."""
+    n = 400
+    rng = np.random.default_rng(7)
+    w = rng.uniform(0.5, 2.0, n)
+    u, v, ww = _cycle(n, w)
+    path = (u[:-1], v[:-1], ww[:-1])
+    k = kappa(n, (u, v, ww), path, tol=1e-12, method="pcg", dense_limit=0)
+    # rank-one update on the path: lambda_max = 1 + w_c * b' L_P^-1 b, L_P shifted
+    L = _dense_lap(n, *path)
+    b = np.zeros(n)
+    b[n - 1], b[0] = 1.0, -1.0
+    want = 1.0 + ww[-1] * b @ np.linalg.solve(L, b)
+    assert abs(k - want) <= 1e-7 * want
