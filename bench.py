@@ -29,8 +29,9 @@ grid: the 128x128 grid with 2,000 insertions in 10 snapshot batches,
   threshold 10, 512 walkers (the plain incremental configuration that also
   runs through the CPU walk path), GPU beside the oracle's whole run.
 
-CPU baselines (the oracle, OpenMP, on the cores this process may use) run
-after every GPU leg.
+CPU baselines (the oracle, OpenMP, on the cores this process may use, in a
+child process with its threads bound to neighbouring cores) run after every
+GPU leg.
 
   python bench.py [--steps K] [--warmup W]
                   [--workload all|batched|served|incremental|decremental|grid]
@@ -199,26 +200,7 @@ def leg_incremental(args, ctx, grid: bool = False) -> dict:
                         "seconds": time.perf_counter() - ts}
         log(f"{leg} kappa: {k_before:.1f} -> {k_after:.1f}")
     if not args.no_cpu:
-        found = head["found"]
-
-        def cpu_job():
-            from oracle import pyoracle as po
-
-            threads = cpu_threads(args)
-            og = po.Graph(wl.H.V, rp, col, w)
-            ts = time.perf_counter()
-            cfound, _ = og.incremental(wl.ins.u, wl.ins.v, wl.ins.w, wl.threshold, wl.walkers,
-                                       seed=args.seed, threads=threads)
-            cdt = time.perf_counter() - ts
-            log(f"{leg} cpu: {m / cdt:.0f} edge-updates/s on {threads} threads")
-            return {"value": m / cdt, "unit": "edge-updates/s", "cores": threads,
-                    "nproc": os.cpu_count(), "kind": "port",
-                    "sample": "the whole workload (every snapshot batch's walks and the "
-                              "sparsifier appends) through the oracle, OpenMP over each "
-                              "batch's edges",
-                    "decisions_identical": bool(np.array_equal(found, cfound))}
-
-        out["_cpu_job"] = cpu_job
+        out["_cpu"] = ("grid" if grid else "incremental", head["found"])
     return out
 
 
@@ -434,20 +416,82 @@ def leg_batched(args, ctx) -> dict:
                            b.view(np.uint32) if b.dtype == np.float32 else b)
             for a, b in zip(pr["H"], head["H"]))
     if not args.no_cpu:
-        gpu_out = head["outcomes"]
-        out["_cpu_job"] = lambda: cpu_stream_batched(wl, args, setting, csr, gpu_out)
+        out["_cpu"] = ("batched", head["outcomes"])
     return out
 
 
 def cpu_threads(args) -> int:
     """Host threads for the CPU baselines: --cpu-threads, else the cores this
     process may use (OMP_NUM_THREADS when the environment sets it: a GPU box
-    allots its cores per GPU that way), else nproc."""
+    allots its cores per GPU that way), else nproc.  Read at start-up, before
+    any OpenMP runtime binds a thread."""
     if args.cpu_threads:
         return args.cpu_threads
     n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     env = os.environ.get("OMP_NUM_THREADS", "")
     return min(n, int(env)) if env.isdigit() and int(env) > 0 else n
+
+
+def cpu_baseline_child(args, leg: str, gpu_outcomes) -> dict:
+    """Runs one leg's CPU baseline in a child process whose OpenMP threads
+    are bound to neighbouring cores (OMP_PROC_BIND=close, OMP_PLACES=cores,
+    OMP_NUM_THREADS = the cores counted at start-up): the binding stays out of
+    this process, whose served loop and GPU threads must not inherit it.  The
+    child regenerates the same seeded workload and compares its outcomes with
+    the GPU's (passed through a temporary file)."""
+    import subprocess
+    import tempfile
+
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "gpu_outcomes.npy")
+        np.save(path, np.asarray(gpu_outcomes))
+        env = dict(os.environ, OMP_PROC_BIND="close", OMP_PLACES="cores",
+                   OMP_NUM_THREADS=str(args.threads_at_start))
+        cmd = [sys.executable, os.path.abspath(__file__), "--cpu-leg", leg, "--gpu-outcomes",
+               path, "--seed", str(args.seed), "--scale", str(args.scale),
+               "--cpu-threads", str(args.threads_at_start)]
+        p = subprocess.run(cmd, env=env, stdout=subprocess.PIPE, text=True, timeout=900)
+        if p.returncode != 0:
+            raise RuntimeError(f"CPU baseline of {leg} failed ({p.returncode})")
+        return json.loads(p.stdout.strip().splitlines()[-1])
+
+
+def cpu_leg_main(args) -> int:
+    """--cpu-leg: one leg's CPU baseline on this process's host cores (no GPU
+    work), printed as one JSON line."""
+    from dygrass_amd import csr_from_edges, synth
+
+    gpu = np.load(args.gpu_outcomes) if args.gpu_outcomes else None
+    threads = args.cpu_threads = cpu_threads(args)  # before any OpenMP runtime starts
+    if args.cpu_leg == "batched":
+        wl = synth.config_batched(args.scale, args.seed)
+        setting, _ = tuned(wl.name, wl.w_ins, wl.w_del)
+        csr = (csr_from_edges(wl.G.V, wl.G.u, wl.G.v, wl.G.w),
+               csr_from_edges(wl.G.V, wl.H.u, wl.H.v, wl.H.w))
+        out = cpu_stream_batched(wl, args, setting, csr, gpu)
+    else:
+        from oracle import pyoracle as po
+
+        grid = args.cpu_leg == "grid"
+        wl = (synth.config_grid_insert if grid else synth.config_rgg_insert)(args.scale,
+                                                                             args.seed)
+        m = len(wl.ins)
+        rp, col, w = csr_from_edges(wl.H.V, wl.H.u, wl.H.v, wl.H.w)
+        og = po.Graph(wl.H.V, rp, col, w)
+        ts = time.perf_counter()
+        cfound, _ = og.incremental(wl.ins.u, wl.ins.v, wl.ins.w, wl.threshold, wl.walkers,
+                                   seed=args.seed, threads=threads)
+        cdt = time.perf_counter() - ts
+        log(f"{args.cpu_leg} cpu: {m / cdt:.0f} edge-updates/s on {threads} threads")
+        out = {"value": m / cdt, "unit": "edge-updates/s", "cores": threads,
+               "nproc": os.cpu_count(), "kind": "port",
+               "sample": "the whole workload (every snapshot batch's walks and the sparsifier "
+                         "appends) through the oracle, OpenMP over each batch's edges, "
+                         f"OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND', 'unset')}",
+               "decisions_identical": (bool(np.array_equal(gpu, cfound))
+                                       if gpu is not None else None)}
+    print(json.dumps(out), flush=True)
+    return 0
 
 
 def cpu_stream_batched(wl, args, setting, csr, gpu_outcomes) -> dict:
@@ -472,9 +516,10 @@ def cpu_stream_batched(wl, args, setting, csr, gpu_outcomes) -> dict:
     orc.close()
     got = np.concatenate(outs)
     n = len(got)
-    same = bool(np.array_equal(got, gpu_outcomes[:n]))
+    same = (bool(np.array_equal(got, gpu_outcomes[:n])) if gpu_outcomes is not None else None)
     log(f"batched cpu: {n / st['total_seconds']:.0f} edge-updates/s on {threads} threads "
-        f"({nb} of {len(wl.batches)} batches, outcomes {'equal' if same else 'DIFFER'})")
+        f"({nb} of {len(wl.batches)} batches, outcomes "
+        f"{'not compared' if same is None else 'equal' if same else 'DIFFER'})")
     return {"value": n / st["total_seconds"], "unit": "edge-updates/s", "cores": threads,
             "nproc": os.cpu_count(), "kind": "port",
             "sample": f"the stream's first {nb} of {len(wl.batches)} batches ({n} updates: "
@@ -704,6 +749,9 @@ def main() -> int:
                     help="served: further arrival rates (per second) for the latency curve")
     ap.add_argument("--plain-requests", type=int, default=40,
                     help="served: requests of the trace the plain form serves")
+    ap.add_argument("--cpu-leg", choices=["batched", "incremental", "grid"],
+                    help=argparse.SUPPRESS)  # child process of a CPU baseline
+    ap.add_argument("--gpu-outcomes", help=argparse.SUPPRESS)
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run one device-to-device copy of this size (a known-size "
                          "dispatch for scaling hardware byte counters)")
@@ -715,10 +763,11 @@ def main() -> int:
               "this build does not implement", file=sys.stderr)
         return 2
 
-    # CPU baselines bind their OpenMP threads to neighbouring cores; they run
-    # after every GPU leg, so the binding (which the main thread takes too)
-    # never reaches the served loop's threads
-    os.environ.setdefault("OMP_PROC_BIND", "close")
+    if args.cpu_leg:
+        return cpu_leg_main(args)
+    # the host cores for the CPU baselines, counted before any OpenMP runtime
+    # (the workload generators use one) binds a thread
+    args.threads_at_start = cpu_threads(args)
     from dygrass_amd import Context, DeviceArray
 
     ctx = Context(0)
@@ -745,9 +794,9 @@ def main() -> int:
                       "decremental": leg_decremental,
                       "grid": lambda a, c: leg_incremental(a, c, grid=True)}[name](args, ctx)
     for name, leg in legs.items():  # CPU baselines, after all GPU work
-        job = leg.pop("_cpu_job", None)
+        job = leg.pop("_cpu", None)
         if job is not None:
-            leg["cpu"] = job()
+            leg["cpu"] = cpu_baseline_child(args, *job)
             leg["speedup_vs_cpu"] = leg["value"] / leg["cpu"]["value"]
     head_name = want[0]
     head = legs[head_name]
