@@ -183,9 +183,8 @@ dyg_status walk_host(dyg_graph g, bool insert, const uint32_t* src, const uint32
     a.hops = d_hops;
     a.path = insert ? nullptr : d_path;
     a.stats = d_stats;
-    DYG_HIP(ctx, hipEventRecord(ctx->ev0, s));
-    DYG_HIP(ctx, insert ? launch_walk_incremental(a, s) : launch_walk_decremental(a, s));
-    DYG_HIP(ctx, hipEventRecord(ctx->ev1, s));
+    DYG_HIP(ctx, insert ? launch_walk_incremental(a, s, ctx->ev0, ctx->ev1)
+                        : launch_walk_decremental(a, s, ctx->ev0, ctx->ev1));
     DYG_HIP(ctx, hipMemcpyAsync(found_at, d_found, n * 4ull, hipMemcpyDeviceToHost, s));
     DYG_HIP(ctx, hipMemcpyAsync(cond, d_cond, n * 4ull, hipMemcpyDeviceToHost, s));
     if (winner) DYG_HIP(ctx, hipMemcpyAsync(winner, d_win, n * 4ull, hipMemcpyDeviceToHost, s));

@@ -56,7 +56,7 @@ dyg_status run_incremental(dyg_graph H, const RunBufs& b, const uint32_t* h_src,
         set_error(ctx, "incremental: event creation failed");
         return DYG_ERR_HIP;
     }
-    auto launch = [&](uint32_t b0, uint32_t n) -> hipError_t {
+    auto launch = [&](uint32_t b0, uint32_t n, hipEvent_t e0, hipEvent_t e1) -> hipError_t {
         WalkLaunch a{};
         a.rows = H->d_rows;
         a.nodesc = H->nodesc;
@@ -75,16 +75,14 @@ dyg_status run_incremental(dyg_graph H, const RunBufs& b, const uint32_t* h_src,
         a.found = b.found + b0;
         a.cond = b.cond + b0;
         a.stats = b.stats;
-        return launch_walk_incremental(a, s);
+        return launch_walk_incremental(a, s, e0, e1);  // the events time the kernel itself
     };
     if (device_edits && m > 0) {
         ret = device_edits_begin(H, I, m);
         uint32_t k = 0;
         for (uint32_t b0 = 0; b0 < m && ret == DYG_OK; b0 += I, ++k) {
             const uint32_t n = std::min(I, m - b0);
-            hipError_t e = hipEventRecord(ctx->evpool[3 * k], s);
-            if (e == hipSuccess) e = launch(b0, n);
-            if (e == hipSuccess) e = hipEventRecord(ctx->evpool[3 * k + 1], s);
+            hipError_t e = launch(b0, n, ctx->evpool[3 * k], ctx->evpool[3 * k + 1]);
             if (e != hipSuccess) {
                 set_error(ctx, std::string("incremental walk: ") + hipGetErrorString(e));
                 ret = DYG_ERR_HIP;
@@ -131,9 +129,7 @@ dyg_status run_incremental(dyg_graph H, const RunBufs& b, const uint32_t* h_src,
         std::vector<float> aw;
         for (uint32_t b0 = 0; b0 < m && ret == DYG_OK; b0 += I) {
             const uint32_t n = std::min(I, m - b0);
-            hipError_t e = hipEventRecord(ctx->ev0, s);
-            if (e == hipSuccess) e = launch(b0, n);
-            if (e == hipSuccess) e = hipEventRecord(ctx->ev1, s);
+            hipError_t e = launch(b0, n, ctx->ev0, ctx->ev1);
             if (e == hipSuccess)
                 e = hipMemcpyAsync(found.data() + b0, b.found + b0, n * 4ull,
                                    hipMemcpyDeviceToHost, s);

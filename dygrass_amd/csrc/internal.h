@@ -181,8 +181,13 @@ struct WalkLaunch {
     const uint32_t* n_dev;   // optional: list length in device memory (<= n; grid sized by n)
     bool nodesc;             // the graph's slots carry no neighbour descriptors
 };
-hipError_t launch_walk_incremental(const WalkLaunch& a, hipStream_t s);
-hipError_t launch_walk_decremental(const WalkLaunch& a, hipStream_t s);
+// ev_begin / ev_end (optional) take the walk kernel's own start and end
+// times (recorded with the dispatch, not as marker packets around it); an
+// empty launch records both where it would have run.
+hipError_t launch_walk_incremental(const WalkLaunch& a, hipStream_t s,
+                                   hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
+hipError_t launch_walk_decremental(const WalkLaunch& a, hipStream_t s,
+                                   hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
 hipError_t launch_heuristic(const uint2* rows, const uint4* adj, const uint32_t* endpoint,
                             const uint64_t* key, uint32_t n, uint32_t walkers, uint64_t seed,
                             uint32_t* path3, hipStream_t s, const uint32_t* n_dev = nullptr,

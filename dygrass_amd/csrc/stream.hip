@@ -666,12 +666,11 @@ int step_class(uint32_t n) {
 dyg_status enqueue_ins_step(dyg_stream s, const SmallInsArgs& a, bool timed) {
     hipStream_t st = s->ctx->stream;
     DYG_HIP(s->ctx, launch_small_ins_pre(a, st));
-    if (timed) DYG_HIP(s->ctx, hipEventRecord(s->wev0, st));
     WalkLaunch w = walk_args(s, s->H, s->sp.ins, a.n);
     w.stats = a.stats;
     w.flags |= DYG_WALK_DECISION_ONLY;  // outcomes need found / not found only
-    DYG_HIP(s->ctx, launch_walk_incremental(w, st));
-    if (timed) DYG_HIP(s->ctx, hipEventRecord(s->wev1, st));
+    DYG_HIP(s->ctx, launch_walk_incremental(w, st, timed ? s->wev0 : nullptr,
+                                            timed ? s->wev1 : nullptr));
     DYG_HIP(s->ctx, launch_small_ins_post(a, st));
     return DYG_OK;
 }
@@ -685,9 +684,8 @@ dyg_status enqueue_del_step(dyg_stream s, const SmallDelArgs& a, bool timed) {
     w.w = nullptr;
     w.path = s->path;
     w.stats = a.stats;
-    if (timed) DYG_HIP(s->ctx, hipEventRecord(s->wev0, st));
-    DYG_HIP(s->ctx, launch_walk_decremental(w, st));
-    if (timed) DYG_HIP(s->ctx, hipEventRecord(s->wev1, st));
+    DYG_HIP(s->ctx, launch_walk_decremental(w, st, timed ? s->wev0 : nullptr,
+                                            timed ? s->wev1 : nullptr));
     DYG_HIP(s->ctx, launch_small_del_post(a, st));
     return DYG_OK;
 }
@@ -879,12 +877,11 @@ dyg_status apply_insert(dyg_stream s, const uint32_t* du, const uint32_t* dv, co
     LAUNCH_1D(k_compact, n, s->keep, n, s->bcnt, du, dv, dw, s->base, false, s->cu, s->cv,
               s->cw, s->ceid, s->sel, s->cnt);
     DYG_HIP(s->ctx, hipGetLastError());
-    if (timed) DYG_HIP(s->ctx, hipEventRecord(s->wev0, st));
     WalkLaunch a = walk_args(s, H, s->sp.ins, n);
     a.stats = dstats;
     a.flags |= DYG_WALK_DECISION_ONLY;  // outcomes need found / not found only
-    DYG_HIP(s->ctx, launch_walk_incremental(a, st));
-    if (timed) DYG_HIP(s->ctx, hipEventRecord(s->wev1, st));
+    DYG_HIP(s->ctx, launch_walk_incremental(a, st, timed ? s->wev0 : nullptr,
+                                            timed ? s->wev1 : nullptr));
     r = device_edits_batch(H, s->cu, s->cv, s->cw, s->cfound, n, s->cnt);
     if (r != DYG_OK) return r;
     LAUNCH_1D(k_outcome, n, s->sel, s->cnt, s->cfound, dcls, kOutFound, kOutAdded);
@@ -1040,9 +1037,8 @@ dyg_status apply_delete(dyg_stream s, const uint32_t* du, const uint32_t* dv, ui
     a.w = nullptr;
     a.path = s->path;
     a.stats = dstats;
-    if (timed) DYG_HIP(s->ctx, hipEventRecord(s->wev0, st));
-    DYG_HIP(s->ctx, launch_walk_decremental(a, st));
-    if (timed) DYG_HIP(s->ctx, hipEventRecord(s->wev1, st));
+    DYG_HIP(s->ctx, launch_walk_decremental(a, st, timed ? s->wev0 : nullptr,
+                                            timed ? s->wev1 : nullptr));
     LAUNCH_1D(k_path_counts, n, s->cfound, s->cnt, n, s->pc, s->nopath, s->bsum);
     LAUNCH_1D(k_recovery_lists, n, s->pc, s->nopath, n, s->bsum, s->cu, s->cv, s->ceid, s->poff,
               s->ep, s->ekey, s->cnt, (uint32_t*)(s->pin_dev + kPinSizes));
@@ -1446,14 +1442,27 @@ dyg_status stream_apply_list(dyg_stream s, uint32_t nb, const int* op, const uin
     auto* rw_d = (unsigned long long*)rd;
     auto* rc_h = (int32_t*)((char*)ctx->report.p + words_bytes);
     auto* rc_d = (int32_t*)((char*)rd + words_bytes);
-    ctx->reserve_events(3 * (size_t)nb + 1);
-    if (ctx->evpool.size() < 3 * (size_t)nb + 1) {
+    // events: the list's start, each batch's walk kernel (start, end, taken
+    // by the dispatch itself), the list's end
+    ctx->reserve_events(2 * (size_t)nb + 2);
+    if (ctx->evpool.size() < 2 * (size_t)nb + 2) {
         set_error(ctx, "stream apply many: event creation failed");
         return DYG_ERR_HIP;
     }
-    hipEvent_t* ev = ctx->evpool.data();  // per batch: start, walk start, walk end; then end
+    hipEvent_t* ev = ctx->evpool.data();
     std::vector<dyg_stream_stats> rs(nb);
     const uint32_t ms = s->sp.del.max_steps, hw = s->sp.heur_walkers;
+    // a batch's worst-case arena growth in slots, as the served loop
+    // estimates it: G (and H) 4 * 2n * (slack + 4) + 8n; a deletion batch's
+    // recovery candidates, n (max_steps - 1) + 4n * heuristic walkers, 8 slots
+    // each in H
+    auto growth = [&](uint32_t b, uint64_t* add_g, uint64_t* add_h) {
+        const uint64_t n = off[b + 1] - off[b];
+        const uint64_t add = 4ull * (2ull * n) * (s->sp.slack + 4) + 8ull * n;
+        const uint64_t cand = n * (ms - 1) + 4ull * n * hw;
+        *add_g = add;
+        *add_h = op[b] == DYG_STREAM_DELETE ? add + 8ull * cand : add;
+    };
     uint64_t est_g = s->G->used_slots, est_h = s->H->used_slots;  // arena bump pointers, upper
     int last = -1;  // last batch enqueued
     // brings the host's view of both arenas up to the last enqueued batch
@@ -1467,6 +1476,7 @@ dyg_status stream_apply_list(dyg_stream s, uint32_t nb, const int* op, const uin
     };
     dyg_status fail = DYG_OK;
     uint32_t done_b = 0;
+    bool started = false;  // the list's start event recorded
     for (uint32_t b = 0; b < nb && fail == DYG_OK; ++b) {
         const uint64_t o = off[b];
         const uint32_t n = (uint32_t)(off[b + 1] - o);
@@ -1475,17 +1485,19 @@ dyg_status stream_apply_list(dyg_stream s, uint32_t nb, const int* op, const uin
             done_b = b + 1;
             continue;
         }
-        // the batch's worst-case arena growth (as the served loop estimates it)
         const bool del = op[b] == DYG_STREAM_DELETE;
-        const uint64_t add = 4ull * (2ull * n) * (s->sp.slack + 4) + 8ull * n;
+        uint64_t add = 0, add_h = 0;
+        growth(b, &add, &add_h);
         const uint64_t cand = (uint64_t)n * (ms - 1) + 4ull * n * hw;
-        const uint64_t add_h = del ? add + 8ull * cand : add;
         if (est_g + add > s->G->cap_slots || est_h + add_h > s->H->cap_slots) {
+            // the estimate ran out: wait for the arenas' true use, and grow
+            // an arena only when that still leaves too little room
             fail = wait_spin(ctx, str, s->ev_sync);
             if (fail == DYG_OK) fail = absorb_last();
-            if (fail == DYG_OK)
+            if (fail == DYG_OK && s->G->used_slots + add > s->G->cap_slots)
                 fail = device_edits_prepare(s->G, s->sp.max_batch, 2ull * n + s->sp.max_batch);
-            if (fail == DYG_OK) fail = device_edits_prepare(s->H, s->cmax, 2ull * n + cand);
+            if (fail == DYG_OK && s->H->used_slots + add_h > s->H->cap_slots)
+                fail = device_edits_prepare(s->H, s->cmax, 2ull * n + cand);
             if (fail != DYG_OK) break;
             est_g = s->G->used_slots;
             est_h = s->H->used_slots;
@@ -1499,9 +1511,12 @@ dyg_status stream_apply_list(dyg_stream s, uint32_t nb, const int* op, const uin
         est_h += add_h;
         const Report rep{rc_d + (o - o0), rw_d + (size_t)b * kListWords, s->G->dev.meta,
                          s->H->dev.meta};
-        s->wev0 = ev[3 * b + 1];
-        s->wev1 = ev[3 * b + 2];
-        fail = hipEventRecord(ev[3 * b], str) == hipSuccess ? DYG_OK : DYG_ERR_HIP;
+        s->wev0 = ev[1 + 2 * b];
+        s->wev1 = ev[2 + 2 * b];
+        if (!started) {
+            fail = hipEventRecord(ev[0], str) == hipSuccess ? DYG_OK : DYG_ERR_HIP;
+            started = true;
+        }
         if (fail == DYG_OK)
             fail = del ? apply_delete(s, u + o, v + o, n, s->cls, s->stats, true, &rs[b], rep,
                                       nullptr)
@@ -1515,10 +1530,11 @@ dyg_status stream_apply_list(dyg_stream s, uint32_t nb, const int* op, const uin
     s->wev0 = ctx->ev0;
     s->wev1 = ctx->ev1;
     // the list's one wait, then the reports of every batch enqueued
-    DYG_HIP(ctx, hipEventRecord(ev[3 * (size_t)nb], str));
+    DYG_HIP(ctx, hipEventRecord(ev[2 * (size_t)nb + 1], str));
     r = wait_spin(ctx, str, s->ev_sync);
     if (r == DYG_OK) r = absorb_last();
     if (fail == DYG_OK) fail = r;
+    hipEvent_t prev_end = nullptr;  // the last batch's walk end
     for (uint32_t b = 0; b < done_b; ++b) {
         const uint64_t o = off[b] - o0;
         const uint32_t n = (uint32_t)rs[b].updates;
@@ -1547,12 +1563,16 @@ dyg_status stream_apply_list(dyg_stream s, uint32_t nb, const int* op, const uin
         x.redirects = wd[1];
         if (wd[3]) x.candidates = wd[3];
         float t = 0.f;
-        if (hipEventElapsedTime(&t, ev[3 * b + 1], ev[3 * b + 2]) == hipSuccess) x.walk_ms = t;
-        // the batch's span on the device timeline: its start to the next one's
+        if (hipEventElapsedTime(&t, ev[1 + 2 * b], ev[2 + 2 * b]) == hipSuccess) x.walk_ms = t;
+        // the batch's span on the device timeline: from the previous batch's
+        // walk end (the list's start for the first) to its own walk end (the
+        // list's end for the last); the spans add up to the list's
         uint32_t nx = b + 1;
         while (nx < done_b && rs[nx].updates == 0) ++nx;
-        const hipEvent_t end = nx < done_b ? ev[3 * nx] : ev[3 * (size_t)nb];
-        if (hipEventElapsedTime(&t, ev[3 * b], end) == hipSuccess) x.total_ms = t;
+        const hipEvent_t begin = prev_end ? prev_end : ev[0];
+        const hipEvent_t end = nx < done_b ? ev[2 + 2 * b] : ev[2 * (size_t)nb + 1];
+        if (hipEventElapsedTime(&t, begin, end) == hipSuccess) x.total_ms = t;
+        prev_end = ev[2 + 2 * b];
         if (st) st[b] = x;
     }
     if (fail != DYG_OK) return fail;

@@ -20,6 +20,8 @@
 // Draws come from Philox4x32-10 keyed by (seed, tag, edge id, walker): each
 // 10-round block yields the next four draws of a walker, so results do not
 // depend on how walkers are mapped to lanes, workgroups or launches.
+#include <hip/hip_ext.h>
+
 #include <atomic>
 
 #include "internal.h"
@@ -246,11 +248,11 @@ __global__ __launch_bounds__((1 << LOGW) < 64 ? 64 : (1 << LOGW)) void k_walk(Wa
 // Work-queue form.  A workgroup of 256 lanes owns `E` consecutive edges and
 // their E*W (edge, walker) tasks, taken from an LDS counter; a lane whose
 // walker ends takes the next task at once, so a wave is never held by its
-// longest walker.  Each wave keeps a ring of up to 127 claimed tasks with
-// their first draw block, filled 64 at a time by the whole wave (one LDS
-// atomic and one Philox block per lane), so refills cost no divergent Philox
-// and the workgroup's pool can be large: many tasks per lane keep the tail
-// after the pool drains short.  Per-edge winner keys and hop counts live in LDS (64-bit LDS max);
+// longest walker.  Each wave keeps a ring of up to 64 claimed tasks with
+// their first draw block; a claim fills the ring up, by the whole wave (one
+// LDS atomic and one Philox block per lane), so refills cost no divergent
+// Philox and the workgroup's pool can be large: many tasks per lane keep the
+// tail after the pool drains short.  Per-edge winner keys and hop counts live in LDS (64-bit LDS max);
 // pruning reads the edge's key.  Results equal the one-workgroup-per-edge
 // kernel's bit for bit: a walker's trajectory depends only on (edge, walker)
 // and the graph.
@@ -266,13 +268,14 @@ __global__ __launch_bounds__((1 << LOGW) < 64 ? 64 : (1 << LOGW)) void k_walk(Wa
 //    loads unused, and a leaf ends the walk on arrival;
 //  * first and later hops share the draw, the load phase and the tests;
 //  * each edge's source row is staged in LDS (a refill reads no global memory);
-//  * draws come from an 8-entry ring of two Philox blocks.  Block 0 of a
-//    task comes with it from the wave's task ring; later blocks
-//    are computed every fourth turn by all active lanes at once, into the
-//    half of the ring already consumed (a walker draws exactly once per
-//    turn, so the half not holding the next draw is always spent then).  The
-//    Philox rounds thus run once per four turns for the whole wave, instead
-//    of whenever any one lane runs dry.
+//  * draws come from an 8-entry ring of two Philox blocks (a column of LDS
+//    in the pipelined loop, registers otherwise).  Block 0 of a task comes
+//    with it from the wave's task ring; later blocks are computed every
+//    fourth turn by all active lanes at once, into the half of the ring
+//    already consumed (a walker draws exactly once per turn, so the half not
+//    holding the next draw is always spent then).  The Philox rounds thus
+//    run once per four turns for the whole wave, instead of whenever any one
+//    lane runs dry.
 // ---------------------------------------------------------------------------
 // The slot a lane reads when its walk has no next hop (the loop issues its
 // loads unconditionally); never written.
@@ -749,14 +752,13 @@ __global__ __launch_bounds__(256) void k_apply_delta(uint2* __restrict__ rows,
 }
 
 template <bool INSERT, bool PRUNE>
-hipError_t launch_walk_t(const WalkLaunch& a, hipStream_t s) {
-    if (a.n == 0) return hipSuccess;
+hipError_t launch_walk_t(const WalkLaunch& a, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     const dim3 grid(a.n);
     switch (a.walkers) {
 #define DYG_CASE(LW)                                                                       \
     case (1u << LW):                                                                       \
-        hipLaunchKernelGGL((k_walk<LW, INSERT, PRUNE>), grid,                              \
-                           dim3((1 << LW) < 64 ? 64 : (1 << LW)), 0, s, a);                \
+        hipExtLaunchKernelGGL((k_walk<LW, INSERT, PRUNE>), grid,                           \
+                              dim3((1 << LW) < 64 ? 64 : (1 << LW)), 0, s, e0, e1, 0, a);  \
         break;
         DYG_CASE(0) DYG_CASE(1) DYG_CASE(2) DYG_CASE(3) DYG_CASE(4) DYG_CASE(5)
         DYG_CASE(6) DYG_CASE(7) DYG_CASE(8) DYG_CASE(9) DYG_CASE(10)
@@ -805,8 +807,7 @@ inline uint32_t pool_edges(uint32_t W, uint32_t requested, uint32_t n, uint32_t 
 }
 
 template <bool INSERT, bool PRUNE, bool DESC>
-hipError_t launch_pool_t(const WalkLaunch& a, hipStream_t s) {
-    if (a.n == 0) return hipSuccess;
+hipError_t launch_pool_t(const WalkLaunch& a, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     uint32_t slots = 2048;
     switch (a.walkers) {
 #define DYG_CASE(LW) \
@@ -822,8 +823,8 @@ hipError_t launch_pool_t(const WalkLaunch& a, hipStream_t s) {
     switch (a.walkers) {
 #define DYG_CASE(LW)                                                                        \
     case (1u << LW):                                                                        \
-        hipLaunchKernelGGL((k_walk_pool<LW, INSERT, PRUNE, DESC>), grid, dim3(kPoolThreads), 0, s, \
-                           a, E);                                                           \
+        hipExtLaunchKernelGGL((k_walk_pool<LW, INSERT, PRUNE, DESC>), grid, dim3(kPoolThreads), \
+                              0, s, e0, e1, 0, a, E);                                       \
         break;
         DYG_CASE(0) DYG_CASE(1) DYG_CASE(2) DYG_CASE(3) DYG_CASE(4) DYG_CASE(5)
         DYG_CASE(6) DYG_CASE(7) DYG_CASE(8) DYG_CASE(9) DYG_CASE(10)
@@ -835,32 +836,43 @@ hipError_t launch_pool_t(const WalkLaunch& a, hipStream_t s) {
 }
 
 template <bool INSERT>
-hipError_t launch_any(const WalkLaunch& a, hipStream_t s) {
+hipError_t launch_any(const WalkLaunch& a, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     const bool prune = !(a.flags & DYG_WALK_NO_PRUNE);
     if constexpr (INSERT) {
         if (a.nodesc) return hipErrorNotSupported;  // insertion walks run on sparsifiers
-    } else {
+    }
+    if (!valid_walkers(a.walkers)) return hipErrorInvalidValue;
+    if (a.n == 0) {  // nothing to launch: the events still mark the place
+        hipError_t e = e0 ? hipEventRecord(e0, s) : hipSuccess;
+        if (e == hipSuccess && e1) e = hipEventRecord(e1, s);
+        return e;
+    }
+    if constexpr (!INSERT) {
         // a graph without descriptors (the stream's dense graph) is walked by
         // the work-queue kernel reading rows[] per hop; same results
         if (a.nodesc)
-            return prune ? launch_pool_t<false, true, false>(a, s)
-                         : launch_pool_t<false, false, false>(a, s);
+            return prune ? launch_pool_t<false, true, false>(a, s, e0, e1)
+                         : launch_pool_t<false, false, false>(a, s, e0, e1);
     }
     if (a.flags & DYG_WALK_PER_EDGE)
-        return prune ? launch_walk_t<INSERT, true>(a, s) : launch_walk_t<INSERT, false>(a, s);
-    return prune ? launch_pool_t<INSERT, true, true>(a, s) : launch_pool_t<INSERT, false, true>(a, s);
+        return prune ? launch_walk_t<INSERT, true>(a, s, e0, e1)
+                     : launch_walk_t<INSERT, false>(a, s, e0, e1);
+    return prune ? launch_pool_t<INSERT, true, true>(a, s, e0, e1)
+                 : launch_pool_t<INSERT, false, true>(a, s, e0, e1);
 }
 
 }  // namespace
 
 bool valid_walkers(uint32_t w) { return w >= 1 && w <= 1024 && (w & (w - 1)) == 0; }
 
-hipError_t launch_walk_incremental(const WalkLaunch& a, hipStream_t s) {
-    return launch_any<true>(a, s);
+hipError_t launch_walk_incremental(const WalkLaunch& a, hipStream_t s, hipEvent_t ev_begin,
+                                   hipEvent_t ev_end) {
+    return launch_any<true>(a, s, ev_begin, ev_end);
 }
 
-hipError_t launch_walk_decremental(const WalkLaunch& a, hipStream_t s) {
-    return launch_any<false>(a, s);
+hipError_t launch_walk_decremental(const WalkLaunch& a, hipStream_t s, hipEvent_t ev_begin,
+                                   hipEvent_t ev_end) {
+    return launch_any<false>(a, s, ev_begin, ev_end);
 }
 
 hipError_t launch_heuristic(const uint2* rows, const uint4* adj, const uint32_t* endpoint,
