@@ -53,6 +53,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 METRIC = "edge updates/s + p50/p99 request latency, served vs plain, at equal kappa"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
+OUT_FOUND, OUT_ADDED, OUT_RECOVERED, OUT_HEURISTIC = 2, 3, 6, 7  # DYG_OUT_* (dygrass.h)
 
 
 def log(msg: str) -> None:
@@ -166,8 +167,14 @@ def leg_incremental(args, ctx, grid: bool = False) -> dict:
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": f"k_walk_pool<W={wl.walkers}, insert, prune>",
-                     "workload": wl.name, "bytes_per_launch": bytes_run / launches,
-                     "avg_launch_ms": head["walk_ms"] / launches},
+                     "workload": wl.name, "launches": launches,
+                     "bytes_per_launch": bytes_run / launches,
+                     "avg_launch_ms": head["walk_ms"] / launches,
+                     "edges_per_launch": m / launches,
+                     "hops_per_launch": stats["hops"] / launches,
+                     "redirects_per_launch": stats["redirects"] / launches,
+                     "bytes_basis": "16 B per neighbour draw + 8 B per redirect + 20 B per "
+                                    "edge"},
     }
     if not args.no_plain:
         plain = run_mode(True, max(1, min(args.steps, 3)), 1)
@@ -555,13 +562,14 @@ def leg_served(args, ctx) -> dict:
         st = Stream(ctx, wl.G.V, g_edges, h_edges, sp(False))
         srv = Server(st, slots=args.slots, max_request=maxr)
         tickets, lat, done_at = [], np.zeros(n), np.zeros(n)
+        got = [None] * n
 
         def collect():
             for i in range(n):
                 while len(tickets) <= i:
                     time.sleep(0.0002)
-                _, res = srv.wait(tickets[i], int(sizes[i]))
-                lat[i] = res["latency_ms"]
+                got[i] = srv.wait(tickets[i], int(sizes[i]))
+                lat[i] = got[i][1]["latency_ms"]
                 done_at[i] = time.perf_counter()
 
         th = threading.Thread(target=collect)
@@ -576,9 +584,14 @@ def leg_served(args, ctx) -> dict:
         span = done_at[-1] - start
         srv.close()
         st.close()
-        return span, lat
+        walks = np.zeros((n, 4))  # per request: walked edges, hops, redirects, walk ms
+        for i, (codes, res) in enumerate(got):
+            walked = np.isin(codes, [OUT_FOUND, OUT_ADDED] if wl.requests[i][0] == "ins"
+                             else [OUT_RECOVERED, OUT_HEURISTIC])
+            walks[i] = (walked.sum(), res["hops"], res["redirects"], res["walk_ms"])
+        return span, lat, walks
 
-    span, lat = serve(wl.arrivals)
+    span, lat, walks = serve(wl.arrivals)
     small = sizes <= 1000
     out = {
         "value": m / span,
@@ -595,7 +608,27 @@ def leg_served(args, ctx) -> dict:
     }
     log(f"served at {args.rate:g}/s: {out['value']:.0f} updates/s, p50 {pct(lat, 50):.2f} ms, "
         f"p99 {pct(lat, 99):.2f} ms (<=1k edges: p99 {pct(lat[small], 99):.2f} ms)")
-    cspan, _ = serve(np.zeros(n))  # every request waiting at t=0: capacity
+    # the deletion walks in the dense graph (the leg's dominant kernel): one
+    # launch per deletion request that walks, timed by events taken with its
+    # dispatch, bytes from its own hop and redirect counters
+    dl = np.array([r[0] == "del" for r in wl.requests]) & (walks[:, 0] > 0)
+    if dl.any():
+        e, h, r_, ms = walks[dl].sum(axis=0)
+        nb = algorithmic_bytes(h, r_, e, per_edge=16.0)
+        achieved = nb / (ms * 1e-3) / 1e9
+        kernel = f"k_walk_pool<{int(np.log2(setting.w_del))},false,true,true>"
+        out["roofline_delete_walks"] = {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kernel, wl.name),
+            "kernel": f"k_walk_pool<W={setting.w_del}, delete, prune>", "workload": wl.name,
+            "launches": int(dl.sum()), "bytes_per_launch": nb / dl.sum(),
+            "avg_launch_ms": ms / dl.sum(), "edges_per_launch": e / dl.sum(),
+            "hops_per_launch": h / dl.sum(), "redirects_per_launch": r_ / dl.sum(),
+            "bytes_basis": "16 B per neighbour draw + 8 B per redirect + 16 B per edge "
+                           "(winner path not counted)"}
+        log(f"served deletion walks: {achieved:.0f} GB/s algorithmic over {int(dl.sum())} "
+            f"launches, {ms / dl.sum() * 1e3:.1f} us per launch")
+    cspan, _, _ = serve(np.zeros(n))  # every request waiting at t=0: capacity
     out["capacity"] = {"value": m / cspan, "span_s": cspan,
                        "requests_per_s": n / cspan,
                        "note": "same trace, all requests arriving at once"}
@@ -603,7 +636,7 @@ def leg_served(args, ctx) -> dict:
     curve = []
     for rate in [float(x) for x in args.rates.split(",") if x.strip()]:
         # the same trace with its arrival times compressed to `rate`
-        rspan, rlat = serve(wl.arrivals * (args.rate / rate))
+        rspan, rlat, _ = serve(wl.arrivals * (args.rate / rate))
         curve.append({"rate": rate, "updates_per_s": m / rspan,
                       "p50_ms": pct(rlat, 50), "p99_ms": pct(rlat, 99),
                       "p50_ms_le_1k": pct(rlat[small], 50), "p99_ms_le_1k": pct(rlat[small], 99)})

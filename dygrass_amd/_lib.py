@@ -67,7 +67,7 @@ class StreamStats(C.Structure):
 
 class ServeResult(C.Structure):
     _fields_ = [("n", C.c_uint32), ("queue_ms", C.c_double), ("latency_ms", C.c_double),
-                ("hops", C.c_uint64)]
+                ("hops", C.c_uint64), ("redirects", C.c_uint64), ("walk_ms", C.c_double)]
 
 
 class MtxInfo(C.Structure):

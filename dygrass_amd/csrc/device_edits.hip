@@ -37,8 +37,8 @@ namespace {
 // row when found[i] == match (-1: no path, append), else a pad key that sorts
 // after every row.  The
 // values carry the entry index, so a stable sort keeps each row's appends in
-// batch order.  Also counts the batch's appends (wave-aggregated atomics)
-// and clears the other parity's counter for the next batch.
+// batch order.  Also counts the batch's appends and clears the other
+// parity's counter for the next batch.
 __global__ __launch_bounds__(256) void k_keys(
     const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
     const int32_t* __restrict__ found, int32_t match, uint32_t n,
@@ -55,9 +55,10 @@ __global__ __launch_bounds__(256) void k_keys(
         vals[2 * i] = 2 * i;
         vals[2 * i + 1] = 2 * i + 1;
     }
-    const unsigned long long b = __ballot(add);
-    if (b && __lane_id() == (uint32_t)(__ffsll((long long)b) - 1)) {
-        const uint32_t c = (uint32_t)__popcll(b);
+    // one pair of atomics per workgroup: per-wave atomics on the same two
+    // words serialise at the cache and cost more than the rest of the kernel
+    const uint32_t c = (uint32_t)__syncthreads_count(add);
+    if (threadIdx.x == 0 && c) {
         atomicAdd(count, 2u * c);
         atomicAdd(added_total, (unsigned long long)c);
     }

@@ -1654,6 +1654,7 @@ struct ServeSlot {
     void* host = nullptr;     // pinned, laid out by pin_layout; device address hdev
     char* hdev = nullptr;
     hipEvent_t e_in = nullptr, e_done = nullptr;
+    hipEvent_t ew0 = nullptr, ew1 = nullptr;  // the request's walk kernel, start and end
 };
 
 struct ServeReq {
@@ -1665,7 +1666,8 @@ struct ServeReq {
     dyg_status status = DYG_OK;
     std::string error;
     std::vector<int32_t> outcome;
-    uint64_t hops = 0;
+    uint64_t hops = 0, redirects = 0;
+    double walk_ms = 0;
     bool copied = false;  // input copied to the slot's device buffer (else read in place)
     // device inputs (dyg_serve_submit_dev): the caller's arrays, read in place
     const uint32_t* du = nullptr;
@@ -1788,10 +1790,16 @@ void serve_worker(dyg_serve_s* v) {
             dw = r->dw;
         }
         const Report rep{dp.cls, dp.words, s->G->dev.meta, s->H->dev.meta};
+        // the walk kernel takes the slot's events with its dispatch (read by
+        // the completion thread, which frees the slot only after that)
+        s->wev0 = sl.ew0;
+        s->wev1 = sl.ew1;
         if (st == DYG_OK)
             st = r->op == DYG_STREAM_INSERT
-                     ? apply_insert(s, du, dv, dw, n, sl.dcls, sl.dstats, false, &rs, rep, nullptr)
-                     : apply_delete(s, du, dv, n, sl.dcls, sl.dstats, false, &rs, rep, nullptr);
+                     ? apply_insert(s, du, dv, dw, n, sl.dcls, sl.dstats, true, &rs, rep, nullptr)
+                     : apply_delete(s, du, dv, n, sl.dcls, sl.dstats, true, &rs, rep, nullptr);
+        s->wev0 = s->ctx->ev0;
+        s->wev1 = s->ctx->ev1;
         if (st == DYG_OK) {
             s->base += n;
             if (hipEventRecord(sl.e_done, cs) != hipSuccess) st = DYG_ERR_HIP;
@@ -1834,6 +1842,9 @@ void serve_completer(dyg_serve_s* v) {
                 const PinLayout hp = pin_layout((char*)sl.host, r->n);
                 r->outcome.assign(hp.cls, hp.cls + r->n);
                 r->hops = hp.words[0];
+                r->redirects = hp.words[1];
+                float t = 0.f;
+                if (hipEventElapsedTime(&t, sl.ew0, sl.ew1) == hipSuccess) r->walk_ms = t;
             }
         }
         r->t_done = now_ms();
@@ -1874,7 +1885,8 @@ extern "C" dyg_status dyg_serve_open(dyg_stream s, uint32_t slots, uint32_t max_
              hipHostMalloc(&sl.host, pin_bytes(max_request), hipHostMallocDefault) == hipSuccess &&
              hipHostGetDevicePointer(&hd, sl.host, 0) == hipSuccess &&
              hipEventCreateWithFlags(&sl.e_in, hipEventDisableTiming) == hipSuccess &&
-             hipEventCreateWithFlags(&sl.e_done, hipEventDisableTiming) == hipSuccess;
+             hipEventCreateWithFlags(&sl.e_done, hipEventDisableTiming) == hipSuccess &&
+             hipEventCreate(&sl.ew0) == hipSuccess && hipEventCreate(&sl.ew1) == hipSuccess;
         sl.hdev = (char*)hd;
     }
     if (ok) ok = hipStreamSynchronize(ctx->stream) == hipSuccess;
@@ -2064,6 +2076,8 @@ extern "C" dyg_status dyg_serve_wait(dyg_serve v, uint64_t ticket, int32_t* outc
         res->queue_ms = r->t_start - r->t_submit;
         res->latency_ms = r->t_done - r->t_submit;
         res->hops = r->hops;
+        res->redirects = r->redirects;
+        res->walk_ms = r->walk_ms;
     }
     delete r;
     return st;
@@ -2093,6 +2107,8 @@ extern "C" void dyg_serve_close(dyg_serve v) {
         if (sl.host) (void)hipHostFree(sl.host);
         if (sl.e_in) (void)hipEventDestroy(sl.e_in);
         if (sl.e_done) (void)hipEventDestroy(sl.e_done);
+        if (sl.ew0) (void)hipEventDestroy(sl.ew0);
+        if (sl.ew1) (void)hipEventDestroy(sl.ew1);
     }
     for (auto& kv : v->reqs) delete kv.second;
     for (ServeReq* r : v->queue) delete r;

@@ -302,7 +302,7 @@ struct Lane {
 // instead of a chain of register selects.
 struct LdsRing {
     uint32_t* col;  // the lane's column; rows kPoolThreads words apart
-    uint32_t q;     // index of the next draw (mod 8)
+    uint32_t used;  // draws taken so far (the next one sits at row used & 7)
     uint32_t blk;   // next block to compute
     uint32_t c2, c3;
 
@@ -311,7 +311,7 @@ struct LdsRing {
         col[kPoolThreads] = b0.y;
         col[2 * kPoolThreads] = b0.z;
         col[3 * kPoolThreads] = b0.w;
-        q = 0;
+        used = 0;
         blk = 1;
         c2 = (uint32_t)id;
         c3 = stream_c3(tag, id);
@@ -326,10 +326,12 @@ struct LdsRing {
         ++blk;
     }
     __device__ __forceinline__ uint32_t next() {
-        const uint32_t v = col[q * kPoolThreads];
-        q = (q + 1u) & 7u;
+        const uint32_t v = col[(used & 7u) * kPoolThreads];
+        ++used;
         return v;
     }
+    // the half the next block goes to has been read (at most 4 draws left)
+    __device__ __forceinline__ bool room() const { return 4u * blk - used <= 4u; }
 };
 
 // Draw ring: r[d & 7] holds draw d of the walker; block b sits in half b & 1.
@@ -427,6 +429,15 @@ __global__ __launch_bounds__(kPoolThreads) void k_walk_pool(WalkLaunch a, uint32
         // a slot arrives (software-pipelined loop)
         LdsRing drw{};
         drw.col = &s_draw[0][tid];
+        // Deletion walks (in the dense graph, too large to stay in cache)
+        // load the row's last slot only when the drawn slot turns out to be
+        // the previous vertex: that hop then completes one turn later, with
+        // the draw kept, while the wave's other lanes go on.  Insertion walks
+        // (in the sparsifier, cache-resident) load both slots at once.
+        constexpr bool kLazy = !INSERT;
+        bool pend = false;   // the slot in flight is the redirect of the current hop
+        uint32_t r_keep = 0; // the next hop's draw, kept while the redirect is in flight
+        uint2 crow{};        // the row the current hop drew from
         for (uint32_t turn = 0;; ++turn) {
             // ---- refill idle lanes from the wave's task ring ----
             const unsigned long long need = __ballot(!active && !drained);
@@ -471,6 +482,7 @@ __global__ __launch_bounds__(kPoolThreads) void k_walk_pool(WalkLaunch a, uint32
                         L.R = 0.0f;
                         L.step = 0;  // fresh: its first slot load is issued this turn
                         drw.start(rblk[q], s_eid[L.edge], kTag);
+                        pend = false;
                         my_hops = 0;
                         active = true;
                     } else {
@@ -482,7 +494,10 @@ __global__ __launch_bounds__(kPoolThreads) void k_walk_pool(WalkLaunch a, uint32
             if (!__any(active)) break;
             // ---- every fourth turn: the next draw block of every active lane
             //      (while the lanes' slot loads are in flight) ----
-            if ((turn & 3u) == 0u && active) drw.append(L.walker, k0, k1);
+            // (a lane that kept draws back for redirects takes a block only
+            // once it has read the half the block goes to)
+            if ((turn & 3u) == 0u && active && (!kLazy || drw.room()))
+                drw.append(L.walker, k0, k1);
             if (!active) continue;
 
             // ---- one hop: take the slot loaded for it in the last turn, issue the
@@ -492,16 +507,23 @@ __global__ __launch_bounds__(kPoolThreads) void k_walk_pool(WalkLaunch a, uint32
             int found = -1;
             const bool fresh = (L.step == 0);  // task started this turn: nothing loaded yet
             const bool first = (L.step == 1);  // the loaded slot is the first hop's
-            bool back = false;
+            bool back = false, defer = false;
             uint4 sl = make_uint4(0u, 0u, 0u, 0u);
             uint2 nrow = L.row;  // the row the next hop draws from
-            const uint32_t r = drw.next();  // read before the wait for the slots
+            // read before the wait for the slots
+            const uint32_t r = (kLazy && pend) ? r_keep : drw.next();
             if (!fresh) {
                 // non-backtracking: a draw of the previous vertex is redirected to
                 // the row's last slot (the first hop draws over the whole row)
-                back = !first && (x.x == L.prev);
-                sl = back ? y : x;
-                nrow = DESC ? make_uint2(sl.z, sl.w) : ld(a.rows, sl.x);
+                if (kLazy && pend) {
+                    sl = x;  // the last slot, loaded in the last turn
+                    back = true;
+                } else {
+                    back = !first && (x.x == L.prev);
+                    sl = (back && !kLazy) ? y : x;
+                    defer = kLazy && back;
+                }
+                nrow = make_uint2(sl.z, sl.w);
             }
             {
                 // one load site for every active lane, with no branch: a lane
@@ -511,8 +533,21 @@ __global__ __launch_bounds__(kPoolThreads) void k_walk_pool(WalkLaunch a, uint32
                 const uint32_t span = go ? (fresh ? nrow.y : nrow.y - 1u) : 1u;
                 const uint32_t i = r % span;
                 const uint4* const base = go ? a.adj : g_no_slot;
-                x = base[go ? nrow.x + i : 0u];
-                y = base[go ? nrow.x + nrow.y - 1u : 0u];
+                if constexpr (kLazy) {
+                    // the next hop's drawn slot, or the redirect of this hop
+                    x = defer ? a.adj[crow.x + crow.y - 1u] : base[go ? nrow.x + i : 0u];
+                    if (!defer) crow = nrow;
+                } else {
+                    x = base[go ? nrow.x + i : 0u];
+                    y = base[go ? nrow.x + nrow.y - 1u : 0u];
+                }
+            }
+            if constexpr (kLazy) {
+                pend = defer;
+                if (defer) {  // this hop completes next turn, with this turn's draw
+                    r_keep = r;
+                    continue;
+                }
             }
             if (fresh) {
                 if (nrow.y == 0u) done = true;  // isolated source: nothing to find

@@ -394,6 +394,8 @@ typedef struct {
     double queue_ms;     /* submit -> launch */
     double latency_ms;   /* submit -> results on the host */
     uint64_t hops;       /* neighbour draws of its walks */
+    uint64_t redirects;  /* non-backtracking redirects of its walks */
+    double walk_ms;      /* its walk kernel's own duration on the device */
 } dyg_serve_result;
 dyg_status dyg_serve_open(dyg_stream s, uint32_t slots, uint32_t max_request,
                           uint64_t budget_bytes, dyg_serve* out);

@@ -1,6 +1,7 @@
 """Full-size checks on the GPU: the 1M-node random geometric graph with 200k
 insertions in 10 snapshot batches (threshold 100, 256 walkers), the 2M-node
-mesh with 100k deletions, and the 4M-point 12-NN batched stream.
+mesh with 100k deletions, the 4M-point 12-NN batched stream (against the C++
+stream oracle and the plain form) and the 8M-vertex served workload.
 
 Every one of the 200k decisions must equal the oracle's, and the final
 sparsifier must equal the oracle's slot for slot.  Size-independent
@@ -156,6 +157,57 @@ def test_full_size_batched_stream_properties(ctx):
     assert np.all(np.isin(ik[io == S.ADDED], hk))
     assert not np.any(np.isin(dk, hk))
     dev.close()
+
+
+@pytest.mark.timeout(600)
+def test_full_size_batched_stream_matches_oracle(ctx):
+    """The batched-stream workload at full size as the benchmark runs it
+    (every batch in device memory, the list applied in one call with the
+    committed autotuned setting) against the C++ stream oracle on the host's
+    cores: every batch's outcome codes equal, and the final G and H equal
+    slot for slot (neighbour order within every row, weight bits)."""
+    from dygrass_amd import DeviceArray, Stream, autotune, csr_from_edges, stream_params, synth
+    from oracle import pyoracle as po
+
+    wl = synth.config_batched(1.0, 0)
+    V = wl.G.V
+    t = autotune.load(wl.name)
+    s = autotune.setting_of(t) if t else autotune.Setting(w_ins=256, w_del=512)
+    csr = (csr_from_edges(V, wl.G.u, wl.G.v, wl.G.w), csr_from_edges(V, wl.H.u, wl.H.v, wl.H.w))
+    maxb = max(len(b[1]) for b in wl.batches)
+    sp = stream_params(w_ins=s.w_ins, w_del=s.w_del, threshold=wl.threshold, seed=0,
+                       max_batch=maxb, max_inserts=wl.updates, steps_ins=s.steps_ins,
+                       steps_del=s.steps_del, block_edges_ins=s.block_ins,
+                       block_edges_del=s.block_del, dense_no_desc=s.dense_no_desc)
+    ops = [b[0] for b in wl.batches]
+    off = np.concatenate([[0], np.cumsum([len(b[1]) for b in wl.batches])]).astype(np.uint64)
+    du = DeviceArray.from_numpy(ctx, np.concatenate([np.asarray(b[1], np.uint32)
+                                                     for b in wl.batches]))
+    dv = DeviceArray.from_numpy(ctx, np.concatenate([np.asarray(b[2], np.uint32)
+                                                     for b in wl.batches]))
+    dw = DeviceArray.from_numpy(ctx, np.concatenate(
+        [np.asarray(b[3], np.float32) if b[3] is not None else np.zeros(len(b[1]), np.float32)
+         for b in wl.batches]))
+    st = Stream(ctx, V, None, None, sp, csr=csr)
+    got, _ = st.apply_many_dev(ops, off, du, dv, dw)
+
+    threads = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        threads = min(threads, int(env))
+    orc = po.StreamCpp(V, thr=wl.threshold, w_ins=s.w_ins, w_del=s.w_del, seed=0,
+                       steps_ins=s.steps_ins, steps_del=s.steps_del, threads=threads, csr=csr)
+    for b, (op, u, v, w) in enumerate(wl.batches):
+        want = orc.apply(op, u, v, w)
+        assert np.array_equal(got[int(off[b]):int(off[b + 1])], want), f"batch {b} ({op})"
+    for which, g in ((0, st.G), (1, st.H)):
+        rp, col, w = g.download()
+        orp, ocol, ow = orc.csr(which)
+        assert np.array_equal(rp, orp) and np.array_equal(col, ocol)
+        assert np.array_equal(f32bits(w), f32bits(ow))
+    assert st.H.check() == 0
+    orc.close()
+    st.close()
 
 
 @pytest.mark.timeout(900)

@@ -81,6 +81,12 @@ def test_serve_alone_equals_in_mix(ctx, grid_wl):
         assert np.array_equal(a, b), i
         assert results[i]["n"] == len(reqs[i][1])
         assert results[i]["latency_ms"] >= results[i]["queue_ms"] >= 0.0
+        # walk counters and the walk kernel's own duration come with the result
+        walked = np.isin(b, [S.FOUND, S.ADDED, S.RECOVERED, S.HEURISTIC]).sum()
+        assert results[i]["redirects"] <= results[i]["hops"]
+        assert (results[i]["hops"] > 0) == (walked > 0)
+        assert 0.0 <= results[i]["walk_ms"] < results[i]["latency_ms"]
+        assert results[i]["walk_ms"] > 0.0 or walked == 0
     _same(alone.G, mixed.G)
     _same(alone.H, mixed.H)
     assert alone.position == mixed.position
