@@ -64,6 +64,68 @@ __global__ __launch_bounds__(256) void k_keys(
     }
 }
 
+__device__ __forceinline__ bool takes(const EditTake& t, uint32_t i) {
+    return t.keep ? (t.keep[i] && t.found[t.jmap[i]] == -1) : t.flag[i] == t.match;
+}
+
+// Presorted edits, first pass: per block of 256 sorted entries, how many
+// this edit takes (tcnt[block]); the batch's entry count and its edge count
+// (one taken entry of even index per edge) are added up, and the other
+// parity's entry counter is cleared for the next batch, as k_keys does.
+__global__ __launch_bounds__(256) void k_take_count(
+    const uint32_t* __restrict__ ent, uint32_t n2, EditTake tk, uint32_t* __restrict__ tcnt,
+    uint32_t* __restrict__ count, uint32_t* __restrict__ count_next,
+    unsigned long long* __restrict__ edges_total) {
+    const uint32_t p = blockIdx.x * 256u + threadIdx.x;
+    if (p == 0) *count_next = 0;
+    uint32_t x = 0;
+    bool tk_p = false;
+    if (p < n2) {
+        x = ent[p];
+        tk_p = takes(tk, x >> 1);
+    }
+    const uint32_t c = (uint32_t)__syncthreads_count(tk_p);
+    const uint32_t ce = (uint32_t)__syncthreads_count(tk_p && !(x & 1u));
+    if (threadIdx.x == 0) {
+        tcnt[blockIdx.x] = c;
+        if (c) atomicAdd(count, c);
+        if (ce) atomicAdd(edges_total, (unsigned long long)ce);
+    }
+}
+
+// Second pass: the taken entries, in sorted order, at their compacted
+// positions (the taken counts of earlier blocks plus a block scan): the keys
+// and values the radix sort of only these entries would give.
+__global__ __launch_bounds__(256) void k_take_compact(
+    const uint32_t* __restrict__ row, const uint32_t* __restrict__ ent, uint32_t n2, EditTake tk,
+    const uint32_t* __restrict__ tcnt, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    using BScan = hipcub::BlockScan<uint32_t, 256>;
+    using BReduce = hipcub::BlockReduce<uint32_t, 256>;
+    __shared__ union {
+        typename BScan::TempStorage scan;
+        typename BReduce::TempStorage reduce;
+    } tmp;
+    __shared__ uint32_t s_prefix;
+    const uint32_t b = blockIdx.x, t = threadIdx.x, p = b * 256u + t;
+    uint32_t part = 0;
+    for (uint32_t q = t; q < b; q += 256u) part += tcnt[q];
+    const uint32_t prefix = BReduce(tmp.reduce).Sum(part);
+    if (t == 0) s_prefix = prefix;
+    __syncthreads();
+    uint32_t x = 0;
+    bool k = false;
+    if (p < n2) {
+        x = ent[p];
+        k = takes(tk, x >> 1);
+    }
+    uint32_t pos = 0, total = 0;
+    BScan(tmp.scan).ExclusiveSum(k ? 1u : 0u, pos, total);
+    if (k) {
+        keys[s_prefix + pos] = row[p];
+        vals[s_prefix + pos] = x;
+    }
+}
+
 __global__ void k_heads(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ count,
                         uint32_t* __restrict__ segstart, uint32_t* __restrict__ segcnt) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -229,7 +291,8 @@ dyg_status device_edits_prepare(dyg_graph g, uint32_t max_batch, uint64_t max_in
                   hipMalloc(&d.keys_out, n2 * 4) == hipSuccess &&
                   hipMalloc(&d.vals_in, n2 * 4) == hipSuccess &&
                   hipMalloc(&d.vals_out, n2 * 4) == hipSuccess &&
-                  hipMalloc(&d.meta, 64) == hipSuccess;
+                  hipMalloc(&d.meta, 64) == hipSuccess &&
+                  hipMalloc(&d.tcnt, (n2 + 255) / 256 * 4) == hipSuccess;
         if (!ok) {
             d.release();
             set_error(ctx, "device edits: allocation failed");
@@ -310,6 +373,33 @@ dyg_status edit_keys(dyg_graph g, const uint32_t* src, const uint32_t* dst, cons
     *count_out = count;
     return DYG_OK;
 }
+
+// edit_keys for a batch whose 2n entries are presorted: the taken ones are
+// compacted (two passes) instead of sorted.
+dyg_status edit_keys_sorted(dyg_graph g, const uint32_t* row, const uint32_t* ent, uint32_t n,
+                            const EditTake& tk, bool insert, uint32_t** count_out) {
+    dyg_ctx ctx = g->ctx;
+    DevEdits& d = g->dev;
+    hipStream_t s = ctx->stream;
+    if (n > d.max_batch) {
+        set_error(ctx, "device edits: batch larger than reserved");
+        return DYG_ERR_ARG;
+    }
+    uint32_t* counts = (uint32_t*)(d.meta + 3);
+    uint32_t* count = counts + (d.seq & 1u);
+    uint32_t* count_next = counts + ((d.seq + 1u) & 1u);
+    ++d.seq;
+    const uint32_t n2 = 2 * n, blocks = (n2 + 255) / 256;
+    hipLaunchKernelGGL(k_take_count, dim3(blocks), dim3(256), 0, s, ent, n2, tk, d.tcnt, count,
+                       count_next, insert ? d.meta + 1 : d.meta + 4);
+    hipLaunchKernelGGL(k_take_compact, dim3(blocks), dim3(256), 0, s, row, ent, n2, tk, d.tcnt,
+                       d.keys_out, d.vals_out);
+    hipLaunchKernelGGL(k_heads, dim3(blocks), dim3(256), 0, s, d.keys_out, count, d.segstart,
+                       d.segcnt);
+    DYG_HIP(ctx, hipGetLastError());
+    *count_out = count;
+    return DYG_OK;
+}
 }  // namespace
 
 dyg_status device_deletes_batch(dyg_graph g, const uint32_t* src, const uint32_t* dst,
@@ -353,6 +443,46 @@ dyg_status device_edits_batch(dyg_graph g, const uint32_t* src, const uint32_t* 
     return DYG_OK;
 }
 
+dyg_status device_edits_sorted(dyg_graph g, const uint32_t* src, const uint32_t* dst,
+                               const float* w, const uint32_t* row, const uint32_t* ent,
+                               uint32_t n, const EditTake& tk) {
+    if (n == 0) return DYG_OK;
+    dyg_ctx ctx = g->ctx;
+    DevEdits& d = g->dev;
+    hipStream_t s = ctx->stream;
+    uint32_t* count = nullptr;
+    dyg_status st = edit_keys_sorted(g, row, ent, n, tk, true, &count);
+    if (st != DYG_OK) return st;
+    const uint32_t blocks = (2 * n + 255) / 256;
+    hipLaunchKernelGGL(k_alloc, dim3(blocks), dim3(256), 0, s, g->d_rows, g->d_adj, d.cap,
+                       d.keys_out, count, d.segstart, d.segcnt, d.rowbase, d.meta,
+                       (unsigned long long)g->cap_slots, g->slack, (uint32_t*)(d.meta + 2));
+    hipLaunchKernelGGL(k_write, dim3(blocks), dim3(256), 0, s, g->d_adj, d.keys_out, d.vals_out,
+                       count, d.segstart, d.segcnt, d.rowbase, src, dst, w);
+    launch_refresh(g, d.keys_out, count, 2 * n, s);
+    DYG_HIP(ctx, hipGetLastError());
+    return DYG_OK;
+}
+
+dyg_status device_deletes_sorted(dyg_graph g, const uint32_t* src, const uint32_t* dst,
+                                 const uint32_t* row, const uint32_t* ent, uint32_t n,
+                                 const EditTake& tk) {
+    if (n == 0) return DYG_OK;
+    dyg_ctx ctx = g->ctx;
+    DevEdits& d = g->dev;
+    hipStream_t s = ctx->stream;
+    uint32_t* count = nullptr;
+    dyg_status st = edit_keys_sorted(g, row, ent, n, tk, false, &count);
+    if (st != DYG_OK) return st;
+    const uint32_t blocks = (2 * n + 255) / 256;
+    hipLaunchKernelGGL(k_del_rows, dim3(blocks), dim3(256), 0, s, g->d_rows, g->d_adj, d.keys_out,
+                       d.vals_out, count, d.segstart, d.segcnt, src, dst,
+                       (uint32_t*)(d.meta + 2));
+    launch_refresh(g, d.keys_out, count, 2 * n, s);
+    DYG_HIP(ctx, hipGetLastError());
+    return DYG_OK;
+}
+
 dyg_status device_edits_absorb(dyg_graph g, const unsigned long long* meta, uint64_t* added_out) {
     const uint32_t err = (uint32_t)meta[2];
     g->used_slots = meta[0];
@@ -380,10 +510,11 @@ dyg_status device_edits_end(dyg_graph g, uint64_t* added_out) {
 }
 
 void DevEdits::release() {
-    void* ps[] = {cap, segstart, segcnt, rowbase, keys_in, keys_out, vals_in, vals_out, meta, temp};
+    void* ps[] = {cap,  segstart, segcnt, rowbase, keys_in, keys_out,
+                  vals_in, vals_out, meta, tcnt, temp};
     for (void* p : ps)
         if (p) (void)hipFree(p);
-    cap = segstart = segcnt = rowbase = keys_in = keys_out = vals_in = vals_out = nullptr;
+    cap = segstart = segcnt = rowbase = keys_in = keys_out = vals_in = vals_out = tcnt = nullptr;
     meta = nullptr;
     temp = nullptr;
     temp_bytes = 0;

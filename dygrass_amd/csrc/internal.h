@@ -75,6 +75,7 @@ struct DevEdits {
     uint64_t base_nslots = 0;  // live slots at device_edits_begin
     uint32_t *keys_in = nullptr, *keys_out = nullptr, *vals_in = nullptr, *vals_out = nullptr;
     unsigned long long* meta = nullptr;  // used slots, added edges, error, counts
+    uint32_t* tcnt = nullptr;  // [blocks of 256 entries] taken counts of presorted edits
     void* temp = nullptr;
     size_t temp_bytes = 0;
     void release();
@@ -149,6 +150,41 @@ dyg_status device_edits_batch(dyg_graph g, const uint32_t* src, const uint32_t* 
 // with flag[i] == 1, in batch order per row.  Each edge must be present.
 dyg_status device_deletes_batch(dyg_graph g, const uint32_t* src, const uint32_t* dst,
                                 const int32_t* flag, uint32_t n, const uint32_t* n_dev);
+// The orders of one batch of n updates computed ahead of it (on another
+// stream, so they overlap the batches before it): its edge keys (min << 32 |
+// max, or all ones for a self-loop) stably sorted with their batch
+// positions, and its 2n row entries (entry 2i is u[i]'s row, 2i+1 is v[i]'s)
+// stably sorted by row with their entry index; bcnt is the batch's cleared
+// array of per-256 kept counts.
+struct BatchOrder {
+    const unsigned long long* key;  // [n]
+    const uint32_t* idx;            // [n]
+    const uint32_t* row;            // [2n]
+    const uint32_t* ent;            // [2n]
+    uint32_t* bcnt;                 // [(n + 255) / 256]
+};
+// Which presorted entries an edit takes, by batch position i = entry >> 1:
+// flag[i] == match, or, when keep is set, keep[i] && found[jmap[i]] == -1
+// (an edge kept for the walk, jmap[i] its place in the kept list, that found
+// no path).
+struct EditTake {
+    const int32_t* flag;
+    int32_t match;
+    const uint8_t* keep;
+    const uint32_t* jmap;
+    const int32_t* found;
+};
+// device_edits_batch / device_deletes_batch over a batch whose entries are
+// presorted (BatchOrder::row / ent): the taken entries, compacted in that
+// order, are the list the sort of only those entries gives, so the results
+// are the same with no sort in the batch's own pipeline.  src, dst, w are
+// the batch's arrays (indexed by batch position).
+dyg_status device_edits_sorted(dyg_graph g, const uint32_t* src, const uint32_t* dst,
+                               const float* w, const uint32_t* row, const uint32_t* ent,
+                               uint32_t n, const EditTake& tk);
+dyg_status device_deletes_sorted(dyg_graph g, const uint32_t* src, const uint32_t* dst,
+                                 const uint32_t* row, const uint32_t* ent, uint32_t n,
+                                 const EditTake& tk);
 // Waits for the stream; updates the host's slot counts; reports the edges
 // appended since begin().  The pipeline stays usable afterwards.
 dyg_status device_edits_end(dyg_graph g, uint64_t* added);
@@ -197,22 +233,37 @@ hipError_t launch_heuristic(const uint2* rows, const uint4* adj, const uint32_t*
 hipError_t launch_apply_delta(uint2* rows, uint4* adj, const uint32_t* delta, uint32_t n_write,
                               uint32_t n_row, hipStream_t s);
 
-// Slot of column b in a row r = (first slot, degree), or -1.  The columns
-// are read eight at a time as independent loads.
+// Slot of column b in a row r = (first slot, degree), or -1: the first match
+// in slot order.  The columns are read sixteen at a time; a round past the
+// row's end rereads its last slot (a later duplicate never wins over it), so
+// the loads need no masks, and the round's matches are gathered into a bit
+// mask before the first one is taken, so no compare sits between two loads:
+// a row of up to sixteen slots costs one round trip to memory.
 __device__ __forceinline__ int slot_of(const uint4* __restrict__ adj, uint2 r, uint32_t b) {
     const uint4* p = adj + r.x;
-    uint32_t q = 0;
-    for (; q + 8 <= r.y; q += 8) {
-        uint32_t c[8];
+    for (uint32_t q = 0; q < r.y; q += 16) {
+        uint32_t c[16];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) c[j] = p[q + j].x;
+        for (uint32_t j = 0; j < 16; ++j) c[j] = p[min(q + j, r.y - 1u)].x;
+        uint32_t m = 0;
 #pragma unroll
-        for (int j = 0; j < 8; ++j)
-            if (c[j] == b) return (int)(q + j);
+        for (uint32_t j = 0; j < 16; ++j) m |= (c[j] == b ? 1u : 0u) << j;
+        if (m) return (int)(q + (uint32_t)__builtin_ctz(m));
     }
-    for (; q < r.y; ++q)
-        if (p[q].x == b) return (int)q;
     return -1;
+}
+
+// Whether the graph holds edge a-b: both row descriptors are loaded whole
+// and together (selecting between the two loaded words keeps the compiler
+// from loading the chosen row's start after the degree compare), then the
+// shorter row is scanned.
+__device__ __forceinline__ bool has_edge(const uint2* __restrict__ rows,
+                                         const uint4* __restrict__ adj, uint32_t a, uint32_t b) {
+    const unsigned long long xa = reinterpret_cast<const unsigned long long*>(rows)[a];
+    const unsigned long long xb = reinterpret_cast<const unsigned long long*>(rows)[b];
+    const bool sa = (xa >> 32) <= (xb >> 32);
+    const unsigned long long x = sa ? xa : xb;
+    return slot_of(adj, make_uint2((uint32_t)x, (uint32_t)(x >> 32)), sa ? b : a) >= 0;
 }
 
 // Moves a row's n slots to a fresh range (relocation to the arena tail; the
@@ -231,23 +282,21 @@ __device__ __forceinline__ void copy_slots(uint4* __restrict__ dst, const uint4*
 }
 
 // In a row (first slot, degree) rx, every slot pointing to r gets r's
-// descriptor rr.  Only the slots' upper halves are written, so the columns
-// are read eight at a time as independent loads (a neighbour's row is one
-// short chain of loads, not one load per slot).
+// descriptor rr.  Only the slots' upper halves are written; the columns are
+// read sixteen at a time as independent loads (a round past the row's end
+// rereads its last slot and writes nothing there), so a neighbour's row is
+// one short chain of loads, not one load per slot.
 __device__ __forceinline__ void refresh_pointers(uint4* __restrict__ adj, uint2 rx, uint32_t r,
                                                  uint2 rr) {
     uint4* p = adj + rx.x;
-    uint32_t k = 0;
-    for (; k + 8 <= rx.y; k += 8) {
-        uint32_t c[8];
+    for (uint32_t k = 0; k < rx.y; k += 16) {
+        uint32_t c[16];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) c[j] = p[k + j].x;
+        for (uint32_t j = 0; j < 16; ++j) c[j] = p[min(k + j, rx.y - 1u)].x;
 #pragma unroll
-        for (int j = 0; j < 8; ++j)
-            if (c[j] == r) *reinterpret_cast<uint2*>(&p[k + j].z) = rr;
+        for (uint32_t j = 0; j < 16; ++j)
+            if (k + j < rx.y && c[j] == r) *reinterpret_cast<uint2*>(&p[k + j].z) = rr;
     }
-    for (; k < rx.y; ++k)
-        if (p[k].x == r) *reinterpret_cast<uint2*>(&p[k].z) = rr;
 }
 
 // Lanes per touched row for the descriptor refresh: the power of two at or

@@ -65,13 +65,6 @@ __device__ __forceinline__ int row_find(const uint2* __restrict__ rows,
     return slot_of(adj, rows[a], b);
 }
 
-// membership through the shorter of the two rows
-__device__ __forceinline__ bool has_edge(const uint2* __restrict__ rows,
-                                         const uint4* __restrict__ adj, uint32_t a, uint32_t b) {
-    return rows[a].y <= rows[b].y ? row_find(rows, adj, a, b) >= 0
-                                  : row_find(rows, adj, b, a) >= 0;
-}
-
 // Also clears bcnt[block] (when given): the per-block counts a later
 // k_compact over the same n reads.
 // Range check of a batch of device inputs (larger than the one-workgroup
@@ -112,6 +105,26 @@ __global__ void k_batch_keys_staged(const uint32_t* __restrict__ u, const uint32
     const uint32_t a = min(x, y), b = max(x, y);
     key[i] = a == b ? kPadKey : ((unsigned long long)a << 32) | b;
     idx[i] = i;
+}
+
+// A batch's orders computed ahead of it (BatchOrder), before their sorts:
+// its edge keys with their positions, its row entries (2i: u[i]'s row, 2i+1:
+// v[i]'s) with their indices, and its cleared per-block kept counts.
+__global__ void k_prep_keys(const uint32_t* __restrict__ u, const uint32_t* __restrict__ v,
+                            uint32_t n, unsigned long long* __restrict__ key,
+                            uint32_t* __restrict__ idx, uint32_t* __restrict__ erow,
+                            uint32_t* __restrict__ eent, uint32_t* __restrict__ bcnt) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (threadIdx.x == 0) bcnt[blockIdx.x] = 0;
+    if (i >= n) return;
+    const uint32_t x = u[i], y = v[i];
+    const uint32_t a = min(x, y), b = max(x, y);
+    key[i] = a == b ? kPadKey : ((unsigned long long)a << 32) | b;
+    idx[i] = i;
+    erow[2 * i] = x;
+    erow[2 * i + 1] = y;
+    eent[2 * i] = 2 * i;
+    eent[2 * i + 1] = 2 * i + 1;
 }
 
 // One thread per distinct key (first position of its run in the sorted order).
@@ -221,13 +234,14 @@ __global__ void k_del_groups(const unsigned long long* __restrict__ key,
 // block b's output offset is the sum of the kept counts of blocks before it
 // (bcnt, counted by the classification kernel), plus its own block scan.
 // Gathers each kept update: (u, v) or, `ordered`, (min, max); its weight;
-// its stream position.  cnt[0] = kept total.  256 threads per block.
+// its stream position; and (jmap set) its place in the kept list by batch
+// position.  cnt[0] = kept total.  256 threads per block.
 __global__ __launch_bounds__(256) void k_compact(
     const uint8_t* __restrict__ keep, uint32_t n, const uint32_t* __restrict__ bcnt,
     const uint32_t* __restrict__ u, const uint32_t* __restrict__ v, const float* __restrict__ w,
     uint64_t base, bool ordered, uint32_t* __restrict__ cu, uint32_t* __restrict__ cv,
     float* __restrict__ cw, uint64_t* __restrict__ ceid, uint32_t* __restrict__ sel,
-    uint32_t* __restrict__ cnt) {
+    uint32_t* __restrict__ cnt, uint32_t* __restrict__ jmap) {
     using BScan = hipcub::BlockScan<uint32_t, 256>;
     using BReduce = hipcub::BlockReduce<uint32_t, 256>;
     __shared__ union {
@@ -252,6 +266,7 @@ __global__ __launch_bounds__(256) void k_compact(
         if (w) cw[pos] = w[i];
         ceid[pos] = base + i;
         sel[pos] = i;
+        if (jmap) jmap[i] = pos;
     }
     if (b == gridDim.x - 1 && t == 0) cnt[0] = s_prefix + total;
 }
@@ -426,6 +441,7 @@ struct dyg_stream_s {
     uint32_t *pc, *poff;
     uint8_t* nopath;
     uint32_t* bcnt;             // [blocks of 256] kept counts for k_compact
+    uint32_t* jmap;             // [max_batch] kept-list place by batch position (BatchOrder)
     unsigned long long* bsum;   // [blocks of 256] path hop and no-path sums
     uint32_t* ep;     // [2 * max_batch]
     uint64_t* ekey;   // [2 * max_batch]
@@ -460,7 +476,25 @@ struct dyg_stream_s {
     // events bracketing the current batch's walk (the context's pair, or a
     // batch's own pair while a list of batches is in flight)
     hipEvent_t wev0 = nullptr, wev1 = nullptr;
+    // Orders of a list's larger batches (BatchOrder), computed on their own
+    // stream while the batches before them run: the orders (grow-only), one
+    // batch's sort inputs and the sorts' scratch, one event per batch.
+    hipStream_t prep = nullptr;
+    DevBuf order;
+    void* prep_mem = nullptr;
+    unsigned long long* pk_in = nullptr;
+    uint32_t *pi_in = nullptr, *pr_in = nullptr, *pe_in = nullptr;
+    void* prep_temp = nullptr;
+    size_t prep_temp_bytes = 0;
+    std::vector<hipEvent_t> prep_ev;
     ~dyg_stream_s() {
+        if (prep) {
+            (void)hipStreamSynchronize(prep);
+            (void)hipStreamDestroy(prep);
+        }
+        for (hipEvent_t e : prep_ev) (void)hipEventDestroy(e);
+        if (prep_mem) (void)hipFree(prep_mem);
+        if (prep_temp) (void)hipFree(prep_temp);
         for (auto& op : steps)
             for (StepGraph& g : op) {
                 if (g.exec) (void)hipGraphExecDestroy(g.exec);
@@ -536,6 +570,7 @@ dyg_status stream_alloc(dyg_stream s) {
         s->poff = ar.take<uint32_t>(n);
         s->nopath = ar.take<uint8_t>(n);
         s->bcnt = ar.take<uint32_t>(((size_t)n + 255) / 256);
+        s->jmap = ar.take<uint32_t>(n);
         s->bsum = ar.take<unsigned long long>(((size_t)n + 255) / 256);
         s->ep = ar.take<uint32_t>(2 * (size_t)n);
         s->ekey = ar.take<uint64_t>(2 * (size_t)n);
@@ -849,40 +884,55 @@ dyg_status apply_insert_small(dyg_stream s, const uint32_t* du, const uint32_t* 
 
 // Both apply_* enqueue a whole batch (no wait except a deletion batch's one
 // read-back of its recovery sizes) ending with the report `rep`.
+// `ord` (larger batches of a list): the batch's orders, computed ahead of it
+// on the prep stream; its own pipeline then sorts nothing.
 dyg_status apply_insert(dyg_stream s, const uint32_t* du, const uint32_t* dv, const float* dw,
                         uint32_t n, int32_t* dcls, unsigned long long* dstats, bool timed,
                         dyg_stream_stats* out, const Report& rep, uint32_t* stage,
-                        bool graph = false, uint32_t vcheck = 0) {
+                        bool graph = false, uint32_t vcheck = 0,
+                        const BatchOrder* ord = nullptr) {
     if (n <= std::min(kSmallBatch, s->sp.max_batch) && !(s->sp.flags & DYG_STREAM_GENERAL_ONLY))
         return apply_insert_small(s, du, dv, dw, n, dcls, dstats, timed, rep, graph, vcheck);
     hipStream_t st = s->ctx->stream;
     dyg_graph G = s->G, H = s->H;
-    if (stage) {
-        LAUNCH_1D(k_batch_keys_staged, n, du, dv, dw, n, s->k_in, s->i_in, s->bcnt, stage,
-                  stage + n, (float*)(stage + 2ull * n));
-        du = stage;
-        dv = stage + n;
-        dw = (const float*)(stage + 2ull * n);
-    } else {
-        LAUNCH_1D(k_batch_keys, n, du, dv, n, s->k_in, s->i_in, s->bcnt);
+    const unsigned long long* key = ord ? ord->key : s->k_out;
+    const uint32_t* idx = ord ? ord->idx : s->i_out;
+    uint32_t* bcnt = ord ? ord->bcnt : s->bcnt;
+    dyg_status r = DYG_OK;
+    if (!ord) {
+        if (stage) {
+            LAUNCH_1D(k_batch_keys_staged, n, du, dv, dw, n, s->k_in, s->i_in, s->bcnt, stage,
+                      stage + n, (float*)(stage + 2ull * n));
+            du = stage;
+            dv = stage + n;
+            dw = (const float*)(stage + 2ull * n);
+        } else {
+            LAUNCH_1D(k_batch_keys, n, du, dv, n, s->k_in, s->i_in, s->bcnt);
+        }
+        r = sort_keys(s, s->k_in, s->k_out, s->i_in, s->i_out, n);
+        if (r != DYG_OK) return r;
     }
-    dyg_status r = sort_keys(s, s->k_in, s->k_out, s->i_in, s->i_out, n);
+    LAUNCH_1D(k_ins_groups, n, key, idx, n, G->d_rows, G->d_adj, H->d_rows, H->d_adj, dcls,
+              s->flag_g, s->keep, s->mode, bcnt);
+    r = ord ? device_edits_sorted(G, du, dv, dw, ord->row, ord->ent, n,
+                                  EditTake{s->flag_g, -1, nullptr, nullptr, nullptr})
+            : device_edits_batch(G, du, dv, dw, s->flag_g, n);
     if (r != DYG_OK) return r;
-    LAUNCH_1D(k_ins_groups, n, s->k_out, s->i_out, n, G->d_rows, G->d_adj, H->d_rows, H->d_adj,
-              dcls, s->flag_g, s->keep, s->mode, s->bcnt);
-    r = device_edits_batch(G, du, dv, dw, s->flag_g, n);
-    if (r != DYG_OK) return r;
-    LAUNCH_1D(k_ins_weights, n, s->k_out, s->i_out, n, s->mode, dw, G->d_rows, G->d_adj,
-              H->d_rows, H->d_adj);
-    LAUNCH_1D(k_compact, n, s->keep, n, s->bcnt, du, dv, dw, s->base, false, s->cu, s->cv,
-              s->cw, s->ceid, s->sel, s->cnt);
+    LAUNCH_1D(k_ins_weights, n, key, idx, n, s->mode, dw, G->d_rows, G->d_adj, H->d_rows,
+              H->d_adj);
+    LAUNCH_1D(k_compact, n, s->keep, n, bcnt, du, dv, dw, s->base, false, s->cu, s->cv, s->cw,
+              s->ceid, s->sel, s->cnt, ord ? s->jmap : nullptr);
     DYG_HIP(s->ctx, hipGetLastError());
     WalkLaunch a = walk_args(s, H, s->sp.ins, n);
     a.stats = dstats;
     a.flags |= DYG_WALK_DECISION_ONLY;  // outcomes need found / not found only
     DYG_HIP(s->ctx, launch_walk_incremental(a, st, timed ? s->wev0 : nullptr,
                                             timed ? s->wev1 : nullptr));
-    r = device_edits_batch(H, s->cu, s->cv, s->cw, s->cfound, n, s->cnt);
+    // the kept edges that found no path, appended to H (from the batch's own
+    // arrays and presorted entries when ordered ahead)
+    r = ord ? device_edits_sorted(H, du, dv, dw, ord->row, ord->ent, n,
+                                  EditTake{nullptr, 0, s->keep, s->jmap, s->cfound})
+            : device_edits_batch(H, s->cu, s->cv, s->cw, s->cfound, n, s->cnt);
     if (r != DYG_OK) return r;
     LAUNCH_1D(k_outcome, n, s->sel, s->cnt, s->cfound, dcls, kOutFound, kOutAdded);
     DYG_HIP(s->ctx, launch_report(rep, dcls, n, dstats, st));
@@ -1008,30 +1058,41 @@ dyg_status apply_delete_small(dyg_stream s, const uint32_t* du, const uint32_t* 
 dyg_status apply_delete(dyg_stream s, const uint32_t* du, const uint32_t* dv, uint32_t n,
                         int32_t* dcls, unsigned long long* dstats, bool timed,
                         dyg_stream_stats* out, const Report& rep, uint32_t* stage,
-                        bool graph = false, uint32_t vcheck = 0) {
+                        bool graph = false, uint32_t vcheck = 0,
+                        const BatchOrder* ord = nullptr) {
     if (n <= std::min(kSmallBatch, s->sp.max_batch) && !(s->sp.flags & DYG_STREAM_GENERAL_ONLY))
         return apply_delete_small(s, du, dv, n, dcls, dstats, timed, out, rep, graph, vcheck);
     hipStream_t st = s->ctx->stream;
     dyg_graph G = s->G, H = s->H;
     const uint32_t ms = s->sp.del.max_steps, hw = s->sp.heur_walkers;
-    if (stage) {
-        LAUNCH_1D(k_batch_keys_staged, n, du, dv, (const float*)nullptr, n, s->k_in, s->i_in,
-                  s->bcnt, stage, stage + n, (float*)nullptr);
-        du = stage;
-        dv = stage + n;
-    } else {
-        LAUNCH_1D(k_batch_keys, n, du, dv, n, s->k_in, s->i_in, s->bcnt);
+    const unsigned long long* key = ord ? ord->key : s->k_out;
+    const uint32_t* idx = ord ? ord->idx : s->i_out;
+    uint32_t* bcnt = ord ? ord->bcnt : s->bcnt;
+    dyg_status r = DYG_OK;
+    if (!ord) {
+        if (stage) {
+            LAUNCH_1D(k_batch_keys_staged, n, du, dv, (const float*)nullptr, n, s->k_in, s->i_in,
+                      s->bcnt, stage, stage + n, (float*)nullptr);
+            du = stage;
+            dv = stage + n;
+        } else {
+            LAUNCH_1D(k_batch_keys, n, du, dv, n, s->k_in, s->i_in, s->bcnt);
+        }
+        r = sort_keys(s, s->k_in, s->k_out, s->i_in, s->i_out, n);
+        if (r != DYG_OK) return r;
     }
-    dyg_status r = sort_keys(s, s->k_in, s->k_out, s->i_in, s->i_out, n);
+    LAUNCH_1D(k_del_groups, n, key, idx, n, G->d_rows, G->d_adj, H->d_rows, H->d_adj, dcls,
+              s->flag_g, s->flag_h, s->keep, bcnt);
+    r = ord ? device_deletes_sorted(G, du, dv, ord->row, ord->ent, n,
+                                    EditTake{s->flag_g, 1, nullptr, nullptr, nullptr})
+            : device_deletes_batch(G, du, dv, s->flag_g, n, nullptr);
     if (r != DYG_OK) return r;
-    LAUNCH_1D(k_del_groups, n, s->k_out, s->i_out, n, G->d_rows, G->d_adj, H->d_rows, H->d_adj,
-              dcls, s->flag_g, s->flag_h, s->keep, s->bcnt);
-    r = device_deletes_batch(G, du, dv, s->flag_g, n, nullptr);
+    r = ord ? device_deletes_sorted(H, du, dv, ord->row, ord->ent, n,
+                                    EditTake{s->flag_h, 1, nullptr, nullptr, nullptr})
+            : device_deletes_batch(H, du, dv, s->flag_h, n, nullptr);
     if (r != DYG_OK) return r;
-    r = device_deletes_batch(H, du, dv, s->flag_h, n, nullptr);
-    if (r != DYG_OK) return r;
-    LAUNCH_1D(k_compact, n, s->keep, n, s->bcnt, du, dv, (const float*)nullptr, s->base, true,
-              s->cu, s->cv, (float*)nullptr, s->ceid, s->sel, s->cnt);
+    LAUNCH_1D(k_compact, n, s->keep, n, bcnt, du, dv, (const float*)nullptr, s->base, true,
+              s->cu, s->cv, (float*)nullptr, s->ceid, s->sel, s->cnt, (uint32_t*)nullptr);
     DYG_HIP(s->ctx, hipGetLastError());
     WalkLaunch a = walk_args(s, G, s->sp.del, n);
     a.w = nullptr;
@@ -1390,6 +1451,79 @@ dyg_status stream_apply(dyg_stream s, int op, const uint32_t* u, const uint32_t*
     return DYG_OK;
 }
 
+// A batch of n updates takes the general pipeline (and an order computed
+// ahead of it in a list) when it is larger than a one-workgroup batch.
+bool general_batch(dyg_stream s, uint64_t n) {
+    return n > std::min(kSmallBatch, s->sp.max_batch) || (s->sp.flags & DYG_STREAM_GENERAL_ONLY);
+}
+
+// Bytes of one batch's BatchOrder arrays (each batch's region 256-aligned).
+size_t order_bytes(uint64_t n) {
+    const size_t b = 8 * n + 4 * n + 8 * n + 8 * n + 4 * ((n + 255) / 256);
+    return (b + 255) & ~(size_t)255;
+}
+
+// The prep stream, one batch's sort inputs and scratch, `bytes` of orders and
+// `events` events.
+dyg_status prep_reserve(dyg_stream s, size_t bytes, size_t events) {
+    dyg_ctx ctx = s->ctx;
+    if (!s->prep) DYG_HIP(ctx, hipStreamCreate(&s->prep));
+    if (!s->prep_mem) {
+        const size_t n = s->sp.max_batch;
+        if (hipMalloc(&s->prep_mem, 8 * n + 4 * n + 8 * n + 8 * n + 1024) != hipSuccess) {
+            s->prep_mem = nullptr;
+            set_error(ctx, "stream: device allocation failed");
+            return DYG_ERR_NOMEM;
+        }
+        char* b = (char*)s->prep_mem;
+        s->pk_in = (unsigned long long*)b;
+        s->pi_in = (uint32_t*)(b + 8 * n);
+        s->pr_in = (uint32_t*)(b + 12 * n);
+        s->pe_in = (uint32_t*)(b + 20 * n);
+        size_t t1 = 0, t2 = 0;
+        DYG_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, t1, s->pk_in, s->pk_in, s->pi_in,
+                                                        s->pi_in, (int)n, 0, 64, s->prep));
+        DYG_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, t2, s->pr_in, s->pr_in, s->pe_in,
+                                                        s->pe_in, (int)(2 * n), 0, 32, s->prep));
+        if (hipMalloc(&s->prep_temp, std::max(t1, t2)) != hipSuccess) {
+            s->prep_temp = nullptr;
+            set_error(ctx, "stream: device allocation failed");
+            return DYG_ERR_NOMEM;
+        }
+        s->prep_temp_bytes = std::max(t1, t2);
+    }
+    DYG_HIP(ctx, s->order.reserve(bytes));
+    while (s->prep_ev.size() < events) {
+        hipEvent_t e = nullptr;
+        DYG_HIP(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        s->prep_ev.push_back(e);
+    }
+    return DYG_OK;
+}
+
+// Enqueues one batch's orders on the prep stream (keys, then the two stable
+// sorts) into `ord`'s arrays at `region`, and records event `ev` after them.
+dyg_status enqueue_order(dyg_stream s, const uint32_t* u, const uint32_t* v, uint32_t n,
+                         char* region, BatchOrder* ord, hipEvent_t ev) {
+    dyg_ctx ctx = s->ctx;
+    auto* key = (unsigned long long*)region;
+    auto* idx = (uint32_t*)(region + 8ull * n);
+    auto* row = (uint32_t*)(region + 12ull * n);
+    auto* ent = (uint32_t*)(region + 20ull * n);
+    auto* bcnt = (uint32_t*)(region + 28ull * n);
+    hipLaunchKernelGGL(k_prep_keys, dim3((n + 255) / 256), dim3(256), 0, s->prep, u, v, n,
+                       s->pk_in, s->pi_in, s->pr_in, s->pe_in, bcnt);
+    size_t tb = s->prep_temp_bytes;
+    DYG_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(s->prep_temp, tb, s->pk_in, key, s->pi_in, idx,
+                                                    (int)n, 0, key_bits(s->G->V), s->prep));
+    tb = s->prep_temp_bytes;
+    DYG_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(s->prep_temp, tb, s->pr_in, row, s->pe_in, ent,
+                                                    (int)(2 * n), 0, row_bits(s->G->V), s->prep));
+    DYG_HIP(ctx, hipEventRecord(ev, s->prep));
+    *ord = BatchOrder{key, idx, row, ent, bcnt};
+    return DYG_OK;
+}
+
 // A list of batches already in device memory, enqueued back to back: the
 // ids of the whole list are checked once before anything changes; each
 // batch reports into its own region of the context's pinned report buffer
@@ -1432,6 +1566,35 @@ dyg_status stream_apply_list(dyg_stream s, uint32_t nb, const int* op, const uin
     if (hp.words[kReportBad]) {
         set_error(ctx, "stream apply many: vertex id out of range");
         return DYG_ERR_ARG;
+    }
+    // the larger batches' orders, computed on the prep stream while the
+    // batches before them run (after the inputs are in place on this stream)
+    std::vector<BatchOrder> ords(nb);
+    std::vector<int> ord_ev(nb, -1);
+    size_t obytes = 0, nev = 0;
+    for (uint32_t b = 0; b < nb; ++b)
+        if (general_batch(s, off[b + 1] - off[b]) && off[b + 1] > off[b]) {
+            obytes += order_bytes(off[b + 1] - off[b]);
+            ++nev;
+        }
+    if (nev) {
+        r = prep_reserve(s, obytes, nev + 1);
+        if (r != DYG_OK) return r;
+        DYG_HIP(ctx, hipEventRecord(s->prep_ev[nev], str));
+        DYG_HIP(ctx, hipStreamWaitEvent(s->prep, s->prep_ev[nev], 0));
+        char* region = (char*)s->order.p;
+        int k = 0;
+        for (uint32_t b = 0; b < nb; ++b) {
+            const uint64_t o = off[b], n = off[b + 1] - o;
+            if (!n || !general_batch(s, n)) continue;
+            r = enqueue_order(s, u + o, v + o, (uint32_t)n, region, &ords[b], s->prep_ev[k]);
+            if (r != DYG_OK) {
+                (void)hipStreamSynchronize(s->prep);
+                return r;
+            }
+            ord_ev[b] = k++;
+            region += order_bytes(n);
+        }
     }
     // report regions: kListWords words per batch, then every outcome code
     const size_t words_bytes = (size_t)nb * kListWords * 8;
@@ -1517,11 +1680,15 @@ dyg_status stream_apply_list(dyg_stream s, uint32_t nb, const int* op, const uin
             fail = hipEventRecord(ev[0], str) == hipSuccess ? DYG_OK : DYG_ERR_HIP;
             started = true;
         }
+        const BatchOrder* ord = ord_ev[b] >= 0 ? &ords[b] : nullptr;
+        if (fail == DYG_OK && ord &&
+            hipStreamWaitEvent(str, s->prep_ev[ord_ev[b]], 0) != hipSuccess)
+            fail = DYG_ERR_HIP;
         if (fail == DYG_OK)
             fail = del ? apply_delete(s, u + o, v + o, n, s->cls, s->stats, true, &rs[b], rep,
-                                      nullptr)
+                                      nullptr, false, 0, ord)
                        : apply_insert(s, u + o, v + o, w + o, n, s->cls, s->stats, true, &rs[b],
-                                      rep, nullptr);
+                                      rep, nullptr, false, 0, ord);
         if (fail != DYG_OK) break;
         last = (int)b;
         done_b = b + 1;
@@ -1529,6 +1696,10 @@ dyg_status stream_apply_list(dyg_stream s, uint32_t nb, const int* op, const uin
     }
     s->wev0 = ctx->ev0;
     s->wev1 = ctx->ev1;
+    if (nev) {  // the prep stream's work is done before the call returns
+        DYG_HIP(ctx, hipEventRecord(s->prep_ev[nev], s->prep));
+        DYG_HIP(ctx, hipStreamWaitEvent(str, s->prep_ev[nev], 0));
+    }
     // the list's one wait, then the reports of every batch enqueued
     DYG_HIP(ctx, hipEventRecord(ev[2 * (size_t)nb + 1], str));
     r = wait_spin(ctx, str, s->ev_sync);

@@ -79,11 +79,6 @@ __device__ __forceinline__ int find_in_row(const uint2* rows, const uint4* adj, 
     return slot_of(adj, rows[a], b);
 }
 
-__device__ __forceinline__ bool holds(const uint2* rows, const uint4* adj, uint32_t a, uint32_t b) {
-    return rows[a].y <= rows[b].y ? find_in_row(rows, adj, a, b) >= 0
-                                  : find_in_row(rows, adj, b, a) >= 0;
-}
-
 __device__ __forceinline__ unsigned long long edge_key(uint32_t u, uint32_t v) {
     const uint32_t lo = min(u, v), hi = max(u, v);
     return lo == hi ? kPad64 : ((unsigned long long)lo << 32) | hi;
@@ -472,13 +467,13 @@ __global__ __launch_bounds__(kThreads) void k_small_ins_pre(SmallInsArgs a) {
             uint32_t q = p + 1;
             while (q < n && sh.key[q] == kp) ++q;
             const uint32_t lo = (uint32_t)(kp >> 32), hi = (uint32_t)kp;
-            const bool in_g = holds(a.G.rows, a.G.adj, lo, hi);
+            const bool in_g = has_edge(a.G.rows, a.G.adj, lo, hi);
             for (uint32_t e = p; e < q; ++e) {
                 const bool first_new = !in_g && e == p;
                 a.cls[sh.idx[e]] = first_new ? 0 : kOutWeight;
                 sh.keep[sh.idx[e]] = first_new ? 1 : 0;
             }
-            if (in_g) sh.mode[p] = 1 | (holds(a.H.rows, a.H.adj, lo, hi) ? 4 : 0);
+            if (in_g) sh.mode[p] = 1 | (has_edge(a.H.rows, a.H.adj, lo, hi) ? 4 : 0);
             else if (q - p > 1) sh.mode[p] = 2;
         } else if (p < n && kp == kPad64) {
             a.cls[sh.idx[p]] = kOutInvalid;
@@ -597,8 +592,8 @@ __global__ __launch_bounds__(kThreads) void k_small_del_pre(SmallDelArgs a) {
             bool in_g = false, in_h = false;
             if (kp != kPad64) {
                 const uint32_t lo = (uint32_t)(kp >> 32), hi = (uint32_t)kp;
-                in_g = holds(a.G.rows, a.G.adj, lo, hi);
-                in_h = in_g && holds(a.H.rows, a.H.adj, lo, hi);
+                in_g = has_edge(a.G.rows, a.G.adj, lo, hi);
+                in_h = in_g && has_edge(a.H.rows, a.H.adj, lo, hi);
             }
             for (uint32_t e = p; e < q; ++e) {
                 const uint32_t i = sh.idx[e];
@@ -748,7 +743,7 @@ __global__ __launch_bounds__(kThreads) void k_small_del_post(SmallDelArgs a) {
         const unsigned long long ks = sh.key[p];
         if (p >= nc || ks == kPad64 || (p > 0 && sh.key[p - 1] == ks)) continue;
         const uint32_t lo = (uint32_t)(ks >> 32), hi = (uint32_t)ks;
-        if (holds(a.H.rows, a.H.adj, lo, hi)) continue;
+        if (has_edge(a.H.rows, a.H.adj, lo, hi)) continue;
         const int s = find_in_row(a.G.rows, a.G.adj, lo, hi);
         if (s < 0) continue;  // unreachable: path and heuristic hops are edges of G
         const uint32_t c = sh.idx[p];

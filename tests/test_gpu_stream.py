@@ -447,3 +447,38 @@ def test_apply_many_dev_checks_ids_first_and_runs_lists_in_turn(ctx, grid_wl):
     _assert_same_graph(split.G, _rows_of(whole.G))
     _assert_same_graph(split.H, _rows_of(whole.H))
     assert split.position == whole.position == int(off[-1])
+
+
+@pytest.mark.parametrize("seed,nodesc", [(31, False), (32, True)])
+def test_apply_many_dev_general_pipeline_matches_oracle(ctx, grid_wl, seed, nodesc):
+    """A list through dyg_stream_apply_many_dev with every batch in the general
+    pipeline: each batch's keys and row entries are sorted ahead of it on the
+    prep stream, and its row edits take the presorted entries instead of
+    sorting their own.  Outcome codes of every batch and the final G and H
+    (slot for slot) equal the oracle's, batch after batch."""
+    from dygrass_amd import Stream, stream_params
+    from oracle import stream as S
+
+    wl = grid_wl
+    batches = mixed_batches(wl, seed, n_ins=400, n_del=250)
+    sp = stream_params(w_ins=64, w_del=64, threshold=10.0, seed=seed, max_batch=1100,
+                       max_inserts=8192, general_only=True, dense_no_desc=nodesc)
+    st = Stream(ctx, wl.G.V, (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w), sp)
+    orc = S.Stream(wl.G.V, (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w), thr=10.0,
+                   w_ins=64, w_del=64, seed=seed)
+    off = np.concatenate([[0], np.cumsum([len(b[1]) for b in batches])]).astype(np.uint64)
+    U = np.concatenate([np.asarray(b[1], np.uint32) for b in batches])
+    Vv = np.concatenate([np.asarray(b[2], np.uint32) for b in batches])
+    Wt = np.concatenate([np.asarray(b[3], np.float32) if b[3] is not None
+                         else np.zeros(len(b[1]), np.float32) for b in batches])
+    got, _ = st.apply_many_dev([b[0] for b in batches], off, _dev(ctx, U, np.uint32),
+                               _dev(ctx, Vv, np.uint32), _dev(ctx, Wt, np.float32))
+    seen = set()
+    for i, (op, u, v, w) in enumerate(batches):
+        want = orc.apply(op, u, v, w)
+        g = got[off[i]:off[i + 1]]
+        assert np.array_equal(g, want), (i, op, np.flatnonzero(g != want)[:10])
+        seen.update(np.unique(g).tolist())
+    _assert_same_graph(st.G, orc.G)
+    _assert_same_graph(st.H, orc.H)
+    assert seen >= {S.WEIGHT, S.FOUND, S.ADDED, S.SKIP, S.G_ONLY, S.RECOVERED, S.INVALID}
