@@ -381,6 +381,8 @@ struct SmallInsArgs {
     const StepDesc* desc;  // when set: u, v, w, n, base and rep's outputs come from here
     uint32_t vcheck;       // nonzero (device inputs): ids must be below it, checked by the
                            // pre kernel before any change; a failing batch changes nothing
+    BatchOrder ord;        // key set: the batch's orders, computed ahead (no sorts of the
+                           // batch's keys or row entries in the pre and post kernels)
 };
 hipError_t launch_small_ins_pre(const SmallInsArgs& a, hipStream_t s);
 hipError_t launch_small_ins_post(const SmallInsArgs& a, hipStream_t s);
@@ -422,7 +424,24 @@ struct SmallDelArgs {
     Report rep;
     const StepDesc* desc;  // when set: u, v, n, base and rep's outputs come from here
     uint32_t vcheck;       // as SmallInsArgs::vcheck
+    BatchOrder ord;        // as SmallInsArgs::ord (the pre kernel's sorts)
 };
+// A list's one-workgroup batches get their orders (BatchOrder: keys and row
+// entries, no kept counts) from one launch, a workgroup per batch, sorting
+// exactly as the pre kernel would.
+struct SmallOrderJob {
+    const uint32_t* u;
+    const uint32_t* v;
+    unsigned long long* key;
+    uint32_t* idx;
+    uint32_t* row;
+    uint32_t* ent;
+    uint32_t n;
+    uint32_t pad;
+};
+// jobs: device array of `count` jobs, all of at most `max_n` updates.
+hipError_t launch_small_order(const SmallOrderJob* jobs, uint32_t count, uint32_t max_n,
+                              int key_bits, int row_bits, hipStream_t s);
 // Candidate edges the post kernel of an n-update deletion batch can hold.
 uint32_t small_del_candidates(uint32_t n);
 hipError_t launch_small_del_pre(const SmallDelArgs& a, hipStream_t s);

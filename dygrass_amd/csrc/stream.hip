@@ -487,6 +487,8 @@ struct dyg_stream_s {
     void* prep_temp = nullptr;
     size_t prep_temp_bytes = 0;
     std::vector<hipEvent_t> prep_ev;
+    Pinned jobs_h;  // the one-workgroup batches' order jobs (SmallOrderJob), host side
+    DevBuf jobs_d;  // and on the device
     ~dyg_stream_s() {
         if (prep) {
             (void)hipStreamSynchronize(prep);
@@ -871,9 +873,10 @@ SmallInsArgs small_ins_args(dyg_stream s, const uint32_t* du, const uint32_t* dv
 dyg_status apply_insert_small(dyg_stream s, const uint32_t* du, const uint32_t* dv,
                               const float* dw, uint32_t n, int32_t* dcls,
                               unsigned long long* dstats, bool timed, const Report& rep,
-                              bool graph, uint32_t vcheck) {
+                              bool graph, uint32_t vcheck, const BatchOrder* ord) {
     SmallInsArgs a = small_ins_args(s, du, dv, dw, n, dcls, dstats, rep);
     a.vcheck = vcheck;
+    if (ord) a.ord = *ord;
     dyg_status r;
     if (graph && timed && run_step_graph(s, true, a, &r)) {
         s->last_graph = true;
@@ -884,15 +887,16 @@ dyg_status apply_insert_small(dyg_stream s, const uint32_t* du, const uint32_t* 
 
 // Both apply_* enqueue a whole batch (no wait except a deletion batch's one
 // read-back of its recovery sizes) ending with the report `rep`.
-// `ord` (larger batches of a list): the batch's orders, computed ahead of it
-// on the prep stream; its own pipeline then sorts nothing.
+// `ord` (batches of a list): the batch's orders, computed ahead of it on the
+// prep stream; its own kernels then sort neither its keys nor its row
+// entries.
 dyg_status apply_insert(dyg_stream s, const uint32_t* du, const uint32_t* dv, const float* dw,
                         uint32_t n, int32_t* dcls, unsigned long long* dstats, bool timed,
                         dyg_stream_stats* out, const Report& rep, uint32_t* stage,
                         bool graph = false, uint32_t vcheck = 0,
                         const BatchOrder* ord = nullptr) {
     if (n <= std::min(kSmallBatch, s->sp.max_batch) && !(s->sp.flags & DYG_STREAM_GENERAL_ONLY))
-        return apply_insert_small(s, du, dv, dw, n, dcls, dstats, timed, rep, graph, vcheck);
+        return apply_insert_small(s, du, dv, dw, n, dcls, dstats, timed, rep, graph, vcheck, ord);
     hipStream_t st = s->ctx->stream;
     dyg_graph G = s->G, H = s->H;
     const unsigned long long* key = ord ? ord->key : s->k_out;
@@ -1029,11 +1033,12 @@ void capture_all_steps(dyg_stream s) {
 dyg_status apply_delete_small(dyg_stream s, const uint32_t* du, const uint32_t* dv, uint32_t n,
                               int32_t* dcls, unsigned long long* dstats, bool timed,
                               dyg_stream_stats* out, const Report& rep, bool graph,
-                              uint32_t vcheck) {
+                              uint32_t vcheck, const BatchOrder* ord) {
     hipStream_t st = s->ctx->stream;
     const uint32_t ms = s->sp.del.max_steps, hw = s->sp.heur_walkers;
     SmallDelArgs a = small_del_args(s, du, dv, n, dcls, dstats, rep);
     a.vcheck = vcheck;
+    if (ord) a.ord = *ord;
     dyg_status r = DYG_OK;
     if (graph && timed && run_step_graph(s, false, a, &r))
         s->last_graph = true;
@@ -1061,7 +1066,8 @@ dyg_status apply_delete(dyg_stream s, const uint32_t* du, const uint32_t* dv, ui
                         bool graph = false, uint32_t vcheck = 0,
                         const BatchOrder* ord = nullptr) {
     if (n <= std::min(kSmallBatch, s->sp.max_batch) && !(s->sp.flags & DYG_STREAM_GENERAL_ONLY))
-        return apply_delete_small(s, du, dv, n, dcls, dstats, timed, out, rep, graph, vcheck);
+        return apply_delete_small(s, du, dv, n, dcls, dstats, timed, out, rep, graph, vcheck,
+                                  ord);
     hipStream_t st = s->ctx->stream;
     dyg_graph G = s->G, H = s->H;
     const uint32_t ms = s->sp.del.max_steps, hw = s->sp.heur_walkers;
@@ -1463,9 +1469,9 @@ size_t order_bytes(uint64_t n) {
     return (b + 255) & ~(size_t)255;
 }
 
-// The prep stream, one batch's sort inputs and scratch, `bytes` of orders and
-// `events` events.
-dyg_status prep_reserve(dyg_stream s, size_t bytes, size_t events) {
+// The prep stream, one batch's sort inputs and scratch, `bytes` of orders,
+// `events` events and room for `jobs` one-workgroup order jobs.
+dyg_status prep_reserve(dyg_stream s, size_t bytes, size_t events, size_t jobs) {
     dyg_ctx ctx = s->ctx;
     if (!s->prep) DYG_HIP(ctx, hipStreamCreate(&s->prep));
     if (!s->prep_mem) {
@@ -1493,6 +1499,10 @@ dyg_status prep_reserve(dyg_stream s, size_t bytes, size_t events) {
         s->prep_temp_bytes = std::max(t1, t2);
     }
     DYG_HIP(ctx, s->order.reserve(bytes));
+    if (jobs) {
+        DYG_HIP(ctx, s->jobs_h.reserve(jobs * sizeof(SmallOrderJob)));
+        DYG_HIP(ctx, s->jobs_d.reserve(jobs * sizeof(SmallOrderJob)));
+    }
     while (s->prep_ev.size() < events) {
         hipEvent_t e = nullptr;
         DYG_HIP(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -1567,33 +1577,75 @@ dyg_status stream_apply_list(dyg_stream s, uint32_t nb, const int* op, const uin
         set_error(ctx, "stream apply many: vertex id out of range");
         return DYG_ERR_ARG;
     }
-    // the larger batches' orders, computed on the prep stream while the
-    // batches before them run (after the inputs are in place on this stream)
+    // every batch's orders, computed on the prep stream while the batches
+    // before it run (after the inputs are in place on this stream): a larger
+    // batch's by its own keys kernel and sorts, all one-workgroup batches'
+    // by one launch per size class (a workgroup each), at the place of the
+    // first of them
     std::vector<BatchOrder> ords(nb);
     std::vector<int> ord_ev(nb, -1);
-    size_t obytes = 0, nev = 0;
-    for (uint32_t b = 0; b < nb; ++b)
-        if (general_batch(s, off[b + 1] - off[b]) && off[b + 1] > off[b]) {
-            obytes += order_bytes(off[b + 1] - off[b]);
-            ++nev;
-        }
+    size_t obytes = 0, ngen = 0, nsmall = 0;
+    for (uint32_t b = 0; b < nb; ++b) {
+        const uint64_t n = off[b + 1] - off[b];
+        if (!n) continue;
+        obytes += order_bytes(n);
+        if (general_batch(s, n))
+            ++ngen;
+        else
+            ++nsmall;
+    }
+    const size_t nev = ngen + (nsmall ? 1 : 0);
     if (nev) {
-        r = prep_reserve(s, obytes, nev + 1);
+        r = prep_reserve(s, obytes, nev + 1, nsmall);
         if (r != DYG_OK) return r;
         DYG_HIP(ctx, hipEventRecord(s->prep_ev[nev], str));
         DYG_HIP(ctx, hipStreamWaitEvent(s->prep, s->prep_ev[nev], 0));
+        // regions in list order; the one-workgroup jobs, smaller class first
+        std::vector<SmallOrderJob> j1, j2;
         char* region = (char*)s->order.p;
-        int k = 0;
         for (uint32_t b = 0; b < nb; ++b) {
             const uint64_t o = off[b], n = off[b + 1] - o;
-            if (!n || !general_batch(s, n)) continue;
-            r = enqueue_order(s, u + o, v + o, (uint32_t)n, region, &ords[b], s->prep_ev[k]);
-            if (r != DYG_OK) {
-                (void)hipStreamSynchronize(s->prep);
-                return r;
-            }
-            ord_ev[b] = k++;
+            if (!n) continue;
+            auto* key = (unsigned long long*)region;
+            auto* idx = (uint32_t*)(region + 8 * n);
+            auto* row = (uint32_t*)(region + 12 * n);
+            auto* ent = (uint32_t*)(region + 20 * n);
+            ords[b] = BatchOrder{key, idx, row, ent, (uint32_t*)(region + 28 * n)};
             region += order_bytes(n);
+            if (general_batch(s, n)) continue;
+            const SmallOrderJob j{u + o, v + o, key, idx, row, ent, (uint32_t)n, 0u};
+            (n <= (uint64_t)kSmallBatch / 2 ? j1 : j2).push_back(j);
+        }
+        auto* jobs = (SmallOrderJob*)s->jobs_h.p;
+        std::copy(j1.begin(), j1.end(), jobs);
+        std::copy(j2.begin(), j2.end(), jobs + j1.size());
+        int k = 0, small_ev = -1;
+        for (uint32_t b = 0; b < nb; ++b) {
+            const uint64_t o = off[b], n = off[b + 1] - o;
+            if (!n) continue;
+            if (general_batch(s, n)) {
+                BatchOrder q{};
+                r = enqueue_order(s, u + o, v + o, (uint32_t)n, (char*)ords[b].key, &q,
+                                  s->prep_ev[k]);
+                if (r != DYG_OK) {
+                    (void)hipStreamSynchronize(s->prep);
+                    return r;
+                }
+                ord_ev[b] = k++;
+                continue;
+            }
+            if (small_ev < 0) {
+                const uint32_t c1 = (uint32_t)j1.size(), n2 = (uint32_t)j2.size();
+                DYG_HIP(ctx, hipMemcpyAsync(s->jobs_d.p, s->jobs_h.p, nsmall * sizeof(SmallOrderJob),
+                                            hipMemcpyHostToDevice, s->prep));
+                auto* dj = (const SmallOrderJob*)s->jobs_d.p;
+                const int kb = key_bits(s->G->V), rb = row_bits(s->G->V);
+                DYG_HIP(ctx, launch_small_order(dj, c1, kSmallBatch / 2, kb, rb, s->prep));
+                DYG_HIP(ctx, launch_small_order(dj + c1, n2, kSmallBatch, kb, rb, s->prep));
+                DYG_HIP(ctx, hipEventRecord(s->prep_ev[k], s->prep));
+                small_ev = k++;
+            }
+            ord_ev[b] = small_ev;
         }
     }
     // report regions: kListWords words per batch, then every outcome code
@@ -1640,6 +1692,7 @@ dyg_status stream_apply_list(dyg_stream s, uint32_t nb, const int* op, const uin
     dyg_status fail = DYG_OK;
     uint32_t done_b = 0;
     bool started = false;  // the list's start event recorded
+    int waited = -1;       // the last prep event this stream waits for
     for (uint32_t b = 0; b < nb && fail == DYG_OK; ++b) {
         const uint64_t o = off[b];
         const uint32_t n = (uint32_t)(off[b + 1] - o);
@@ -1681,9 +1734,10 @@ dyg_status stream_apply_list(dyg_stream s, uint32_t nb, const int* op, const uin
             started = true;
         }
         const BatchOrder* ord = ord_ev[b] >= 0 ? &ords[b] : nullptr;
-        if (fail == DYG_OK && ord &&
-            hipStreamWaitEvent(str, s->prep_ev[ord_ev[b]], 0) != hipSuccess)
-            fail = DYG_ERR_HIP;
+        if (fail == DYG_OK && ord && ord_ev[b] != waited) {
+            if (hipStreamWaitEvent(str, s->prep_ev[ord_ev[b]], 0) != hipSuccess) fail = DYG_ERR_HIP;
+            waited = ord_ev[b];
+        }
         if (fail == DYG_OK)
             fail = del ? apply_delete(s, u + o, v + o, n, s->cls, s->stats, true, &rs[b], rep,
                                       nullptr, false, 0, ord)

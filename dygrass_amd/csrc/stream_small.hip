@@ -319,6 +319,150 @@ __device__ void block_delete(const BlockGraph& g, typename Shape<IPT>::TempU& tm
     __syncthreads();
 }
 
+// A batch's orders computed ahead of it (BatchOrder) into LDS: the sorted
+// keys and positions (pads past n, as sort_keys64 leaves them) and the row
+// entries sorted by row (p < 2n).
+template <int IPT>
+__device__ void load_order(const BatchOrder& o, unsigned long long* key, uint32_t* idx,
+                           uint32_t* srow, uint32_t* sent, uint32_t t, uint32_t n) {
+    constexpr int kIPT = IPT, kEPT = 2 * IPT;
+#pragma unroll
+    for (int k = 0; k < kIPT; ++k) {
+        const uint32_t p = kIPT * t + k;
+        key[p] = p < n ? o.key[p] : kPad64;
+        idx[p] = p < n ? o.idx[p] : p;
+    }
+#pragma unroll
+    for (int q = 0; q < kEPT; ++q) {
+        const uint32_t p = kEPT * t + q;
+        if (p < 2 * n) {
+            srow[p] = o.row[p];
+            sent[p] = o.ent[p];
+        }
+    }
+    __syncthreads();
+}
+
+// block_append over presorted entries: srow/sent hold every entry of the
+// first `span` edges (entry 2e: src[e]'s row, 2e+1: dst[e]'s) sorted by row,
+// entry order within a row; the lane at the start of a row's run appends the
+// run's entries of edges with tk[e] set, in order.  The rows, their appends
+// and their order are those of block_append over the taken entries alone.
+template <int IPT>
+__device__ void block_append_sorted(const BlockGraph& g, Scan::TempStorage& scan,
+                                    const uint32_t* srow, const uint32_t* sent,
+                                    const uint32_t* src, const uint32_t* dst, const float* w,
+                                    const uint8_t* tk, uint32_t t, uint32_t span,
+                                    uint32_t* __restrict__ touched,
+                                    uint32_t* __restrict__ ntouched) {
+    constexpr int kIPT = IPT, kEPT = 2 * IPT;
+    const uint32_t lim = 2 * span;
+    uint32_t mine = 0;
+#pragma unroll
+    for (int k = 0; k < kIPT; ++k) {
+        const uint32_t e = kIPT * t + k;
+        mine += (e < span && tk[e]) ? 1u : 0u;
+    }
+    bool h[kEPT];
+    uint32_t r[kEPT];
+#pragma unroll
+    for (int q = 0; q < kEPT; ++q) {
+        const uint32_t p = kEPT * t + q;
+        h[q] = false;
+        r[q] = 0;
+        if (p >= lim) continue;
+        const uint32_t rr = srow[p];
+        if (p > 0 && srow[p - 1] == rr) continue;  // not the start of its row's run
+        uint32_t end = p + 1, cnt = 0;
+        while (end < lim && srow[end] == rr) ++end;
+        for (uint32_t z = p; z < end; ++z) cnt += tk[sent[z] >> 1] ? 1u : 0u;
+        if (!cnt) continue;
+        h[q] = true;
+        r[q] = rr;
+        uint2 row = g.rows[rr];
+        const uint32_t d0 = row.y;
+        const uint32_t c = g.cap[rr];
+        if ((unsigned long long)d0 + cnt > c) {  // relocate to the arena tail, order kept
+            unsigned long long ncap = 2ull * c;
+            ncap = max(ncap, (unsigned long long)d0 + cnt + g.slack);
+            ncap = max(ncap, 4ull);
+            const unsigned long long ns = atomicAdd(&g.meta[0], ncap);
+            if (ns + ncap > g.cap_slots) {
+                atomicOr((uint32_t*)(g.meta + 2), 1u);
+                continue;
+            }
+            copy_slots(g.adj + ns, g.adj + row.x, d0);
+            row.x = (uint32_t)ns;
+            g.cap[rr] = (uint32_t)ncap;
+        }
+        uint32_t o = row.x + d0;
+        for (uint32_t z = p; z < end; ++z) {
+            const uint32_t x = sent[z], e = x >> 1;
+            if (!tk[e]) continue;
+            const uint32_t nbr = (x & 1u) ? src[e] : dst[e];
+            g.adj[o++] = make_uint4(nbr, __float_as_uint(w[e]), 0u, 0u);
+        }
+        g.rows[rr] = make_uint2(row.x, d0 + cnt);
+    }
+    if (mine) atomicAdd(&g.meta[1], (unsigned long long)mine);
+    list_rows(scan, h, r, t, touched, ntouched);
+    __syncthreads();
+}
+
+// block_delete over presorted entries (as block_append_sorted): each row's
+// removals of edges with tk[e] set, in entry order, by swap-with-last.
+template <int IPT>
+__device__ void block_delete_sorted(const BlockGraph& g, Scan::TempStorage& scan,
+                                    const uint32_t* srow, const uint32_t* sent,
+                                    const uint32_t* src, const uint32_t* dst, const uint8_t* tk,
+                                    uint32_t t, uint32_t span, uint32_t* __restrict__ touched,
+                                    uint32_t* __restrict__ ntouched) {
+    constexpr int kIPT = IPT, kEPT = 2 * IPT;
+    const uint32_t lim = 2 * span;
+    uint32_t mine = 0;
+#pragma unroll
+    for (int k = 0; k < kIPT; ++k) {
+        const uint32_t e = kIPT * t + k;
+        mine += (e < span && tk[e]) ? 1u : 0u;
+    }
+    bool h[kEPT];
+    uint32_t r[kEPT];
+#pragma unroll
+    for (int q = 0; q < kEPT; ++q) {
+        const uint32_t p = kEPT * t + q;
+        h[q] = false;
+        r[q] = 0;
+        if (p >= lim) continue;
+        const uint32_t rr = srow[p];
+        if (p > 0 && srow[p - 1] == rr) continue;
+        uint32_t end = p + 1;
+        bool any = false;
+        while (end < lim && srow[end] == rr) ++end;
+        for (uint32_t z = p; z < end; ++z) any = any || tk[sent[z] >> 1];
+        if (!any) continue;
+        h[q] = true;
+        r[q] = rr;
+        uint2 row = g.rows[rr];
+        for (uint32_t z = p; z < end; ++z) {
+            const uint32_t x = sent[z], e = x >> 1;
+            if (!tk[e]) continue;
+            const uint32_t nbr = (x & 1u) ? src[e] : dst[e];
+            const int sl = slot_of(g.adj, row, nbr);
+            if (sl < 0) {  // not present: the caller's lists were inconsistent
+                atomicOr((uint32_t*)(g.meta + 2), 2u);
+                continue;
+            }
+            const uint32_t last = row.y - 1u;
+            if ((uint32_t)sl != last) g.adj[row.x + sl] = g.adj[row.x + last];
+            row.y = last;
+        }
+        g.rows[rr] = row;
+    }
+    if (mine) atomicAdd(&g.meta[4], (unsigned long long)mine);
+    list_rows(scan, h, r, t, touched, ntouched);
+    __syncthreads();
+}
+
 __device__ void add_weights_seq(const BlockGraph& g, uint32_t a, uint32_t b,
                                 const uint32_t* idx, uint32_t p0, uint32_t p1,
                                 const float* __restrict__ w) {
@@ -457,7 +601,10 @@ __global__ __launch_bounds__(kThreads) void k_small_ins_pre(SmallInsArgs a) {
         sh.mode[i] = 0;
     }
     if (a.vcheck && reject_batch<IPT>(ut, vt, n, a, t)) return;
-    sort_keys64<IPT>(sh.tmp, sh.key, sh.idx, kt, t, n, a.key_bits);
+    if (a.ord.key)
+        load_order<IPT>(a.ord, sh.key, sh.idx, sh.srow, sh.sent, t, n);
+    else
+        sort_keys64<IPT>(sh.tmp, sh.key, sh.idx, kt, t, n, a.key_bits);
     // ---- classify each distinct edge, occurrences in batch order ----
 #pragma unroll
     for (int k = 0; k < kIPT; ++k) {
@@ -487,8 +634,12 @@ __global__ __launch_bounds__(kThreads) void k_small_ins_pre(SmallInsArgs a) {
     bool take[kIPT];
 #pragma unroll
     for (int k = 0; k < kIPT; ++k) take[k] = kIPT * t + k < n && sh.keep[kIPT * t + k];
-    block_append<IPT>(a.G, sh.tmp, sh.srow, sh.sent, sh.src, sh.dst, sh.w, take, t, n, a.row_bits,
-                 a.touched, a.cnt + 4);
+    if (a.ord.key)
+        block_append_sorted<IPT>(a.G, sh.tmp.scan, sh.srow, sh.sent, sh.src, sh.dst, sh.w,
+                                 sh.keep, t, n, a.touched, a.cnt + 4);
+    else
+        block_append<IPT>(a.G, sh.tmp, sh.srow, sh.sent, sh.src, sh.dst, sh.w, take, t, n,
+                          a.row_bits, a.touched, a.cnt + 4);
     // ---- weight updates (after the appends: a repeated new edge adds to it) ----
 #pragma unroll
     for (int k = 0; k < kIPT; ++k) {
@@ -534,13 +685,47 @@ __global__ __launch_bounds__(kThreads) void k_small_ins_post(SmallInsArgs a) {
         const uint32_t j = kIPT * t + k;
         live[k] = j < m;
         add[k] = live[k] && a.found[j] == -1;
-        sh.src[j] = live[k] ? a.cu[j] : 0u;
-        sh.dst[j] = live[k] ? a.cv[j] : 0u;
-        sh.w[j] = live[k] ? a.cw[j] : 0.0f;
     }
-    __syncthreads();
-    block_append<IPT>(a.H, sh.tmp, sh.srow, sh.sent, sh.src, sh.dst, sh.w, add, t, m, a.row_bits,
-                 a.touched, a.cnt + 4);
+    if (a.ord.key) {
+        // presorted entries are by batch position: the kept edges go back
+        // to their batch positions, flagged when they found no path
+        const uint32_t n = a.n;
+#pragma unroll
+        for (int k = 0; k < kIPT; ++k) sh.keep[kIPT * t + k] = 0;
+#pragma unroll
+        for (int q = 0; q < 2 * kIPT; ++q) {
+            const uint32_t p = 2 * kIPT * t + q;
+            if (p < 2 * n) {
+                sh.srow[p] = a.ord.row[p];
+                sh.sent[p] = a.ord.ent[p];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kIPT; ++k) {
+            const uint32_t j = kIPT * t + k;
+            if (!live[k]) continue;
+            const uint32_t e = a.sel[j];
+            sh.src[e] = a.cu[j];
+            sh.dst[e] = a.cv[j];
+            sh.w[e] = a.cw[j];
+            sh.keep[e] = add[k] ? 1 : 0;
+        }
+        __syncthreads();
+        block_append_sorted<IPT>(a.H, sh.tmp.scan, sh.srow, sh.sent, sh.src, sh.dst, sh.w,
+                                 sh.keep, t, n, a.touched, a.cnt + 4);
+    } else {
+#pragma unroll
+        for (int k = 0; k < kIPT; ++k) {
+            const uint32_t j = kIPT * t + k;
+            sh.src[j] = live[k] ? a.cu[j] : 0u;
+            sh.dst[j] = live[k] ? a.cv[j] : 0u;
+            sh.w[j] = live[k] ? a.cw[j] : 0.0f;
+        }
+        __syncthreads();
+        block_append<IPT>(a.H, sh.tmp, sh.srow, sh.sent, sh.src, sh.dst, sh.w, add, t, m,
+                          a.row_bits, a.touched, a.cnt + 4);
+    }
 #pragma unroll
     for (int k = 0; k < kIPT; ++k)
         if (live[k]) a.cls[a.sel[kIPT * t + k]] = add[k] ? kOutAdded : kOutFound;
@@ -580,7 +765,10 @@ __global__ __launch_bounds__(kThreads) void k_small_del_pre(SmallDelArgs a) {
         sh.fh[i] = 0;
     }
     if (a.vcheck && reject_batch<IPT>(ut, vt, n, a, t)) return;
-    sort_keys64<IPT>(sh.tmp, sh.key, sh.idx, kt, t, n, a.key_bits);
+    if (a.ord.key)
+        load_order<IPT>(a.ord, sh.key, sh.idx, sh.srow, sh.sent, t, n);
+    else
+        sort_keys64<IPT>(sh.tmp, sh.key, sh.idx, kt, t, n, a.key_bits);
     // ---- classify: the first occurrence of an edge G holds is deleted ----
 #pragma unroll
     for (int k = 0; k < kIPT; ++k) {
@@ -614,10 +802,17 @@ __global__ __launch_bounds__(kThreads) void k_small_del_pre(SmallDelArgs a) {
         dg[k] = i < n && sh.fg[i];
         dh[k] = i < n && sh.fh[i];
     }
-    block_delete<IPT>(a.G, sh.tmp, sh.srow, sh.sent, sh.src, sh.dst, dg, t, n, a.row_bits,
-                 a.touched_g, a.cnt + 4);
-    block_delete<IPT>(a.H, sh.tmp, sh.srow, sh.sent, sh.src, sh.dst, dh, t, n, a.row_bits,
-                 a.touched_h, a.cnt + 5);
+    if (a.ord.key) {
+        block_delete_sorted<IPT>(a.G, sh.tmp.scan, sh.srow, sh.sent, sh.src, sh.dst, sh.fg, t, n,
+                                 a.touched_g, a.cnt + 4);
+        block_delete_sorted<IPT>(a.H, sh.tmp.scan, sh.srow, sh.sent, sh.src, sh.dst, sh.fh, t, n,
+                                 a.touched_h, a.cnt + 5);
+    } else {
+        block_delete<IPT>(a.G, sh.tmp, sh.srow, sh.sent, sh.src, sh.dst, dg, t, n, a.row_bits,
+                          a.touched_g, a.cnt + 4);
+        block_delete<IPT>(a.H, sh.tmp, sh.srow, sh.sent, sh.src, sh.dst, dh, t, n, a.row_bits,
+                          a.touched_h, a.cnt + 5);
+    }
     // ---- compact the edges H held (batch order), as (min, max), for the walk ----
     uint32_t in[kIPT], pos[kIPT], total = 0;
 #pragma unroll
@@ -769,6 +964,57 @@ __global__ __launch_bounds__(kThreads) void k_small_del_post(SmallDelArgs a) {
     report_out(a.rep, a.cls, a.n, a.stats, t, kThreads);
 }
 
+// ------------------------------------------------------------ batch orders
+template <int IPT>
+struct OrderShared {
+    static constexpr uint32_t kItems = Shape<IPT>::kItems;
+    typename Shape<IPT>::TempU tmp;
+    unsigned long long key[kItems];
+    uint32_t idx[kItems];
+    uint32_t srow[2 * kItems];
+    uint32_t sent[2 * kItems];
+};
+
+// One workgroup per job: the batch's keys and its row entries, sorted by the
+// pre kernel's own sorts (so the orders are the ones it would compute).
+template <int IPT>
+__global__ __launch_bounds__(kThreads) void k_small_order(const SmallOrderJob* __restrict__ jobs,
+                                                          int key_bits, int row_bits) {
+    constexpr int kIPT = IPT, kEPT = 2 * IPT;
+    __shared__ OrderShared<IPT> sh;
+    const SmallOrderJob j = jobs[blockIdx.x];
+    const uint32_t t = threadIdx.x, n = j.n;
+    const uint32_t pad = row_pad(row_bits);
+    unsigned long long kt[kIPT];
+    uint32_t re[kEPT];
+#pragma unroll
+    for (int k = 0; k < kIPT; ++k) {
+        const uint32_t i = kIPT * t + k;
+        const uint32_t x = i < n ? j.u[i] : 0u, y = i < n ? j.v[i] : 0u;
+        kt[k] = i < n ? edge_key(x, y) : kPad64;
+        re[2 * k] = i < n ? x : pad;
+        re[2 * k + 1] = i < n ? y : pad;
+    }
+    sort_keys64<IPT>(sh.tmp, sh.key, sh.idx, kt, t, n, key_bits);
+#pragma unroll
+    for (int k = 0; k < kIPT; ++k) {
+        const uint32_t p = kIPT * t + k;
+        if (p < n) {
+            j.key[p] = sh.key[p];
+            j.idx[p] = sh.idx[p];
+        }
+    }
+    sort_rows<IPT>(sh.tmp, sh.srow, sh.sent, re, t, n, row_bits);
+#pragma unroll
+    for (int q = 0; q < kEPT; ++q) {
+        const uint32_t p = kEPT * t + q;
+        if (p < 2 * n) {
+            j.row[p] = sh.srow[p];
+            j.ent[p] = sh.sent[p];
+        }
+    }
+}
+
 // lpr lanes per listed row: every slot pointing to the row gets its final
 // descriptor (a lane per neighbour).  blockIdx.y selects one of two lists.
 struct RefreshList {
@@ -853,6 +1099,18 @@ hipError_t launch_small_del_post(const SmallDelArgs& a, hipStream_t s) {
         hipLaunchKernelGGL(k_small_del_post<2>, dim3(1), dim3(kThreads), 0, s, a);
     refresh({a.H.rows, a.H.adj, a.touched_x, a.cnt + 6, a.H.lpr}, nullptr,
             2 * small_del_candidates(a.n), s);
+    return hipGetLastError();
+}
+
+hipError_t launch_small_order(const SmallOrderJob* jobs, uint32_t count, uint32_t max_n,
+                              int key_bits, int row_bits, hipStream_t s) {
+    if (!count) return hipSuccess;
+    if (max_n <= (uint32_t)kThreads)
+        hipLaunchKernelGGL(k_small_order<1>, dim3(count), dim3(kThreads), 0, s, jobs, key_bits,
+                           row_bits);
+    else
+        hipLaunchKernelGGL(k_small_order<2>, dim3(count), dim3(kThreads), 0, s, jobs, key_bits,
+                           row_bits);
     return hipGetLastError();
 }
 

@@ -449,20 +449,24 @@ def test_apply_many_dev_checks_ids_first_and_runs_lists_in_turn(ctx, grid_wl):
     assert split.position == whole.position == int(off[-1])
 
 
-@pytest.mark.parametrize("seed,nodesc", [(31, False), (32, True)])
-def test_apply_many_dev_general_pipeline_matches_oracle(ctx, grid_wl, seed, nodesc):
-    """A list through dyg_stream_apply_many_dev with every batch in the general
-    pipeline: each batch's keys and row entries are sorted ahead of it on the
-    prep stream, and its row edits take the presorted entries instead of
-    sorting their own.  Outcome codes of every batch and the final G and H
-    (slot for slot) equal the oracle's, batch after batch."""
+@pytest.mark.parametrize("seed,nodesc,general", [(31, False, True), (32, True, True),
+                                                  (33, False, False), (34, True, False)])
+def test_apply_many_dev_orders_ahead_match_oracle(ctx, grid_wl, seed, nodesc, general):
+    """A list through dyg_stream_apply_many_dev, every batch in the general
+    pipeline or in the one-workgroup kernels: each batch's keys and row
+    entries are sorted ahead of it on the prep stream, and its kernels take
+    the presorted orders instead of sorting their own.  Outcome codes of every
+    batch and the final G and H (slot for slot) equal the oracle's, batch
+    after batch."""
     from dygrass_amd import Stream, stream_params
     from oracle import stream as S
 
     wl = grid_wl
     batches = mixed_batches(wl, seed, n_ins=400, n_del=250)
-    sp = stream_params(w_ins=64, w_del=64, threshold=10.0, seed=seed, max_batch=1100,
-                       max_inserts=8192, general_only=True, dense_no_desc=nodesc)
+    # a batch in the larger one-workgroup class (two updates per lane)
+    batches.insert(2, ("ins", wl.ins.u[800:2000], wl.ins.v[800:2000], wl.ins.w[800:2000]))
+    sp = stream_params(w_ins=64, w_del=64, threshold=10.0, seed=seed, max_batch=1300,
+                       max_inserts=8192, general_only=general, dense_no_desc=nodesc)
     st = Stream(ctx, wl.G.V, (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w), sp)
     orc = S.Stream(wl.G.V, (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w), thr=10.0,
                    w_ins=64, w_del=64, seed=seed)
