@@ -442,6 +442,9 @@ struct SmallOrderJob {
 // jobs: device array of `count` jobs, all of at most `max_n` updates.
 hipError_t launch_small_order(const SmallOrderJob* jobs, uint32_t count, uint32_t max_n,
                               int key_bits, int row_bits, hipStream_t s);
+// One job (j.n <= kSmallBatch) passed by value.
+hipError_t launch_small_order_one(const SmallOrderJob& j, int key_bits, int row_bits,
+                                  hipStream_t s);
 // Candidate edges the post kernel of an n-update deletion batch can hold.
 uint32_t small_del_candidates(uint32_t n);
 hipError_t launch_small_del_pre(const SmallDelArgs& a, hipStream_t s);

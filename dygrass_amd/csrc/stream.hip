@@ -400,6 +400,45 @@ __global__ void k_cand_groups(const unsigned long long* __restrict__ key,
 }
 
 // Device buffers of one stream, carved from one allocation.
+// Scratch for computing one batch's orders (BatchOrder) of up to n updates on
+// a stream of its own: the keys' and row entries' sort inputs and the
+// sorts' hipCUB scratch.
+struct OrderScratch {
+    void* mem = nullptr;
+    unsigned long long* key = nullptr;
+    uint32_t *idx = nullptr, *row = nullptr, *ent = nullptr;
+    void* temp = nullptr;
+    size_t temp_bytes = 0;
+    hipError_t alloc(uint32_t n, hipStream_t st) {
+        hipError_t e = hipMalloc(&mem, 28ull * n + 1024);
+        if (e != hipSuccess) {
+            mem = nullptr;
+            return e;
+        }
+        char* b = (char*)mem;
+        key = (unsigned long long*)b;
+        idx = (uint32_t*)(b + 8ull * n);
+        row = (uint32_t*)(b + 12ull * n);
+        ent = (uint32_t*)(b + 20ull * n);
+        size_t t1 = 0, t2 = 0;
+        e = hipcub::DeviceRadixSort::SortPairs(nullptr, t1, key, key, idx, idx, (int)n, 0, 64, st);
+        if (e == hipSuccess)
+            e = hipcub::DeviceRadixSort::SortPairs(nullptr, t2, row, row, ent, ent, (int)(2 * n), 0,
+                                                   32, st);
+        if (e == hipSuccess) e = hipMalloc(&temp, std::max(t1, t2));
+        if (e != hipSuccess) {
+            temp = nullptr;
+            return e;
+        }
+        temp_bytes = std::max(t1, t2);
+        return hipSuccess;
+    }
+    ~OrderScratch() {
+        if (mem) (void)hipFree(mem);
+        if (temp) (void)hipFree(temp);
+    }
+};
+
 struct Arena {
     char* base = nullptr;
     size_t off = 0, cap = 0;
@@ -481,11 +520,7 @@ struct dyg_stream_s {
     // batch's sort inputs and the sorts' scratch, one event per batch.
     hipStream_t prep = nullptr;
     DevBuf order;
-    void* prep_mem = nullptr;
-    unsigned long long* pk_in = nullptr;
-    uint32_t *pi_in = nullptr, *pr_in = nullptr, *pe_in = nullptr;
-    void* prep_temp = nullptr;
-    size_t prep_temp_bytes = 0;
+    OrderScratch oscr;
     std::vector<hipEvent_t> prep_ev;
     Pinned jobs_h;  // the one-workgroup batches' order jobs (SmallOrderJob), host side
     DevBuf jobs_d;  // and on the device
@@ -495,8 +530,6 @@ struct dyg_stream_s {
             (void)hipStreamDestroy(prep);
         }
         for (hipEvent_t e : prep_ev) (void)hipEventDestroy(e);
-        if (prep_mem) (void)hipFree(prep_mem);
-        if (prep_temp) (void)hipFree(prep_temp);
         for (auto& op : steps)
             for (StepGraph& g : op) {
                 if (g.exec) (void)hipGraphExecDestroy(g.exec);
@@ -1474,29 +1507,9 @@ size_t order_bytes(uint64_t n) {
 dyg_status prep_reserve(dyg_stream s, size_t bytes, size_t events, size_t jobs) {
     dyg_ctx ctx = s->ctx;
     if (!s->prep) DYG_HIP(ctx, hipStreamCreate(&s->prep));
-    if (!s->prep_mem) {
-        const size_t n = s->sp.max_batch;
-        if (hipMalloc(&s->prep_mem, 8 * n + 4 * n + 8 * n + 8 * n + 1024) != hipSuccess) {
-            s->prep_mem = nullptr;
-            set_error(ctx, "stream: device allocation failed");
-            return DYG_ERR_NOMEM;
-        }
-        char* b = (char*)s->prep_mem;
-        s->pk_in = (unsigned long long*)b;
-        s->pi_in = (uint32_t*)(b + 8 * n);
-        s->pr_in = (uint32_t*)(b + 12 * n);
-        s->pe_in = (uint32_t*)(b + 20 * n);
-        size_t t1 = 0, t2 = 0;
-        DYG_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, t1, s->pk_in, s->pk_in, s->pi_in,
-                                                        s->pi_in, (int)n, 0, 64, s->prep));
-        DYG_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, t2, s->pr_in, s->pr_in, s->pe_in,
-                                                        s->pe_in, (int)(2 * n), 0, 32, s->prep));
-        if (hipMalloc(&s->prep_temp, std::max(t1, t2)) != hipSuccess) {
-            s->prep_temp = nullptr;
-            set_error(ctx, "stream: device allocation failed");
-            return DYG_ERR_NOMEM;
-        }
-        s->prep_temp_bytes = std::max(t1, t2);
+    if (!s->oscr.mem && s->oscr.alloc(s->sp.max_batch, s->prep) != hipSuccess) {
+        set_error(ctx, "stream: device allocation failed");
+        return DYG_ERR_NOMEM;
     }
     DYG_HIP(ctx, s->order.reserve(bytes));
     if (jobs) {
@@ -1511,27 +1524,34 @@ dyg_status prep_reserve(dyg_stream s, size_t bytes, size_t events, size_t jobs) 
     return DYG_OK;
 }
 
-// Enqueues one batch's orders on the prep stream (keys, then the two stable
-// sorts) into `ord`'s arrays at `region`, and records event `ev` after them.
-dyg_status enqueue_order(dyg_stream s, const uint32_t* u, const uint32_t* v, uint32_t n,
-                         char* region, BatchOrder* ord, hipEvent_t ev) {
+// Enqueues one batch's orders on stream `st` (keys, then the two stable
+// sorts, with scratch `sc`) into the arrays of `region` (order_bytes(n)),
+// then records `ev` (when given) after them.
+dyg_status enqueue_order(dyg_stream s, OrderScratch& sc, hipStream_t st, const uint32_t* u,
+                         const uint32_t* v, uint32_t n, char* region, hipEvent_t ev) {
     dyg_ctx ctx = s->ctx;
     auto* key = (unsigned long long*)region;
     auto* idx = (uint32_t*)(region + 8ull * n);
     auto* row = (uint32_t*)(region + 12ull * n);
     auto* ent = (uint32_t*)(region + 20ull * n);
     auto* bcnt = (uint32_t*)(region + 28ull * n);
-    hipLaunchKernelGGL(k_prep_keys, dim3((n + 255) / 256), dim3(256), 0, s->prep, u, v, n,
-                       s->pk_in, s->pi_in, s->pr_in, s->pe_in, bcnt);
-    size_t tb = s->prep_temp_bytes;
-    DYG_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(s->prep_temp, tb, s->pk_in, key, s->pi_in, idx,
-                                                    (int)n, 0, key_bits(s->G->V), s->prep));
-    tb = s->prep_temp_bytes;
-    DYG_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(s->prep_temp, tb, s->pr_in, row, s->pe_in, ent,
-                                                    (int)(2 * n), 0, row_bits(s->G->V), s->prep));
-    DYG_HIP(ctx, hipEventRecord(ev, s->prep));
-    *ord = BatchOrder{key, idx, row, ent, bcnt};
+    hipLaunchKernelGGL(k_prep_keys, dim3((n + 255) / 256), dim3(256), 0, st, u, v, n, sc.key,
+                       sc.idx, sc.row, sc.ent, bcnt);
+    size_t tb = sc.temp_bytes;
+    DYG_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(sc.temp, tb, sc.key, key, sc.idx, idx, (int)n,
+                                                    0, key_bits(s->G->V), st));
+    tb = sc.temp_bytes;
+    DYG_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(sc.temp, tb, sc.row, row, sc.ent, ent,
+                                                    (int)(2 * n), 0, row_bits(s->G->V), st));
+    if (ev) DYG_HIP(ctx, hipEventRecord(ev, st));
     return DYG_OK;
+}
+
+// The arrays of a batch's orders at `region` (order_bytes(n)).
+BatchOrder order_at(char* region, uint64_t n) {
+    return BatchOrder{(unsigned long long*)region, (uint32_t*)(region + 8 * n),
+                      (uint32_t*)(region + 12 * n), (uint32_t*)(region + 20 * n),
+                      (uint32_t*)(region + 28 * n)};
 }
 
 // A list of batches already in device memory, enqueued back to back: the
@@ -1606,14 +1626,13 @@ dyg_status stream_apply_list(dyg_stream s, uint32_t nb, const int* op, const uin
         for (uint32_t b = 0; b < nb; ++b) {
             const uint64_t o = off[b], n = off[b + 1] - o;
             if (!n) continue;
-            auto* key = (unsigned long long*)region;
-            auto* idx = (uint32_t*)(region + 8 * n);
-            auto* row = (uint32_t*)(region + 12 * n);
-            auto* ent = (uint32_t*)(region + 20 * n);
-            ords[b] = BatchOrder{key, idx, row, ent, (uint32_t*)(region + 28 * n)};
+            ords[b] = order_at(region, n);
             region += order_bytes(n);
             if (general_batch(s, n)) continue;
-            const SmallOrderJob j{u + o, v + o, key, idx, row, ent, (uint32_t)n, 0u};
+            const BatchOrder& q = ords[b];
+            const SmallOrderJob j{u + o, v + o, const_cast<unsigned long long*>(q.key),
+                                  const_cast<uint32_t*>(q.idx), const_cast<uint32_t*>(q.row),
+                                  const_cast<uint32_t*>(q.ent), (uint32_t)n, 0u};
             (n <= (uint64_t)kSmallBatch / 2 ? j1 : j2).push_back(j);
         }
         auto* jobs = (SmallOrderJob*)s->jobs_h.p;
@@ -1624,9 +1643,8 @@ dyg_status stream_apply_list(dyg_stream s, uint32_t nb, const int* op, const uin
             const uint64_t o = off[b], n = off[b + 1] - o;
             if (!n) continue;
             if (general_batch(s, n)) {
-                BatchOrder q{};
-                r = enqueue_order(s, u + o, v + o, (uint32_t)n, (char*)ords[b].key, &q,
-                                  s->prep_ev[k]);
+                r = enqueue_order(s, s->oscr, s->prep, u + o, v + o, (uint32_t)n,
+                                  (char*)ords[b].key, s->prep_ev[k]);
                 if (r != DYG_OK) {
                     (void)hipStreamSynchronize(s->prep);
                     return r;
@@ -1880,6 +1898,7 @@ struct ServeSlot {
     char* hdev = nullptr;
     hipEvent_t e_in = nullptr, e_done = nullptr;
     hipEvent_t ew0 = nullptr, ew1 = nullptr;  // the request's walk kernel, start and end
+    char* omem = nullptr;  // the request's orders (BatchOrder), computed at submit
 };
 
 struct ServeReq {
@@ -1894,6 +1913,7 @@ struct ServeReq {
     uint64_t hops = 0, redirects = 0;
     double walk_ms = 0;
     bool copied = false;  // input copied to the slot's device buffer (else read in place)
+    bool ordered = false; // its orders computed on the copy stream (e_in follows them)
     // device inputs (dyg_serve_submit_dev): the caller's arrays, read in place
     const uint32_t* du = nullptr;
     const uint32_t* dv = nullptr;
@@ -1919,6 +1939,7 @@ struct dyg_serve_s {
     unsigned long long* chk = nullptr;
     unsigned long long* chk_dev = nullptr;
     hipEvent_t e_chk = nullptr;
+    OrderScratch oscr;  // the copy stream's scratch for requests' orders
     std::deque<ServeReq*> queue, inflight;
     std::unordered_map<uint64_t, ServeReq*> reqs;
     std::thread worker, completer;
@@ -1998,8 +2019,10 @@ void serve_worker(dyg_serve_s* v) {
             st = v->failed;
         }
         if (st == DYG_OK) st = ensure_headroom(v, *r);
-        if (st == DYG_OK && r->copied && hipStreamWaitEvent(cs, sl.e_in, 0) != hipSuccess)
+        if (st == DYG_OK && (r->copied || r->ordered) &&
+            hipStreamWaitEvent(cs, sl.e_in, 0) != hipSuccess)
             st = DYG_ERR_HIP;
+        const BatchOrder ord = order_at(sl.omem, r->n);
         dyg_stream_stats rs{};
         const uint32_t n = r->n;
         const PinLayout dp = pin_layout(sl.hdev, n);
@@ -2019,10 +2042,13 @@ void serve_worker(dyg_serve_s* v) {
         // the completion thread, which frees the slot only after that)
         s->wev0 = sl.ew0;
         s->wev1 = sl.ew1;
+        const BatchOrder* po = r->ordered ? &ord : nullptr;
         if (st == DYG_OK)
             st = r->op == DYG_STREAM_INSERT
-                     ? apply_insert(s, du, dv, dw, n, sl.dcls, sl.dstats, true, &rs, rep, nullptr)
-                     : apply_delete(s, du, dv, n, sl.dcls, sl.dstats, true, &rs, rep, nullptr);
+                     ? apply_insert(s, du, dv, dw, n, sl.dcls, sl.dstats, true, &rs, rep, nullptr,
+                                    false, 0, po)
+                     : apply_delete(s, du, dv, n, sl.dcls, sl.dstats, true, &rs, rep, nullptr,
+                                    false, 0, po);
         s->wev0 = s->ctx->ev0;
         s->wev1 = s->ctx->ev1;
         if (st == DYG_OK) {
@@ -2111,9 +2137,11 @@ extern "C" dyg_status dyg_serve_open(dyg_stream s, uint32_t slots, uint32_t max_
              hipHostGetDevicePointer(&hd, sl.host, 0) == hipSuccess &&
              hipEventCreateWithFlags(&sl.e_in, hipEventDisableTiming) == hipSuccess &&
              hipEventCreateWithFlags(&sl.e_done, hipEventDisableTiming) == hipSuccess &&
-             hipEventCreate(&sl.ew0) == hipSuccess && hipEventCreate(&sl.ew1) == hipSuccess;
+             hipEventCreate(&sl.ew0) == hipSuccess && hipEventCreate(&sl.ew1) == hipSuccess &&
+             hipMalloc(&sl.omem, order_bytes(max_request)) == hipSuccess;
         sl.hdev = (char*)hd;
     }
+    if (ok) ok = v->oscr.alloc(max_request, v->copy) == hipSuccess;
     if (ok) ok = hipStreamSynchronize(ctx->stream) == hipSuccess;
     size_t mfree = 0, mtotal = 0;
     if (ok) ok = hipMemGetInfo(&mfree, &mtotal) == hipSuccess;
@@ -2232,9 +2260,27 @@ dyg_status serve_submit(dyg_serve v, int op, const uint32_t* u, const uint32_t* 
                 copy_ok = false;
             } else {
                 copy_ok = hipMemcpyAsync(sl.din, hp.u, n * (ins ? 12ull : 8ull),
-                                         hipMemcpyHostToDevice, v->copy) == hipSuccess &&
-                          hipEventRecord(sl.e_in, v->copy) == hipSuccess;
+                                         hipMemcpyHostToDevice, v->copy) == hipSuccess;
             }
+        }
+        if (copy_ok) {
+            // the request's orders, on the copy stream behind its input: they
+            // overlap the requests ahead of it on the device
+            const uint32_t* ou = r->copied ? sl.din : dev ? u : pin_layout(sl.hdev, n).u;
+            const uint32_t* ov = r->copied ? sl.din + n : dev ? w_v : pin_layout(sl.hdev, n).v;
+            if (general_batch(v->s, n)) {
+                copy_ok = enqueue_order(v->s, v->oscr, v->copy, ou, ov, n, sl.omem, nullptr) ==
+                          DYG_OK;
+            } else {
+                const BatchOrder q = order_at(sl.omem, n);
+                const SmallOrderJob j{ou, ov, const_cast<unsigned long long*>(q.key),
+                                      const_cast<uint32_t*>(q.idx), const_cast<uint32_t*>(q.row),
+                                      const_cast<uint32_t*>(q.ent), n, 0u};
+                copy_ok = launch_small_order_one(j, key_bits(v->s->G->V), row_bits(v->s->G->V),
+                                                 v->copy) == hipSuccess;
+            }
+            copy_ok = copy_ok && hipEventRecord(sl.e_in, v->copy) == hipSuccess;
+            r->ordered = copy_ok;
         }
         if (!copy_ok) {
             // the request never reaches the worker; the device's state is no
@@ -2326,7 +2372,7 @@ extern "C" void dyg_serve_close(dyg_serve v) {
     }
     if (v->copy) (void)hipStreamSynchronize(v->copy);
     for (ServeSlot& sl : v->slots) {
-        void* ps[] = {sl.din, sl.dcls, sl.dstats};
+        void* ps[] = {sl.din, sl.dcls, sl.dstats, sl.omem};
         for (void* p : ps)
             if (p) (void)hipFree(p);
         if (sl.host) (void)hipHostFree(sl.host);

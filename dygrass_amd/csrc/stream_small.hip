@@ -975,14 +975,12 @@ struct OrderShared {
     uint32_t sent[2 * kItems];
 };
 
-// One workgroup per job: the batch's keys and its row entries, sorted by the
-// pre kernel's own sorts (so the orders are the ones it would compute).
+// A job's orders: the batch's keys and its row entries, sorted by the pre
+// kernel's own sorts (so the orders are the ones it would compute).
 template <int IPT>
-__global__ __launch_bounds__(kThreads) void k_small_order(const SmallOrderJob* __restrict__ jobs,
-                                                          int key_bits, int row_bits) {
+__device__ void small_order(OrderShared<IPT>& sh, const SmallOrderJob& j, int key_bits,
+                            int row_bits) {
     constexpr int kIPT = IPT, kEPT = 2 * IPT;
-    __shared__ OrderShared<IPT> sh;
-    const SmallOrderJob j = jobs[blockIdx.x];
     const uint32_t t = threadIdx.x, n = j.n;
     const uint32_t pad = row_pad(row_bits);
     unsigned long long kt[kIPT];
@@ -1013,6 +1011,23 @@ __global__ __launch_bounds__(kThreads) void k_small_order(const SmallOrderJob* _
             j.ent[p] = sh.sent[p];
         }
     }
+}
+
+// One workgroup per job of a device array (a list's one-workgroup batches).
+template <int IPT>
+__global__ __launch_bounds__(kThreads) void k_small_order(const SmallOrderJob* __restrict__ jobs,
+                                                          int key_bits, int row_bits) {
+    __shared__ OrderShared<IPT> sh;
+    const SmallOrderJob j = jobs[blockIdx.x];
+    small_order<IPT>(sh, j, key_bits, row_bits);
+}
+
+// One job given by value (a served request).
+template <int IPT>
+__global__ __launch_bounds__(kThreads) void k_small_order_one(SmallOrderJob j, int key_bits,
+                                                              int row_bits) {
+    __shared__ OrderShared<IPT> sh;
+    small_order<IPT>(sh, j, key_bits, row_bits);
 }
 
 // lpr lanes per listed row: every slot pointing to the row gets its final
@@ -1110,6 +1125,17 @@ hipError_t launch_small_order(const SmallOrderJob* jobs, uint32_t count, uint32_
                            row_bits);
     else
         hipLaunchKernelGGL(k_small_order<2>, dim3(count), dim3(kThreads), 0, s, jobs, key_bits,
+                           row_bits);
+    return hipGetLastError();
+}
+
+hipError_t launch_small_order_one(const SmallOrderJob& j, int key_bits, int row_bits,
+                                  hipStream_t s) {
+    if (j.n <= (uint32_t)kThreads)
+        hipLaunchKernelGGL(k_small_order_one<1>, dim3(1), dim3(kThreads), 0, s, j, key_bits,
+                           row_bits);
+    else
+        hipLaunchKernelGGL(k_small_order_one<2>, dim3(1), dim3(kThreads), 0, s, j, key_bits,
                            row_bits);
     return hipGetLastError();
 }
