@@ -68,61 +68,135 @@ __device__ __forceinline__ bool takes(const EditTake& t, uint32_t i) {
     return t.keep ? (t.keep[i] && t.found[t.jmap[i]] == -1) : t.flag[i] == t.match;
 }
 
-// Presorted edits, first pass: per block of 256 sorted entries, how many
-// this edit takes (tcnt[block]); the batch's entry count and its edge count
-// (one taken entry of even index per edge) are added up, and the other
-// parity's entry counter is cleared for the next batch, as k_keys does.
-__global__ __launch_bounds__(256) void k_take_count(
-    const uint32_t* __restrict__ ent, uint32_t n2, EditTake tk, uint32_t* __restrict__ tcnt,
-    uint32_t* __restrict__ count, uint32_t* __restrict__ count_next,
-    unsigned long long* __restrict__ edges_total) {
-    const uint32_t p = blockIdx.x * 256u + threadIdx.x;
-    if (p == 0) *count_next = 0;
-    uint32_t x = 0;
-    bool tk_p = false;
-    if (p < n2) {
-        x = ent[p];
-        tk_p = takes(tk, x >> 1);
-    }
-    const uint32_t c = (uint32_t)__syncthreads_count(tk_p);
-    const uint32_t ce = (uint32_t)__syncthreads_count(tk_p && !(x & 1u));
-    if (threadIdx.x == 0) {
-        tcnt[blockIdx.x] = c;
-        if (c) atomicAdd(count, c);
-        if (ce) atomicAdd(edges_total, (unsigned long long)ce);
-    }
+// Presorted edits (BatchOrder row entries, all 2n of them sorted by row):
+// the thread of each row's first entry does the row's whole edit -- the
+// taken entries of its run, in order -- so no compaction, head or count
+// kernels are needed; it marks its position in `mark` for the descriptor
+// refresh (every position is written: 1 at a touched row's first entry).
+// Edges edited are counted once, at their even entry, one atomic per block.
+constexpr uint32_t kTakeBlock = 1024;
+
+__device__ __forceinline__ uint32_t run_end(const uint32_t* __restrict__ row, uint32_t p,
+                                            uint32_t n2) {
+    const uint32_t r = row[p];
+    uint32_t e = p + 1;
+    while (e < n2 && row[e] == r) ++e;
+    return e;
 }
 
-// Second pass: the taken entries, in sorted order, at their compacted
-// positions (the taken counts of earlier blocks plus a block scan): the keys
-// and values the radix sort of only these entries would give.
-__global__ __launch_bounds__(256) void k_take_compact(
+// Appends (k_alloc + k_write of the sorted path): relocation to the arena
+// tail when the row is full, order kept; the neighbour's descriptor is
+// written by the refresh of that neighbour.
+__global__ __launch_bounds__(kTakeBlock) void k_take_append(
+    uint2* __restrict__ rows, uint4* __restrict__ adj, uint32_t* __restrict__ cap,
     const uint32_t* __restrict__ row, const uint32_t* __restrict__ ent, uint32_t n2, EditTake tk,
-    const uint32_t* __restrict__ tcnt, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
-    using BScan = hipcub::BlockScan<uint32_t, 256>;
-    using BReduce = hipcub::BlockReduce<uint32_t, 256>;
-    __shared__ union {
-        typename BScan::TempStorage scan;
-        typename BReduce::TempStorage reduce;
-    } tmp;
-    __shared__ uint32_t s_prefix;
-    const uint32_t b = blockIdx.x, t = threadIdx.x, p = b * 256u + t;
-    uint32_t part = 0;
-    for (uint32_t q = t; q < b; q += 256u) part += tcnt[q];
-    const uint32_t prefix = BReduce(tmp.reduce).Sum(part);
-    if (t == 0) s_prefix = prefix;
-    __syncthreads();
-    uint32_t x = 0;
-    bool k = false;
+    const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
+    const float* __restrict__ w, uint8_t* __restrict__ mark, unsigned long long* __restrict__ used,
+    unsigned long long cap_slots, uint32_t slack, uint32_t* __restrict__ err,
+    unsigned long long* __restrict__ edges_total) {
+    const uint32_t p = blockIdx.x * kTakeBlock + threadIdx.x;
+    bool even_taken = false, touched = false;
     if (p < n2) {
-        x = ent[p];
-        k = takes(tk, x >> 1);
+        const uint32_t x = ent[p];
+        even_taken = !(x & 1u) && takes(tk, x >> 1);
+        const uint32_t r = row[p];
+        if (p == 0 || row[p - 1] != r) {
+            const uint32_t end = run_end(row, p, n2);
+            uint32_t cnt = 0;
+            for (uint32_t z = p; z < end; ++z) cnt += takes(tk, ent[z] >> 1) ? 1u : 0u;
+            if (cnt) {
+                touched = true;
+                uint2 rw = rows[r];
+                const uint32_t d0 = rw.y, c = cap[r];
+                bool ok = true;
+                if ((unsigned long long)d0 + cnt > c) {
+                    unsigned long long ncap = 2ull * c;
+                    ncap = max(ncap, (unsigned long long)d0 + cnt + slack);
+                    ncap = max(ncap, 4ull);
+                    const unsigned long long ns = atomicAdd(used, ncap);
+                    if (ns + ncap > cap_slots) {
+                        atomicOr(err, 1u);
+                        ok = false;
+                    } else {
+                        copy_slots(adj + ns, adj + rw.x, d0);
+                        rw.x = (uint32_t)ns;
+                        cap[r] = (uint32_t)ncap;
+                    }
+                }
+                if (ok) {
+                    uint32_t o = rw.x + d0;
+                    for (uint32_t z = p; z < end; ++z) {
+                        const uint32_t y = ent[z], k = y >> 1;
+                        if (!takes(tk, k)) continue;
+                        const uint32_t nbr = (y & 1u) ? src[k] : dst[k];
+                        adj[o++] = make_uint4(nbr, __float_as_uint(w[k]), 0u, 0u);
+                    }
+                    rows[r] = make_uint2(rw.x, d0 + cnt);
+                } else {
+                    touched = false;
+                }
+            }
+        }
+        mark[p] = touched ? 1 : 0;
     }
-    uint32_t pos = 0, total = 0;
-    BScan(tmp.scan).ExclusiveSum(k ? 1u : 0u, pos, total);
-    if (k) {
-        keys[s_prefix + pos] = row[p];
-        vals[s_prefix + pos] = x;
+    const uint32_t c = (uint32_t)__syncthreads_count(even_taken);
+    if (threadIdx.x == 0 && c) atomicAdd(edges_total, (unsigned long long)c);
+}
+
+// Deletions (k_del_rows of the sorted path): the run's taken entries in
+// order, each by swap-with-last.
+__global__ __launch_bounds__(kTakeBlock) void k_take_delete(
+    uint2* __restrict__ rows, uint4* __restrict__ adj, const uint32_t* __restrict__ row,
+    const uint32_t* __restrict__ ent, uint32_t n2, EditTake tk, const uint32_t* __restrict__ src,
+    const uint32_t* __restrict__ dst, uint8_t* __restrict__ mark, uint32_t* __restrict__ err,
+    unsigned long long* __restrict__ edges_total) {
+    const uint32_t p = blockIdx.x * kTakeBlock + threadIdx.x;
+    bool even_taken = false, touched = false;
+    if (p < n2) {
+        const uint32_t x = ent[p];
+        even_taken = !(x & 1u) && takes(tk, x >> 1);
+        const uint32_t r = row[p];
+        if (p == 0 || row[p - 1] != r) {
+            const uint32_t end = run_end(row, p, n2);
+            uint2 rw = rows[r];
+            for (uint32_t z = p; z < end; ++z) {
+                const uint32_t y = ent[z], k = y >> 1;
+                if (!takes(tk, k)) continue;
+                touched = true;
+                const uint32_t nbr = (y & 1u) ? src[k] : dst[k];
+                const int sl = slot_of(adj, rw, nbr);
+                if (sl < 0) {  // not present: the caller's lists were inconsistent
+                    atomicOr(err, 2u);
+                    continue;
+                }
+                const uint32_t last = rw.y - 1u;
+                if ((uint32_t)sl != last) adj[rw.x + sl] = adj[rw.x + last];
+                rw.y = last;
+            }
+            if (touched) rows[r] = rw;
+        }
+        mark[p] = touched ? 1 : 0;
+    }
+    const uint32_t c = (uint32_t)__syncthreads_count(even_taken);
+    if (threadIdx.x == 0 && c) atomicAdd(edges_total, (unsigned long long)c);
+}
+
+// The descriptor refresh of the sorted path: LPR lanes per marked entry (a
+// touched row's first entry), as k_refresh.
+template <uint32_t LPR>
+__global__ __launch_bounds__(256) void k_refresh_marked(const uint2* __restrict__ rows,
+                                                        uint4* __restrict__ adj,
+                                                        const uint32_t* __restrict__ row,
+                                                        const uint8_t* __restrict__ mark,
+                                                        uint32_t n2) {
+    const uint32_t p = (blockIdx.x * 256u + threadIdx.x) / LPR;
+    const uint32_t lane = threadIdx.x & (LPR - 1u);
+    if (p >= n2 || !mark[p]) return;
+    const uint32_t r = row[p];
+    const uint2 rr = rows[r];
+    for (uint32_t q = lane; q < rr.y; q += LPR) {
+        const uint32_t x = adj[rr.x + q].x;
+        refresh_pointers(adj, rows[x], r, rr);
     }
 }
 
@@ -262,6 +336,23 @@ void launch_refresh(dyg_graph g, const uint32_t* keys, const uint32_t* count, ui
 #undef DYG_REFRESH
 }
 
+void launch_refresh_marked(dyg_graph g, const uint32_t* row, uint32_t n2, hipStream_t s) {
+    if (g->nodesc) return;
+    const uint32_t lpr = refresh_lanes(g);
+    const uint32_t blocks = (uint32_t)(((uint64_t)n2 * lpr + 255) / 256);
+    if (blocks == 0) return;
+#define DYG_REFRESH_M(L)                                                                          \
+    case L:                                                                                       \
+        hipLaunchKernelGGL(k_refresh_marked<L>, dim3(blocks), dim3(256), 0, s, g->d_rows,         \
+                           g->d_adj, row, g->dev.mark, n2);                                       \
+        break;
+    switch (lpr) {
+        DYG_REFRESH_M(4) DYG_REFRESH_M(8) DYG_REFRESH_M(16) DYG_REFRESH_M(32)
+        default: DYG_REFRESH_M(64)
+    }
+#undef DYG_REFRESH_M
+}
+
 }  // namespace
 
 // Device state of a graph in device-edit mode (see internal.h).
@@ -292,7 +383,7 @@ dyg_status device_edits_prepare(dyg_graph g, uint32_t max_batch, uint64_t max_in
                   hipMalloc(&d.vals_in, n2 * 4) == hipSuccess &&
                   hipMalloc(&d.vals_out, n2 * 4) == hipSuccess &&
                   hipMalloc(&d.meta, 64) == hipSuccess &&
-                  hipMalloc(&d.tcnt, (n2 + 255) / 256 * 4) == hipSuccess;
+                  hipMalloc(&d.mark, n2) == hipSuccess;
         if (!ok) {
             d.release();
             set_error(ctx, "device edits: allocation failed");
@@ -374,32 +465,6 @@ dyg_status edit_keys(dyg_graph g, const uint32_t* src, const uint32_t* dst, cons
     return DYG_OK;
 }
 
-// edit_keys for a batch whose 2n entries are presorted: the taken ones are
-// compacted (two passes) instead of sorted.
-dyg_status edit_keys_sorted(dyg_graph g, const uint32_t* row, const uint32_t* ent, uint32_t n,
-                            const EditTake& tk, bool insert, uint32_t** count_out) {
-    dyg_ctx ctx = g->ctx;
-    DevEdits& d = g->dev;
-    hipStream_t s = ctx->stream;
-    if (n > d.max_batch) {
-        set_error(ctx, "device edits: batch larger than reserved");
-        return DYG_ERR_ARG;
-    }
-    uint32_t* counts = (uint32_t*)(d.meta + 3);
-    uint32_t* count = counts + (d.seq & 1u);
-    uint32_t* count_next = counts + ((d.seq + 1u) & 1u);
-    ++d.seq;
-    const uint32_t n2 = 2 * n, blocks = (n2 + 255) / 256;
-    hipLaunchKernelGGL(k_take_count, dim3(blocks), dim3(256), 0, s, ent, n2, tk, d.tcnt, count,
-                       count_next, insert ? d.meta + 1 : d.meta + 4);
-    hipLaunchKernelGGL(k_take_compact, dim3(blocks), dim3(256), 0, s, row, ent, n2, tk, d.tcnt,
-                       d.keys_out, d.vals_out);
-    hipLaunchKernelGGL(k_heads, dim3(blocks), dim3(256), 0, s, d.keys_out, count, d.segstart,
-                       d.segcnt);
-    DYG_HIP(ctx, hipGetLastError());
-    *count_out = count;
-    return DYG_OK;
-}
 }  // namespace
 
 dyg_status device_deletes_batch(dyg_graph g, const uint32_t* src, const uint32_t* dst,
@@ -449,17 +514,17 @@ dyg_status device_edits_sorted(dyg_graph g, const uint32_t* src, const uint32_t*
     if (n == 0) return DYG_OK;
     dyg_ctx ctx = g->ctx;
     DevEdits& d = g->dev;
+    if (n > d.max_batch) {
+        set_error(ctx, "device edits: batch larger than reserved");
+        return DYG_ERR_ARG;
+    }
     hipStream_t s = ctx->stream;
-    uint32_t* count = nullptr;
-    dyg_status st = edit_keys_sorted(g, row, ent, n, tk, true, &count);
-    if (st != DYG_OK) return st;
-    const uint32_t blocks = (2 * n + 255) / 256;
-    hipLaunchKernelGGL(k_alloc, dim3(blocks), dim3(256), 0, s, g->d_rows, g->d_adj, d.cap,
-                       d.keys_out, count, d.segstart, d.segcnt, d.rowbase, d.meta,
-                       (unsigned long long)g->cap_slots, g->slack, (uint32_t*)(d.meta + 2));
-    hipLaunchKernelGGL(k_write, dim3(blocks), dim3(256), 0, s, g->d_adj, d.keys_out, d.vals_out,
-                       count, d.segstart, d.segcnt, d.rowbase, src, dst, w);
-    launch_refresh(g, d.keys_out, count, 2 * n, s);
+    const uint32_t n2 = 2 * n;
+    hipLaunchKernelGGL(k_take_append, dim3((n2 + kTakeBlock - 1) / kTakeBlock), dim3(kTakeBlock),
+                       0, s, g->d_rows, g->d_adj, d.cap, row, ent, n2, tk, src, dst, w, d.mark,
+                       d.meta, (unsigned long long)g->cap_slots, g->slack,
+                       (uint32_t*)(d.meta + 2), d.meta + 1);
+    launch_refresh_marked(g, row, n2, s);
     DYG_HIP(ctx, hipGetLastError());
     return DYG_OK;
 }
@@ -470,15 +535,16 @@ dyg_status device_deletes_sorted(dyg_graph g, const uint32_t* src, const uint32_
     if (n == 0) return DYG_OK;
     dyg_ctx ctx = g->ctx;
     DevEdits& d = g->dev;
+    if (n > d.max_batch) {
+        set_error(ctx, "device edits: batch larger than reserved");
+        return DYG_ERR_ARG;
+    }
     hipStream_t s = ctx->stream;
-    uint32_t* count = nullptr;
-    dyg_status st = edit_keys_sorted(g, row, ent, n, tk, false, &count);
-    if (st != DYG_OK) return st;
-    const uint32_t blocks = (2 * n + 255) / 256;
-    hipLaunchKernelGGL(k_del_rows, dim3(blocks), dim3(256), 0, s, g->d_rows, g->d_adj, d.keys_out,
-                       d.vals_out, count, d.segstart, d.segcnt, src, dst,
-                       (uint32_t*)(d.meta + 2));
-    launch_refresh(g, d.keys_out, count, 2 * n, s);
+    const uint32_t n2 = 2 * n;
+    hipLaunchKernelGGL(k_take_delete, dim3((n2 + kTakeBlock - 1) / kTakeBlock), dim3(kTakeBlock),
+                       0, s, g->d_rows, g->d_adj, row, ent, n2, tk, src, dst, d.mark,
+                       (uint32_t*)(d.meta + 2), d.meta + 4);
+    launch_refresh_marked(g, row, n2, s);
     DYG_HIP(ctx, hipGetLastError());
     return DYG_OK;
 }
@@ -511,10 +577,11 @@ dyg_status device_edits_end(dyg_graph g, uint64_t* added_out) {
 
 void DevEdits::release() {
     void* ps[] = {cap,  segstart, segcnt, rowbase, keys_in, keys_out,
-                  vals_in, vals_out, meta, tcnt, temp};
+                  vals_in, vals_out, meta, mark, temp};
     for (void* p : ps)
         if (p) (void)hipFree(p);
-    cap = segstart = segcnt = rowbase = keys_in = keys_out = vals_in = vals_out = tcnt = nullptr;
+    cap = segstart = segcnt = rowbase = keys_in = keys_out = vals_in = vals_out = nullptr;
+    mark = nullptr;
     meta = nullptr;
     temp = nullptr;
     temp_bytes = 0;

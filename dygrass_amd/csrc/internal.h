@@ -75,7 +75,7 @@ struct DevEdits {
     uint64_t base_nslots = 0;  // live slots at device_edits_begin
     uint32_t *keys_in = nullptr, *keys_out = nullptr, *vals_in = nullptr, *vals_out = nullptr;
     unsigned long long* meta = nullptr;  // used slots, added edges, error, counts
-    uint32_t* tcnt = nullptr;  // [blocks of 256 entries] taken counts of presorted edits
+    uint8_t* mark = nullptr;   // [2 * max_batch] touched rows of a presorted edit (by position)
     void* temp = nullptr;
     size_t temp_bytes = 0;
     void release();
@@ -175,10 +175,11 @@ struct EditTake {
     const int32_t* found;
 };
 // device_edits_batch / device_deletes_batch over a batch whose entries are
-// presorted (BatchOrder::row / ent): the taken entries, compacted in that
-// order, are the list the sort of only those entries gives, so the results
-// are the same with no sort in the batch's own pipeline.  src, dst, w are
-// the batch's arrays (indexed by batch position).
+// presorted (BatchOrder::row / ent): each row's run of entries holds its
+// taken entries in the order the sort of only those entries gives, so one
+// thread per row edits the row as the sorted pipeline does, with no sort in
+// the batch's own pipeline.  src, dst, w are the batch's arrays (indexed by
+// batch position).
 dyg_status device_edits_sorted(dyg_graph g, const uint32_t* src, const uint32_t* dst,
                                const float* w, const uint32_t* row, const uint32_t* ent,
                                uint32_t n, const EditTake& tk);

@@ -1604,6 +1604,7 @@ dyg_status stream_apply_list(dyg_stream s, uint32_t nb, const int* op, const uin
     // first of them
     std::vector<BatchOrder> ords(nb);
     std::vector<int> ord_ev(nb, -1);
+    std::vector<uint32_t> gens;  // the larger batches, in list order
     size_t obytes = 0, ngen = 0, nsmall = 0;
     for (uint32_t b = 0; b < nb; ++b) {
         const uint64_t n = off[b + 1] - off[b];
@@ -1638,34 +1639,44 @@ dyg_status stream_apply_list(dyg_stream s, uint32_t nb, const int* op, const uin
         auto* jobs = (SmallOrderJob*)s->jobs_h.p;
         std::copy(j1.begin(), j1.end(), jobs);
         std::copy(j2.begin(), j2.end(), jobs + j1.size());
-        int k = 0, small_ev = -1;
-        for (uint32_t b = 0; b < nb; ++b) {
-            const uint64_t o = off[b], n = off[b + 1] - o;
-            if (!n) continue;
-            if (general_batch(s, n)) {
-                r = enqueue_order(s, s->oscr, s->prep, u + o, v + o, (uint32_t)n,
-                                  (char*)ords[b].key, s->prep_ev[k]);
-                if (r != DYG_OK) {
-                    (void)hipStreamSynchronize(s->prep);
-                    return r;
-                }
-                ord_ev[b] = k++;
-                continue;
-            }
-            if (small_ev < 0) {
-                const uint32_t c1 = (uint32_t)j1.size(), n2 = (uint32_t)j2.size();
-                DYG_HIP(ctx, hipMemcpyAsync(s->jobs_d.p, s->jobs_h.p, nsmall * sizeof(SmallOrderJob),
-                                            hipMemcpyHostToDevice, s->prep));
-                auto* dj = (const SmallOrderJob*)s->jobs_d.p;
-                const int kb = key_bits(s->G->V), rb = row_bits(s->G->V);
-                DYG_HIP(ctx, launch_small_order(dj, c1, kSmallBatch / 2, kb, rb, s->prep));
-                DYG_HIP(ctx, launch_small_order(dj + c1, n2, kSmallBatch, kb, rb, s->prep));
-                DYG_HIP(ctx, hipEventRecord(s->prep_ev[k], s->prep));
-                small_ev = k++;
-            }
-            ord_ev[b] = small_ev;
+        if (nsmall) {  // first: the one launch per size class for every small batch
+            const uint32_t c1 = (uint32_t)j1.size(), c2 = (uint32_t)j2.size();
+            DYG_HIP(ctx, hipMemcpyAsync(s->jobs_d.p, s->jobs_h.p, nsmall * sizeof(SmallOrderJob),
+                                        hipMemcpyHostToDevice, s->prep));
+            auto* dj = (const SmallOrderJob*)s->jobs_d.p;
+            const int kb = key_bits(s->G->V), rb = row_bits(s->G->V);
+            DYG_HIP(ctx, launch_small_order(dj, c1, kSmallBatch / 2, kb, rb, s->prep));
+            DYG_HIP(ctx, launch_small_order(dj + c1, c2, kSmallBatch, kb, rb, s->prep));
+            DYG_HIP(ctx, hipEventRecord(s->prep_ev[ngen], s->prep));
+            for (uint32_t b = 0; b < nb; ++b)
+                if (off[b + 1] > off[b] && !general_batch(s, off[b + 1] - off[b]))
+                    ord_ev[b] = (int)ngen;
         }
+        for (uint32_t b = 0, k = 0; b < nb; ++b)
+            if (off[b + 1] > off[b] && general_batch(s, off[b + 1] - off[b])) {
+                gens.push_back(b);
+                ord_ev[b] = (int)k++;
+            }
     }
+    // a larger batch's orders are enqueued a few larger batches ahead of its
+    // own kernels, so the host's enqueueing of orders overlaps the device's
+    // work instead of holding back the first batch
+    constexpr size_t kOrderAhead = 3;
+    size_t next_gen = 0;
+    auto orders_ahead = [&](uint32_t b) -> dyg_status {
+        while (next_gen < gens.size() &&
+               (gens[next_gen] <= b || next_gen < kOrderAhead ||
+                gens[next_gen - kOrderAhead] <= b)) {
+            const uint32_t g = gens[next_gen];
+            const uint64_t o = off[g];
+            dyg_status q = enqueue_order(s, s->oscr, s->prep, u + o, v + o,
+                                         (uint32_t)(off[g + 1] - o), (char*)ords[g].key,
+                                         s->prep_ev[ord_ev[g]]);
+            if (q != DYG_OK) return q;
+            ++next_gen;
+        }
+        return DYG_OK;
+    };
     // report regions: kListWords words per batch, then every outcome code
     const size_t words_bytes = (size_t)nb * kListWords * 8;
     DYG_HIP(ctx, ctx->report.reserve(words_bytes + total * 4 + 256));
@@ -1751,6 +1762,7 @@ dyg_status stream_apply_list(dyg_stream s, uint32_t nb, const int* op, const uin
             fail = hipEventRecord(ev[0], str) == hipSuccess ? DYG_OK : DYG_ERR_HIP;
             started = true;
         }
+        if (fail == DYG_OK) fail = orders_ahead(b);
         const BatchOrder* ord = ord_ev[b] >= 0 ? &ords[b] : nullptr;
         if (fail == DYG_OK && ord && ord_ev[b] != waited) {
             if (hipStreamWaitEvent(str, s->prep_ev[ord_ev[b]], 0) != hipSuccess) fail = DYG_ERR_HIP;
