@@ -34,6 +34,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -1306,6 +1307,11 @@ dyg_status plain_delete(dyg_stream s, const uint32_t* u, const uint32_t* v, uint
 
 }  // namespace
 
+namespace {
+size_t order_bytes(uint64_t n);
+dyg_status prep_reserve(dyg_stream s, size_t bytes, size_t events, size_t jobs);
+}  // namespace
+
 extern "C" dyg_status dyg_stream_create(dyg_ctx ctx, uint32_t V, const uint64_t* g_row_ptr,
                                         const uint32_t* g_col, const float* g_w,
                                         const uint64_t* h_row_ptr, const uint32_t* h_col,
@@ -1335,6 +1341,13 @@ extern "C" dyg_status dyg_stream_create(dyg_ctx ctx, uint32_t V, const uint64_t*
         if (r == DYG_OK) r = device_edits_begin(s->H, s->cmax, sp->max_inserts);
         if (r == DYG_OK && (sp->flags & DYG_STREAM_GRAPH) && !(sp->flags & DYG_STREAM_GENERAL_ONLY))
             capture_all_steps(s);
+        // lists compute their batches' orders on the prep stream: its
+        // buffers for lists of up to max_inserts updates (at most 4M; larger
+        // lists grow them), 64 larger batches and 256 one-workgroup batches,
+        // so that a list call allocates nothing
+        const uint64_t upd = std::min<uint64_t>(
+            std::max<uint64_t>(sp->max_inserts, sp->max_batch), 4ull << 20);
+        if (r == DYG_OK) r = prep_reserve(s, order_bytes(upd) + 64 * 256, 65, 256);
     }
     if (r != DYG_OK) {
         delete s;
@@ -1658,25 +1671,6 @@ dyg_status stream_apply_list(dyg_stream s, uint32_t nb, const int* op, const uin
                 ord_ev[b] = (int)k++;
             }
     }
-    // a larger batch's orders are enqueued a few larger batches ahead of its
-    // own kernels, so the host's enqueueing of orders overlaps the device's
-    // work instead of holding back the first batch
-    constexpr size_t kOrderAhead = 3;
-    size_t next_gen = 0;
-    auto orders_ahead = [&](uint32_t b) -> dyg_status {
-        while (next_gen < gens.size() &&
-               (gens[next_gen] <= b || next_gen < kOrderAhead ||
-                gens[next_gen - kOrderAhead] <= b)) {
-            const uint32_t g = gens[next_gen];
-            const uint64_t o = off[g];
-            dyg_status q = enqueue_order(s, s->oscr, s->prep, u + o, v + o,
-                                         (uint32_t)(off[g + 1] - o), (char*)ords[g].key,
-                                         s->prep_ev[ord_ev[g]]);
-            if (q != DYG_OK) return q;
-            ++next_gen;
-        }
-        return DYG_OK;
-    };
     // report regions: kListWords words per batch, then every outcome code
     const size_t words_bytes = (size_t)nb * kListWords * 8;
     DYG_HIP(ctx, ctx->report.reserve(words_bytes + total * 4 + 256));
@@ -1722,6 +1716,39 @@ dyg_status stream_apply_list(dyg_stream s, uint32_t nb, const int* op, const uin
     uint32_t done_b = 0;
     bool started = false;  // the list's start event recorded
     int waited = -1;       // the last prep event this stream waits for
+    // The larger batches' orders are enqueued by a second host thread, in
+    // list order, while this thread enqueues the batches themselves: the prep
+    // stream gets all its work at once (its kernels run early, between and
+    // beside the first batches' kernels) and no batch waits for the host to
+    // enqueue orders.  `recorded` publishes how many of their events are
+    // recorded; a batch's wait on its event is enqueued only after that.  The
+    // thread starts after the last early return and is joined after the loop.
+    std::atomic<size_t> recorded{0};
+    dyg_status order_fail = DYG_OK;
+    std::thread order_thread;
+    if (!gens.empty()) {
+        order_thread = std::thread([&] {
+            (void)hipSetDevice(ctx->device);
+            for (size_t g = 0; g < gens.size(); ++g) {
+                const uint32_t b = gens[g];
+                const uint64_t o = off[b];
+                const dyg_status q =
+                    enqueue_order(s, s->oscr, s->prep, u + o, v + o, (uint32_t)(off[b + 1] - o),
+                                  (char*)ords[b].key, s->prep_ev[ord_ev[b]]);
+                if (q != DYG_OK) {
+                    order_fail = q;
+                    recorded.store(gens.size() + 1, std::memory_order_release);
+                    return;
+                }
+                recorded.store(g + 1, std::memory_order_release);
+            }
+        });
+    }
+    auto order_ready = [&](int k) -> dyg_status {
+        size_t c;
+        while ((c = recorded.load(std::memory_order_acquire)) <= (size_t)k) std::this_thread::yield();
+        return c > gens.size() ? (order_fail != DYG_OK ? order_fail : DYG_ERR_HIP) : DYG_OK;
+    };
     for (uint32_t b = 0; b < nb && fail == DYG_OK; ++b) {
         const uint64_t o = off[b];
         const uint32_t n = (uint32_t)(off[b + 1] - o);
@@ -1762,8 +1789,8 @@ dyg_status stream_apply_list(dyg_stream s, uint32_t nb, const int* op, const uin
             fail = hipEventRecord(ev[0], str) == hipSuccess ? DYG_OK : DYG_ERR_HIP;
             started = true;
         }
-        if (fail == DYG_OK) fail = orders_ahead(b);
         const BatchOrder* ord = ord_ev[b] >= 0 ? &ords[b] : nullptr;
+        if (fail == DYG_OK && ord && general_batch(s, n)) fail = order_ready(ord_ev[b]);
         if (fail == DYG_OK && ord && ord_ev[b] != waited) {
             if (hipStreamWaitEvent(str, s->prep_ev[ord_ev[b]], 0) != hipSuccess) fail = DYG_ERR_HIP;
             waited = ord_ev[b];
@@ -1780,6 +1807,7 @@ dyg_status stream_apply_list(dyg_stream s, uint32_t nb, const int* op, const uin
     }
     s->wev0 = ctx->ev0;
     s->wev1 = ctx->ev1;
+    if (order_thread.joinable()) order_thread.join();
     if (nev) {  // the prep stream's work is done before the call returns
         DYG_HIP(ctx, hipEventRecord(s->prep_ev[nev], s->prep));
         DYG_HIP(ctx, hipStreamWaitEvent(str, s->prep_ev[nev], 0));
