@@ -376,31 +376,54 @@ __global__ void k_cand_heur(const uint32_t* __restrict__ path3, uint32_t n3, uin
     xv[base + 2 * g + 1] = t[2];
 }
 
-// First occurrence of each candidate edge that H lacks: append, with G's weight.
-__global__ void k_cand_groups(const unsigned long long* __restrict__ key,
-                              const uint32_t* __restrict__ idx, uint32_t n,
-                              const uint2* __restrict__ grows, const uint4* __restrict__ gadj,
-                              const uint2* __restrict__ hrows, const uint4* __restrict__ hadj,
-                              int32_t* __restrict__ flag, float* __restrict__ xw) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const unsigned long long k = key[p];
-    const bool head = p == 0 || key[p - 1] != k;
-    bool add = false;
-    if (head && k != kPadKey) {
-        const uint32_t a = (uint32_t)(k >> 32), b = (uint32_t)k;
-        if (!has_edge(hrows, hadj, a, b)) {
-            const int q = row_find(grows, gadj, a, b);
-            if (q >= 0) {  // always: path and heuristic hops are edges of G
-                add = true;
-                xw[idx[p]] = __uint_as_float(gadj[grows[a].x + q].y);
-            }
-        }
-    }
-    flag[idx[p]] = add ? -1 : 0;
-}
 
 // Device buffers of one stream, carved from one allocation.
+// Recovery candidates as row entries (entry 2c: xu[c]'s row, 2c+1: xv[c]'s),
+// with their indices, for one stable sort by row.
+__global__ void k_cand_entries(const uint32_t* __restrict__ xu, const uint32_t* __restrict__ xv,
+                               uint32_t n, uint32_t* __restrict__ erow,
+                               uint32_t* __restrict__ eent) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    erow[2 * c] = xu[c];
+    erow[2 * c + 1] = xv[c];
+    eent[2 * c] = 2 * c;
+    eent[2 * c + 1] = 2 * c + 1;
+}
+
+// Each candidate c = (a, b), decided at its even entry (row a's run of the
+// row-sorted entries, in candidate order): appended to H (flag -1, with G's
+// weight) when it is the first candidate of edge {a, b} -- no earlier entry
+// of the run has neighbour b, the same first occurrence a sort by edge key
+// gives -- and H lacks the edge.  Every candidate's flag is written.
+__global__ void k_cand_decide(const uint32_t* __restrict__ erow, const uint32_t* __restrict__ eent,
+                              uint32_t n2, const uint32_t* __restrict__ xu,
+                              const uint32_t* __restrict__ xv, const uint2* __restrict__ grows,
+                              const uint4* __restrict__ gadj, const uint2* __restrict__ hrows,
+                              const uint4* __restrict__ hadj, int32_t* __restrict__ flag,
+                              float* __restrict__ xw) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n2) return;
+    const uint32_t x = eent[p];
+    if (x & 1u) return;
+    const uint32_t c = x >> 1, a = erow[p], b = xv[c];
+    int32_t f = 0;
+    bool first = a != b;  // a self-loop (an isolated endpoint's heuristic) is never an edge
+    for (uint32_t q = p; first && q > 0 && erow[q - 1] == a; --q) {
+        const uint32_t y = eent[q - 1], k = y >> 1;
+        first = ((y & 1u) ? xu[k] : xv[k]) != b;
+    }
+    if (first && !has_edge(hrows, hadj, a, b)) {
+        const uint2 ra = grows[a];
+        const int sl = slot_of(gadj, ra, b);
+        if (sl >= 0) {  // always: path and heuristic hops are edges of G
+            f = -1;
+            xw[c] = __uint_as_float(gadj[ra.x + sl].y);
+        }
+    }
+    flag[c] = f;
+}
+
 // Scratch for computing one batch's orders (BatchOrder) of up to n updates on
 // a stream of its own: the keys' and row entries' sort inputs and the
 // sorts' hipCUB scratch.
@@ -491,6 +514,7 @@ struct dyg_stream_s {
     int32_t* xflag;
     unsigned long long *xk_in, *xk_out;
     uint32_t *xi_in, *xi_out;
+    uint32_t *xr_in, *xr_out, *xe_in, *xe_out;  // [2 * cmax] candidates' row entries
     unsigned long long* stats;  // [4] walk counters
     // pinned: inputs (u, v, w), outcomes, report words, recovery sizes; the
     // device reaches it at pin_dev
@@ -620,6 +644,10 @@ dyg_status stream_alloc(dyg_stream s) {
         s->xk_out = ar.take<unsigned long long>(cm);
         s->xi_in = ar.take<uint32_t>(cm);
         s->xi_out = ar.take<uint32_t>(cm);
+        s->xr_in = ar.take<uint32_t>(2 * cm);
+        s->xr_out = ar.take<uint32_t>(2 * cm);
+        s->xe_in = ar.take<uint32_t>(2 * cm);
+        s->xe_out = ar.take<uint32_t>(2 * cm);
         s->stats = ar.take<unsigned long long>(4);
         if (pass == 0) {
             ar.cap = ar.off + 256;
@@ -631,11 +659,15 @@ dyg_status stream_alloc(dyg_stream s) {
             ar.base = (char*)s->mem;
         }
     }
-    // hipCUB scratch of the key sorts
-    size_t t = 0;
+    // hipCUB scratch of the key sorts and of the candidates' row-entry sort
+    size_t t = 0, t2 = 0;
     DYG_HIP(s->ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, t, s->xk_in, s->xk_out, s->xi_in,
                                                        s->xi_out, (int)std::max<uint64_t>(cm, n),
                                                        0, 64, s->ctx->stream));
+    DYG_HIP(s->ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, t2, s->xr_in, s->xr_out, s->xe_in,
+                                                       s->xe_out, (int)std::min<uint64_t>(2 * cm, 0x7FFFFFFFull),
+                                                       0, 32, s->ctx->stream));
+    t = std::max(t, t2);
     if (hipMalloc(&s->temp, t) != hipSuccess) {
         s->temp = nullptr;
         set_error(s->ctx, "stream: device allocation failed");
@@ -993,14 +1025,20 @@ dyg_status recover_general(dyg_stream s, uint32_t n, const uint32_t* hc, int32_t
         return DYG_ERR_STATE;
     }
     LAUNCH_1D(k_cand_heur, n3, s->path3, n3, npath, s->xu, s->xv);
-    LAUNCH_1D(k_batch_keys, ncand, s->xu, s->xv, ncand, s->xk_in, s->xi_in, (uint32_t*)nullptr);
+    // the candidates' row entries, sorted once: each candidate is decided in
+    // its row's run (first occurrence of its edge, H lacking it), and the
+    // appends to H take the same presorted entries
+    LAUNCH_1D(k_cand_entries, ncand, s->xu, s->xv, ncand, s->xr_in, s->xe_in);
     if (ncand) {
-        dyg_status r = sort_keys(s, s->xk_in, s->xk_out, s->xi_in, s->xi_out, ncand);
-        if (r != DYG_OK) return r;
+        size_t tb = s->temp_bytes;
+        DYG_HIP(s->ctx, hipcub::DeviceRadixSort::SortPairs(s->temp, tb, s->xr_in, s->xr_out, s->xe_in,
+                                                           s->xe_out, (int)(2 * ncand), 0,
+                                                           row_bits(G->V), st));
     }
-    LAUNCH_1D(k_cand_groups, ncand, s->xk_out, s->xi_out, ncand, G->d_rows, G->d_adj, H->d_rows,
-              H->d_adj, s->xflag, s->xw);
-    dyg_status r = device_edits_batch(H, s->xu, s->xv, s->xw, s->xflag, ncand);
+    LAUNCH_1D(k_cand_decide, 2 * ncand, s->xr_out, s->xe_out, 2 * ncand, s->xu, s->xv, G->d_rows,
+              G->d_adj, H->d_rows, H->d_adj, s->xflag, s->xw);
+    dyg_status r = device_edits_sorted(H, s->xu, s->xv, s->xw, s->xr_out, s->xe_out, ncand,
+                                       EditTake{s->xflag, -1, nullptr, nullptr, nullptr});
     if (r != DYG_OK) return r;
     LAUNCH_1D(k_outcome, n, s->sel, s->cnt, s->cfound, dcls, kOutRecovered, kOutHeuristic);
     DYG_HIP(s->ctx, launch_report(rep, dcls, n, dstats, st));
