@@ -339,6 +339,21 @@ struct Report {
 // Copies cls[0..n) and the counters out as described above.
 hipError_t launch_report(const Report& r, const int32_t* cls, uint32_t n,
                          unsigned long long* stats, hipStream_t s);
+// Lane i of `lanes` copies its share of the report (see Report).
+__device__ __forceinline__ void report_out(const Report& r, const int32_t* __restrict__ cls,
+                                           uint32_t n, unsigned long long* __restrict__ stats,
+                                           uint32_t i, uint32_t lanes) {
+    for (uint32_t k = i; k < n; k += lanes) r.cls[k] = cls[k];
+    if (i < 4) {
+        r.words[i] = stats[i];
+        stats[i] = 0;
+    } else if (i < 9) {
+        r.words[i] = __hip_atomic_load(r.mg + (i - 4), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else if (i < (uint32_t)kReportWords) {
+        r.words[i] = __hip_atomic_load(r.mh + (i - 9), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
 
 // The per-batch values of a one-workgroup step run as a captured graph,
 // whose kernel arguments are fixed at capture: the host writes them into

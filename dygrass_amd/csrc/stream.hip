@@ -182,23 +182,30 @@ __device__ void add_weights(const uint2* __restrict__ rows, uint4* __restrict__ 
     sb->y = __float_as_uint(x);
 }
 
-__global__ void k_ins_weights(const unsigned long long* __restrict__ key,
-                              const uint32_t* __restrict__ idx, uint32_t n,
-                              const uint8_t* __restrict__ mode, const float* __restrict__ w,
-                              const uint2* __restrict__ grows, uint4* __restrict__ gadj,
-                              const uint2* __restrict__ hrows, uint4* __restrict__ hadj) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const unsigned long long k = key[p];
-    if (p > 0 && key[p - 1] == k) return;  // not a head: mode[p] is not written
-    const uint8_t md = mode[p];
+// An insertion batch's weight updates (after G's appends: a repeated new
+// edge adds to the weight it was appended with), run by k_compact's threads
+// at sorted position p (the head of each key's run).
+struct InsWeights {
+    const unsigned long long* key;  // null: a deletion batch, nothing to do
+    const uint32_t* idx;
+    const uint8_t* mode;
+    const float* w;
+    const uint2* grows;
+    uint4* gadj;
+    const uint2* hrows;
+    uint4* hadj;
+};
+__device__ void ins_weights_at(const InsWeights& t, uint32_t p, uint32_t n) {
+    const unsigned long long k = t.key[p];
+    if (p > 0 && t.key[p - 1] == k) return;  // not a head: mode[p] is not written
+    const uint8_t md = t.mode[p];
     if (md == 0) return;
     uint32_t q = p + 1;
-    while (q < n && key[q] == k) ++q;
+    while (q < n && t.key[q] == k) ++q;
     const uint32_t a = (uint32_t)(k >> 32), b = (uint32_t)k;
     const uint32_t e0 = (md & 1) ? p : p + 1;  // a new edge already holds its first weight
-    add_weights(grows, gadj, a, b, idx, e0, q, w);
-    if (md & 4) add_weights(hrows, hadj, a, b, idx, e0, q, w);
+    add_weights(t.grows, t.gadj, a, b, t.idx, e0, q, t.w);
+    if (md & 4) add_weights(t.hrows, t.hadj, a, b, t.idx, e0, q, t.w);
 }
 
 __global__ void k_del_groups(const unsigned long long* __restrict__ key,
@@ -236,13 +243,14 @@ __global__ void k_del_groups(const unsigned long long* __restrict__ key,
 // (bcnt, counted by the classification kernel), plus its own block scan.
 // Gathers each kept update: (u, v) or, `ordered`, (min, max); its weight;
 // its stream position; and (jmap set) its place in the kept list by batch
-// position.  cnt[0] = kept total.  256 threads per block.
+// position.  cnt[0] = kept total.  With wt.key set (an insertion batch) the
+// same threads also apply the weight updates.  256 threads per block.
 __global__ __launch_bounds__(256) void k_compact(
     const uint8_t* __restrict__ keep, uint32_t n, const uint32_t* __restrict__ bcnt,
     const uint32_t* __restrict__ u, const uint32_t* __restrict__ v, const float* __restrict__ w,
     uint64_t base, bool ordered, uint32_t* __restrict__ cu, uint32_t* __restrict__ cv,
     float* __restrict__ cw, uint64_t* __restrict__ ceid, uint32_t* __restrict__ sel,
-    uint32_t* __restrict__ cnt, uint32_t* __restrict__ jmap) {
+    uint32_t* __restrict__ cnt, uint32_t* __restrict__ jmap, InsWeights wt) {
     using BScan = hipcub::BlockScan<uint32_t, 256>;
     using BReduce = hipcub::BlockReduce<uint32_t, 256>;
     __shared__ union {
@@ -270,6 +278,20 @@ __global__ __launch_bounds__(256) void k_compact(
         if (jmap) jmap[i] = pos;
     }
     if (b == gridDim.x - 1 && t == 0) cnt[0] = s_prefix + total;
+    if (wt.key && i < n) ins_weights_at(wt, i, n);
+}
+
+// The batch's outcome codes straight into its report (k_outcome and k_report
+// in one launch): a kept update's code comes from its walk (jmap: its place in
+// the kept list), every other one's from the classification; block 0 also
+// copies the counters.
+__global__ __launch_bounds__(256) void k_outcome_report(
+    Report r, const int32_t* __restrict__ cls, const uint8_t* __restrict__ keep,
+    const uint32_t* __restrict__ jmap, const int32_t* __restrict__ found, uint32_t n,
+    int32_t if_found, int32_t if_not, unsigned long long* __restrict__ stats) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < n) r.cls[i] = keep[i] ? (found[jmap[i]] == -1 ? if_not : if_found) : cls[i];
+    if (blockIdx.x == 0) report_out(r, cls, 0, stats, threadIdx.x, 256);
 }
 
 __global__ void k_outcome(const uint32_t* __restrict__ sel, const uint32_t* __restrict__ nsel,
@@ -988,10 +1010,9 @@ dyg_status apply_insert(dyg_stream s, const uint32_t* du, const uint32_t* dv, co
                                   EditTake{s->flag_g, -1, nullptr, nullptr, nullptr})
             : device_edits_batch(G, du, dv, dw, s->flag_g, n);
     if (r != DYG_OK) return r;
-    LAUNCH_1D(k_ins_weights, n, key, idx, n, s->mode, dw, G->d_rows, G->d_adj, H->d_rows,
-              H->d_adj);
     LAUNCH_1D(k_compact, n, s->keep, n, bcnt, du, dv, dw, s->base, false, s->cu, s->cv, s->cw,
-              s->ceid, s->sel, s->cnt, ord ? s->jmap : nullptr);
+              s->ceid, s->sel, s->cnt, s->jmap,
+              InsWeights{key, idx, s->mode, dw, G->d_rows, G->d_adj, H->d_rows, H->d_adj});
     DYG_HIP(s->ctx, hipGetLastError());
     WalkLaunch a = walk_args(s, H, s->sp.ins, n);
     a.stats = dstats;
@@ -1004,8 +1025,8 @@ dyg_status apply_insert(dyg_stream s, const uint32_t* du, const uint32_t* dv, co
                                   EditTake{nullptr, 0, s->keep, s->jmap, s->cfound})
             : device_edits_batch(H, s->cu, s->cv, s->cw, s->cfound, n, s->cnt);
     if (r != DYG_OK) return r;
-    LAUNCH_1D(k_outcome, n, s->sel, s->cnt, s->cfound, dcls, kOutFound, kOutAdded);
-    DYG_HIP(s->ctx, launch_report(rep, dcls, n, dstats, st));
+    hipLaunchKernelGGL(k_outcome_report, dim3((n + 255) / 256), dim3(256), 0, st, rep, dcls,
+                       s->keep, s->jmap, s->cfound, n, kOutFound, kOutAdded, dstats);
     (void)out;
     return DYG_OK;
 }
@@ -1013,8 +1034,10 @@ dyg_status apply_insert(dyg_stream s, const uint32_t* du, const uint32_t* dv, co
 // Recovery of a deletion batch once the walks' candidate list sizes are in
 // `hc` (kept, no-path, endpoints, path hops) and the path hops are in xu/xv:
 // heuristic hops, first occurrences H lacks, appended with G's weight.
+// kept_map: keep[] and jmap[] describe the batch (the general pipeline's
+// k_compact wrote them), so the outcome codes go out with the report.
 dyg_status recover_general(dyg_stream s, uint32_t n, const uint32_t* hc, int32_t* dcls,
-                           unsigned long long* dstats, dyg_stream_stats* out, const Report& rep) {
+                           unsigned long long* dstats, dyg_stream_stats* out, const Report& rep, bool kept_map) {
     hipStream_t st = s->ctx->stream;
     dyg_graph G = s->G, H = s->H;
     const uint32_t hw = s->sp.heur_walkers;
@@ -1040,8 +1063,13 @@ dyg_status recover_general(dyg_stream s, uint32_t n, const uint32_t* hc, int32_t
     dyg_status r = device_edits_sorted(H, s->xu, s->xv, s->xw, s->xr_out, s->xe_out, ncand,
                                        EditTake{s->xflag, -1, nullptr, nullptr, nullptr});
     if (r != DYG_OK) return r;
-    LAUNCH_1D(k_outcome, n, s->sel, s->cnt, s->cfound, dcls, kOutRecovered, kOutHeuristic);
-    DYG_HIP(s->ctx, launch_report(rep, dcls, n, dstats, st));
+    if (kept_map) {
+        hipLaunchKernelGGL(k_outcome_report, dim3((n + 255) / 256), dim3(256), 0, st, rep, dcls,
+                           s->keep, s->jmap, s->cfound, n, kOutRecovered, kOutHeuristic, dstats);
+    } else {
+        LAUNCH_1D(k_outcome, n, s->sel, s->cnt, s->cfound, dcls, kOutRecovered, kOutHeuristic);
+        DYG_HIP(s->ctx, launch_report(rep, dcls, n, dstats, st));
+    }
     if (out) out->candidates = ncand;
     return DYG_OK;
 }
@@ -1129,7 +1157,7 @@ dyg_status apply_delete_small(dyg_stream s, const uint32_t* du, const uint32_t* 
                                      s->sp.del.seed, s->path3, st, s->cnt + 2, s->G->nodesc));
     LAUNCH_1D(k_cand_paths, n, s->cfound, s->cnt, s->path, ms, s->poff, s->xu, s->xv);
     DYG_HIP(s->ctx, hipGetLastError());
-    return recover_general(s, n, hc, dcls, dstats, out, rep);
+    return recover_general(s, n, hc, dcls, dstats, out, rep, false);
 }
 
 dyg_status apply_delete(dyg_stream s, const uint32_t* du, const uint32_t* dv, uint32_t n,
@@ -1170,7 +1198,7 @@ dyg_status apply_delete(dyg_stream s, const uint32_t* du, const uint32_t* dv, ui
             : device_deletes_batch(H, du, dv, s->flag_h, n, nullptr);
     if (r != DYG_OK) return r;
     LAUNCH_1D(k_compact, n, s->keep, n, bcnt, du, dv, (const float*)nullptr, s->base, true,
-              s->cu, s->cv, (float*)nullptr, s->ceid, s->sel, s->cnt, (uint32_t*)nullptr);
+              s->cu, s->cv, (float*)nullptr, s->ceid, s->sel, s->cnt, s->jmap, InsWeights{});
     DYG_HIP(s->ctx, hipGetLastError());
     WalkLaunch a = walk_args(s, G, s->sp.del, n);
     a.w = nullptr;
@@ -1189,7 +1217,7 @@ dyg_status apply_delete(dyg_stream s, const uint32_t* du, const uint32_t* dv, ui
     // the one wait of a deletion batch: recovery list sizes
     r = wait_spin(s->ctx, st, s->ev_sync);
     if (r != DYG_OK) return r;
-    return recover_general(s, n, hc, dcls, dstats, out, rep);
+    return recover_general(s, n, hc, dcls, dstats, out, rep, true);
 }
 
 // ---------------------------------------------------------------------------

@@ -476,21 +476,6 @@ __device__ void add_weights_seq(const BlockGraph& g, uint32_t a, uint32_t b,
     sb->y = __float_as_uint(x);
 }
 
-// Lane i of `lanes` copies its share of the report (see Report).
-__device__ __forceinline__ void report_out(const Report& r, const int32_t* __restrict__ cls,
-                                           uint32_t n, unsigned long long* __restrict__ stats,
-                                           uint32_t i, uint32_t lanes) {
-    for (uint32_t k = i; k < n; k += lanes) r.cls[k] = cls[k];
-    if (i < 4) {
-        r.words[i] = stats[i];
-        stats[i] = 0;
-    } else if (i < 9) {
-        r.words[i] = __hip_atomic_load(r.mg + (i - 4), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    } else if (i < (uint32_t)kReportWords) {
-        r.words[i] = __hip_atomic_load(r.mh + (i - 9), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-}
-
 // ---------------------------------------------------------------- insertions
 template <int IPT>
 struct InsShared {
