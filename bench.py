@@ -243,7 +243,7 @@ def sp_of(setting, threshold: float, seed: int, max_batch: int, max_inserts: int
                          block_edges_ins=0 if plain else s.block_ins,
                          block_edges_del=0 if plain else s.block_del,
                          dense_no_desc=s.dense_no_desc and not plain,
-                         graph=s.graph and not plain)
+                         graph=s.graph and not plain, small_1024=s.small_1024)
 
 
 def upload_batches(ctx, batches) -> list:

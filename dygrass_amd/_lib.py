@@ -80,6 +80,7 @@ DYG_STREAM_PLAIN = 1
 DYG_STREAM_GENERAL_ONLY = 2
 DYG_STREAM_DENSE_NO_DESC = 4
 DYG_STREAM_GRAPH = 8
+DYG_STREAM_SMALL_1024 = 16
 DYG_NOALLOC_MAX_EDGES = 1048576
 OUTCOMES = {1: "weight", 2: "found", 3: "added", 4: "skip", 5: "g_only", 6: "recovered",
             7: "heuristic", 8: "invalid"}

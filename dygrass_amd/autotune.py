@@ -8,13 +8,14 @@ command line (random_walk_incremental/main.cu:240-262, `<#walkers>` at most
 Those fixed values are the tuner's *anchor*: the quality the tuned setting
 must keep.
 
-Eight knobs, in two kinds:
+Nine knobs, in two kinds:
 
 * Shape knobs change no result, only the time, because every draw is keyed
   by (edge, walker, step) and never by where a walker runs: edges per
   workgroup of the work-queue walk kernel for insertion walks and for
   deletion walks, whether the dense graph G keeps neighbour descriptors in
-  its slots, and whether one-workgroup batch steps run as captured hipGraphs.  They are chosen by timing alone, and every shape's outcome
+  its slots, whether one-workgroup batch steps run as captured hipGraphs,
+  and whether the one-workgroup path stops at 1024 updates.  They are chosen by timing alone, and every shape's outcome
   codes and final sparsifier must equal the first shape's (checked; a
   mismatch is an error, not a tuning result).
 * Quality knobs change which paths the walks find: walkers per edge and
@@ -75,6 +76,7 @@ class Setting:
     block_del: int = 0        # edges per workgroup, deletion walks (0 = automatic)
     dense_no_desc: bool = False
     graph: bool = False       # one-workgroup batch steps as captured hipGraphs
+    small_1024: bool = False  # one-workgroup path for batches of at most 1024 updates
 
     def quality(self) -> tuple:
         return (self.w_ins, self.w_del, self.steps_ins, self.steps_del)
@@ -94,6 +96,7 @@ QUALITY_GRID = {
 SHAPE_GRID = {
     "dense_no_desc": (False, True),
     "graph": (False, True),
+    "small_1024": (False, True),
     "block_ins": (0, 1, 2, 4, 8, 16),
     "block_del": (0, 1, 2, 4, 8, 16),
 }
@@ -250,7 +253,8 @@ def stream_params_for(s: Setting, threshold: float, seed: int, max_batch: int, m
                          max_batch=max_batch, max_inserts=max_inserts,
                          steps_ins=s.steps_ins, steps_del=s.steps_del,
                          block_edges_ins=s.block_ins, block_edges_del=s.block_del,
-                         dense_no_desc=s.dense_no_desc, graph=s.graph)
+                         dense_no_desc=s.dense_no_desc, graph=s.graph,
+                         small_1024=s.small_1024)
 
 
 class StreamTrials:
@@ -336,6 +340,32 @@ def setting_of(tuning: dict, which: str = "chosen") -> Setting:
     return Setting(**tuning[which]["setting"])
 
 
+def retune_shapes(run, edges_h0: int, tuning: dict, *, shape_grid=None, repeats: int = 2,
+                  log=None) -> dict:
+    """The shape knobs again, from a tuning's chosen setting (its quality
+    knobs kept): a shape knob changes no result (every trial's outcome codes
+    and final sparsifier must equal the chosen setting's), so the tuning's
+    growth and kappa still hold and only the shape fields of `chosen` and its
+    times change.  For a shape knob added after a tuning was made, without
+    repeating the quality search and its kappa solves."""
+    import copy
+
+    base = setting_of(tuning)
+    res = tune(run, None, edges_h0, anchor=base, quality_grid={}, shape_grid=shape_grid,
+               repeats=repeats, log=log)
+    growth = res["chosen"]["growth"]
+    if growth != tuning["chosen"]["growth"]:
+        raise TuneError(f"the chosen setting grows the sparsifier by {growth}, the tuning "
+                        f"recorded {tuning['chosen']['growth']}: a different stream")
+    out = copy.deepcopy(tuning)
+    out["chosen"]["setting"] = res["chosen"]["setting"]
+    out["chosen"]["seconds"] = res["chosen"]["seconds"]
+    out["chosen"]["walk_ms"] = res["chosen"]["walk_ms"]
+    out["shape_retune"] = {"from": base.asdict(), "base_seconds": res["anchor"]["seconds"],
+                           "trials": res["trials"]}
+    return out
+
+
 def check_kappa(tr: StreamTrials, tuning: dict, method: str, log,
                 which_settings=("anchor", "chosen")) -> dict:
     """kappa of the anchor's and the chosen setting's final graphs on the
@@ -398,6 +428,9 @@ def main(argv=None) -> int:
                          "this tuning file, on its calibration stream (written to --out)")
     ap.add_argument("--kappa-of", choices=["anchor", "chosen", "both"], default="both",
                     help="check-kappa: which settings to compute (others keep the file's)")
+    ap.add_argument("--retune-shapes", metavar="TUNING",
+                    help="re-run only the shape knobs from this tuning's chosen setting "
+                         "(its quality knobs, growth and kappa kept)")
     ap.add_argument("--kappa-method", choices=["lu", "pcg"], default=None,
                     help="how kappa applies L_H^-1 (default: lu for batched, pcg for served)")
     a = ap.parse_args(argv)
@@ -422,6 +455,18 @@ def main(argv=None) -> int:
     ctx = Context(0)
     tr = StreamTrials(ctx, wl.G.V, (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w),
                       batches, wl.threshold, a.seed, kappa_method=method)
+    if a.retune_shapes:
+        sg = SHAPE_GRID if a.workload == "batched" else \
+            {k: v for k, v in SHAPE_GRID.items() if k != "graph"}
+        res = retune_shapes(tr.run, tr.edges_h0, load(name, a.retune_shapes), shape_grid=sg,
+                            repeats=a.repeats, log=log)
+        res["shape_retune"]["command"] = ("python -m dygrass_amd.autotune "
+                                          + " ".join(argv or sys.argv[1:]))
+        with open(a.out, "w") as f:
+            json.dump(res, f, indent=1)
+        log(f"chosen {res['chosen']['setting']}")
+        ctx.close()
+        return 0
     if a.check_kappa:
         which = ("anchor", "chosen") if a.kappa_of == "both" else (a.kappa_of,)
         res = check_kappa(tr, load(name, a.check_kappa), method, log, which)

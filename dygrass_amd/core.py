@@ -386,7 +386,8 @@ def stream_params(w_ins: int = 256, w_del: int = 512, threshold: float = 100.0, 
                   block_edges: int = 0, general_only: bool = False,
                   dense_no_desc: bool = False, steps_ins: int | None = None,
                   steps_del: int | None = None, block_edges_ins: int | None = None,
-                  block_edges_del: int | None = None, graph: bool = False) -> L.StreamParams:
+                  block_edges_del: int | None = None, graph: bool = False,
+                  small_1024: bool = False) -> L.StreamParams:
     """Stream parameters; plain=True selects the reference-shaped form (host
     edge maps and edit plans, one workgroup per edge, a wait per step);
     general_only=True keeps small batches on the general pipeline;
@@ -395,7 +396,8 @@ def stream_params(w_ins: int = 256, w_del: int = 512, threshold: float = 100.0, 
     block_edges_* set the insertion and deletion walks apart (default:
     max_steps and block_edges for both); graph=True runs batches of at most
     2048 updates as captured hipGraphs of their one-workgroup step (apply()
-    only; the served loop launches directly)."""
+    only; the served loop launches directly); small_1024=True keeps the
+    one-workgroup path to batches of at most 1024 updates."""
     sp = L.StreamParams()
     sp.ins = params(w_ins, max_steps if steps_ins is None else steps_ins, threshold, seed,
                     block_edges=block_edges if block_edges_ins is None else block_edges_ins)
@@ -406,7 +408,8 @@ def stream_params(w_ins: int = 256, w_del: int = 512, threshold: float = 100.0, 
     sp.flags = ((L.DYG_STREAM_PLAIN if plain else 0)
                 | (L.DYG_STREAM_GENERAL_ONLY if general_only else 0)
                 | (L.DYG_STREAM_DENSE_NO_DESC if dense_no_desc else 0)
-                | (L.DYG_STREAM_GRAPH if graph else 0))
+                | (L.DYG_STREAM_GRAPH if graph else 0)
+                | (L.DYG_STREAM_SMALL_1024 if small_1024 else 0))
     return sp
 
 

@@ -616,6 +616,14 @@ PinLayout pin_layout(char* base, uint32_t n) {
     return l;
 }
 
+// Largest batch of the one-workgroup path (0: every batch takes the general
+// pipeline).
+uint32_t small_limit(const dyg_stream_s* s) {
+    if (s->sp.flags & DYG_STREAM_GENERAL_ONLY) return 0;
+    const uint32_t top = (s->sp.flags & DYG_STREAM_SMALL_1024) ? kSmallBatch / 2 : kSmallBatch;
+    return std::min(top, s->sp.max_batch);
+}
+
 int key_bits(uint32_t V) { return std::min(64, 32 + 33 - __builtin_clz(std::max<uint32_t>(V, 1))); }
 int row_bits(uint32_t V) { return std::min(32, 33 - __builtin_clz(std::max<uint32_t>(V, 1))); }
 
@@ -983,7 +991,7 @@ dyg_status apply_insert(dyg_stream s, const uint32_t* du, const uint32_t* dv, co
                         dyg_stream_stats* out, const Report& rep, uint32_t* stage,
                         bool graph = false, uint32_t vcheck = 0,
                         const BatchOrder* ord = nullptr) {
-    if (n <= std::min(kSmallBatch, s->sp.max_batch) && !(s->sp.flags & DYG_STREAM_GENERAL_ONLY))
+    if (n <= small_limit(s))
         return apply_insert_small(s, du, dv, dw, n, dcls, dstats, timed, rep, graph, vcheck, ord);
     hipStream_t st = s->ctx->stream;
     dyg_graph G = s->G, H = s->H;
@@ -1115,7 +1123,7 @@ SmallDelArgs small_del_args(dyg_stream s, const uint32_t* du, const uint32_t* dv
 // graph launch, and only a layout change (arena growth) captures again.
 void capture_all_steps(dyg_stream s) {
     const Report rep{nullptr, nullptr, s->G->dev.meta, s->H->dev.meta};
-    const uint32_t top = std::min(kSmallBatch, s->sp.max_batch);
+    const uint32_t top = small_limit(s);
     for (int c = 0; c < dyg_stream_s::kStepClasses; ++c) {
         if (c > 0 && kStepClass[c - 1] >= top) break;  // no batch reaches this class
         const uint32_t cm = kStepClass[c];
@@ -1165,7 +1173,7 @@ dyg_status apply_delete(dyg_stream s, const uint32_t* du, const uint32_t* dv, ui
                         dyg_stream_stats* out, const Report& rep, uint32_t* stage,
                         bool graph = false, uint32_t vcheck = 0,
                         const BatchOrder* ord = nullptr) {
-    if (n <= std::min(kSmallBatch, s->sp.max_batch) && !(s->sp.flags & DYG_STREAM_GENERAL_ONLY))
+    if (n <= small_limit(s))
         return apply_delete_small(s, du, dv, n, dcls, dstats, timed, out, rep, graph, vcheck,
                                   ord);
     hipStream_t st = s->ctx->stream;
@@ -1494,7 +1502,7 @@ dyg_status stream_apply(dyg_stream s, int op, const uint32_t* u, const uint32_t*
         const PinLayout hp = pin_layout((char*)s->pin, n), dp = pin_layout(s->pin_dev, n);
         const bool ins = op == DYG_STREAM_INSERT;
         const bool small =
-            n <= std::min(kSmallBatch, s->sp.max_batch) && !(s->sp.flags & DYG_STREAM_GENERAL_ONLY);
+            n <= small_limit(s);
         hp.words[kReportBad] = 0;
         if (dev && !small) {
             // a larger batch of device inputs: its ids are checked before it starts
@@ -1572,7 +1580,7 @@ dyg_status stream_apply(dyg_stream s, int op, const uint32_t* u, const uint32_t*
 // A batch of n updates takes the general pipeline (and an order computed
 // ahead of it in a list) when it is larger than a one-workgroup batch.
 bool general_batch(dyg_stream s, uint64_t n) {
-    return n > std::min(kSmallBatch, s->sp.max_batch) || (s->sp.flags & DYG_STREAM_GENERAL_ONLY);
+    return n > small_limit(s);
 }
 
 // Bytes of one batch's BatchOrder arrays (each batch's region 256-aligned).
@@ -2358,7 +2366,7 @@ dyg_status serve_submit(dyg_serve v, int op, const uint32_t* u, const uint32_t* 
             std::memcpy(hp.u, u, n * 4ull);
             std::memcpy(hp.v, w_v, n * 4ull);
             if (ins) std::memcpy(hp.w, w, n * 4ull);
-            r->copied = !(ins && n <= kSmallBatch && !(v->s->sp.flags & DYG_STREAM_GENERAL_ONLY));
+            r->copied = !(ins && n <= small_limit(v->s));
         }
         if (r->copied) {
             if (v->fail_next_copy) {  // testing hook: report a failure, issue nothing

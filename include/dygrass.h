@@ -327,6 +327,11 @@ typedef struct {
  * directly.  Same results.  (No reference counterpart: the reference launches
  * each kernel on the default stream.) */
 #define DYG_STREAM_GRAPH 8u
+/* Device form only: the one-workgroup path takes batches of at most 1024
+ * updates (one per lane) instead of 2048; larger ones take the general
+ * pipeline, whose kernels spread a batch's row scans over the whole GPU.
+ * Same results; a shape knob of the autotuner.  (No reference counterpart.) */
+#define DYG_STREAM_SMALL_1024 16u
 typedef struct {
     uint64_t updates, weight_updates, found, added, skipped, g_only, recovered, heuristic,
         invalid;

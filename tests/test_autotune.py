@@ -145,3 +145,33 @@ def test_growth_only_tuning():
     assert Setting(**res["chosen"]["setting"]) == Setting(w_ins=128, w_del=128, steps_ins=32,
                                                           steps_del=32)
     assert res["chosen"]["kappa"] is None and res["kappa_checks"] == []
+
+
+def test_retune_shapes_keeps_quality_growth_and_kappa():
+    """A shape knob added after a tuning: only the shape knobs run again, from
+    the tuning's chosen setting; its quality knobs, growth and kappa stay, and
+    a tuning whose growth the stream does not reproduce is refused."""
+    import copy
+
+    from dygrass_amd.autotune import retune_shapes
+
+    res = tune(model_run, model_kappa, H0, repeats=1)
+    old = copy.deepcopy(res)
+    old["chosen"]["setting"]["block_ins"] = 0  # an older shape choice
+
+    def run(s):  # the newer knob is faster in this model
+        t = model_run(s)
+        return dataclasses.replace(t, seconds=t.seconds - (1e-6 if s.small_1024 else 0.0))
+
+    new = retune_shapes(run, H0, old, repeats=1)
+    chosen = Setting(**new["chosen"]["setting"])
+    assert chosen.quality() == Setting(**old["chosen"]["setting"]).quality()
+    assert chosen.block_ins == 4 and chosen.small_1024 and chosen.dense_no_desc
+    assert new["chosen"]["kappa"] == old["chosen"]["kappa"]
+    assert new["chosen"]["growth"] == old["chosen"]["growth"]
+    assert new["anchor"] == old["anchor"]
+    assert new["shape_retune"]["from"] == old["chosen"]["setting"]
+    bad = copy.deepcopy(old)
+    bad["chosen"]["growth"] += 1
+    with pytest.raises(TuneError, match="different stream"):
+        retune_shapes(run, H0, bad, repeats=1)

@@ -449,15 +449,19 @@ def test_apply_many_dev_checks_ids_first_and_runs_lists_in_turn(ctx, grid_wl):
     assert split.position == whole.position == int(off[-1])
 
 
-@pytest.mark.parametrize("seed,nodesc,general", [(31, False, True), (32, True, True),
-                                                  (33, False, False), (34, True, False)])
-def test_apply_many_dev_orders_ahead_match_oracle(ctx, grid_wl, seed, nodesc, general):
+@pytest.mark.parametrize("seed,nodesc,general,small_1024",
+                         [(31, False, True, False), (32, True, True, False),
+                          (33, False, False, False), (34, True, False, False),
+                          (35, False, False, True)])
+def test_apply_many_dev_orders_ahead_match_oracle(ctx, grid_wl, seed, nodesc, general,
+                                                  small_1024):
     """A list through dyg_stream_apply_many_dev, every batch in the general
     pipeline or in the one-workgroup kernels: each batch's keys and row
     entries are sorted ahead of it on the prep stream, and its kernels take
     the presorted orders instead of sorting their own.  Outcome codes of every
     batch and the final G and H (slot for slot) equal the oracle's, batch
-    after batch."""
+    after batch.  small_1024: the 1200-update batch takes the general pipeline
+    between one-workgroup batches."""
     from dygrass_amd import Stream, stream_params
     from oracle import stream as S
 
@@ -466,7 +470,8 @@ def test_apply_many_dev_orders_ahead_match_oracle(ctx, grid_wl, seed, nodesc, ge
     # a batch in the larger one-workgroup class (two updates per lane)
     batches.insert(2, ("ins", wl.ins.u[800:2000], wl.ins.v[800:2000], wl.ins.w[800:2000]))
     sp = stream_params(w_ins=64, w_del=64, threshold=10.0, seed=seed, max_batch=1300,
-                       max_inserts=8192, general_only=general, dense_no_desc=nodesc)
+                       max_inserts=8192, general_only=general, dense_no_desc=nodesc,
+                       small_1024=small_1024)
     st = Stream(ctx, wl.G.V, (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w), sp)
     orc = S.Stream(wl.G.V, (wl.G.u, wl.G.v, wl.G.w), (wl.H.u, wl.H.v, wl.H.w), thr=10.0,
                    w_ins=64, w_del=64, seed=seed)
