@@ -233,6 +233,9 @@ extern "C" dyg_status dyg_ctx_create(int device, dyg_ctx* out) {
         delete c;
         return DYG_ERR_HIP;
     }
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess)
+        c->clock_khz = khz;
     // the fixed staging array's address on this device (device current);
     // a failure here is retried by the first call that needs it
     (void)scratch_setup(device);

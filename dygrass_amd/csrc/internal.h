@@ -43,6 +43,7 @@ struct dyg_ctx_s {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double clock_khz = 0.0;  // the device's wall clock rate (wall_clock64 ticks per ms)
     // written by the caller's thread and, while a server is open, by its
     // worker threads: always under err_mu
     std::mutex err_mu;
@@ -330,6 +331,10 @@ constexpr int kReportWords = 14;
 // when a batch of device inputs held a vertex id out of range (the batch then
 // changed nothing).
 constexpr int kReportBad = 14;
+// And after it the device's wall clock (wall_clock64, `dyg_ctx_s::clock_khz`
+// ticks per ms) when the batch's last kernel reported it: consecutive
+// batches' clocks give their spans without events between their kernels.
+constexpr int kReportClock = 15;
 struct Report {
     int32_t* cls;                  // [n] outcome codes
     unsigned long long* words;     // [kReportWords]
@@ -351,6 +356,8 @@ __device__ __forceinline__ void report_out(const Report& r, const int32_t* __res
         r.words[i] = __hip_atomic_load(r.mg + (i - 4), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else if (i < (uint32_t)kReportWords) {
         r.words[i] = __hip_atomic_load(r.mh + (i - 9), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else if (i == (uint32_t)kReportClock) {
+        r.words[i] = wall_clock64();
     }
 }
 

@@ -70,6 +70,11 @@ __device__ __forceinline__ int row_find(const uint2* __restrict__ rows,
 // k_compact over the same n reads.
 // Range check of a batch of device inputs (larger than the one-workgroup
 // path, which checks inside its pre kernel): any id >= V sets *bad.
+// The device's wall clock into *w (a list's start, for its batches' spans).
+__global__ void k_clock(unsigned long long* w) {
+    if (threadIdx.x == 0) *w = wall_clock64();
+}
+
 __global__ void k_check_batch(const uint32_t* __restrict__ u, const uint32_t* __restrict__ v,
                               uint32_t n, uint32_t V, unsigned long long* bad) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
@@ -1754,8 +1759,8 @@ dyg_status stream_apply_list(dyg_stream s, uint32_t nb, const int* op, const uin
     auto* rw_d = (unsigned long long*)rd;
     auto* rc_h = (int32_t*)((char*)ctx->report.p + words_bytes);
     auto* rc_d = (int32_t*)((char*)rd + words_bytes);
-    // events: the list's start, each batch's walk kernel (start, end, taken
-    // by the dispatch itself), the list's end
+    // events: each insertion batch's walk kernel (start, end, taken by the
+    // dispatch itself)
     ctx->reserve_events(2 * (size_t)nb + 2);
     if (ctx->evpool.size() < 2 * (size_t)nb + 2) {
         set_error(ctx, "stream apply many: event creation failed");
@@ -1788,7 +1793,7 @@ dyg_status stream_apply_list(dyg_stream s, uint32_t nb, const int* op, const uin
     };
     dyg_status fail = DYG_OK;
     uint32_t done_b = 0;
-    bool started = false;  // the list's start event recorded
+    bool started = false;  // the list's start clock enqueued
     int waited = -1;       // the last prep event this stream waits for
     // The larger batches' orders are enqueued by a second host thread, in
     // list order, while this thread enqueues the batches themselves: the prep
@@ -1857,10 +1862,14 @@ dyg_status stream_apply_list(dyg_stream s, uint32_t nb, const int* op, const uin
         est_h += add_h;
         const Report rep{rc_d + (o - o0), rw_d + (size_t)b * kListWords, s->G->dev.meta,
                          s->H->dev.meta};
-        s->wev0 = ev[1 + 2 * b];
-        s->wev1 = ev[2 + 2 * b];
-        if (!started) {
-            fail = hipEventRecord(ev[0], str) == hipSuccess ? DYG_OK : DYG_ERR_HIP;
+        // insertion walks (the stream's dominant kernel) are timed by events
+        // taken with their dispatch; deletion walks are not (events on their
+        // dispatch cost about 13 us per batch here)
+        s->wev0 = del ? nullptr : ev[1 + 2 * b];
+        s->wev1 = del ? nullptr : ev[2 + 2 * b];
+        if (!started) {  // the list's start on the device clock
+            hipLaunchKernelGGL(k_clock, dim3(1), dim3(64), 0, str, dp.words + kReportClock);
+            fail = hipGetLastError() == hipSuccess ? DYG_OK : DYG_ERR_HIP;
             started = true;
         }
         const BatchOrder* ord = ord_ev[b] >= 0 ? &ords[b] : nullptr;
@@ -1887,11 +1896,10 @@ dyg_status stream_apply_list(dyg_stream s, uint32_t nb, const int* op, const uin
         DYG_HIP(ctx, hipStreamWaitEvent(str, s->prep_ev[nev], 0));
     }
     // the list's one wait, then the reports of every batch enqueued
-    DYG_HIP(ctx, hipEventRecord(ev[2 * (size_t)nb + 1], str));
     r = wait_spin(ctx, str, s->ev_sync);
     if (r == DYG_OK) r = absorb_last();
     if (fail == DYG_OK) fail = r;
-    hipEvent_t prev_end = nullptr;  // the last batch's walk end
+    unsigned long long prev_clk = hp.words[kReportClock];  // the list's start
     for (uint32_t b = 0; b < done_b; ++b) {
         const uint64_t o = off[b] - o0;
         const uint32_t n = (uint32_t)rs[b].updates;
@@ -1920,16 +1928,16 @@ dyg_status stream_apply_list(dyg_stream s, uint32_t nb, const int* op, const uin
         x.redirects = wd[1];
         if (wd[3]) x.candidates = wd[3];
         float t = 0.f;
-        if (hipEventElapsedTime(&t, ev[1 + 2 * b], ev[2 + 2 * b]) == hipSuccess) x.walk_ms = t;
-        // the batch's span on the device timeline: from the previous batch's
-        // walk end (the list's start for the first) to its own walk end (the
-        // list's end for the last); the spans add up to the list's
-        uint32_t nx = b + 1;
-        while (nx < done_b && rs[nx].updates == 0) ++nx;
-        const hipEvent_t begin = prev_end ? prev_end : ev[0];
-        const hipEvent_t end = nx < done_b ? ev[2 + 2 * b] : ev[2 * (size_t)nb + 1];
-        if (hipEventElapsedTime(&t, begin, end) == hipSuccess) x.total_ms = t;
-        prev_end = ev[2 + 2 * b];
+        if (op[b] != DYG_STREAM_DELETE &&
+            hipEventElapsedTime(&t, ev[1 + 2 * b], ev[2 + 2 * b]) == hipSuccess)
+            x.walk_ms = t;
+        // the batch's span on the device clock: from the previous batch's
+        // report (the list's start for the first) to its own; the spans add
+        // up to the list's
+        const unsigned long long clk = wd[kReportClock];
+        if (ctx->clock_khz > 0 && clk >= prev_clk)
+            x.total_ms = (double)(clk - prev_clk) / ctx->clock_khz;
+        prev_clk = clk;
         if (st) st[b] = x;
     }
     if (fail != DYG_OK) return fail;

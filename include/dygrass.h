@@ -369,9 +369,13 @@ dyg_status dyg_stream_apply_dev(dyg_stream s, int op, const uint32_t* u, const u
  * of nb calls of dyg_stream_apply_dev in order.  The batches are enqueued
  * back to back: the host waits only where a deletion batch's recovery list
  * sizes must reach it, and once at the end; each batch reports into its own
- * region and times its walk with its own events.  outcome (optional, host)
- * receives every update's code at its position, st (optional) nb per-batch
- * stats (total_ms: the batch's span on the device timeline).  Offsets, and
+ * region, and an insertion batch times its walk with events taken with the
+ * walk's dispatch (a deletion batch's walk_ms is 0 here: events on its
+ * dispatch would cost more than its walk's share of the time).  Each
+ * batch's orders (its sorted keys and row entries) are computed ahead on a
+ * second stream.  outcome (optional, host) receives every update's code at
+ * its position, st (optional) nb per-batch stats (total_ms: the batch's
+ * span on the device clock, from the previous batch's end).  Offsets, and
  * then every vertex id of the list (on the device), are checked first
  * (DYG_ERR_ARG, nothing applied); a later failure stops the call with the
  * batches before it applied. */

@@ -393,10 +393,17 @@ def test_apply_many_dev_matches_apply_dev(ctx, grid_wl):
     assert many.position == 0
     got, gst = many.apply_many_dev(ops, off, du, dv, dw)
     assert np.array_equal(got, np.concatenate(want))
-    for a, b in zip(gst, wst):
+    for op, a, b in zip(ops, gst, wst):
         for k in ("updates", "found", "added", "skipped", "g_only", "recovered", "heuristic",
                   "weight_updates", "invalid"):
             assert a[k] == b[k], k
+        # spans from the device clock; insertion walks timed, deletion walks not
+        assert a["total_ms"] > 0.0
+        if op == "ins":
+            assert a["walk_ms"] > 0.0 or a["hops"] == 0
+            assert a["walk_ms"] < a["total_ms"] or a["hops"] == 0
+        else:
+            assert a["walk_ms"] == 0.0
     _assert_same_graph(many.G, _rows_of(one.G))
     _assert_same_graph(many.H, _rows_of(one.H))
     assert many.position == one.position
