@@ -2068,6 +2068,10 @@ struct dyg_serve_s {
     bool stop = false, worker_done = false;
     dyg_status failed = DYG_OK;
     std::string fail_msg;
+    // the first ticket the failure stops: requests admitted before it go on
+    // (a failed input copy never reached the device; a request that failed
+    // on the device stops everything after it)
+    uint64_t fail_ticket = ~0ull;
     uint64_t next_ticket = 0;
 };
 
@@ -2134,11 +2138,11 @@ void serve_worker(dyg_serve_s* v) {
         ServeSlot& sl = v->slots[r->slot];
         r->t_start = now_ms();
         // a request whose submit failed (its input never reached the device)
-        // is never applied, and neither is anything after a failure
+        // is never applied, and neither is anything admitted after a failure
         dyg_status st = r->status;
         if (st == DYG_OK) {
             std::lock_guard<std::mutex> lk(v->mu);
-            st = v->failed;
+            if (r->ticket >= v->fail_ticket) st = v->failed;
         }
         if (st == DYG_OK) st = ensure_headroom(v, *r);
         if (st == DYG_OK && (r->copied || r->ordered) &&
@@ -2187,6 +2191,7 @@ void serve_worker(dyg_serve_s* v) {
             if (v->failed == DYG_OK) {
                 v->failed = st;
                 v->fail_msg = r->error;
+                v->fail_ticket = r->ticket + 1;
             }
             v->gate.fail(v->failed);
         }
@@ -2411,6 +2416,7 @@ dyg_status serve_submit(dyg_serve v, int op, const uint32_t* u, const uint32_t* 
             if (v->failed == DYG_OK) {
                 v->failed = DYG_ERR_HIP;
                 v->fail_msg = "serve submit: input copy failed";
+                v->fail_ticket = v->next_ticket;  // this request took no ticket
             }
             v->gate.fail(v->failed);
             set_error(ctx, "serve submit: input copy failed");
