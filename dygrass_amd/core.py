@@ -413,6 +413,16 @@ def stream_params(w_ins: int = 256, w_del: int = 512, threshold: float = 100.0, 
     return sp
 
 
+_STATS_NAMES = [k for k, _ in L.StreamStats._fields_]
+_STATS_DTYPE = np.ctypeslib.as_array((L.StreamStats * 1)()).dtype
+
+
+def _stats_dicts(sts, nb: int) -> list:
+    """Per-batch dyg_stream_stats as dicts, read through one structured view."""
+    rows = np.frombuffer(sts, dtype=_STATS_DTYPE, count=nb).tolist()
+    return [dict(zip(_STATS_NAMES, r)) for r in rows]
+
+
 class Stream:
     """Mixed insertion/deletion update stream on device-resident G and H
     (dyg_stream; dynamic_update/main.cu:92-124 batch loop semantics)."""
@@ -484,7 +494,7 @@ class Stream:
         check(self._lib.dyg_stream_apply_many_dev(self.handle, nb, L.addr(code), L.addr(off),
                                                   addr(u), addr(v), addr(w), L.addr(out), sts),
               self.ctx.handle)
-        stats = [{k: getattr(sts[b], k) for k, _ in L.StreamStats._fields_} for b in range(nb)]
+        stats = _stats_dicts(sts, nb)
         return out[:int(off[-1])], stats
 
     def close(self) -> None:

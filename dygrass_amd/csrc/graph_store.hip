@@ -69,10 +69,13 @@ inline uint32_t fbits(float f) {
 }
 
 // Grow the slot arena to hold at least `need` slots (device copy of the used
-// prefix).  The host mirror grows when the host next plans an edit.
+// prefix).  The host mirror grows when the host next plans an edit.  A growth
+// leaves an eighth of `need` (or half the old size) on top, so that the small
+// rises of a headroom reservation after each call (slots in use grow with
+// every relocated row) do not each cost a reallocation and a copy.
 dyg_status grow_arena(dyg_graph g, uint64_t need) {
     if (need <= g->cap_slots) return DYG_OK;
-    uint64_t ncap = std::max<uint64_t>(need, g->cap_slots + g->cap_slots / 2 + 1024);
+    uint64_t ncap = std::max<uint64_t>(need + need / 8, g->cap_slots + g->cap_slots / 2 + 1024);
     if (ncap > 0xFFFFFFFFull) ncap = 0xFFFFFFFFull;
     if (need > ncap) {
         set_error(g->ctx, "slot arena exceeds 2^32 slots");

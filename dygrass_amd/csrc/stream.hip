@@ -1416,8 +1416,14 @@ extern "C" dyg_status dyg_stream_create(dyg_ctx ctx, uint32_t V, const uint64_t*
     if (!plain) {
         if (r == DYG_OK && (sp->flags & DYG_STREAM_DENSE_NO_DESC)) s->G->nodesc = true;
         if (r == DYG_OK) r = stream_alloc(s);
-        if (r == DYG_OK) r = device_edits_begin(s->G, sp->max_batch, sp->max_inserts);
-        if (r == DYG_OK) r = device_edits_begin(s->H, s->cmax, sp->max_inserts);
+        // arena room for the run's expected appends, and at least what each
+        // call's closing reservation asks for (a batch for G, a deletion
+        // batch's recovery candidates for H): calls then do not reallocate
+        if (r == DYG_OK)
+            r = device_edits_begin(s->G, sp->max_batch,
+                                   std::max<uint64_t>(sp->max_inserts, sp->max_batch));
+        if (r == DYG_OK)
+            r = device_edits_begin(s->H, s->cmax, std::max<uint64_t>(sp->max_inserts, s->cmax));
         if (r == DYG_OK && (sp->flags & DYG_STREAM_GRAPH) && !(sp->flags & DYG_STREAM_GENERAL_ONLY))
             capture_all_steps(s);
         // lists compute their batches' orders on the prep stream: its
@@ -1437,6 +1443,26 @@ extern "C" dyg_status dyg_stream_create(dyg_ctx ctx, uint32_t V, const uint64_t*
 }
 
 namespace {
+
+// Outcome codes from pinned memory to the caller (when `out`) and their counts
+// by class: one bulk copy, then counts over the copy in four independent
+// tables (a single table serialises on repeated codes).
+void count_codes(const int32_t* codes, int32_t* out, uint32_t n, uint64_t counts[16]) {
+    if (out) {
+        std::memcpy(out, codes, (size_t)n * 4);
+        codes = out;
+    }
+    uint32_t t[4][16] = {};
+    uint32_t i = 0;
+    for (; i + 4 <= n; i += 4) {
+        ++t[0][codes[i] & 15];
+        ++t[1][codes[i + 1] & 15];
+        ++t[2][codes[i + 2] & 15];
+        ++t[3][codes[i + 3] & 15];
+    }
+    for (; i < n; ++i) ++t[0][codes[i] & 15];
+    for (int c = 0; c < 16; ++c) counts[c] += (uint64_t)t[0][c] + t[1][c] + t[2][c] + t[3][c];
+}
 
 // One batch through the stream; u, v, w are host arrays, or device pointers
 // when `dev` (then the device checks the ids before anything changes).
@@ -1546,12 +1572,7 @@ dyg_status stream_apply(dyg_stream s, int op, const uint32_t* u, const uint32_t*
         r = device_edits_absorb(s->G, hp.words + 4, nullptr);
         if (r == DYG_OK) r = device_edits_absorb(s->H, hp.words + 9, nullptr);
         if (r != DYG_OK) return r;
-        // outcome codes to the caller and their counts, in one pass
-        for (uint32_t i = 0; i < n; ++i) {
-            const int32_t c = hp.cls[i];
-            if (outcome) outcome[i] = c;
-            counts[c & 15]++;
-        }
+        count_codes(hp.cls, outcome, n, counts);
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, s->wev0, s->wev1) == hipSuccess) rs.walk_ms = ms;
         rs.hops = hst[0];
@@ -1683,12 +1704,9 @@ dyg_status stream_apply_list(dyg_stream s, uint32_t nb, const int* op, const uin
                            v + o0 + c, k, s->G->V, dp.words + kReportBad);
     }
     DYG_HIP(ctx, hipGetLastError());
-    dyg_status r = wait_spin(ctx, str, s->ev_sync);
-    if (r != DYG_OK) return r;
-    if (hp.words[kReportBad]) {
-        set_error(ctx, "stream apply many: vertex id out of range");
-        return DYG_ERR_ARG;
-    }
+    // (its result is read once the host has set up the orders below: they
+    // only read the ids, never index by them, and change nothing)
+    dyg_status r = DYG_OK;
     // every batch's orders, computed on the prep stream while the batches
     // before it run (after the inputs are in place on this stream): a larger
     // batch's by its own keys kernel and sorts, all one-workgroup batches'
@@ -1801,7 +1819,8 @@ dyg_status stream_apply_list(dyg_stream s, uint32_t nb, const int* op, const uin
     // beside the first batches' kernels) and no batch waits for the host to
     // enqueue orders.  `recorded` publishes how many of their events are
     // recorded; a batch's wait on its event is enqueued only after that.  The
-    // thread starts after the last early return and is joined after the loop.
+    // thread starts after the last allocation and is joined after the loop
+    // (or before the return of a list whose ids fail the check).
     std::atomic<size_t> recorded{0};
     dyg_status order_fail = DYG_OK;
     std::thread order_thread;
@@ -1822,6 +1841,17 @@ dyg_status stream_apply_list(dyg_stream s, uint32_t nb, const int* op, const uin
                 recorded.store(g + 1, std::memory_order_release);
             }
         });
+    }
+    // the ids' check, read while the prep stream starts on the orders
+    r = wait_spin(ctx, str, s->ev_sync);
+    if (r == DYG_OK && hp.words[kReportBad]) {
+        set_error(ctx, "stream apply many: vertex id out of range");
+        r = DYG_ERR_ARG;
+    }
+    if (r != DYG_OK) {
+        if (order_thread.joinable()) order_thread.join();
+        if (s->prep) (void)hipStreamSynchronize(s->prep);
+        return r;
     }
     auto order_ready = [&](int k) -> dyg_status {
         size_t c;
@@ -1909,11 +1939,7 @@ dyg_status stream_apply_list(dyg_stream s, uint32_t nb, const int* op, const uin
         }
         const unsigned long long* wd = rw_h + (size_t)b * kListWords;
         uint64_t counts[16] = {};
-        for (uint32_t i = 0; i < n; ++i) {
-            const int32_t c = rc_h[o + i];
-            if (outcome) outcome[o + i] = c;
-            counts[c & 15]++;
-        }
+        count_codes(rc_h + o, outcome ? outcome + o : nullptr, n, counts);
         dyg_stream_stats& x = rs[b];
         x.weight_updates = counts[kOutWeight];
         x.found = counts[kOutFound];
