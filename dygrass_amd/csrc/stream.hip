@@ -39,6 +39,8 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <functional>
+#include <memory>
 #include <mutex>
 #include <thread>
 #include <unordered_map>
@@ -490,6 +492,69 @@ struct OrderScratch {
     }
 };
 
+// A host thread kept for the life of a stream, that runs one job at a time
+// (a list's order enqueues): handing it a job costs a wake-up instead of a
+// thread creation inside the call.
+class Helper {
+  public:
+    explicit Helper(int device) : device_(device) {}
+    ~Helper() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        if (th_.joinable()) th_.join();
+    }
+    // Starts the thread (once); false when it could not be started.
+    bool start() {
+        if (th_.joinable()) return true;
+        try {
+            th_ = std::thread([this] { loop(); });
+        } catch (...) {
+            return false;
+        }
+        return true;
+    }
+    // Hands `f` to the thread; the previous job has been waited for.
+    void post(std::function<void()> f) {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            job_ = std::move(f);
+            busy_ = true;
+        }
+        cv_.notify_all();
+    }
+    // Returns once the posted job has finished.
+    void wait() {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return !busy_; });
+    }
+
+  private:
+    void loop() {
+        (void)hipSetDevice(device_);
+        std::unique_lock<std::mutex> lk(mu_);
+        for (;;) {
+            cv_.wait(lk, [&] { return stop_ || job_ != nullptr; });
+            if (job_ == nullptr) return;  // stopping, nothing left to run
+            std::function<void()> f = std::move(job_);
+            job_ = nullptr;
+            lk.unlock();
+            f();
+            lk.lock();
+            busy_ = false;
+            cv_.notify_all();
+        }
+    }
+    int device_;
+    std::thread th_;
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::function<void()> job_;
+    bool busy_ = false, stop_ = false;
+};
+
 struct Arena {
     char* base = nullptr;
     size_t off = 0, cap = 0;
@@ -576,7 +641,10 @@ struct dyg_stream_s {
     std::vector<hipEvent_t> prep_ev;
     Pinned jobs_h;  // the one-workgroup batches' order jobs (SmallOrderJob), host side
     DevBuf jobs_d;  // and on the device
+    // enqueues a list's larger batches' orders beside the list's own thread
+    std::unique_ptr<Helper> helper;
     ~dyg_stream_s() {
+        helper.reset();  // its last job has been waited for
         if (prep) {
             (void)hipStreamSynchronize(prep);
             (void)hipStreamDestroy(prep);
@@ -1433,6 +1501,13 @@ extern "C" dyg_status dyg_stream_create(dyg_ctx ctx, uint32_t V, const uint64_t*
         const uint64_t upd = std::min<uint64_t>(
             std::max<uint64_t>(sp->max_inserts, sp->max_batch), 4ull << 20);
         if (r == DYG_OK) r = prep_reserve(s, order_bytes(upd) + 64 * 256, 65, 256);
+        if (r == DYG_OK) {
+            s->helper.reset(new Helper(ctx->device));
+            if (!s->helper->start()) {
+                set_error(ctx, "stream: helper thread could not start");
+                r = DYG_ERR_STATE;
+            }
+        }
     }
     if (r != DYG_OK) {
         delete s;
@@ -1813,20 +1888,19 @@ dyg_status stream_apply_list(dyg_stream s, uint32_t nb, const int* op, const uin
     uint32_t done_b = 0;
     bool started = false;  // the list's start clock enqueued
     int waited = -1;       // the last prep event this stream waits for
-    // The larger batches' orders are enqueued by a second host thread, in
-    // list order, while this thread enqueues the batches themselves: the prep
-    // stream gets all its work at once (its kernels run early, between and
-    // beside the first batches' kernels) and no batch waits for the host to
-    // enqueue orders.  `recorded` publishes how many of their events are
+    // The larger batches' orders are enqueued by the stream's helper thread,
+    // in list order, while this thread enqueues the batches themselves: the
+    // prep stream gets all its work at once (its kernels run early, between
+    // and beside the first batches' kernels) and no batch waits for the host
+    // to enqueue orders.  `recorded` publishes how many of their events are
     // recorded; a batch's wait on its event is enqueued only after that.  The
-    // thread starts after the last allocation and is joined after the loop
+    // job is posted after the last allocation and waited for after the loop
     // (or before the return of a list whose ids fail the check).
     std::atomic<size_t> recorded{0};
     dyg_status order_fail = DYG_OK;
-    std::thread order_thread;
-    if (!gens.empty()) {
-        order_thread = std::thread([&] {
-            (void)hipSetDevice(ctx->device);
+    const bool helped = !gens.empty();
+    if (helped) {
+        s->helper->post([&] {
             for (size_t g = 0; g < gens.size(); ++g) {
                 const uint32_t b = gens[g];
                 const uint64_t o = off[b];
@@ -1849,7 +1923,7 @@ dyg_status stream_apply_list(dyg_stream s, uint32_t nb, const int* op, const uin
         r = DYG_ERR_ARG;
     }
     if (r != DYG_OK) {
-        if (order_thread.joinable()) order_thread.join();
+        if (helped) s->helper->wait();
         if (s->prep) (void)hipStreamSynchronize(s->prep);
         return r;
     }
@@ -1920,7 +1994,7 @@ dyg_status stream_apply_list(dyg_stream s, uint32_t nb, const int* op, const uin
     }
     s->wev0 = ctx->ev0;
     s->wev1 = ctx->ev1;
-    if (order_thread.joinable()) order_thread.join();
+    if (helped) s->helper->wait();
     if (nev) {  // the prep stream's work is done before the call returns
         DYG_HIP(ctx, hipEventRecord(s->prep_ev[nev], s->prep));
         DYG_HIP(ctx, hipStreamWaitEvent(str, s->prep_ev[nev], 0));
@@ -1929,42 +2003,65 @@ dyg_status stream_apply_list(dyg_stream s, uint32_t nb, const int* op, const uin
     r = wait_spin(ctx, str, s->ev_sync);
     if (r == DYG_OK) r = absorb_last();
     if (fail == DYG_OK) fail = r;
-    unsigned long long prev_clk = hp.words[kReportClock];  // the list's start
-    for (uint32_t b = 0; b < done_b; ++b) {
-        const uint64_t o = off[b] - o0;
-        const uint32_t n = (uint32_t)rs[b].updates;
-        if (n == 0) {
-            if (st) st[b] = rs[b];
-            continue;
+    // each batch's span on the device clock runs from the previous nonempty
+    // batch's report (the list's start for the first) to its own; the spans
+    // add up to the list's
+    std::vector<unsigned long long> pclk(done_b);
+    {
+        unsigned long long c = hp.words[kReportClock];  // the list's start
+        for (uint32_t b = 0; b < done_b; ++b) {
+            pclk[b] = c;
+            if (rs[b].updates) c = rw_h[(size_t)b * kListWords + kReportClock];
         }
-        const unsigned long long* wd = rw_h + (size_t)b * kListWords;
-        uint64_t counts[16] = {};
-        count_codes(rc_h + o, outcome ? outcome + o : nullptr, n, counts);
-        dyg_stream_stats& x = rs[b];
-        x.weight_updates = counts[kOutWeight];
-        x.found = counts[kOutFound];
-        x.added = counts[kOutAdded];
-        x.skipped = counts[kOutSkip];
-        x.g_only = counts[kOutGOnly];
-        x.recovered = counts[kOutRecovered];
-        x.heuristic = counts[kOutHeuristic];
-        x.invalid = n - (x.weight_updates + x.found + x.added + x.skipped + x.g_only +
-                         x.recovered + x.heuristic);
-        x.hops = wd[0];
-        x.redirects = wd[1];
-        if (wd[3]) x.candidates = wd[3];
-        float t = 0.f;
-        if (op[b] != DYG_STREAM_DELETE &&
-            hipEventElapsedTime(&t, ev[1 + 2 * b], ev[2 + 2 * b]) == hipSuccess)
-            x.walk_ms = t;
-        // the batch's span on the device clock: from the previous batch's
-        // report (the list's start for the first) to its own; the spans add
-        // up to the list's
-        const unsigned long long clk = wd[kReportClock];
-        if (ctx->clock_khz > 0 && clk >= prev_clk)
-            x.total_ms = (double)(clk - prev_clk) / ctx->clock_khz;
-        prev_clk = clk;
-        if (st) st[b] = x;
+    }
+    auto read_reports = [&](uint32_t b0, uint32_t b1) {
+        for (uint32_t b = b0; b < b1; ++b) {
+            const uint64_t o = off[b] - o0;
+            const uint32_t n = (uint32_t)rs[b].updates;
+            if (n == 0) {
+                if (st) st[b] = rs[b];
+                continue;
+            }
+            const unsigned long long* wd = rw_h + (size_t)b * kListWords;
+            uint64_t counts[16] = {};
+            count_codes(rc_h + o, outcome ? outcome + o : nullptr, n, counts);
+            dyg_stream_stats& x = rs[b];
+            x.weight_updates = counts[kOutWeight];
+            x.found = counts[kOutFound];
+            x.added = counts[kOutAdded];
+            x.skipped = counts[kOutSkip];
+            x.g_only = counts[kOutGOnly];
+            x.recovered = counts[kOutRecovered];
+            x.heuristic = counts[kOutHeuristic];
+            x.invalid = n - (x.weight_updates + x.found + x.added + x.skipped + x.g_only +
+                             x.recovered + x.heuristic);
+            x.hops = wd[0];
+            x.redirects = wd[1];
+            if (wd[3]) x.candidates = wd[3];
+            float t = 0.f;
+            if (op[b] != DYG_STREAM_DELETE &&
+                hipEventElapsedTime(&t, ev[1 + 2 * b], ev[2 + 2 * b]) == hipSuccess)
+                x.walk_ms = t;
+            const unsigned long long clk = wd[kReportClock];
+            if (ctx->clock_khz > 0 && clk >= pclk[b])
+                x.total_ms = (double)(clk - pclk[b]) / ctx->clock_khz;
+            if (st) st[b] = x;
+        }
+    };
+    // a long list's reports are read by two threads (this one and the
+    // helper, each about half the updates): the outcome codes come from
+    // pinned memory the device wrote, so the copy runs at memory speed per
+    // thread
+    uint64_t done_upd = 0;
+    for (uint32_t b = 0; b < done_b; ++b) done_upd += rs[b].updates;
+    uint32_t mid = 0;
+    for (uint64_t acc = 0; mid < done_b && 2 * acc < done_upd; ++mid) acc += rs[mid].updates;
+    if (s->helper && done_upd >= (1u << 16) && mid > 0 && mid < done_b) {
+        s->helper->post([&] { read_reports(0, mid); });
+        read_reports(mid, done_b);
+        s->helper->wait();
+    } else {
+        read_reports(0, done_b);
     }
     if (fail != DYG_OK) return fail;
     // a batch's worth of arena headroom for what comes next
