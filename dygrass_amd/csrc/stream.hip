@@ -2214,8 +2214,13 @@ uint64_t request_need(const dyg_serve_s* v, int op, uint32_t n) {
 // could not hold the next request's worst case.
 dyg_status ensure_headroom(dyg_serve_s* v, const ServeReq& r) {
     dyg_stream s = v->s;
+    // the request's own worst case (a deletion request's recovery candidates
+    // are bounded by its size, as in a list), not a largest batch's
     const uint64_t add = 4ull * (2ull * r.n) * (s->sp.slack + 4) + 8ull * r.n;
-    const uint64_t add_h = r.op == DYG_STREAM_DELETE ? add + 8ull * s->cmax : add;
+    const uint64_t cand =
+        (uint64_t)r.n * (s->sp.del.max_steps - 1) + 4ull * r.n * s->sp.heur_walkers;
+    const uint64_t add_h = r.op == DYG_STREAM_DELETE ? add + 8ull * std::min<uint64_t>(cand, s->cmax)
+                                                     : add;
     if (v->used_est_g + add <= s->G->cap_slots && v->used_est_h + add_h <= s->H->cap_slots) {
         v->used_est_g += add;
         v->used_est_h += add_h;
