@@ -277,10 +277,6 @@ __global__ __launch_bounds__((1 << LOGW) < 64 ? 64 : (1 << LOGW)) void k_walk(Wa
 //    run once per four turns for the whole wave, instead of whenever any one
 //    lane runs dry.
 // ---------------------------------------------------------------------------
-// The slot a lane reads when its walk has no next hop (the loop issues its
-// loads unconditionally); never written.
-__device__ uint4 g_no_slot[1];
-
 constexpr int kPoolThreads = 256;
 constexpr int kPoolWaves = kPoolThreads / 64;
 constexpr float kPruneSlack = 1.0f + 0x1p-20f;
@@ -527,19 +523,22 @@ __global__ __launch_bounds__(kPoolThreads) void k_walk_pool(WalkLaunch a, uint32
             }
             {
                 // one load site for every active lane, with no branch: a lane
-                // whose walk cannot go on (isolated source, leaf) reads the dummy
-                // slot instead
-                const bool go = fresh ? nrow.y >= 1u : nrow.y >= 2u;
-                const uint32_t span = go ? (fresh ? nrow.y : nrow.y - 1u) : 1u;
+                // whose walk cannot go on (isolated source, leaf) reads the
+                // arena's slot 0 instead (every arena holds at least 4096
+                // slots) and never uses it
+                // the first hop draws over the whole row, later ones over all
+                // but the last slot: d = 0 for a fresh task, else 1
+                const uint32_t d = (uint32_t)min(L.step, 1);
+                const bool go = nrow.y > d;
+                const uint32_t span = go ? nrow.y - d : 1u;
                 const uint32_t i = r % span;
-                const uint4* const base = go ? a.adj : g_no_slot;
                 if constexpr (kLazy) {
                     // the next hop's drawn slot, or the redirect of this hop
-                    x = defer ? a.adj[crow.x + crow.y - 1u] : base[go ? nrow.x + i : 0u];
+                    x = a.adj[defer ? crow.x + crow.y - 1u : (go ? nrow.x + i : 0u)];
                     if (!defer) crow = nrow;
                 } else {
-                    x = base[go ? nrow.x + i : 0u];
-                    y = base[go ? nrow.x + nrow.y - 1u : 0u];
+                    x = a.adj[go ? nrow.x + i : 0u];
+                    y = a.adj[go ? nrow.x + nrow.y - 1u : 0u];
                 }
             }
             if constexpr (kLazy) {
