@@ -1290,14 +1290,17 @@ dyg_status apply_delete(dyg_stream s, const uint32_t* du, const uint32_t* dv, ui
     LAUNCH_1D(k_path_counts, n, s->cfound, s->cnt, n, s->pc, s->nopath, s->bsum);
     LAUNCH_1D(k_recovery_lists, n, s->pc, s->nopath, n, s->bsum, s->cu, s->cv, s->ceid, s->poff,
               s->ep, s->ekey, s->cnt, (uint32_t*)(s->pin_dev + kPinSizes));
+    // the one wait of a deletion batch is for the recovery list sizes, which
+    // this kernel writes: the host waits for it (not for the heuristic walks
+    // and path candidates enqueued behind it) and enqueues the recovery
+    // while those run
+    DYG_HIP(s->ctx, hipEventRecord(s->ev_sync, st));
     const uint32_t* hc = (const uint32_t*)((char*)s->pin + kPinSizes);
     DYG_HIP(s->ctx, launch_heuristic(G->d_rows, G->d_adj, s->ep, s->ekey, 2 * n, hw,
                                      s->sp.del.seed, s->path3, st, s->cnt + 2, G->nodesc));
     LAUNCH_1D(k_cand_paths, n, s->cfound, s->cnt, s->path, ms, s->poff, s->xu, s->xv);
     DYG_HIP(s->ctx, hipGetLastError());
-    // the one wait of a deletion batch: recovery list sizes
-    r = wait_spin(s->ctx, st, s->ev_sync);
-    if (r != DYG_OK) return r;
+    DYG_HIP(s->ctx, spin_event(s->ev_sync));
     return recover_general(s, n, hc, dcls, dstats, out, rep, true);
 }
 
