@@ -411,6 +411,25 @@ def leg_batched(args, ctx) -> dict:
     if tuning.get("source"):
         # the stream is deterministic: this run's growth is the tuning run's
         out["growth_matches_tuning"] = out["growth_H"] == tuning["growth"]
+        # the same stream at the reference's walker counts and step budgets
+        # (the tuning's anchor: 512 walkers, 100 steps for both kinds), the
+        # tuned launch shape kept: what the chosen budgets are measured against
+        from dygrass_amd import autotune
+
+        anc = dict(tuning["anchor"])
+        ref = autotune.Setting(**{**setting.asdict(), **{k: anc[k] for k in
+                                                        ("w_ins", "w_del", "steps_ins",
+                                                         "steps_del")}})
+        run_stream(ctx, wl, False, args.seed, ref, csr, dev_list)  # warm-up
+        rr = [run_stream(ctx, wl, False, args.seed, ref, csr, dev_list) for _ in range(2)]
+        rt = sum(r["seconds"] for r in rr) / len(rr)
+        log(f"batched at the reference's budgets: {rt * 1e3:.1f} ms")
+        out["reference_budgets"] = {
+            "value": m / rt, "ms_per_step": rt * 1e3, "setting": ref.asdict(),
+            "insertion_walk_kernel_ms_per_step": rr[-1]["walk_ms"],
+            "growth_H": int(len(rr[-1]["H"][1]) // 2 - len(wl.H)),
+            "growth_matches_anchor": int(len(rr[-1]["H"][1]) // 2 - len(wl.H))
+            == tuning["growth_anchor"]}
     if not args.no_plain:
         pr = run_stream(ctx, wl, True, args.seed, setting, csr)
         log(f"batched plain: {pr['seconds'] * 1e3:.1f} ms")
