@@ -344,7 +344,12 @@ typedef struct {
     uint64_t redirects;   /* non-backtracking redirects to a row's last slot (each one
                              more slot read) in the batch's walks */
 } dyg_stream_stats;
-/* G and H in CSR (slot order as given), uploaded and owned by the stream. */
+/* G and H in CSR (slot order as given), uploaded and owned by the stream.
+ * The device form reserves both slot arenas at creation for max_inserts
+ * appends and for what each call keeps in hand for the next (a largest batch
+ * in G, a deletion batch's recovery candidates in H), and starts one host
+ * helper thread (a list call hands it the orders of its larger batches; it
+ * ends with dyg_stream_free). */
 dyg_status dyg_stream_create(dyg_ctx ctx, uint32_t V, const uint64_t* g_row_ptr,
                              const uint32_t* g_col, const float* g_w, const uint64_t* h_row_ptr,
                              const uint32_t* h_col, const float* h_w,
