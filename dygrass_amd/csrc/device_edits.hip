@@ -549,6 +549,96 @@ dyg_status device_deletes_sorted(dyg_graph g, const uint32_t* src, const uint32_
     return DYG_OK;
 }
 
+namespace {
+
+// Keys of a run's row entries (device_edits_order_run): batch b = i / I in
+// the bits above rb, the row below; the value is the entry's index within
+// its batch.
+__global__ __launch_bounds__(256) void k_run_keys(const uint32_t* __restrict__ src,
+                                                  const uint32_t* __restrict__ dst, uint32_t m,
+                                                  uint32_t I, int rb, uint32_t* __restrict__ key,
+                                                  uint32_t* __restrict__ val) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= m) return;
+    const uint32_t b = i / I, l = i - b * I;
+    const uint32_t hi = rb < 32 ? b << rb : 0u;
+    key[2 * i] = hi | src[i];
+    key[2 * i + 1] = hi | dst[i];
+    val[2 * i] = 2 * l;
+    val[2 * i + 1] = 2 * l + 1;
+}
+
+// The sorted keys' row bits alone (the batch bits only grouped the entries).
+__global__ __launch_bounds__(256) void k_run_rows(uint32_t* __restrict__ key, uint32_t n2,
+                                                  uint32_t mask) {
+    const uint32_t p = blockIdx.x * 256u + threadIdx.x;
+    if (p < n2) key[p] &= mask;
+}
+
+int bits_for(uint64_t x) {  // bits that hold every value below x (x >= 1)
+    int b = 0;
+    while (b < 64 && (x - 1) >> b) ++b;
+    return b;
+}
+
+size_t run_array_bytes(uint64_t m) { return ((2 * m * 4 + 255) / 256) * 256; }
+
+}  // namespace
+
+dyg_status device_edits_reserve_run(dyg_graph g, uint64_t m) {
+    DevEdits& d = g->dev;
+    if (m == 0 || m <= d.run_cap) return DYG_OK;
+    dyg_ctx ctx = g->ctx;
+    if (2 * m > 0x7FFFFFFFull) {
+        set_error(ctx, "device edits: run too large to order at once");
+        return DYG_ERR_ARG;
+    }
+    size_t tb = 0;
+    DYG_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, tb, (uint32_t*)nullptr,
+                                                    (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                                    (uint32_t*)nullptr, (int)(2 * m), 0, 32,
+                                                    ctx->stream));
+    DYG_HIP(ctx, d.run.reserve(4 * run_array_bytes(m) + tb));
+    d.run_cap = m;
+    d.run_temp_bytes = tb;
+    return DYG_OK;
+}
+
+dyg_status device_edits_order_run(dyg_graph g, const uint32_t* src, const uint32_t* dst,
+                                  uint32_t m, uint32_t I, const uint32_t** row,
+                                  const uint32_t** ent, bool* ok) {
+    *ok = false;
+    if (m == 0 || I == 0) return DYG_OK;
+    const int rb = bits_for(std::max<uint32_t>(g->V, 2)), bb = bits_for((m + I - 1) / I);
+    if (rb + bb > 32) return DYG_OK;
+    dyg_status st = device_edits_reserve_run(g, m);
+    if (st != DYG_OK) return st;
+    dyg_ctx ctx = g->ctx;
+    DevEdits& d = g->dev;
+    hipStream_t s = ctx->stream;
+    const size_t ab = run_array_bytes(m);
+    char* base = (char*)d.run.p;
+    auto* k_in = (uint32_t*)base;
+    auto* v_in = (uint32_t*)(base + ab);
+    auto* k_out = (uint32_t*)(base + 2 * ab);
+    auto* v_out = (uint32_t*)(base + 3 * ab);
+    void* temp = base + 4 * ab;
+    const uint32_t n2 = 2 * m;
+    hipLaunchKernelGGL(k_run_keys, dim3((m + 255) / 256), dim3(256), 0, s, src, dst, m, I, rb,
+                       k_in, v_in);
+    size_t tb = d.run_temp_bytes;
+    DYG_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(temp, tb, k_in, k_out, v_in, v_out, (int)n2,
+                                                    0, rb + bb, s));
+    if (bb > 0)
+        hipLaunchKernelGGL(k_run_rows, dim3((n2 + 255) / 256), dim3(256), 0, s, k_out, n2,
+                           rb < 32 ? (1u << rb) - 1u : 0xFFFFFFFFu);
+    DYG_HIP(ctx, hipGetLastError());
+    *row = k_out;
+    *ent = v_out;
+    *ok = true;
+    return DYG_OK;
+}
+
 dyg_status device_edits_absorb(dyg_graph g, const unsigned long long* meta, uint64_t* added_out) {
     const uint32_t err = (uint32_t)meta[2];
     g->used_slots = meta[0];

@@ -6,9 +6,11 @@
 // then appended in batch order.  Edge i's draws are keyed by i.
 //
 // Two ways to apply a batch's appends, same resulting graph:
-//   device (default)  compaction, ordering and writes run as kernels on the
-//                     context stream right after the walk kernel, so the
-//                     whole run is one uninterrupted stream of launches;
+//   device (default)  the run's row entries are sorted once, by batch and
+//                     row, before the first walk; each batch's appends then
+//                     run as two kernels on the context stream right after
+//                     its walk kernel, so the whole run is one uninterrupted
+//                     stream of launches;
 //   host              the decisions are copied back after each batch and the
 //                     host plans the appends (the reference's shape).
 #include <algorithm>
@@ -79,6 +81,14 @@ dyg_status run_incremental(dyg_graph H, const RunBufs& b, const uint32_t* h_src,
     };
     if (device_edits && m > 0) {
         ret = device_edits_begin(H, I, m);
+        // every batch's row entries, sorted once before the first walk: a
+        // batch's appends are then two launches (take the entries of the
+        // edges that found no path, refresh the touched rows) with no sort
+        // of their own
+        const uint32_t *orow = nullptr, *oent = nullptr;
+        bool presorted = false;
+        if (ret == DYG_OK)
+            ret = device_edits_order_run(H, b.src, b.dst, m, I, &orow, &oent, &presorted);
         uint32_t k = 0;
         for (uint32_t b0 = 0; b0 < m && ret == DYG_OK; b0 += I, ++k) {
             const uint32_t n = std::min(I, m - b0);
@@ -88,7 +98,11 @@ dyg_status run_incremental(dyg_graph H, const RunBufs& b, const uint32_t* h_src,
                 ret = DYG_ERR_HIP;
                 break;
             }
-            ret = device_edits_batch(H, b.src + b0, b.dst + b0, b.w + b0, b.found + b0, n);
+            ret = presorted
+                      ? device_edits_sorted(H, b.src + b0, b.dst + b0, b.w + b0, orow + 2ull * b0,
+                                            oent + 2ull * b0, n,
+                                            EditTake{b.found + b0, -1, nullptr, nullptr, nullptr})
+                      : device_edits_batch(H, b.src + b0, b.dst + b0, b.w + b0, b.found + b0, n);
             if (ret == DYG_OK && hipEventRecord(ctx->evpool[3 * k + 2], s) != hipSuccess) {
                 set_error(ctx, "incremental: event record failed");
                 ret = DYG_ERR_HIP;
@@ -217,7 +231,10 @@ using namespace dyg;
 extern "C" dyg_status dyg_graph_reserve(dyg_graph g, uint64_t max_inserts, uint32_t max_batch) {
     if (!g) return DYG_ERR_ARG;
     if (g->keyed) return DYG_OK;  // keyed graphs take host-planned edits
-    return device_edits_prepare(g, max_batch, max_inserts);
+    const dyg_status st = device_edits_prepare(g, max_batch, max_inserts);
+    if (st != DYG_OK) return st;
+    // the run-wide order of a run of max_inserts updates
+    return max_inserts <= 0x3FFFFFFFull ? device_edits_reserve_run(g, max_inserts) : DYG_OK;
 }
 
 extern "C" dyg_status dyg_incremental_update(dyg_graph H, const uint32_t* src, const uint32_t* dst,

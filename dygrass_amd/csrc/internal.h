@@ -79,6 +79,11 @@ struct DevEdits {
     uint8_t* mark = nullptr;   // [2 * max_batch] touched rows of a presorted edit (by position)
     void* temp = nullptr;
     size_t temp_bytes = 0;
+    // a whole run's row entries, every batch's sorted ahead of its walks
+    // (device_edits_order_run): keys and values in and out, then sort scratch
+    DevBuf run;
+    uint64_t run_cap = 0;  // updates the buffer holds
+    size_t run_temp_bytes = 0;
     void release();
     ~DevEdits() { release(); }
 };
@@ -147,6 +152,16 @@ dyg_status device_edits_prepare(dyg_graph g, uint32_t max_batch, uint64_t max_in
 dyg_status device_edits_batch(dyg_graph g, const uint32_t* src, const uint32_t* dst,
                               const float* w, const int32_t* found, uint32_t n,
                               const uint32_t* n_dev = nullptr);
+// The row entries of a run of m updates in batches of I (batch b = i / I),
+// computed in one sort ahead of the run's walks: entry 2i / 2i+1 of update i
+// is src[i]'s / dst[i]'s row, valued by its entry index within its batch;
+// batch b's 2n entries end up at [2bI, 2bI + 2n) of *row / *ent, by row and
+// then by entry, as BatchOrder::row / ent are for one batch.  *ok is false
+// (and nothing enqueued) when batch and row bits exceed 32.
+dyg_status device_edits_reserve_run(dyg_graph g, uint64_t m);
+dyg_status device_edits_order_run(dyg_graph g, const uint32_t* src, const uint32_t* dst,
+                                  uint32_t m, uint32_t I, const uint32_t** row,
+                                  const uint32_t** ent, bool* ok);
 // Swap-with-last removal of (src[i], dst[i]) for every i < n (or < *n_dev)
 // with flag[i] == 1, in batch order per row.  Each edge must be present.
 dyg_status device_deletes_batch(dyg_graph g, const uint32_t* src, const uint32_t* dst,

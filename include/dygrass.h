@@ -228,8 +228,9 @@ dyg_status dyg_incremental_update_dev(dyg_graph H, const uint32_t* src, const ui
                                       const dyg_walk_params* p, int32_t* found_at, float* cond,
                                       dyg_run_stats* st);
 /* Pre-allocate for device-side edits: slot-arena room for `max_inserts`
- * appended edges and the ordering buffers for batches of up to `max_batch`
- * edges, so that a following dyg_incremental_update* allocates nothing.
+ * appended edges, the ordering buffers for batches of up to `max_batch`
+ * edges and for a run of `max_inserts` updates sorted at once, so that a
+ * following dyg_incremental_update* allocates nothing.
  * No counterpart in the reference (its arena is fixed at 2x, unchecked:
  * random_walk_incremental/gpu_graph.cuh:210,357-384). */
 dyg_status dyg_graph_reserve(dyg_graph g, uint64_t max_inserts, uint32_t max_batch);
