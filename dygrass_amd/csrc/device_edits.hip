@@ -74,7 +74,7 @@ __device__ __forceinline__ bool takes(const EditTake& t, uint32_t i) {
 // kernels are needed; it marks its position in `mark` for the descriptor
 // refresh (every position is written: 1 at a touched row's first entry).
 // Edges edited are counted once, at their even entry, one atomic per block.
-constexpr uint32_t kTakeBlock = 1024;
+constexpr uint32_t kTakeBlock = 256;
 
 __device__ __forceinline__ uint32_t run_end(const uint32_t* __restrict__ row, uint32_t p,
                                             uint32_t n2) {
