@@ -127,6 +127,37 @@ def test_incremental_update_matches_oracle_run(ctx, grid_wl, per_edge, host_edit
     assert H.check() == 0
 
 
+@pytest.mark.parametrize("divisor", [1, 7, 64])
+def test_incremental_run_order_any_batching(ctx, grid_wl, divisor):
+    """The device-edited run sorts every batch's row entries at once, keyed
+    by batch and row: one batch (no batch bits), uneven batches (a short last
+    one) and many small ones all give the oracle's decisions and slot order."""
+    from dygrass_amd import DeviceGraph, incremental_update, params
+
+    po = _oracle()
+    rp, col, w = csr(grid_wl.H)
+    H = DeviceGraph(ctx, grid_wl.H.V, rp, col, w, slack=1)
+    u, v, x = grid_wl.ins.u, grid_wl.ins.v, grid_wl.ins.w
+    print(f"incremental run order: divisor {divisor}", flush=True)
+    found, stats = incremental_update(H, u, v, x, params(128, 100, 10.0, seed=3),
+                                      divisor=divisor)
+    og = po.Graph(grid_wl.H.V, rp, col, w)
+    starts = po.batches(len(u), divisor)
+    expect = np.zeros(len(u), np.int32)
+    for b0, b1 in zip(starts[:-1], starts[1:]):
+        f, _, _, _ = og.walk_inc_batch(u[b0:b1], v[b0:b1], x[b0:b1], 10.0, 128, seed=3,
+                                       edge_id=np.arange(b0, b1), threads=8)
+        expect[b0:b1] = f
+        for i in np.nonzero(f == -1)[0]:
+            og.insert(int(u[b0 + i]), int(v[b0 + i]), float(x[b0 + i]))
+    assert np.array_equal(found, expect)
+    assert stats["batches"] == len(starts) - 1
+    g, o = H.download(), og.export()
+    assert np.array_equal(g[0], o[0]) and np.array_equal(g[1], o[1])
+    assert np.array_equal(f32bits(g[2]), f32bits(o[2]))
+    assert H.check() == 0
+
+
 def test_device_edits_chain_with_host_edits(ctx, grid_wl):
     """Two device-edited runs back to back, then host-planned inserts on the
     same graph (the host re-reads the device state first)."""
