@@ -608,7 +608,8 @@ dyg_status device_edits_order_run(dyg_graph g, const uint32_t* src, const uint32
                                   uint32_t m, uint32_t I, const uint32_t** row,
                                   const uint32_t** ent, bool* ok) {
     *ok = false;
-    if (m == 0 || I == 0) return DYG_OK;
+    // (a run too long for one sort's item count keeps the per-batch pipeline)
+    if (m == 0 || I == 0 || 2ull * m > 0x7FFFFFFFull) return DYG_OK;
     const int rb = bits_for(std::max<uint32_t>(g->V, 2)), bb = bits_for((m + I - 1) / I);
     if (rb + bb > 32) return DYG_OK;
     dyg_status st = device_edits_reserve_run(g, m);

@@ -157,7 +157,8 @@ dyg_status device_edits_batch(dyg_graph g, const uint32_t* src, const uint32_t* 
 // is src[i]'s / dst[i]'s row, valued by its entry index within its batch;
 // batch b's 2n entries end up at [2bI, 2bI + 2n) of *row / *ent, by row and
 // then by entry, as BatchOrder::row / ent are for one batch.  *ok is false
-// (and nothing enqueued) when batch and row bits exceed 32.
+// (and nothing enqueued) when batch and row bits exceed 32 or 2m entries
+// exceed one sort's item count.
 dyg_status device_edits_reserve_run(dyg_graph g, uint64_t m);
 dyg_status device_edits_order_run(dyg_graph g, const uint32_t* src, const uint32_t* dst,
                                   uint32_t m, uint32_t I, const uint32_t** row,
